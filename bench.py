@@ -1,0 +1,194 @@
+"""Benchmark of the MI355X RK3 step on its headline workload.
+
+Workload (N = 1 GPU): 512^3 homogeneous shear in the Rogallo frame (F_TYPE = 2,
+NS_BF_ROT, S = 1, nu = 0.005, RK3_TIMESTEP = DECIDE with F_REMESH_T = 250, i.e.
+dt = 0.002) with 1,048,576 passive tracers (P_INTERP_ORDER = 4), seeded
+type-10 initial field (k0 = 6).  The timed value is the FP32-mixed form; the
+FP64 form of the same case is timed beside it.  A "step" is one g_RK3Calc
+(three RK stages) with all data resident in HBM.
+
+Prints one JSON line (see the metric/config keys).  --gpus N > 1 is refused:
+the path runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pandora-2.0_amd"))
+
+METRIC = "grid-pt RK3 steps/s at 512³ shear, FP32-mixed vs FP64 GPU; % HBM peak"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def model_bytes_per_step(g, csz, rsz, ntr):
+    """Algorithmic HBM bytes of one RK3 step in the reference's slab dataflow
+    (compact wave state + 1r2w intermediates, x-y planes on chip):
+    3 * ((15 Mw + 18 M1) * s_c + 96 N_tr + 3 N^3 s_r)."""
+    n3 = g.nx * g.ny * g.nz
+    return 3 * ((15 * g.mw + 18 * g.m1) * csz + (96 * ntr + 3 * n3 * rsz if ntr else 0))
+
+
+def kernel_bytes(g, name, csz, rsz, ntr):
+    """Compulsory bytes one launch of each kernel must move (its inputs + outputs)."""
+    m2 = g.nz * g.ny * g.nki
+    n3 = g.nx * g.ny * g.nz
+    return {
+        "zinv": (3 * g.mw + 6 * g.m1) * csz,
+        "yinv": (6 * g.m1 + 6 * m2) * csz,
+        "xpass": 9 * m2 * csz + (3 * n3 * rsz if ntr else 0),
+        "yfwd": (3 * m2 + 3 * g.m1) * csz,
+        "zfwd": (3 * g.m1 + 3 * g.mw) * csz,
+        "formf": 15 * g.mw * csz,
+        "tracers": 96 * ntr + 3 * n3 * rsz,
+    }.get(name)
+
+
+def run_gpu(pa, n, precision, steps, warmup, ntr, profile):
+    g = pa.Pandora(n, f_type=pa.F_SHEAR_DNS, rk3_timestep=pa.RK3_TIMESTEP_DECIDE, deform_s=1.0,
+                   remesh_t=250, nu=0.005, precision=precision)
+    g.init_field(2, 2.0 / 3.0, 6.0)
+    if ntr:
+        g.init_particles(102, ntr)
+    t = 0.0
+    if warmup:
+        t = g.run(1, warmup, t)
+    g.synchronize()
+    if profile:
+        g.profile(True)
+    t0 = time.perf_counter()
+    t = g.run(1 + warmup, steps, t)
+    g.synchronize()
+    t1 = time.perf_counter()
+    prof = g.profile_get() if profile else {}
+    ok = g.check_finite()
+    g.close()
+    return (t1 - t0), prof, ok
+
+
+def cpu_baseline(n, ntr):
+    """One RK3 stage (a third of a step) of the same workload on the FP64 oracle,
+    OpenMP over the host cores; scaled to grid-point-steps/s."""
+    sys.path.insert(0, ROOT)
+    import numpy as np
+
+    import oracle
+    import pandora_amd as pa
+
+    oracle.build()
+    cores = min(os.cpu_count() or 1, 16)
+    u = pa.init_field_host(n, 2, 2.0 / 3.0, 6.0)
+    o = oracle.Oracle(n, f_type=2, rk3_timestep=1, remesh_t=250, deform_s=1.0, nu=0.005,
+                      nthreads=cores)
+    o.set_state(*u)
+    del u
+    rng = np.random.default_rng(102)
+    dx = pa.TWOPI_REF / n
+    o.set_particles(rng.uniform(-0.5 * dx, pa.TWOPI_REF - 0.5 * dx, size=(ntr, 3)))
+    t0 = time.perf_counter()
+    o.stage(1, 1, 0.0, 0.0)
+    t1 = time.perf_counter()
+    o.close()
+    stage_s = t1 - t0
+    return {"value": n**3 / (3.0 * stage_s), "unit": "grid-point-steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"one RK3 stage (1/3 step) of the {n}^3 shear + {ntr} tracer workload on the "
+                      f"FP64 C oracle, {cores} OpenMP threads: {stage_s:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--tracers", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this path runs on one GPU (the whole box stays resident on one "
+              "MI355X); multi-GPU runs are not supported", file=sys.stderr)
+        sys.exit(2)
+    import pandora_amd as pa
+
+    n, ntr = args.n, args.tracers
+    print(f"[bench] FP32-mixed {n}^3 shear + {ntr} tracers: {args.warmup} warmup, {args.steps} timed",
+          file=sys.stderr, flush=True)
+    dt32, prof, ok32 = run_gpu(pa, n, pa.PDR_FP32_MIXED, args.steps, args.warmup, ntr, True)
+    sps32 = args.steps / dt32
+    fp64 = None
+    if not args.no_fp64:
+        k64 = max(3, args.steps // 3)
+        print(f"[bench] FP64 {n}^3: 1 warmup, {k64} timed", file=sys.stderr, flush=True)
+        dt64, _, ok64 = run_gpu(pa, n, pa.PDR_FP64, k64, 1, ntr, False)
+        sps64 = k64 / dt64
+        fp64 = {"steps_per_s": sps64, "grid_point_steps_per_s": sps64 * n**3, "ms_per_step": 1e3 / sps64,
+                "finite": bool(ok64)}
+
+    class G:
+        pass
+
+    g = G()
+    g.nx = g.ny = g.nz = n
+    g.nki, g.nkj, g.nkk = n // 3 + 1, 2 * (n // 3) + 1, 2 * (n // 3) + 1
+    g.mw = g.nki * g.nkj * g.nkk
+    g.m1 = g.nki * g.nkj * n
+    dom, best = None, -1.0
+    for k, (cnt, ms) in prof.items():
+        if cnt and ms > best and kernel_bytes(g, k, 8, 4, ntr):
+            dom, best = k, ms
+    roof = None
+    kernels = {}
+    for k, (cnt, ms) in prof.items():
+        if cnt:
+            b = kernel_bytes(g, k, 8, 4, ntr)
+            kernels[k] = {"launches": cnt, "avg_ms": ms / cnt,
+                          "GBps": (b / (ms / cnt * 1e-3) / 1e9) if b else None}
+    if dom:
+        cnt, ms = prof[dom]
+        avg = ms / cnt * 1e-3
+        b = kernel_bytes(g, dom, 8, 4, ntr)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": b / avg / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": b / avg / HBM_PEAK, "traffic": None,
+                "bytes_per_launch": b, "avg_launch_ms": avg * 1e3}
+    bstep = model_bytes_per_step(g, 8, 4, ntr)
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] CPU baseline (FP64 oracle, one stage)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(n, ntr)
+    out = {
+        "metric": METRIC,
+        "value": sps32 * n**3,
+        "unit": "grid-point-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt32 / args.steps,
+        "steps_per_s": sps32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-mixed (f64 frame, dt, viscous factors, tracer positions)",
+        "data": "synthetic: seeded Rogallo type-10 field (k0=6) + uniform random tracers",
+        "config": {"workload": f"{n}^3 homogeneous shear, Rogallo frame BF_ROT, S=1, nu=0.005, "
+                               f"dt=0.002 (F_REMESH_T=250), {ntr} tracers, P_INTERP_ORDER=4",
+                   "n": n, "tracers": ntr, "precision": "fp32-mixed"},
+        "fp64": fp64,
+        "fp32_over_fp64": (sps32 / fp64["steps_per_s"]) if fp64 else None,
+        "step_model": {"bytes_per_step": bstep, "achieved_GBps": bstep * sps32 / 1e9,
+                       "frac_of_peak": bstep * sps32 / HBM_PEAK},
+        "kernels": kernels,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "finite": bool(ok32),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

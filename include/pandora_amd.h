@@ -1,0 +1,230 @@
+/*
+ * pandora_amd.h -- C-ABI of the MI355X-native PANDORA 2.0 pseudo-spectral RK3 step.
+ *
+ * The whole periodic (or Rogallo shear-deforming) box lives on one GPU.  The
+ * Fortran host keeps its own drivers and calls these entry points through
+ * ISO_C_BINDING (pandora-2.0_amd/fortran/pdr_rk3_amd.f90 is the shim that keeps
+ * the reference's g_RK3Calc / f_RK3Stage* / p_RK3SolveParticle* signatures).
+ *
+ * Conventions
+ *  - Every function returns PDR_OK (0) or a negative PDR_ERR_* code; the text
+ *    of the last error of the calling thread is returned by pdr_last_error().
+ *    The Fortran shim maps the code onto the reference's `ierr` argument.
+ *  - Host arrays are borrowed for the duration of a call; the context owns all
+ *    device memory.  Calls are synchronous with respect to host buffers.
+ *  - Wave-space host arrays use the reference's pf_arrays layout: Fortran
+ *    arr(0:1, 0:NKK-1, 0:NKJ-1, 0:NKI-1) of real(C_DOUBLE), i.e. (re, im)
+ *    fastest, then k, then j, then i (src/pf_arrays.f90:267-280, 761-810),
+ *    with k,j = 0..K positive and K+1..2K holding -K..-1, i = 0..KI.
+ *  - 3x3 tensors bmat/binv/amat are Fortran column-major (src/pg_domain.f90:78-87).
+ *  - Particle arrays are the reference's particle_xp(1:3, np) (src/pp_arrays.f90:25-34).
+ *  - One context per host thread; a context is not thread-safe (neither is the
+ *    reference's module state).
+ */
+#ifndef PANDORA_AMD_H
+#define PANDORA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDR_OK               0
+#define PDR_ERR_ARG         -1   /* invalid argument or parameter combination  */
+#define PDR_ERR_HIP         -2   /* HIP runtime error (message has details)    */
+#define PDR_ERR_NOMEM       -3   /* device or host allocation failed           */
+#define PDR_ERR_UNSUPPORTED -4   /* option the reference has but this build lacks */
+#define PDR_ERR_IO          -5   /* restart file could not be read / written   */
+#define PDR_ERR_STATE       -6   /* call out of order (e.g. no particles set)  */
+#define PDR_ERR_NONFINITE   -7   /* a NaN/Inf was detected in the state        */
+
+#define PDR_FP64        0        /* every field and transform in FP64 (parity form) */
+#define PDR_FP32_MIXED  1        /* FFTs, products and state in FP32; frame, dt,
+                                    viscous factors and tracer positions in FP64 */
+
+/* Reference parameter values (src/pg_parameters.f90) */
+#define PDR_F_ISOTROPIC           1
+#define PDR_F_SHEAR_DNS           2
+#define PDR_F_SHEAR_RDT_VISCOUS   3
+#define PDR_F_SHEAR_RDT_INVISCID  4
+#define PDR_NS_BF_ROT             1
+#define PDR_RK3_TIMESTEP_DECIDE   1
+#define PDR_RK3_TIMESTEP_FIXED    2
+#define PDR_RK3_TIMESTEP_CFL      3
+
+typedef struct pdr_ctx pdr_ctx;
+
+/* Run parameters: the subset of the reference's namelist globals
+ * (src/pg_parameters.f90:24-300, read in src/pg_control.f90:385-436)
+ * that the time step uses. */
+typedef struct pdr_params {
+  int32_t nodes_x, nodes_y, nodes_z; /* NODES_X/Y/Z: 2^a * {1,3,5} (radix 2/3/5 FFTs)  */
+  int32_t f_type;                    /* F_TYPE: 1 isotropic, 2 shear DNS, 3/4 shear RDT */
+  int32_t ns_deformed;               /* NS_DEFORMED: 1 = NS_BF_ROT                      */
+  int32_t dealiasing;                /* DEALIASING: 1 = cubic 2/3 rule by storage       */
+  int32_t rk3_timestep;              /* RK3_TIMESTEP: 1 DECIDE, 2 FIXED, 3 CFL          */
+  int32_t f_remesh_t;                /* F_REMESH_T (shear, DECIDE)                      */
+  int32_t b110;                      /* B110 (box stretch in x1; 1 for all configs)     */
+  int32_t p_interp_order;            /* P_INTERP_ORDER: even, 2..8 (reference default 4) */
+  int32_t precision;                 /* PDR_FP64 or PDR_FP32_MIXED                      */
+  int32_t reserved;                  /* must be 0                                       */
+  double  rk3_dt;                    /* RK3_DT  (FIXED)                                 */
+  double  rk3_cfl;                   /* RK3_CFL (CFL, DECIDE-isotropic)                 */
+  double  f_nu;                      /* F_NU                                            */
+  double  f_deform_s;                /* F_DEFORM_S: amat(1,3) = dU1/dx3                 */
+  double  force_tf;                  /* FORCE_TF: dt cap in DECIDE-isotropic mode       */
+} pdr_params;
+
+/* ---- library / context -------------------------------------------------- */
+
+/* Version string of this library. No reference counterpart. */
+const char* pdr_version(void);
+
+/* Text of the last error on this thread ("" if none). No reference counterpart
+ * (the reference prints and calls MPI_Abort). */
+const char* pdr_last_error(void);
+
+/* Create a context: validates parameters, computes the domain (g_DomainCalc,
+ * src/pg_domain.f90:147-243), allocates the resident device state and work
+ * buffers (f_ArraysInit, src/pf_arrays.f90:165) and plans the transforms
+ * (f_FftwInit, src/pf_fftw.f90:98-315).  State and RK registers start at zero,
+ * bmat/binv/amat at their g_DomainCalc values, dt at 99999.0 (src/pg_rk3.f90:92). */
+int pdr_create(pdr_ctx** out, const pdr_params* params);
+
+/* Release all device and host memory of a context (ProgramFinalise,
+ * src/pg_control.f90:200-259). */
+int pdr_destroy(pdr_ctx* ctx);
+
+/* Sizes: out[0..7] = NODES_KI, NODES_KJ, NODES_KK, Mw (modes per wave field),
+ * M1 (modes per 1r2w field), kkmax (spectrum bins), device bytes held,
+ * number of tracers.  (src/pg_domain.f90:220-228, src/pf_fluidstats.f90:343-374) */
+int pdr_sizes(const pdr_ctx* ctx, int64_t out[8]);
+
+/* ---- state ---------------------------------------------------------------- */
+
+/* Copy the velocity u1_3w, u2_3w, u3_3w in (pf_arrays layout, FP64) to the device
+ * (the DMDAVecGetArray/VecLoad path of src/pg_restart.f90:39-97). */
+int pdr_upload_state(pdr_ctx* ctx, const double* u1, const double* u2, const double* u3);
+/* Copy the velocity back to the host (pf_arrays layout, FP64). */
+int pdr_download_state(pdr_ctx* ctx, double* u1, double* u2, double* u3);
+/* Low-storage RK registers rk3_3w_1..3 (src/pf_arrays.f90:777-781). */
+int pdr_upload_rk3(pdr_ctx* ctx, const double* r1, const double* r2, const double* r3);
+int pdr_download_rk3(pdr_ctx* ctx, double* r1, double* r2, double* r3);
+
+/* Shear-frame tensors bmat, binv (Fortran column-major, FP64; g_domain module
+ * variables, src/pg_domain.f90:78-87). */
+int pdr_set_frame(pdr_ctx* ctx, const double bmat[9], const double binv[9]);
+int pdr_get_frame(const pdr_ctx* ctx, double bmat[9], double binv[9]);
+
+/* sim_dt, the SAVEd step width of g_RK3Calc (src/pg_rk3.f90:92). */
+int pdr_set_dt(pdr_ctx* ctx, double dt);
+int pdr_get_dt(const pdr_ctx* ctx, double* dt);
+
+/* ---- time stepping ------------------------------------------------------- */
+
+/* One Runge-Kutta stage: f_RK3Stage1/2/3(tstep, time, dt, ierr)
+ * (src/pf_rk3.f90:40, 121, 161).  Stage 1 also does the shear remesh check
+ * (f_RK3Remesh, src/pf_rk3.f90:3958) and sets *dt (f_RK3Tstep, 3852-3948).
+ * Tracers, if set, are advanced inside the stage as in f_RK3RealSpace
+ * (src/pf_rk3.f90:1145-1176).  The state-space statistics calls of the
+ * reference stage are not made (see pdr_spectrum). */
+int pdr_stage(pdr_ctx* ctx, int32_t stage, int32_t tstep, double time, double* dt);
+
+/* Remesh check and remesh of the Rogallo frame: f_RK3Remesh(tstep, ierr)
+ * (src/pf_rk3.f90:3958-4049) with f_RK3RemeshWaveSpace (4056-4178) and
+ * p_ControlRemesh (src/pp_control.f90:1658-1743).  pdr_stage(1, ...) calls it
+ * first; *done (may be NULL) is set to 1 if this tstep remeshed. */
+int pdr_remesh(pdr_ctx* ctx, int32_t tstep, int32_t* done);
+
+/* Shear-frame update after a stage: g_RK3UpdateB(tstep, dt, simtime)
+ * (src/pg_rk3.f90:280-385).  FP64 on the host. */
+int pdr_update_b(pdr_ctx* ctx, int32_t tstep, double simtime);
+
+/* One full time step: g_RK3Calc(tstep, simtime, ierr) (src/pg_rk3.f90:82-129). */
+int pdr_step(pdr_ctx* ctx, int32_t tstep, double* simtime);
+
+/* nsteps consecutive g_RK3Calc calls (tstep0 .. tstep0+nsteps-1) without a host
+ * round trip between steps: the body of TIMELOOP (src/pg_control.f90:177-194)
+ * minus its reporting. */
+int pdr_run(pdr_ctx* ctx, int32_t tstep0, int32_t nsteps, double* simtime);
+
+/* Wait for all queued device work of the context. */
+int pdr_synchronize(pdr_ctx* ctx);
+
+/* Checks that the velocity state is finite (no NaN/Inf); returns PDR_OK or
+ * PDR_ERR_NONFINITE.  Replaces the reference's absence of such a check
+ * (failure detection for long runs). */
+int pdr_check_finite(pdr_ctx* ctx);
+
+/* ---- tracers ------------------------------------------------------------- */
+
+/* Set np passive tracers at positions xp(1:3, np) (moving frame), zeroing
+ * particle_up_rk3: p_ControlInitialiseParticles' result (src/pp_control.f90:589-727)
+ * handed over by the host.  np = 0 removes them. */
+int pdr_particles_upload(pdr_ctx* ctx, const double* xp, int64_t np);
+/* Read positions xp(1:3,np) and, if up_rk3 != NULL, particle_up_rk3(1:3,np). */
+int pdr_particles_download(pdr_ctx* ctx, double* xp, double* up_rk3);
+
+/* ---- initial conditions, restart, statistics ----------------------------- */
+
+/* Seeded initial field: Rogallo alpha/beta solenoidal construction of
+ * f_InitialiseFluid (src/pf_initialise.f90:97-512) with the spectrum of
+ * f_InitialiseSpec type 10, Ahmed/Elghobashi (src/pf_initialise.f90:1081-1083):
+ * E(k) = 16 sqrt(2/pi) (3/2) U0 k^4/kp^5 exp(-2k^2/kp^2).  The gfortran
+ * intrinsic RNG is replaced by a counter-based generator, so fields are
+ * reproducible from (seed) on any machine but are not the reference's stream.
+ * Hermitian symmetry on the i = 0 plane is enforced as in the reference. */
+int pdr_init_field(pdr_ctx* ctx, uint64_t seed, double init_u0, double init_kp);
+
+/* Host-only form of pdr_init_field: fills u1..u3 (pf_arrays layout for an
+ * nx*ny*nz grid, 2*NKI*NKJ*NKK doubles each) without a context or a GPU, for
+ * hosts that build the initial state in their own arrays (f_InitialiseFluid,
+ * src/pf_initialise.f90:97-512). */
+int pdr_init_field_host(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double init_u0,
+                        double init_kp, double* u1, double* u2, double* u3);
+
+/* Uniform random tracer positions in [-dx/2, L-dx/2)^3 (P_INIT_RANDOM,
+ * src/pp_control.f90:657-686) from a counter-based generator. */
+int pdr_init_particles(pdr_ctx* ctx, uint64_t seed, int64_t np);
+
+/* PETSc-binary restart of u1_3w, u2_3w, u3_3w: g_RestartPetscFluid
+ * (src/pg_restart.f90:39-194).  Three records of {int32 1211214 (VEC_FILE_CLASSID),
+ * PetscInt length, big-endian float64 data in DMDA natural order}; int64_header
+ * selects 64-bit PetscInt (--with-64-bit-indices builds). */
+int pdr_restart_write(pdr_ctx* ctx, const char* path, int32_t int64_header);
+/* Reads either header width (detected from the file size). */
+int pdr_restart_read(pdr_ctx* ctx, const char* path);
+
+/* Host-only restart I/O of three n-double records (n = 2*NKI*NKJ*NKK), same
+ * format as pdr_restart_write/read; no context or GPU needed. */
+int pdr_restart_write_host(const char* path, int64_t n, const double* u1, const double* u2,
+                           const double* u3, int32_t int64_header);
+int pdr_restart_read_host(const char* path, int64_t n, double* u1, double* u2, double* u3);
+
+/* Wave-space statistics of the current velocity: f_FluidstatsFsPrimaryQuants
+ * (src/pf_fluidstats.f90:1423-1576) before mode-density normalisation.
+ * ek, dk, dsk have kkmax entries (shell sums of energy, 2 nu k^2 E and
+ * 4 nu^2 k^4 E); tot[6] = {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}.
+ * Sums are FP64 with compensated (Kahan-Neumaier) accumulation. */
+int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6]);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+
+/* Copy 1r2w intermediate field f (0..5 = u1..u6_1r2w after the z pass of
+ * f_RK3PrepareRealSpace, src/pf_rk3.f90:205-948) to the host as
+ * (0:1, 0:NKJ-1, 0:NZ-1, 0:NKI-1) FP64.  Runs pass 1 of a stage on the
+ * current state; the state is not modified. */
+int pdr_debug_prepare(pdr_ctx* ctx, int32_t field, double* out);
+
+/* Per-kernel timing with HIP events on the context's stream.  enable != 0
+ * starts collection; pdr_profile_get returns, for up to n kernel classes,
+ * names (comma-separated into names[namelen]), launch counts and total ms. */
+int pdr_profile_enable(pdr_ctx* ctx, int32_t enable);
+int pdr_profile_get(pdr_ctx* ctx, int32_t n, char* names, int32_t namelen,
+                    int64_t* counts, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PANDORA_AMD_H */

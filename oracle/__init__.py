@@ -1,0 +1,235 @@
+"""FP64 CPU oracle for PANDORA 2.0's pseudo-spectral RK3 step.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()`` and
+by ``bench.py``'s cpu_baseline leg, where it is the parity checker (and the
+"port" CPU baseline).  The product in ``pandora-2.0_amd/`` never imports it.
+
+The C restatement lives in ``oracle/pdr_oracle.c`` (each routine cites the
+reference file:line it follows); this module is a thin ctypes wrapper.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liborc.so")
+_lib = None
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("nodes_x", C.c_int32), ("nodes_y", C.c_int32), ("nodes_z", C.c_int32),
+        ("f_type", C.c_int32), ("ns_deformed", C.c_int32), ("rk3_timestep", C.c_int32),
+        ("rk3_dt", C.c_double), ("rk3_cfl", C.c_double), ("f_nu", C.c_double),
+        ("f_deform_s", C.c_double), ("f_remesh_t", C.c_int32), ("b110", C.c_int32),
+        ("force_tf", C.c_double), ("p_interp_order", C.c_int32), ("nthreads", C.c_int32),
+    ]
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        vp = C.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [C.POINTER(OrcParams)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_sizes.argtypes = [vp, np.ctypeslib.ndpointer(dtype=np.int64)]
+        for name in ("orc_set_state", "orc_get_state", "orc_set_rk", "orc_get_rk", "orc_nl_forward"):
+            getattr(L, name).argtypes = [vp, dp, dp, dp]
+        L.orc_set_frame.argtypes = [vp, dp, dp]
+        L.orc_get_frame.argtypes = [vp, dp, dp]
+        L.orc_set_dt.argtypes = [vp, C.c_double]
+        L.orc_get_dt.restype = C.c_double
+        L.orc_get_dt.argtypes = [vp]
+        L.orc_get_ucfl.restype = C.c_double
+        L.orc_get_ucfl.argtypes = [vp]
+        L.orc_step.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
+        L.orc_stage.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_double)]
+        L.orc_update_b.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_prepare_realspace.argtypes = [vp]
+        L.orc_realspace.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.orc_get_1r2w.argtypes = [vp, C.c_int, dp]
+        L.orc_remesh.restype = C.c_int
+        L.orc_remesh.argtypes = [vp, C.c_int]
+        L.orc_remesh_target.restype = C.c_int
+        L.orc_remesh_target.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.orc_set_particles.argtypes = [vp, dp, C.c_int64]
+        L.orc_get_particles.argtypes = [vp, dp, dp]
+        L.orc_get_tracer_field.argtypes = [vp, dp, dp, dp]
+        L.orc_interpolate.argtypes = [vp, dp, dp, dp, dp, C.c_int64, dp]
+        L.orc_correct_position.argtypes = [vp, dp, C.c_int64]
+        L.orc_fft.argtypes = [C.c_int, C.c_int, dp, dp]
+        L.orc_spectrum.argtypes = [vp, dp, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def fft(x: np.ndarray, sign: int) -> np.ndarray:
+    """Unnormalised complex DFT with exponent sign*2*pi*i*n*k/N (FFTW convention)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    inp = x.view(np.float64).copy()
+    out = np.zeros_like(inp)
+    lib().orc_fft(len(x), sign, inp, out)
+    return out.view(np.complex128)
+
+
+def remesh_target(kmax: int, i: int, k: int) -> int:
+    return lib().orc_remesh_target(kmax, i, k)
+
+
+class Oracle:
+    """FP64 reference-step restatement.  Wave arrays are complex128 numpy
+    arrays shaped (NKI, NKJ, NKK) = Fortran arr(0:1, k, j, i) in C order."""
+
+    def __init__(self, n=None, *, nx=None, ny=None, nz=None, f_type=1, ns_deformed=1,
+                 rk3_timestep=2, rk3_dt=0.005, rk3_cfl=0.75, nu=0.01, deform_s=0.0,
+                 remesh_t=0, b110=1, force_tf=0.0, interp_order=4, nthreads=0):
+        nx = nx or n
+        ny = ny or n
+        nz = nz or n
+        self.p = OrcParams(nx, ny, nz, f_type, ns_deformed, rk3_timestep, rk3_dt, rk3_cfl,
+                           nu, deform_s, remesh_t, b110, force_tf, interp_order, nthreads)
+        self.h = lib().orc_create(C.byref(self.p))
+        s = np.zeros(6, dtype=np.int64)
+        lib().orc_sizes(self.h, s)
+        self.nki, self.nkj, self.nkk, self.mw, self.m1, self.kkmax = (int(v) for v in s)
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (self.nki, self.nkj, self.nkk)
+        self.np = 0
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state -------------------------------------------------------------
+    def _wave(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.shape)
+        return a.view(np.float64).reshape(-1)
+
+    def set_state(self, u1, u2, u3):
+        lib().orc_set_state(self.h, self._wave(u1), self._wave(u2), self._wave(u3))
+
+    def get_state(self):
+        out = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        lib().orc_get_state(self.h, *[o.view(np.float64).reshape(-1) for o in out])
+        return out
+
+    def set_rk(self, r1, r2, r3):
+        lib().orc_set_rk(self.h, self._wave(r1), self._wave(r2), self._wave(r3))
+
+    def get_rk(self):
+        out = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        lib().orc_get_rk(self.h, *[o.view(np.float64).reshape(-1) for o in out])
+        return out
+
+    def set_frame(self, bmat, binv):
+        lib().orc_set_frame(self.h, np.ascontiguousarray(bmat, np.float64).reshape(9),
+                            np.ascontiguousarray(binv, np.float64).reshape(9))
+
+    def get_frame(self):
+        b = np.zeros(9)
+        bi = np.zeros(9)
+        lib().orc_get_frame(self.h, b, bi)
+        return b, bi
+
+    @property
+    def dt(self):
+        return lib().orc_get_dt(self.h)
+
+    @dt.setter
+    def dt(self, v):
+        lib().orc_set_dt(self.h, float(v))
+
+    # ---- stepping ----------------------------------------------------------
+    def step(self, tstep: int, simtime: float) -> float:
+        t = C.c_double(simtime)
+        lib().orc_step(self.h, tstep, C.byref(t))
+        return t.value
+
+    def stage(self, stage: int, tstep: int, time: float, dt: float) -> float:
+        d = C.c_double(dt)
+        lib().orc_stage(self.h, stage, tstep, time, C.byref(d))
+        return d.value
+
+    def prepare_realspace(self):
+        lib().orc_prepare_realspace(self.h)
+
+    def realspace(self, stage: int, tstep: int = 0, dt: float = 0.0) -> float:
+        d = C.c_double(dt)
+        lib().orc_realspace(self.h, stage, tstep, C.byref(d))
+        return d.value
+
+    def nl_forward(self):
+        out = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        lib().orc_nl_forward(self.h, *[o.view(np.float64).reshape(-1) for o in out])
+        return out
+
+    def get_1r2w(self, field: int) -> np.ndarray:
+        out = np.zeros((self.nki, self.nz, self.nkj), np.complex128)
+        lib().orc_get_1r2w(self.h, field, out.view(np.float64).reshape(-1))
+        return out
+
+    @property
+    def ucfl(self):
+        return lib().orc_get_ucfl(self.h)
+
+    def remesh(self, tstep: int) -> bool:
+        return bool(lib().orc_remesh(self.h, tstep))
+
+    # ---- tracers -----------------------------------------------------------
+    def set_particles(self, xp):
+        xp = np.ascontiguousarray(xp, np.float64).reshape(-1, 3)
+        self.np = xp.shape[0]
+        lib().orc_set_particles(self.h, xp.reshape(-1), self.np)
+
+    def get_particles(self):
+        xp = np.zeros((self.np, 3))
+        ur = np.zeros((self.np, 3))
+        lib().orc_get_particles(self.h, xp.reshape(-1), ur.reshape(-1))
+        return xp, ur
+
+    def tracer_field(self):
+        out = [np.zeros((self.nz, self.ny, self.nx)) for _ in range(3)]
+        lib().orc_get_tracer_field(self.h, *[o.reshape(-1) for o in out])
+        return out
+
+    def interpolate(self, f1, f2, f3, xp):
+        xp = np.ascontiguousarray(xp, np.float64).reshape(-1, 3)
+        uf = np.zeros_like(xp)
+        fs = [np.ascontiguousarray(f, np.float64).reshape(-1) for f in (f1, f2, f3)]
+        lib().orc_interpolate(self.h, *fs, xp.reshape(-1), xp.shape[0], uf.reshape(-1))
+        return uf
+
+    def correct_position(self, xp):
+        xp = np.ascontiguousarray(xp, np.float64).reshape(-1, 3).copy()
+        lib().orc_correct_position(self.h, xp.reshape(-1), xp.shape[0])
+        return xp
+
+    # ---- statistics --------------------------------------------------------
+    def spectrum(self):
+        ek = np.zeros(self.kkmax)
+        dk = np.zeros(self.kkmax)
+        ds = np.zeros(self.kkmax)
+        tot = np.zeros(6)
+        lib().orc_spectrum(self.h, ek, dk, ds, tot)
+        return ek, dk, ds, tot

@@ -1,0 +1,1044 @@
+/*
+ * pdr_oracle.c -- FP64 CPU restatement of PANDORA 2.0's pseudo-spectral RK3 step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the parity checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; the product never
+ * links or calls it.
+ *
+ * Each routine below follows the reference operation for operation (citations
+ * are file:line in the reference repository).  The FFT is this file's own
+ * recursive mixed-radix Cooley-Tukey transform with twiddles evaluated in long
+ * double; it follows the FFTW conventions the reference uses (sign -1 forward,
+ * +1 backward, unnormalised, c2r ignores the imaginary part of the DC term) and
+ * is checked against brute-force DFTs in tests/.
+ *
+ * Parity pins (tests/test_oracle.py): reference known answer for the nonlinear
+ * term (src/ptf_rk3.f90:1030-1198), brute-force DFTs and 3-D direct sums,
+ * single-mode viscous decay under the Williamson scheme, inviscid energy
+ * conservation of the truncated nonlinear term, S=0 shear == isotropic,
+ * shear energy budget, remesh map enumeration, Neville polynomial reproduction,
+ * uniform-flow tracer advection.
+ */
+#include "pdr_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Constants: src/pg_constants.f90:26-141.  LONGONEPI is assigned from a      */
+/* default-real (float32) literal, so ONEPI/TWOPI are float32 pi widened.     */
+/* ------------------------------------------------------------------------- */
+#define ONEPI_REF ((double)3.14159265358979323846f)
+#define TWOPI_REF (2.0 * ONEPI_REF)
+static const double THIRD = 1.0 / 3.0;
+static const double FIVE_OVER_NINE = 5.0 / 9.0;
+static const double FIFTEEN_OVER_SIXTEEN = 15.0 / 16.0;
+static const double ONEFIVETHREE_OVER_ONETWOEIGHT = 153.0 / 128.0;
+static const double EIGHT_OVER_FIFTEEN = 8.0 / 15.0;
+
+/* ------------------------------------------------------------------------- */
+/* FFT: recursive mixed-radix decimation in time                              */
+/* ------------------------------------------------------------------------- */
+typedef struct fftp {
+  int n;
+  int nf;
+  int f[48];
+  double* tw; /* tw[2m], tw[2m+1] = cos, sin of -2 pi m / n */
+} fftp;
+
+static void fftp_init(fftp* p, int n) {
+  p->n = n;
+  p->nf = 0;
+  int m = n;
+  while (m % 4 == 0) { p->f[p->nf++] = 4; m /= 4; }
+  while (m % 2 == 0) { p->f[p->nf++] = 2; m /= 2; }
+  for (int q = 3; m > 1; q += 2) {
+    while (m % q == 0) { p->f[p->nf++] = q; m /= q; }
+  }
+  p->tw = (double*)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1));
+  const long double pil = 3.14159265358979323846264338327950288L;
+  for (int k = 0; k < n; k++) {
+    long double a = -2.0L * pil * (long double)k / (long double)n;
+    p->tw[2 * k] = (double)cosl(a);
+    p->tw[2 * k + 1] = (double)sinl(a);
+  }
+}
+
+static void fftp_free(fftp* p) { free(p->tw); p->tw = NULL; }
+
+static void fft_rec(const fftp* p, const double* in, long is, double* out, int n, int fi,
+                    int tws, int sign) {
+  if (n == 1) { out[0] = in[0]; out[1] = in[1]; return; }
+  const int r = p->f[fi];
+  const int m = n / r;
+  for (int q = 0; q < r; q++)
+    fft_rec(p, in + 2 * q * is, is * r, out + 2 * (long)q * m, m, fi + 1, tws * r, sign);
+  const int N = p->n;
+  const double sg = (sign > 0) ? -1.0 : 1.0; /* conj of table for sign +1 */
+  double t[2 * 64];
+  for (int k = 0; k < m; k++) {
+    for (int q = 0; q < r; q++) {
+      double yr = out[2 * (q * m + k)], yi = out[2 * (q * m + k) + 1];
+      long e = ((long)q * k * tws) % N;
+      double wr = p->tw[2 * e], wi = sg * p->tw[2 * e + 1];
+      t[2 * q] = yr * wr - yi * wi;
+      t[2 * q + 1] = yr * wi + yi * wr;
+    }
+    for (int s = 0; s < r; s++) {
+      double ar = 0.0, ai = 0.0;
+      for (int q = 0; q < r; q++) {
+        long e = ((long)((q * s) % r) * (N / r)) % N;
+        double wr = p->tw[2 * e], wi = sg * p->tw[2 * e + 1];
+        ar += t[2 * q] * wr - t[2 * q + 1] * wi;
+        ai += t[2 * q] * wi + t[2 * q + 1] * wr;
+      }
+      out[2 * (k + s * m)] = ar;
+      out[2 * (k + s * m) + 1] = ai;
+    }
+  }
+}
+
+/* out must not alias in */
+static void fft_exec(const fftp* p, const double* in, double* out, int sign) {
+  fft_rec(p, in, 1, out, p->n, 0, 1, sign);
+}
+
+void orc_fft(int n, int sign, const double* in, double* out) {
+  fftp p;
+  fftp_init(&p, n);
+  fft_exec(&p, in, out, sign);
+  fftp_free(&p);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Context                                                                    */
+/* ------------------------------------------------------------------------- */
+struct orc {
+  orc_params p;
+  int nx, ny, nz;
+  int ki, kj, kk;       /* KI_MAX, KJ_MAX, KK_MAX   src/pg_domain.f90:220-228 */
+  int nki, nkj, nkk;    /* NODES_KI, NODES_KJ, NODES_KK */
+  long mw, m1;
+  int kkmax;
+  double bmat[9], binv[9], amat[9];
+  int bval[9], aval[9];
+  double dt;            /* sim_dt, SAVEd in g_RK3Calc (src/pg_rk3.f90:92) */
+  double ucfl;
+  double dx, dy, dz, lx, ly, lz;
+  double* u[3];
+  double* rk[3];
+  double* w[6];         /* u1..u6_1r2w */
+  double* ur[3];        /* real-space velocity for tracers */
+  long np;
+  double* xp;
+  double* up;
+  double* uprk;
+  fftp fz, fy, fx;
+  int nthreads;
+};
+
+#define T9(a, m, n) ((a)[((m) - 1) + 3 * ((n) - 1)])
+
+static int nthr(const orc* o) {
+#ifdef _OPENMP
+  return o->nthreads;
+#else
+  (void)o;
+  return 1;
+#endif
+}
+
+orc* orc_create(const orc_params* p) {
+  orc* o = (orc*)calloc(1, sizeof(orc));
+  o->p = *p;
+  o->nx = p->nodes_x; o->ny = p->nodes_y; o->nz = p->nodes_z;
+  /* src/pg_domain.f90:220-228 */
+  o->ki = o->nx / 3; o->kj = o->ny / 3; o->kk = o->nz / 3;
+  o->nki = o->ki + 1; o->nkj = 2 * o->kj + 1; o->nkk = 2 * o->kk + 1;
+  o->mw = (long)o->nki * o->nkj * o->nkk;
+  o->m1 = (long)o->nki * o->nkj * o->nz;
+  /* src/pf_fluidstats.f90:343-374 (k_min = 1) */
+  o->kkmax = (int)lround(sqrt((double)(o->ki * o->ki + o->kj * o->kj + o->kk * o->kk)));
+  /* src/pg_domain.f90:168-216 */
+  o->lx = TWOPI_REF; o->ly = TWOPI_REF; o->lz = TWOPI_REF;
+  o->dx = o->lx / (double)o->nx; o->dy = o->ly / (double)o->ny; o->dz = o->lz / (double)o->nz;
+  for (int a = 0; a < 9; a++) { o->bmat[a] = o->binv[a] = o->amat[a] = 0.0; o->bval[a] = o->aval[a] = 0; }
+  const int b110 = p->b110 > 0 ? p->b110 : 1;
+  T9(o->binv, 1, 1) = (double)b110; T9(o->bval, 1, 1) = 1;
+  T9(o->binv, 2, 2) = 1.0; T9(o->bval, 2, 2) = 1;
+  T9(o->binv, 3, 3) = 1.0; T9(o->bval, 3, 3) = 1;
+  if (p->f_type != 1 && p->f_type != 5) {
+    T9(o->bval, 1, 3) = 1;
+    T9(o->amat, 1, 3) = p->f_deform_s;
+    T9(o->aval, 1, 3) = 1;
+  }
+  T9(o->bmat, 1, 1) = 1.0 / T9(o->binv, 1, 1);
+  T9(o->bmat, 2, 2) = 1.0 / T9(o->binv, 2, 2);
+  T9(o->bmat, 3, 3) = 1.0 / T9(o->binv, 3, 3);
+  o->dt = 99999.0;
+  for (int c = 0; c < 3; c++) {
+    o->u[c] = (double*)calloc((size_t)(2 * o->mw), sizeof(double));
+    o->rk[c] = (double*)calloc((size_t)(2 * o->mw), sizeof(double));
+    o->ur[c] = NULL;
+  }
+  for (int f = 0; f < 6; f++) o->w[f] = (double*)calloc((size_t)(2 * o->m1), sizeof(double));
+  fftp_init(&o->fz, o->nz);
+  fftp_init(&o->fy, o->ny);
+  fftp_init(&o->fx, o->nx);
+#ifdef _OPENMP
+  o->nthreads = p->nthreads > 0 ? p->nthreads : omp_get_max_threads();
+#else
+  o->nthreads = 1;
+#endif
+  return o;
+}
+
+void orc_destroy(orc* o) {
+  if (!o) return;
+  for (int c = 0; c < 3; c++) { free(o->u[c]); free(o->rk[c]); free(o->ur[c]); }
+  for (int f = 0; f < 6; f++) free(o->w[f]);
+  free(o->xp); free(o->up); free(o->uprk);
+  fftp_free(&o->fz); fftp_free(&o->fy); fftp_free(&o->fx);
+  free(o);
+}
+
+void orc_sizes(const orc* o, int64_t* s) {
+  s[0] = o->nki; s[1] = o->nkj; s[2] = o->nkk; s[3] = o->mw; s[4] = o->m1; s[5] = o->kkmax;
+}
+
+void orc_set_state(orc* o, const double* a, const double* b, const double* c) {
+  size_t n = sizeof(double) * 2 * (size_t)o->mw;
+  memcpy(o->u[0], a, n); memcpy(o->u[1], b, n); memcpy(o->u[2], c, n);
+}
+void orc_get_state(const orc* o, double* a, double* b, double* c) {
+  size_t n = sizeof(double) * 2 * (size_t)o->mw;
+  memcpy(a, o->u[0], n); memcpy(b, o->u[1], n); memcpy(c, o->u[2], n);
+}
+void orc_set_rk(orc* o, const double* a, const double* b, const double* c) {
+  size_t n = sizeof(double) * 2 * (size_t)o->mw;
+  memcpy(o->rk[0], a, n); memcpy(o->rk[1], b, n); memcpy(o->rk[2], c, n);
+}
+void orc_get_rk(const orc* o, double* a, double* b, double* c) {
+  size_t n = sizeof(double) * 2 * (size_t)o->mw;
+  memcpy(a, o->rk[0], n); memcpy(b, o->rk[1], n); memcpy(c, o->rk[2], n);
+}
+void orc_set_frame(orc* o, const double* b, const double* bi) {
+  memcpy(o->bmat, b, 9 * sizeof(double)); memcpy(o->binv, bi, 9 * sizeof(double));
+}
+void orc_get_frame(const orc* o, double* b, double* bi) {
+  memcpy(b, o->bmat, 9 * sizeof(double)); memcpy(bi, o->binv, 9 * sizeof(double));
+}
+void orc_set_dt(orc* o, double dt) { o->dt = dt; }
+double orc_get_dt(const orc* o) { return o->dt; }
+double orc_get_ucfl(const orc* o) { return o->ucfl; }
+void orc_get_1r2w(const orc* o, int f, double* out) {
+  memcpy(out, o->w[f], sizeof(double) * 2 * (size_t)o->m1);
+}
+
+/* f_ArraysWavenumber, src/pf_arrays.f90:1825-1834 (K0 = 1) */
+static inline double wvn(int k, int kmax) {
+  return (k <= kmax) ? (double)k : (double)(k - (2 * kmax + 1));
+}
+
+/* ------------------------------------------------------------------------- */
+/* f_RK3Derivative (src/pf_rk3.f90:3735-3743) and vorticity forms            */
+/* ------------------------------------------------------------------------- */
+static inline void deriv(const double* u, double k, double* d) {
+  d[1] = k * u[0];
+  d[0] = -k * u[1];
+}
+
+/* f_RK3VorticityBFRot, src/pf_rk3.f90:3786-3843.  om is (0:1,1:3) row major [c][2]. */
+static void vort_bfrot(const orc* o, const double* un, const double* k, int n, double om[3][2]) {
+  for (int c = 0; c < 3; c++) om[c][0] = om[c][1] = 0.0;
+  const double* B = o->bmat;
+  const double* BI = o->binv;
+  const int* bv = o->bval;
+  for (int m = 1; m <= 3; m++) {
+    if (T9(bv, m, 1)) {
+      if (T9(bv, 3, n)) {
+        om[1][1] = om[1][1] - (T9(B, m, 1) * T9(BI, 3, n) * k[m - 1] * un[0]);
+        om[1][0] = om[1][0] + (T9(B, m, 1) * T9(BI, 3, n) * k[m - 1] * un[1]);
+      }
+      if (T9(bv, 2, n)) {
+        om[2][1] = om[2][1] + (T9(B, m, 1) * T9(BI, 2, n) * k[m - 1] * un[0]);
+        om[2][0] = om[2][0] - (T9(B, m, 1) * T9(BI, 2, n) * k[m - 1] * un[1]);
+      }
+    }
+    if (T9(bv, m, 2)) {
+      if (T9(bv, 3, n)) {
+        om[0][1] = om[0][1] + (T9(B, m, 2) * T9(BI, 3, n) * k[m - 1] * un[0]);
+        om[0][0] = om[0][0] - (T9(B, m, 2) * T9(BI, 3, n) * k[m - 1] * un[1]);
+      }
+      if (T9(bv, 1, n)) {
+        om[2][1] = om[2][1] - (T9(B, m, 2) * T9(BI, 1, n) * k[m - 1] * un[0]);
+        om[2][0] = om[2][0] + (T9(B, m, 2) * T9(BI, 1, n) * k[m - 1] * un[1]);
+      }
+    }
+    if (T9(bv, m, 3)) {
+      if (T9(bv, 2, n)) {
+        om[0][1] = om[0][1] - (T9(B, m, 3) * T9(BI, 2, n) * k[m - 1] * un[0]);
+        om[0][0] = om[0][0] + (T9(B, m, 3) * T9(BI, 2, n) * k[m - 1] * un[1]);
+      }
+      if (T9(bv, 1, n)) {
+        om[1][1] = om[1][1] + (T9(B, m, 3) * T9(BI, 1, n) * k[m - 1] * un[0]);
+        om[1][0] = om[1][0] - (T9(B, m, 3) * T9(BI, 1, n) * k[m - 1] * un[1]);
+      }
+    }
+  }
+}
+
+/* Vorticity of one mode, accumulated component by component as in
+ * f_RK3PrepareVelocity (src/pf_rk3.f90:336-723). */
+static void mode_vorticity(const orc* o, const double* u1, const double* u2, const double* u3,
+                           double ki_, double kj_, double kk_, double om[3][2]) {
+  if (o->p.f_type == 1) {
+    double dudy[2], dudz[2], dvdx[2], dvdz[2], dwdx[2], dwdy[2];
+    deriv(u1, kj_, dudy); deriv(u1, kk_, dudz);
+    om[0][0] = 0.0; om[0][1] = 0.0;
+    om[1][0] = dudz[0]; om[1][1] = dudz[1];
+    om[2][0] = -dudy[0]; om[2][1] = -dudy[1];
+    deriv(u2, ki_, dvdx); deriv(u2, kk_, dvdz);
+    om[0][0] = om[0][0] - dvdz[0]; om[0][1] = om[0][1] - dvdz[1];
+    om[2][0] = om[2][0] + dvdx[0]; om[2][1] = om[2][1] + dvdx[1];
+    deriv(u3, ki_, dwdx); deriv(u3, kj_, dwdy);
+    om[0][0] = om[0][0] + dwdy[0]; om[0][1] = om[0][1] + dwdy[1];
+    om[1][0] = om[1][0] - dwdx[0]; om[1][1] = om[1][1] - dwdx[1];
+  } else {
+    const double kv[3] = {ki_, kj_, kk_};
+    double t[3][2];
+    vort_bfrot(o, u1, kv, 1, t);
+    for (int c = 0; c < 3; c++) { om[c][0] = t[c][0]; om[c][1] = t[c][1]; }
+    vort_bfrot(o, u2, kv, 2, t);
+    for (int c = 0; c < 3; c++) { om[c][0] = om[c][0] + t[c][0]; om[c][1] = om[c][1] + t[c][1]; }
+    vort_bfrot(o, u3, kv, 3, t);
+    for (int c = 0; c < 3; c++) { om[c][0] = om[c][0] + t[c][0]; om[c][1] = om[c][1] + t[c][1]; }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* f_RK3PrepareRealSpace: vorticity, zero pad in k, z FFT (sign +1), store  */
+/* in the 1r2w arrays.  src/pf_rk3.f90:205-948, src/pf_fftw.f90:586-624     */
+/* ------------------------------------------------------------------------- */
+void orc_prepare_realspace(orc* o) {
+  const int nz = o->nz, nkj = o->nkj, nkk = o->nkk, K = o->kk;
+#pragma omp parallel num_threads(nthr(o))
+  {
+    double* in = (double*)malloc(sizeof(double) * 2 * nz);
+    double* out = (double*)malloc(sizeof(double) * 2 * nz);
+    double* om = (double*)malloc(sizeof(double) * 2 * (size_t)nkk * 3);
+#pragma omp for schedule(dynamic)
+    for (int i = 0; i < o->nki; i++) {
+      const double wi = wvn(i, o->ki);
+      for (int j = 0; j < nkj; j++) {
+        const double wj = wvn(j, o->kj);
+        const long base = ((long)i * nkj + j) * nkk;
+        for (int k = 0; k < nkk; k++) {
+          const double wk = wvn(k, K);
+          double t[3][2];
+          mode_vorticity(o, &o->u[0][2 * (base + k)], &o->u[1][2 * (base + k)],
+                         &o->u[2][2 * (base + k)], wi, wj, wk, t);
+          for (int c = 0; c < 3; c++) { om[2 * (c * nkk + k)] = t[c][0]; om[2 * (c * nkk + k) + 1] = t[c][1]; }
+        }
+        for (int f = 0; f < 6; f++) {
+          const double* src = (f < 3) ? &o->u[f][2 * base] : &om[2 * (f - 3) * nkk];
+          /* zero-padded buffer: K+1 positive, NZ-2K-1 zeros, K negative */
+          for (int z = 0; z < 2 * nz; z++) in[z] = 0.0;
+          for (int k = 0; k <= K; k++) { in[2 * k] = src[2 * k]; in[2 * k + 1] = src[2 * k + 1]; }
+          for (int k = K + 1; k < nkk; k++) {
+            int z = nz - nkk + k;
+            in[2 * z] = src[2 * k]; in[2 * z + 1] = src[2 * k + 1];
+          }
+          fft_exec(&o->fz, in, out, +1);
+          double* dst = o->w[f];
+          for (int z = 0; z < nz; z++) {
+            long d = (((long)i * nz + z) * nkj + j) * 2;
+            dst[d] = out[2 * z]; dst[d + 1] = out[2 * z + 1];
+          }
+        }
+      }
+    }
+    free(in); free(out); free(om);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Nonlinear products in real space                                          */
+/* ------------------------------------------------------------------------- */
+/* f_RK3ComputeNonlinearIsotropic, src/pf_rk3.f90:1597-1628 */
+static inline void nl_iso(const double* u, const double* w, double* n) {
+  n[0] = u[2] * w[1] - u[1] * w[2];
+  n[1] = u[0] * w[2] - u[2] * w[0];
+  n[2] = u[1] * w[0] - u[0] * w[1];
+}
+/* f_RK3ComputeNonlinearBFRot, src/pf_rk3.f90:1634-1711 */
+static inline void nl_bfrot(const orc* o, const double* u, const double* om, double* n) {
+  double n1 = 0.0, n2 = 0.0, n3 = 0.0;
+  const double* BI = o->binv;
+  const int* bv = o->bval;
+  for (int m = 1; m <= 3; m++) {
+    if (T9(bv, 1, m)) {
+      n3 = n3 - (om[1] * T9(BI, 1, m) * u[m - 1]);
+      n2 = n2 + (om[2] * T9(BI, 1, m) * u[m - 1]);
+    }
+    if (T9(bv, 2, m)) {
+      n1 = n1 - (om[2] * T9(BI, 2, m) * u[m - 1]);
+      n3 = n3 + (om[0] * T9(BI, 2, m) * u[m - 1]);
+    }
+    if (T9(bv, 3, m)) {
+      n1 = n1 + (om[1] * T9(BI, 3, m) * u[m - 1]);
+      n2 = n2 - (om[0] * T9(BI, 3, m) * u[m - 1]);
+    }
+  }
+  n[0] = n1; n[1] = n2; n[2] = n3;
+}
+
+/* ------------------------------------------------------------------------- */
+/* f_RK3RealSpace: per z-plane y/x inverse transforms, nonlinear term,       */
+/* forward x/y transforms with truncation.                                   */
+/* src/pf_rk3.f90:960-1237, 1245-1498, 1780-1940; src/pf_fftw.f90:632-834   */
+/* ------------------------------------------------------------------------- */
+void orc_realspace(orc* o, int stage, int tstep, double* dt) {
+  (void)tstep;
+  const int nx = o->nx, ny = o->ny, nz = o->nz, nki = o->nki, nkj = o->nkj, KJ = o->kj;
+  const long plane = (long)nx * ny;
+  const int want_tracers = (o->np > 0);
+  if (want_tracers) {
+    for (int c = 0; c < 3; c++)
+      if (!o->ur[c]) o->ur[c] = (double*)malloc(sizeof(double) * (size_t)plane * nz);
+  }
+  double umax_all = 0.0;
+#pragma omp parallel num_threads(nthr(o)) reduction(max : umax_all)
+  {
+    double* in = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+    double* out = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+    double* pc = (double*)malloc(sizeof(double) * 2 * (size_t)ny * nki);   /* ji plane */
+    double* r = (double*)malloc(sizeof(double) * 6 * (size_t)plane);       /* u1..u6_2r */
+#pragma omp for schedule(dynamic)
+    for (int z = 0; z < nz; z++) {
+      for (int f = 0; f < 6; f++) {
+        /* y transforms for each i (src/pf_rk3.f90:1437-1498, pf_fftw.f90:673-759) */
+        for (int i = 0; i < nki; i++) {
+          for (int y = 0; y < 2 * ny; y++) in[y] = 0.0;
+          const double* src = &o->w[f][(((long)i * nz + z) * nkj) * 2];
+          for (int j = 0; j <= KJ; j++) { in[2 * j] = src[2 * j]; in[2 * j + 1] = src[2 * j + 1]; }
+          for (int j = KJ + 1; j < nkj; j++) {
+            int y = ny - nkj + j;
+            in[2 * y] = src[2 * j]; in[2 * y + 1] = src[2 * j + 1];
+          }
+          fft_exec(&o->fy, in, out, +1);
+          for (int y = 0; y < ny; y++) {
+            pc[2 * ((long)y * nki + i)] = out[2 * y];
+            pc[2 * ((long)y * nki + i) + 1] = out[2 * y + 1];
+          }
+        }
+        /* zero pad i to NX/2+1, c2r in x (imaginary part of the DC term ignored) */
+        for (int y = 0; y < ny; y++) {
+          for (int x = 0; x < 2 * nx; x++) in[x] = 0.0;
+          for (int i = 0; i < nki; i++) {
+            double re = pc[2 * ((long)y * nki + i)], im = pc[2 * ((long)y * nki + i) + 1];
+            if (i == 0) { in[0] = re; in[1] = 0.0; continue; }
+            in[2 * i] = re; in[2 * i + 1] = im;
+            in[2 * (nx - i)] = re; in[2 * (nx - i) + 1] = -im;
+          }
+          fft_exec(&o->fx, in, out, +1);
+          double* rr = &r[f * plane + (long)y * nx];
+          for (int x = 0; x < nx; x++) rr[x] = out[2 * x];
+        }
+      }
+      /* ucfl (src/pf_fluidstats.f90:914-917): computed at stage 1 */
+      if (stage == 1) {
+        for (long q = 0; q < plane; q++) {
+          double uu = fabs(r[q]) + fabs(r[plane + q]) + fabs(r[2 * plane + q]);
+          if (uu > umax_all) umax_all = uu;
+        }
+      }
+      /* nonlinear term */
+      for (long q = 0; q < plane; q++) {
+        double uu[3] = {r[q], r[plane + q], r[2 * plane + q]};
+        double ww[3] = {r[3 * plane + q], r[4 * plane + q], r[5 * plane + q]};
+        double nn[3];
+        if (o->p.f_type == 1) nl_iso(uu, ww, nn); else nl_bfrot(o, uu, ww, nn);
+        r[3 * plane + q] = nn[0]; r[4 * plane + q] = nn[1]; r[5 * plane + q] = nn[2];
+      }
+      if (want_tracers) {
+        for (int c = 0; c < 3; c++) memcpy(&o->ur[c][(long)z * plane], &r[c * plane], sizeof(double) * plane);
+      }
+      /* forward: r2c in x, keep i <= KI, y FFT (sign -1), truncate j */
+      for (int f = 3; f < 6; f++) {
+        for (int y = 0; y < ny; y++) {
+          const double* rr = &r[f * plane + (long)y * nx];
+          for (int x = 0; x < nx; x++) { in[2 * x] = rr[x]; in[2 * x + 1] = 0.0; }
+          fft_exec(&o->fx, in, out, -1);
+          for (int i = 0; i < nki; i++) {
+            pc[2 * ((long)y * nki + i)] = out[2 * i];
+            pc[2 * ((long)y * nki + i) + 1] = out[2 * i + 1];
+          }
+        }
+        for (int i = 0; i < nki; i++) {
+          for (int y = 0; y < ny; y++) {
+            in[2 * y] = pc[2 * ((long)y * nki + i)];
+            in[2 * y + 1] = pc[2 * ((long)y * nki + i) + 1];
+          }
+          fft_exec(&o->fy, in, out, -1);
+          double* dst = &o->w[f][(((long)i * nz + z) * nkj) * 2];
+          for (int j = 0; j <= KJ; j++) { dst[2 * j] = out[2 * j]; dst[2 * j + 1] = out[2 * j + 1]; }
+          for (int j = KJ + 1; j < nkj; j++) {
+            int y = ny - nkj + j;
+            dst[2 * j] = out[2 * y]; dst[2 * j + 1] = out[2 * y + 1];
+          }
+        }
+      }
+    }
+    free(in); free(out); free(pc); free(r);
+  }
+  if (stage == 1) {
+    o->ucfl = umax_all;
+    /* f_RK3Tstep, src/pf_rk3.f90:3852-3948 */
+    switch (o->p.rk3_timestep) {
+      case 2: *dt = o->p.rk3_dt; break;
+      case 3: *dt = (o->ucfl == 0.0) ? (double)0.02f : o->p.rk3_cfl * o->dx / o->ucfl; break;
+      default:
+        if (o->p.f_type == 1) {
+          if (o->ucfl == 0.0) *dt = (double)0.02f;
+          else {
+            *dt = o->p.rk3_cfl * o->dx / o->ucfl;
+            if (*dt > o->p.force_tf) *dt = o->p.force_tf;
+          }
+        } else {
+          *dt = ((0.5 / o->p.f_deform_s) / (double)o->p.f_remesh_t);
+        }
+        break;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* z forward transform of u4..u6_1r2w for one i slice, x 1/(NX NY NZ),       */
+/* truncated to the stored k modes.  src/pf_rk3.f90:2148-2316,              */
+/* src/pf_fftw.f90:295-297, 841-879.  nl[c] is [j][k] complex.               */
+/* ------------------------------------------------------------------------- */
+static void kforward_slice(const orc* o, int i, double* in, double* out, double* nl[3]) {
+  const int nz = o->nz, nkj = o->nkj, nkk = o->nkk, K = o->kk;
+  const double norm = 1.0 / ((double)o->nx * (double)o->ny * (double)o->nz);
+  for (int c = 0; c < 3; c++) {
+    const double* src = o->w[3 + c];
+    for (int j = 0; j < nkj; j++) {
+      for (int z = 0; z < nz; z++) {
+        long s = (((long)i * nz + z) * nkj + j) * 2;
+        in[2 * z] = src[s]; in[2 * z + 1] = src[s + 1];
+      }
+      fft_exec(&o->fz, in, out, -1);
+      for (int z = 0; z < 2 * nz; z++) out[z] = out[z] * norm;
+      double* d = &nl[c][(long)j * nkk * 2];
+      for (int k = 0; k <= K; k++) { d[2 * k] = out[2 * k]; d[2 * k + 1] = out[2 * k + 1]; }
+      for (int k = K + 1; k < nkk; k++) {
+        int z = nz - nkk + k;
+        d[2 * k] = out[2 * z]; d[2 * k + 1] = out[2 * z + 1];
+      }
+    }
+  }
+}
+
+void orc_nl_forward(orc* o, double* a, double* b, double* c) {
+  double* dst[3] = {a, b, c};
+  const long slab = (long)o->nkj * o->nkk;
+#pragma omp parallel num_threads(nthr(o))
+  {
+    double* in = (double*)malloc(sizeof(double) * 2 * o->nz);
+    double* out = (double*)malloc(sizeof(double) * 2 * o->nz);
+#pragma omp for
+    for (int i = 0; i < o->nki; i++) {
+      double* nl[3] = {&dst[0][i * slab * 2], &dst[1][i * slab * 2], &dst[2][i * slab * 2]};
+      kforward_slice(o, i, in, out, nl);
+    }
+    free(in); free(out);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* FormF terms for the BF_ROT shear form: src/pf_rk3.f90:2864-3209           */
+/* ------------------------------------------------------------------------- */
+static void bf_nonlinear(const orc* o, const double nl[3][2], double f[3][2]) {
+  const double* B = o->bmat;
+  const int* bv = o->bval;
+  for (int i = 1; i <= 3; i++) {
+    f[i - 1][0] = 0.0; f[i - 1][1] = 0.0;
+    for (int c = 1; c <= 3; c++) {
+      if (T9(bv, i, c)) {
+        f[i - 1][0] = f[i - 1][0] + T9(B, i, c) * nl[c - 1][0];
+        f[i - 1][1] = f[i - 1][1] + T9(B, i, c) * nl[c - 1][1];
+      }
+    }
+  }
+}
+
+static void bf_viscous(const orc* o, const double u[3][2], const double* k, double v[3][2]) {
+  const double* B = o->bmat;
+  const int* bv = o->bval;
+  const double nu = o->p.f_nu;
+  for (int c = 0; c < 3; c++) v[c][0] = v[c][1] = 0.0;
+  for (int l = 1; l <= 3; l++)
+    for (int j = 1; j <= 3; j++)
+      if (T9(bv, l, j))
+        for (int n = 1; n <= 3; n++)
+          if (T9(bv, n, j)) {
+            double coef = nu * T9(B, l, j) * T9(B, n, j) * k[l - 1] * k[n - 1];
+            for (int c = 0; c < 3; c++) {
+              v[c][0] = v[c][0] - coef * u[c][0];
+              v[c][1] = v[c][1] - coef * u[c][1];
+            }
+          }
+}
+
+static void bf_pressure(const orc* o, const double u[3][2], const double* k, const double nl[3][2],
+                        double pf[3][2]) {
+  const double* B = o->bmat;
+  const double* BI = o->binv;
+  const double* A = o->amat;
+  const int* bv = o->bval;
+  const int* av = o->aval;
+  double p1[3] = {0, 0, 0}, p2[2] = {0, 0}, p3[2] = {0, 0}, p4 = 0.0;
+  for (int m = 1; m <= 3; m++) {
+    p2[0] = p2[0] + (k[m - 1] * nl[m - 1][0]);
+    p2[1] = p2[1] + (k[m - 1] * nl[m - 1][1]);
+    for (int r = 1; r <= 3; r++) {
+      if (!T9(bv, m, r)) continue;
+      for (int s = 1; s <= 3; s++) {
+        if (!T9(av, r, s)) continue;
+        for (int kk = 1; kk <= 3; kk++) {
+          if (!T9(bv, s, kk)) continue;
+          double cf = k[m - 1] * T9(B, m, r) * T9(A, r, s) * T9(BI, s, kk);
+          p3[0] = p3[0] + (2.0 * (cf * u[kk - 1][0]));
+          p3[1] = p3[1] + (2.0 * (cf * u[kk - 1][1]));
+        }
+      }
+      for (int kk = 1; kk <= 3; kk++)
+        if (T9(bv, kk, r)) p4 = p4 + (k[m - 1] * T9(B, m, r) * T9(B, kk, r) * k[kk - 1]);
+    }
+  }
+  for (int i = 1; i <= 3; i++)
+    for (int p = 1; p <= 3; p++)
+      if (T9(bv, i, p))
+        for (int j = 1; j <= 3; j++)
+          if (T9(bv, j, p)) p1[i - 1] = p1[i - 1] + T9(B, i, p) * T9(B, j, p) * k[j - 1];
+  if (p4 > 0.0) {
+    for (int c = 0; c < 3; c++) {
+      pf[c][0] = (p1[c] / p4) * (p2[0] + p3[0]);
+      pf[c][1] = (p1[c] / p4) * (p2[1] + p3[1]);
+    }
+  } else {
+    for (int c = 0; c < 3; c++) pf[c][0] = pf[c][1] = 0.0;
+  }
+}
+
+static void bf_source(const orc* o, const double u[3][2], double h[3][2]) {
+  const double* B = o->bmat;
+  const double* BI = o->binv;
+  const double* A = o->amat;
+  const int* bv = o->bval;
+  const int* av = o->aval;
+  for (int c = 0; c < 3; c++) h[c][0] = h[c][1] = 0.0;
+  for (int i = 1; i <= 3; i++)
+    for (int r = 1; r <= 3; r++)
+      if (T9(bv, i, r))
+        for (int j = 1; j <= 3; j++)
+          if (T9(av, r, j))
+            for (int c = 1; c <= 3; c++)
+              if (T9(bv, j, c)) {
+                double cf = T9(B, i, r) * T9(A, r, j) * T9(BI, j, c);
+                h[i - 1][0] = h[i - 1][0] + (2.0 * (cf * u[c - 1][0]));
+                h[i - 1][1] = h[i - 1][1] + (2.0 * (cf * u[c - 1][1]));
+              }
+}
+
+/* f_RK3RungeKutta (src/pf_rk3.f90:3457-3498) + f_RK3ContinuityFixIsotropic (3504-3531) */
+static inline void rk_update(double* rk[3], double* u[3], long e, const double tot[3][2], double beta,
+                             double gdt, const double* kv, double kmag2) {
+  for (int c = 0; c < 3; c++) {
+    rk[c][e] = (beta * rk[c][e]) + tot[c][0];
+    rk[c][e + 1] = (beta * rk[c][e + 1]) + tot[c][1];
+  }
+  for (int c = 0; c < 3; c++) {
+    u[c][e] = u[c][e] + (gdt * rk[c][e]);
+    u[c][e + 1] = u[c][e + 1] + (gdt * rk[c][e + 1]);
+  }
+  double kd0 = kv[0] * u[0][e] + kv[1] * u[1][e] + kv[2] * u[2][e];
+  double kd1 = kv[0] * u[0][e + 1] + kv[1] * u[1][e + 1] + kv[2] * u[2][e + 1];
+  for (int c = 0; c < 3; c++) {
+    u[c][e] = u[c][e] - (kv[c] * kd0 / kmag2);
+    u[c][e + 1] = u[c][e + 1] - (kv[c] * kd1 / kmag2);
+  }
+}
+
+/* f_RK3SolveNavierStokes, src/pf_rk3.f90:1996-2142 with f_RK3FormFIsotropic
+ * (2323-2487) and f_RK3FormFBFRot (2494-2673).  Forcing is zero (decaying
+ * runs, FORCE_STOP = 0); the two-way coupling term is zero. */
+static void solve_ns(orc* o, double dt, int stage) {
+  double beta = 0.0, gamma = THIRD;
+  if (stage == 2) { beta = -FIVE_OVER_NINE; gamma = FIFTEEN_OVER_SIXTEEN; }
+  if (stage == 3) { beta = -ONEFIVETHREE_OVER_ONETWOEIGHT; gamma = EIGHT_OVER_FIFTEEN; }
+  const double gdt = gamma * dt;
+  const int nkj = o->nkj, nkk = o->nkk;
+  const long slab = (long)nkj * nkk;
+#pragma omp parallel num_threads(nthr(o))
+  {
+    double* in = (double*)malloc(sizeof(double) * 2 * o->nz);
+    double* out = (double*)malloc(sizeof(double) * 2 * o->nz);
+    double* nlbuf = (double*)malloc(sizeof(double) * 2 * 3 * slab);
+    double* nl[3] = {nlbuf, nlbuf + 2 * slab, nlbuf + 4 * slab};
+#pragma omp for schedule(dynamic)
+    for (int i = 0; i < o->nki; i++) {
+      kforward_slice(o, i, in, out, nl);
+      const double wi = wvn(i, o->ki);
+      for (int j = 0; j < nkj; j++) {
+        const double wj = wvn(j, o->kj);
+        for (int k = 0; k < nkk; k++) {
+          const double wk = wvn(k, o->kk);
+          const double kmag2 = (wi * wi) + (wj * wj) + (wk * wk);
+          if (!(kmag2 > 0.0)) continue;
+          const long e = (((long)i * nkj + j) * nkk + k) * 2;
+          const long q = ((long)j * nkk + k) * 2;
+          const double kv[3] = {wi, wj, wk};
+          double tot[3][2];
+          double uu[3][2], nn[3][2];
+          for (int c = 0; c < 3; c++) {
+            uu[c][0] = o->u[c][e]; uu[c][1] = o->u[c][e + 1];
+            nn[c][0] = nl[c][q]; nn[c][1] = nl[c][q + 1];
+          }
+          if (o->p.f_type == 1) {
+            double kdn0 = (wi * nn[0][0]) + (wj * nn[1][0]) + (wk * nn[2][0]);
+            double kdn1 = (wi * nn[0][1]) + (wj * nn[1][1]) + (wk * nn[2][1]);
+            for (int c = 0; c < 3; c++) {
+              double visc0 = -(o->p.f_nu * kmag2 * uu[c][0]);
+              double visc1 = -(o->p.f_nu * kmag2 * uu[c][1]);
+              double pr0 = (kv[c] / kmag2) * kdn0;
+              double pr1 = (kv[c] / kmag2) * kdn1;
+              tot[c][0] = ((visc0 - nn[c][0]) + pr0) + 0.0;
+              tot[c][1] = ((visc1 - nn[c][1]) + pr1) + 0.0;
+            }
+          } else {
+            double nlf[3][2], pf[3][2], hom[3][2], visc[3][2];
+            if (o->p.f_type == 2) bf_nonlinear(o, nn, nlf);
+            else for (int c = 0; c < 3; c++) nlf[c][0] = nlf[c][1] = 0.0;
+            bf_pressure(o, uu, kv, nlf, pf);
+            bf_source(o, uu, hom);
+            if (o->p.f_type == 2 || o->p.f_type == 3) bf_viscous(o, uu, kv, visc);
+            for (int c = 0; c < 3; c++)
+              for (int r = 0; r < 2; r++) {
+                if (o->p.f_type == 2) tot[c][r] = (((visc[c][r] - nlf[c][r]) + pf[c][r]) + 0.0) - hom[c][r];
+                else if (o->p.f_type == 3) tot[c][r] = ((visc[c][r] + pf[c][r]) + 0.0) - hom[c][r];
+                else tot[c][r] = pf[c][r] - hom[c][r];
+              }
+          }
+          rk_update(o->rk, o->u, e, tot, beta, gdt, kv, kmag2);
+        }
+      }
+    }
+    free(in); free(out); free(nlbuf);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Remesh: src/pf_rk3.f90:3958-4178, src/pp_mpi.f90:1062-1092                */
+/* ------------------------------------------------------------------------- */
+int orc_remesh_target(int kmax, int i, int k) {
+  const int nodes = 2 * kmax + 1;
+  int actualk = (k > kmax) ? k - nodes : k;
+  int knew = actualk - i;
+  if (knew >= -kmax) {
+    if (knew < 0) knew += nodes;
+    return knew;
+  }
+  return -1;
+}
+
+static void wrap_one(const orc* o, double* x);
+
+int orc_remesh(orc* o, int tstep) {
+  if (o->p.f_type == 1 || o->p.rk3_timestep != 1) return 0;
+  const int b110 = o->p.b110 > 0 ? o->p.b110 : 1;
+  const int a = tstep - (b110 * o->p.f_remesh_t) - 1;
+  const int b = 2 * b110 * o->p.f_remesh_t;
+  if ((a % b) != 0) return 0; /* Fortran mod truncates like C % */
+  /* particles first: x1 += x3, wrap x1 */
+  for (long q = 0; q < o->np; q++) {
+    double* x = &o->xp[3 * q];
+    x[0] = x[0] + x[2];
+    const double x1min = -0.5 * o->dx, x1max = x1min + o->lx;
+    if (x[0] < x1min) x[0] = x[0] + o->lx;
+    if (x[0] >= x1max) x[0] = x[0] - o->lx;
+  }
+  T9(o->binv, 1, 3) = ((double)b110 * 0.5) / T9(o->bmat, 3, 3);
+  T9(o->bmat, 1, 3) = -T9(o->bmat, 1, 1) * (-((double)b110 * 0.5));
+  const int nkk = o->nkk, nkj = o->nkj, K = o->kk;
+  double* buf = (double*)malloc(sizeof(double) * 6 * nkk);
+  for (int i = 0; i < o->nki; i++)
+    for (int j = 0; j < nkj; j++) {
+      for (int q = 0; q < 6 * nkk; q++) buf[q] = 0.0;
+      const long base = (((long)i * nkj + j) * nkk) * 2;
+      for (int k = 0; k < nkk; k++) {
+        int kn = orc_remesh_target(K, i, k);
+        if (kn < 0) continue;
+        for (int c = 0; c < 3; c++) {
+          buf[2 * (c * nkk + kn)] = o->u[c][base + 2 * k];
+          buf[2 * (c * nkk + kn) + 1] = o->u[c][base + 2 * k + 1];
+        }
+      }
+      for (int k = 0; k < nkk; k++)
+        for (int r = 0; r < 2; r++) {
+          double b1 = buf[2 * k + r], b2 = buf[2 * (nkk + k) + r], b3 = buf[2 * (2 * nkk + k) + r];
+          o->u[0][base + 2 * k + r] = (o->p.ns_deformed == 1) ? b1 + b3 : b1;
+          o->u[1][base + 2 * k + r] = b2;
+          o->u[2][base + 2 * k + r] = b3;
+        }
+    }
+  free(buf);
+  T9(o->binv, 1, 3) = (-((double)b110 * 0.5)) / T9(o->bmat, 3, 3);
+  return 1;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Tracers                                                                    */
+/* ------------------------------------------------------------------------- */
+void orc_set_particles(orc* o, const double* xp, int64_t np) {
+  free(o->xp); free(o->up); free(o->uprk);
+  o->np = np;
+  o->xp = (double*)malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
+  o->up = (double*)calloc(3 * (size_t)(np > 0 ? np : 1), sizeof(double));
+  o->uprk = (double*)calloc(3 * (size_t)(np > 0 ? np : 1), sizeof(double));
+  memcpy(o->xp, xp, sizeof(double) * 3 * (size_t)np);
+}
+
+void orc_get_particles(const orc* o, double* xp, double* uprk) {
+  if (xp) memcpy(xp, o->xp, sizeof(double) * 3 * (size_t)o->np);
+  if (uprk) memcpy(uprk, o->uprk, sizeof(double) * 3 * (size_t)o->np);
+}
+
+void orc_get_tracer_field(const orc* o, double* a, double* b, double* c) {
+  size_t n = sizeof(double) * (size_t)o->nx * o->ny * o->nz;
+  if (o->ur[0]) { memcpy(a, o->ur[0], n); memcpy(b, o->ur[1], n); memcpy(c, o->ur[2], n); }
+}
+
+static inline long pmod(long a, long n) { long r = a % n; return r < 0 ? r + n : r; }
+
+/* p_InterpFindMolecule + p_InterpNeville3D, src/pp_interp.f90:406-560, with
+ * LX_HALO from g_DomainInitHalo (src/pg_domain.f90:360-384). */
+static void interp_one(const orc* o, const double* f[3], const double* xp, double* uf) {
+  const int P = o->p.p_interp_order > 0 ? o->p.p_interp_order : 4;
+  const int nhalo = P / 2;
+  double lxh = (double)nhalo * o->dx, lyh = (double)nhalo * o->dy, lzh = (double)nhalo * o->dz;
+  double xd1, xd2, xd3;
+  if (P % 2 == 0) {
+    lxh = lxh - (0.5 * o->dx); lyh = lyh - (0.5 * o->dy); lzh = lzh - (0.5 * o->dz);
+    xd1 = xp[0] + lxh + o->dx + (0.5 * o->dx);
+    xd2 = xp[1] + lyh + o->dy + (0.5 * o->dy);
+    xd3 = xp[2] + lzh + o->dz + (0.5 * o->dz);
+  } else {
+    xd1 = xp[0] + lxh + (0.5 * o->dx);
+    xd2 = xp[1] + lyh + (0.5 * o->dy);
+    xd3 = xp[2] + lzh + (0.5 * o->dz);
+  }
+  const int i0 = (int)(xd1 / o->dx) - nhalo - nhalo;
+  const int j0 = (int)(xd2 / o->dy) - nhalo - nhalo;
+  const int k0 = (int)(xd3 / o->dz) - nhalo - nhalo;
+  double ix[16], iy[16], iz[16];
+  for (int m = 0; m < P; m++) {
+    ix[m] = (double)(i0 + m) * o->dx;
+    iy[m] = (double)(j0 + m) * o->dy;
+    iz[m] = (double)(k0 + m) * o->dz;
+  }
+  /* v[c][x][z][y] as interpolationvalues(c, x, z, y) */
+  double v[3][8][8][8];
+  for (int a = 0; a < P; a++)
+    for (int c2 = 0; c2 < P; c2++)
+      for (int b = 0; b < P; b++) {
+        long gx = pmod(i0 + a, o->nx), gz = pmod(k0 + c2, o->nz), gy = pmod(j0 + b, o->ny);
+        long idx = (gz * o->ny + gy) * o->nx + gx;
+        for (int c = 0; c < 3; c++) v[c][a][c2][b] = f[c][idx];
+      }
+  for (int col = 1; col <= P - 1; col++)
+    for (int nv = 1; nv <= P - col; nv++) {
+      double d1 = xp[1] - iy[nv + col - 1];
+      double d2 = iy[nv - 1] - xp[1];
+      double d3 = iy[nv - 1] - iy[nv + col - 1];
+      for (int zi = 0; zi < P; zi++)
+        for (int xi = 0; xi < P; xi++)
+          for (int c = 0; c < 3; c++)
+            v[c][xi][zi][nv - 1] = ((d1 * v[c][xi][zi][nv - 1]) + (d2 * v[c][xi][zi][nv])) / d3;
+    }
+  for (int col = 1; col <= P - 1; col++)
+    for (int nv = 1; nv <= P - col; nv++) {
+      double d1 = xp[2] - iz[nv + col - 1];
+      double d2 = iz[nv - 1] - xp[2];
+      double d3 = iz[nv - 1] - iz[nv + col - 1];
+      for (int xi = 0; xi < P; xi++)
+        for (int c = 0; c < 3; c++)
+          v[c][xi][nv - 1][0] = ((d1 * v[c][xi][nv - 1][0]) + (d2 * v[c][xi][nv][0])) / d3;
+    }
+  for (int col = 1; col <= P - 1; col++)
+    for (int nv = 1; nv <= P - col; nv++) {
+      double d1 = xp[0] - ix[nv + col - 1];
+      double d2 = ix[nv - 1] - xp[0];
+      double d3 = ix[nv - 1] - ix[nv + col - 1];
+      for (int c = 0; c < 3; c++)
+        v[c][nv - 1][0][0] = ((d1 * v[c][nv - 1][0][0]) + (d2 * v[c][nv][0][0])) / d3;
+    }
+  for (int c = 0; c < 3; c++) uf[c] = v[c][0][0][0];
+}
+
+void orc_interpolate(const orc* o, const double* f1, const double* f2, const double* f3,
+                     const double* xp, int64_t np, double* uf) {
+  const double* f[3] = {f1, f2, f3};
+#pragma omp parallel for num_threads(nthr(o))
+  for (long q = 0; q < np; q++) interp_one(o, f, &xp[3 * q], &uf[3 * q]);
+}
+
+/* p_MPICorrectPosition, src/pp_mpi.f90:1011-1057 (x3 upper test is '>') */
+static void wrap_one(const orc* o, double* x) {
+  const double x1min = -0.5 * o->dx, x1max = x1min + o->lx;
+  const double x2min = -0.5 * o->dy, x2max = x2min + o->ly;
+  const double x3min = -0.5 * o->dz, x3max = x3min + o->lz;
+  if (x[0] < x1min) x[0] = x[0] + o->lx;
+  if (x[0] >= x1max) x[0] = x[0] - o->lx;
+  if (x[1] < x2min) x[1] = x[1] + o->ly;
+  if (x[1] >= x2max) x[1] = x[1] - o->ly;
+  if (x[2] < x3min) x[2] = x[2] + o->lz;
+  if (x[2] > x3max) x[2] = x[2] - o->lz;
+}
+
+void orc_correct_position(const orc* o, double* xp, int64_t np) {
+  for (long q = 0; q < np; q++) wrap_one(o, &xp[3 * q]);
+}
+
+/* p_RK3SolveParticle / p_RK3SolveParticleBF tracer branch (dp = 0),
+ * src/pp_rk3.f90:57-165, 169-303, 989-1043; then p_ControlFinishTimeStep wrap. */
+static void tracers_stage(orc* o, int stage, double dt) {
+  double beta = 0.0, gamma = THIRD;
+  if (stage == 2) { beta = -FIVE_OVER_NINE; gamma = FIFTEEN_OVER_SIXTEEN; }
+  if (stage == 3) { beta = -ONEFIVETHREE_OVER_ONETWOEIGHT; gamma = EIGHT_OVER_FIFTEEN; }
+  const double* f[3] = {o->ur[0], o->ur[1], o->ur[2]};
+  const int bf = (o->p.f_type != 1);
+#pragma omp parallel for num_threads(nthr(o))
+  for (long q = 0; q < o->np; q++) {
+    double* x = &o->xp[3 * q];
+    double* up = &o->up[3 * q];
+    double* ur = &o->uprk[3 * q];
+    double uf[3];
+    interp_one(o, f, x, uf);
+    if (!bf) {
+      for (int c = 0; c < 3; c++) up[c] = uf[c];
+      for (int c = 0; c < 3; c++) ur[c] = (beta * ur[c]) + up[c];
+      for (int c = 0; c < 3; c++) x[c] = x[c] + (gamma * dt * ur[c]);
+    } else {
+      double xl[3] = {0, 0, 0}, ul[3] = {0, 0, 0}, xm[3] = {0, 0, 0};
+      for (int m = 1; m <= 3; m++)
+        for (int n = 1; n <= 3; n++)
+          if (T9(o->bval, m, n)) {
+            xl[m - 1] = xl[m - 1] + T9(o->binv, m, n) * x[n - 1];
+            ul[m - 1] = ul[m - 1] + T9(o->binv, m, n) * uf[n - 1];
+          }
+      for (int c = 0; c < 3; c++) up[c] = ul[c];
+      for (int c = 0; c < 3; c++) ur[c] = (beta * ur[c]) + up[c];
+      for (int c = 0; c < 3; c++) xl[c] = xl[c] + (gamma * dt * ur[c]);
+      for (int m = 1; m <= 3; m++)
+        for (int n = 1; n <= 3; n++)
+          if (T9(o->bval, m, n)) xm[m - 1] = xm[m - 1] + T9(o->bmat, m, n) * xl[n - 1];
+      for (int c = 0; c < 3; c++) x[c] = xm[c];
+    }
+    wrap_one(o, x);
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Stage / step drivers                                                       */
+/* ------------------------------------------------------------------------- */
+void orc_stage(orc* o, int stage, int tstep, double time, double* dt) {
+  (void)time;
+  if (stage == 1) orc_remesh(o, tstep);
+  orc_prepare_realspace(o);
+  orc_realspace(o, stage, tstep, dt);
+  if (o->np > 0) tracers_stage(o, stage, *dt);
+  solve_ns(o, *dt, stage);
+}
+
+/* g_RK3UpdateB, src/pg_rk3.f90:280-385 */
+void orc_update_b(orc* o, int tstep, double simtime) {
+  if (o->p.f_type == 1) return;
+  const int b110 = o->p.b110 > 0 ? o->p.b110 : 1;
+  double localtime;
+  if (o->p.rk3_timestep == 1) {
+    if (tstep <= (b110 * o->p.f_remesh_t)) {
+      localtime = simtime;
+    } else {
+      double a = (double)(tstep - (b110 * o->p.f_remesh_t) - 1);
+      double b = (double)(2 * (b110 * o->p.f_remesh_t));
+      int localtstep = (int)(a / b);
+      localtime = simtime - ((double)(localtstep + 1) / o->p.f_deform_s * (double)b110);
+    }
+  } else {
+    localtime = simtime;
+  }
+  T9(o->bmat, 1, 3) = -T9(o->bmat, 1, 1) * o->p.f_deform_s * localtime;
+  T9(o->binv, 1, 3) = o->p.f_deform_s * localtime / T9(o->bmat, 3, 3);
+}
+
+void orc_step(orc* o, int tstep, double* simtime) {
+  const double tstart = *simtime;
+  orc_stage(o, 1, tstep, *simtime, &o->dt);
+  *simtime = tstart + (1.0 / 3.0) * o->dt;
+  orc_update_b(o, tstep, *simtime);
+  orc_stage(o, 2, tstep, *simtime, &o->dt);
+  *simtime = tstart + (3.0 / 4.0) * o->dt;
+  orc_update_b(o, tstep, *simtime);
+  orc_stage(o, 3, tstep, *simtime, &o->dt);
+  *simtime = tstart + o->dt;
+  orc_update_b(o, tstep, *simtime);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Wave-space statistics: src/pf_fluidstats.f90:1423-1576                     */
+/* ------------------------------------------------------------------------- */
+void orc_spectrum(const orc* o, double* ek, double* dk, double* dsk, double* tot) {
+  const int kkmax = o->kkmax;
+  for (int q = 0; q < kkmax; q++) ek[q] = dk[q] = dsk[q] = 0.0;
+  double s11 = 0, s22 = 0, s33 = 0;
+  const double nu = o->p.f_nu;
+  for (int i = 0; i < o->nki; i++) {
+    const double wi = wvn(i, o->ki);
+    for (int j = 0; j < o->nkj; j++) {
+      const double wj = wvn(j, o->kj);
+      for (int k = 0; k < o->nkk; k++) {
+        const double wk = wvn(k, o->kk);
+        const double km = sqrt((wi * wi) + (wj * wj) + (wk * wk));
+        const long e = (((long)i * o->nkj + j) * o->nkk + k) * 2;
+        const double* a = &o->u[0][e];
+        const double* b = &o->u[1][e];
+        const double* c = &o->u[2][e];
+        double p11 = a[0] * a[0] + a[1] * a[1];
+        double p22 = b[0] * b[0] + b[1] * b[1];
+        double p33 = c[0] * c[0] + c[1] * c[1];
+        if (i != 0) { p11 = p11 * 2.0; p22 = p22 * 2.0; p33 = p33 * 2.0; }
+        s11 += p11; s22 += p22; s33 += p33;
+        const double e_ = 0.5 * (p11 + p22 + p33);
+        const double d_ = 2.0 * km * km * nu * e_;
+        const double ds = 4.0 * km * km * km * km * nu * nu * e_;
+        /* the reference scans every bin kk = 1..kkmax with [kk-1/2, kk+1/2);
+         * only the bins next to nint(|k|) can match */
+        const int kc = (int)floor(km + 0.5);
+        for (int kk = kc - 1; kk <= kc + 1; kk++) {
+          if (kk < 1 || kk > kkmax) continue;
+          const double up_ = (double)kk + 0.5, lo_ = (double)kk - 0.5;
+          if (km >= lo_ && km < up_) { ek[kk - 1] += e_; dk[kk - 1] += d_; dsk[kk - 1] += ds; }
+        }
+      }
+    }
+  }
+  double fk = 0, fd = 0, fs = 0;
+  for (int q = 0; q < kkmax; q++) { fk += ek[q]; fd += dk[q]; fs += dsk[q]; }
+  tot[0] = fk; tot[1] = fd; tot[2] = fs; tot[3] = s11; tot[4] = s22; tot[5] = s33;
+}

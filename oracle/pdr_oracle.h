@@ -1,0 +1,98 @@
+/*
+ * pdr_oracle.h -- FP64 CPU restatement of PANDORA 2.0's pseudo-spectral RK3 step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and by
+ * nothing else.  The product (pandora-2.0_amd/) never links, loads or calls it.
+ *
+ * Layouts follow the reference's PETSc DMDA arrays (0-based global indices,
+ * Fortran column-major, (re,im) interleaved fastest):
+ *   wave state  u{1,2,3}_3w, rk3_3w_{1,2,3}: (0:1, 0:NKK-1, 0:NKJ-1, 0:NKI-1)
+ *                                            src/pf_arrays.f90:267-280, 761-810
+ *   1r2w        u{1..6}_1r2w              : (0:1, 0:NKJ-1, 0:NZ-1, 0:NKI-1)
+ *                                            src/pf_arrays.f90:292-305
+ *   tracer field u{1,2,3}_3r (no halo)   : (0:NX-1, 0:NY-1, 0:NZ-1), x fastest
+ *                                            (the reference stores (x,z,y) with
+ *                                            a periodic halo, src/pf_arrays.f90:315-328;
+ *                                            values are identical)
+ *   particles   particle_xp(1:3, np)      : src/pp_arrays.f90:25-34
+ * 3x3 tensors are Fortran column-major: T(m,n) = t[(m-1) + 3*(n-1)].
+ */
+#ifndef PDR_ORACLE_H
+#define PDR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+  int32_t nodes_x, nodes_y, nodes_z;
+  int32_t f_type;          /* 1 iso, 2 shear DNS, 3 shear RDT viscous, 4 shear RDT inviscid */
+  int32_t ns_deformed;     /* 1 BF_ROT (only form restated here for shear) */
+  int32_t rk3_timestep;    /* 1 DECIDE, 2 FIXED, 3 CFL */
+  double  rk3_dt;
+  double  rk3_cfl;
+  double  f_nu;
+  double  f_deform_s;
+  int32_t f_remesh_t;
+  int32_t b110;
+  double  force_tf;
+  int32_t p_interp_order;  /* even orders (4 is the reference default) */
+  int32_t nthreads;        /* 0 = OpenMP default */
+} orc_params;
+
+typedef struct orc orc;
+
+orc*   orc_create(const orc_params* p);
+void   orc_destroy(orc* o);
+void   orc_sizes(const orc* o, int64_t* out6);   /* NKI,NKJ,NKK, Mw, M1, kkmax */
+
+void   orc_set_state(orc* o, const double* u1, const double* u2, const double* u3);
+void   orc_get_state(const orc* o, double* u1, double* u2, double* u3);
+void   orc_set_rk(orc* o, const double* r1, const double* r2, const double* r3);
+void   orc_get_rk(const orc* o, double* r1, double* r2, double* r3);
+void   orc_set_frame(orc* o, const double* bmat9, const double* binv9);
+void   orc_get_frame(const orc* o, double* bmat9, double* binv9);
+void   orc_set_dt(orc* o, double dt);
+double orc_get_dt(const orc* o);
+
+/* g_RK3Calc (src/pg_rk3.f90:82-129): one full RK3 step, updates simtime. */
+void   orc_step(orc* o, int tstep, double* simtime);
+/* f_RK3Stage{1,2,3} (src/pf_rk3.f90:40,121,161) */
+void   orc_stage(orc* o, int stage, int tstep, double time, double* dt);
+/* g_RK3UpdateB (src/pg_rk3.f90:280-385) */
+void   orc_update_b(orc* o, int tstep, double simtime);
+
+/* Pieces of one stage, exposed for the reference's own known-answer tests. */
+void   orc_prepare_realspace(orc* o);                          /* src/pf_rk3.f90:205-329 */
+void   orc_realspace(orc* o, int stage, int tstep, double* dt); /* src/pf_rk3.f90:960-1237 */
+void   orc_nl_forward(orc* o, double* nl1, double* nl2, double* nl3); /* kRs2Fs of u4..u6_1r2w */
+void   orc_get_1r2w(const orc* o, int field, double* out);     /* field 0..5 = u1..u6_1r2w */
+double orc_get_ucfl(const orc* o);
+
+/* Remesh (src/pf_rk3.f90:3958-4178); returns 1 if a remesh happened. */
+int    orc_remesh(orc* o, int tstep);
+int    orc_remesh_target(int kk_max, int i, int k);            /* new k index or -1 */
+
+/* Tracers (src/pp_control.f90:518-580, src/pp_rk3.f90:57-303, src/pp_mpi.f90:1011-1092) */
+void   orc_set_particles(orc* o, const double* xp, int64_t np);
+void   orc_get_particles(const orc* o, double* xp, double* uprk);
+void   orc_get_tracer_field(const orc* o, double* u1r, double* u2r, double* u3r);
+/* Interpolate a user field f (3 real components, each NX*NY*NZ, x fastest) */
+void   orc_interpolate(const orc* o, const double* f1, const double* f2, const double* f3,
+                       const double* xp, int64_t np, double* uf);
+void   orc_correct_position(const orc* o, double* xp, int64_t np);
+
+/* Plain complex FFT used by the oracle (out = sum_n in[n] exp(sign*2*pi*i*n*k/n)). */
+void   orc_fft(int n, int sign, const double* in, double* out);
+
+/* Wave-space statistics (src/pf_fluidstats.f90:1423-1576): per-shell E(k), D(k),
+ * disskew shells and totals.  ek/dk/dsk have kkmax entries; tot[6] =
+ * {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}. Mode density is not applied. */
+void   orc_spectrum(const orc* o, double* ek, double* dk, double* dsk, double* tot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

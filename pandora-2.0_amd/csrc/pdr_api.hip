@@ -1,0 +1,694 @@
+// pdr_api.hip -- C-ABI of the MI355X RK3 step: context, state transfer, stage
+// orchestration, shear-frame clock and remesh decision (all FP64 on the host).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pdr_internal.hpp"
+
+using namespace pdr;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int pdr::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+#define HIPCHK(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) return fail(PDR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define T9(a, m, n) ((a)[((m)-1) + 3 * ((n)-1)])
+
+// Reference RK3 coefficients (src/pg_constants.f90:56-62, src/pf_rk3.f90:3478-3488)
+static void rk_coeffs(int stage, double* beta, double* gamma) {
+  if (stage == 1) {
+    *beta = 0.0;
+    *gamma = 1.0 / 3.0;
+  } else if (stage == 2) {
+    *beta = -(5.0 / 9.0);
+    *gamma = 15.0 / 16.0;
+  } else {
+    *beta = -(153.0 / 128.0);
+    *gamma = 8.0 / 15.0;
+  }
+}
+
+// float32 pi widened to double (LONGONEPI is assigned from a default-real
+// literal, src/pg_constants.f90:126)
+static const double kTwoPiRef = 2.0 * double(3.14159265358979323846f);
+
+namespace pdr {
+
+void fft_plan(FftPlan* p, int n) {
+  p->n = n;
+  p->npass = 0;
+  int m = n;
+  while (m % 8 == 0 && p->npass < kMaxPasses) { p->radix[p->npass++] = 8; m /= 8; }
+  if (m % 4 == 0) { p->radix[p->npass++] = 4; m /= 4; }
+  if (m % 2 == 0) { p->radix[p->npass++] = 2; m /= 2; }
+  while (m % 3 == 0 && p->npass < kMaxPasses) { p->radix[p->npass++] = 3; m /= 3; }
+  while (m % 5 == 0 && p->npass < kMaxPasses) { p->radix[p->npass++] = 5; m /= 5; }
+}
+
+void prof_begin(pdr_ctx* c, int cls) {
+  if (!c->prof) return;
+  hipEvent_t a, b;
+  if (c->ev_pool.size() >= 2) {
+    a = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    b = c->ev_pool.back();
+    c->ev_pool.pop_back();
+  } else {
+    if (hipEventCreate(&a) != hipSuccess) return;
+    if (hipEventCreate(&b) != hipSuccess) return;
+  }
+  (void)hipEventRecord(a, c->stream);
+  c->ev_pending.push_back({cls, {a, b}});
+}
+
+void prof_end(pdr_ctx* c, int cls) {
+  if (!c->prof || c->ev_pending.empty()) return;
+  auto& last = c->ev_pending.back();
+  if (last.first != cls) return;
+  (void)hipEventRecord(last.second.second, c->stream);
+}
+
+}  // namespace pdr
+
+static bool radix235(int n) {
+  if (n < 2) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+// g_DomainCalc frame initialisation (src/pg_domain.f90:177-216)
+static void frame_init(pdr_ctx* c) {
+  for (int a = 0; a < 9; ++a) {
+    c->bmat[a] = c->binv[a] = c->amat[a] = 0.0;
+    c->bval[a] = c->aval[a] = 0;
+  }
+  const int b110 = c->p.b110 > 0 ? c->p.b110 : 1;
+  T9(c->binv, 1, 1) = double(b110);
+  T9(c->bval, 1, 1) = 1;
+  T9(c->binv, 2, 2) = 1.0;
+  T9(c->bval, 2, 2) = 1;
+  T9(c->binv, 3, 3) = 1.0;
+  T9(c->bval, 3, 3) = 1;
+  if (c->p.f_type != PDR_F_ISOTROPIC) {
+    T9(c->bval, 1, 3) = 1;
+    T9(c->amat, 1, 3) = c->p.f_deform_s;
+    T9(c->aval, 1, 3) = 1;
+  }
+  T9(c->bmat, 1, 1) = 1.0 / T9(c->binv, 1, 1);
+  T9(c->bmat, 2, 2) = 1.0 / T9(c->binv, 2, 2);
+  T9(c->bmat, 3, 3) = 1.0 / T9(c->binv, 3, 3);
+}
+
+// Operator tensors from the current frame (see Coef).
+static Coef make_coef(const pdr_ctx* c) {
+  Coef f;
+  std::memset(&f, 0, sizeof(f));
+  const double* B = c->bmat;
+  const double* BI = c->binv;
+  const double* A = c->amat;
+  const int* bv = c->bval;
+  const int* av = c->aval;
+  auto G = [&](int cc, int n, int m) -> double& { return f.G[((cc - 1) * 3 + (n - 1)) * 3 + (m - 1)]; };
+  for (int n = 1; n <= 3; ++n)
+    for (int m = 1; m <= 3; ++m) {
+      if (T9(bv, m, 1)) {
+        if (T9(bv, 3, n)) G(2, n, m) -= T9(B, m, 1) * T9(BI, 3, n);
+        if (T9(bv, 2, n)) G(3, n, m) += T9(B, m, 1) * T9(BI, 2, n);
+      }
+      if (T9(bv, m, 2)) {
+        if (T9(bv, 3, n)) G(1, n, m) += T9(B, m, 2) * T9(BI, 3, n);
+        if (T9(bv, 1, n)) G(3, n, m) -= T9(B, m, 2) * T9(BI, 1, n);
+      }
+      if (T9(bv, m, 3)) {
+        if (T9(bv, 2, n)) G(1, n, m) -= T9(B, m, 3) * T9(BI, 2, n);
+        if (T9(bv, 1, n)) G(2, n, m) += T9(B, m, 3) * T9(BI, 1, n);
+      }
+    }
+  for (int r = 1; r <= 3; ++r)
+    for (int m = 1; m <= 3; ++m) {
+      f.V[3 * (r - 1) + (m - 1)] = T9(bv, r, m) ? T9(BI, r, m) : 0.0;
+      f.Bm[3 * (r - 1) + (m - 1)] = T9(bv, r, m) ? T9(B, r, m) : 0.0;
+    }
+  for (int l = 1; l <= 3; ++l)
+    for (int n = 1; n <= 3; ++n) {
+      double s = 0.0;
+      for (int j = 1; j <= 3; ++j)
+        if (T9(bv, l, j) && T9(bv, n, j)) s += T9(B, l, j) * T9(B, n, j);
+      f.M[3 * (l - 1) + (n - 1)] = s;
+    }
+  for (int m = 1; m <= 3; ++m)
+    for (int cc = 1; cc <= 3; ++cc) {
+      double s = 0.0, h = 0.0;
+      for (int r = 1; r <= 3; ++r)
+        for (int s2 = 1; s2 <= 3; ++s2) {
+          if (T9(bv, m, r) && T9(av, r, s2) && T9(bv, s2, cc)) s += T9(B, m, r) * T9(A, r, s2) * T9(BI, s2, cc);
+        }
+      f.P[3 * (m - 1) + (cc - 1)] = s;
+      for (int r = 1; r <= 3; ++r)
+        for (int j = 1; j <= 3; ++j)
+          if (T9(bv, m, r) && T9(av, r, j) && T9(bv, j, cc)) h += T9(B, m, r) * T9(A, r, j) * T9(BI, j, cc);
+      f.H[3 * (m - 1) + (cc - 1)] = h;
+    }
+  f.nu = c->p.f_nu;
+  f.use_nl = (c->p.f_type == PDR_F_ISOTROPIC || c->p.f_type == PDR_F_SHEAR_DNS) ? 1 : 0;
+  f.use_visc = (c->p.f_type != PDR_F_SHEAR_RDT_INVISCID) ? 1 : 0;
+  f.has_hom = (c->p.f_type != PDR_F_ISOTROPIC) ? 1 : 0;
+  return f;
+}
+
+static TracerFrame make_tracer_frame(const pdr_ctx* c) {
+  TracerFrame t;
+  std::memset(&t, 0, sizeof(t));
+  for (int m = 1; m <= 3; ++m)
+    for (int n = 1; n <= 3; ++n) {
+      t.binv[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->binv, m, n) : 0.0;
+      t.bmat[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->bmat, m, n) : 0.0;
+    }
+  t.bf = (c->p.f_type != PDR_F_ISOTROPIC) ? 1 : 0;
+  t.order = c->p.p_interp_order;
+  t.lx = kTwoPiRef;
+  t.ly = kTwoPiRef;
+  t.lz = kTwoPiRef;
+  t.dx = t.lx / double(c->g.nx);
+  t.dy = t.ly / double(c->g.ny);
+  t.dz = t.lz / double(c->g.nz);
+  return t;
+}
+
+template <typename T>
+static void make_twiddles(int n, std::vector<unsigned char>& out) {
+  out.resize(sizeof(cx<T>) * size_t(n));
+  cx<T>* w = reinterpret_cast<cx<T>*>(out.data());
+  const long double pil = 3.14159265358979323846264338327950288L;
+  for (int m = 0; m < n; ++m) {
+    const long double a = -2.0L * pil * (long double)m / (long double)n;
+    w[m].x = T(double(cosl(a)));
+    w[m].y = T(double(sinl(a)));
+  }
+}
+
+static int upload_tw(pdr_ctx* c, int n, void** dst) {
+  std::vector<unsigned char> h;
+  if (c->prec)
+    make_twiddles<float>(n, h);
+  else
+    make_twiddles<double>(n, h);
+  HIPCHK(hipMalloc(dst, h.size()));
+  c->dev_bytes += h.size();
+  HIPCHK(hipMemcpy(*dst, h.data(), h.size(), hipMemcpyHostToDevice));
+  return PDR_OK;
+}
+
+static int dalloc(pdr_ctx* c, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(PDR_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " +
+                                   hipGetErrorString(e));
+  }
+  c->dev_bytes += bytes;
+  return PDR_OK;
+}
+
+extern "C" {
+
+const char* pdr_version(void) { return "pandora-amd 0.1 (gfx950, HIP)"; }
+
+const char* pdr_last_error(void) { return g_err.c_str(); }
+
+int pdr_destroy(pdr_ctx* c) {
+  if (!c) return PDR_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int q = 0; q < 3; ++q) {
+    if (c->u[q]) (void)hipFree(c->u[q]);
+    if (c->rk[q]) (void)hipFree(c->rk[q]);
+  }
+  if (c->w1) (void)hipFree(c->w1);
+  if (c->w2) (void)hipFree(c->w2);
+  if (c->ur) (void)hipFree(c->ur);
+  if (c->xp) (void)hipFree(c->xp);
+  if (c->uprk) (void)hipFree(c->uprk);
+  if (c->umax) (void)hipFree(c->umax);
+  if (c->red) (void)hipFree(c->red);
+  if (c->tab.twz) (void)hipFree(c->tab.twz);
+  if (c->tab.twy) (void)hipFree(c->tab.twy);
+  if (c->tab.twx) (void)hipFree(c->tab.twx);
+  for (auto& e : c->ev_pending) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PDR_OK;
+}
+
+int pdr_create(pdr_ctx** out, const pdr_params* p) {
+  if (!out || !p) return fail(PDR_ERR_ARG, "pdr_create: null argument");
+  *out = nullptr;
+  const int nx = p->nodes_x, ny = p->nodes_y, nz = p->nodes_z;
+  if (!radix235(nx) || !radix235(ny) || !radix235(nz) || nx % 2 || nx < 8 || ny < 8 || nz < 8)
+    return fail(PDR_ERR_UNSUPPORTED, "grid sizes must be even, >= 8 and of the form 2^a 3^b 5^c");
+  if (p->f_type < 1 || p->f_type > 4)
+    return fail(PDR_ERR_UNSUPPORTED, "F_TYPE must be 1 (isotropic) or 2..4 (shear DNS/RDT)");
+  if (p->f_type != PDR_F_ISOTROPIC && p->ns_deformed != PDR_NS_BF_ROT)
+    return fail(PDR_ERR_UNSUPPORTED, "shear flows support NS_DEFORMED = 1 (BF_ROT) only");
+  if (p->dealiasing != 1) return fail(PDR_ERR_UNSUPPORTED, "DEALIASING must be 1 (2/3 rule by storage)");
+  if (p->rk3_timestep < 1 || p->rk3_timestep > 3) return fail(PDR_ERR_ARG, "RK3_TIMESTEP must be 1, 2 or 3");
+  if (p->rk3_cfl >= 1.0) return fail(PDR_ERR_ARG, "STOP: Courant number is greater than 1.0 (g_RK3Check)");
+  if (p->f_type != PDR_F_ISOTROPIC && p->rk3_timestep == PDR_RK3_TIMESTEP_DECIDE &&
+      (p->f_remesh_t <= 0 || p->f_deform_s == 0.0))
+    return fail(PDR_ERR_ARG, "shear with RK3_TIMESTEP=DECIDE needs F_REMESH_T > 0 and F_DEFORM_S != 0");
+  if (p->p_interp_order < 2 || p->p_interp_order > 8 || (p->p_interp_order & 1))
+    return fail(PDR_ERR_UNSUPPORTED, "P_INTERP_ORDER must be even, 2..8");
+  if (p->precision != PDR_FP64 && p->precision != PDR_FP32_MIXED)
+    return fail(PDR_ERR_ARG, "precision must be PDR_FP64 or PDR_FP32_MIXED");
+  if (p->b110 != 1) return fail(PDR_ERR_UNSUPPORTED, "B110 must be 1");
+
+  pdr_ctx* c = new pdr_ctx();
+  c->p = *p;
+  c->prec = p->precision;
+  c->csz = c->prec ? 8 : 16;
+  c->rsz = c->prec ? 4 : 8;
+  Geo& g = c->g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.ki = nx / 3;
+  g.kj = ny / 3;
+  g.kk = nz / 3;
+  g.nki = g.ki + 1;
+  g.nkj = 2 * g.kj + 1;
+  g.nkk = 2 * g.kk + 1;
+  g.mw = (long long)g.nki * g.nkj * g.nkk;
+  g.m1 = (long long)g.nki * g.nz * g.nkj;
+  g.m2 = (long long)g.nz * g.ny * g.nki;
+  g.nr = (long long)nx * ny * nz;
+  frame_init(c);
+  c->dt = 99999.0;
+
+  int dev = 0;
+  hipError_t he = hipGetDevice(&dev);
+  if (he != hipSuccess) {
+    delete c;
+    return fail(PDR_ERR_HIP, std::string("no HIP device: ") + hipGetErrorString(he));
+  }
+  int lds = 65536;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) lds = 65536;
+  c->lds_max = size_t(lds);
+  // keep two workgroups per CU (160 KB of LDS per CU on gfx950)
+  const size_t target = std::min<size_t>(c->lds_max, 81920);
+  auto pick = [&](int len, int nlines) {
+    int t = 16;
+    while (t > 1 && (2 * size_t(t) * (len + 1) * c->csz > target || t > nlines)) t /= 2;
+    return t;
+  };
+  c->tj_z = pick(nz, g.nkj);
+  c->ti_y = pick(ny, g.nki);
+  if (6 * size_t(nx + 1) * c->csz > c->lds_max || 2 * size_t(nz + 1) * c->csz > c->lds_max ||
+      2 * size_t(ny + 1) * c->csz > c->lds_max) {
+    delete c;
+    return fail(PDR_ERR_UNSUPPORTED, "grid too large for the LDS of this device");
+  }
+  fft_plan(&c->pz, nz);
+  fft_plan(&c->py, ny);
+  fft_plan(&c->px, nx);
+
+  int rc = PDR_OK;
+  he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    delete c;
+    return fail(PDR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+  }
+  const size_t wbytes = size_t(g.mw) * c->csz;
+  for (int q = 0; q < 3 && rc == PDR_OK; ++q) {
+    rc = dalloc(c, &c->u[q], wbytes);
+    if (rc == PDR_OK) rc = dalloc(c, &c->rk[q], wbytes);
+  }
+  if (rc == PDR_OK) rc = dalloc(c, &c->w1, 6 * size_t(g.m1) * c->csz);
+  const size_t w2n = std::max<size_t>(6 * size_t(g.m2), 3 * size_t(g.mw));
+  if (rc == PDR_OK) rc = dalloc(c, &c->w2, w2n * c->csz);
+  if (rc == PDR_OK) rc = dalloc(c, (void**)&c->umax, 256);
+  if (rc == PDR_OK) rc = upload_tw(c, nz, &c->tab.twz);
+  if (rc == PDR_OK) rc = upload_tw(c, ny, &c->tab.twy);
+  if (rc == PDR_OK) rc = upload_tw(c, nx, &c->tab.twx);
+  if (rc == PDR_OK) {
+    for (int q = 0; q < 3; ++q) {
+      if (hipMemsetAsync(c->u[q], 0, wbytes, c->stream) != hipSuccess ||
+          hipMemsetAsync(c->rk[q], 0, wbytes, c->stream) != hipSuccess)
+        rc = fail(PDR_ERR_HIP, "zeroing state failed");
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(PDR_ERR_HIP, "stream sync failed");
+  }
+  if (rc != PDR_OK) {
+    std::string keep = g_err;
+    pdr_destroy(c);
+    g_err = keep;
+    return rc;
+  }
+  *out = c;
+  return PDR_OK;
+}
+
+int pdr_sizes(const pdr_ctx* c, int64_t out[8]) {
+  if (!c || !out) return fail(PDR_ERR_ARG, "pdr_sizes: null argument");
+  const Geo& g = c->g;
+  out[0] = g.nki;
+  out[1] = g.nkj;
+  out[2] = g.nkk;
+  out[3] = g.mw;
+  out[4] = g.m1;
+  out[5] = (int64_t)llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk)));
+  out[6] = (int64_t)c->dev_bytes;
+  out[7] = c->np;
+  return PDR_OK;
+}
+
+static int put_field(pdr_ctx* c, void* dst, const double* src, long long n) {
+  if (!src) return fail(PDR_ERR_ARG, "null host array");
+  if (c->prec == PDR_FP64) {
+    HIPCHK(hipMemcpy(dst, src, size_t(n) * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(static_cast<size_t>(n));
+    for (long long q = 0; q < n; ++q) tmp[size_t(q)] = float(src[q]);
+    HIPCHK(hipMemcpy(dst, tmp.data(), size_t(n) * sizeof(float), hipMemcpyHostToDevice));
+  }
+  return PDR_OK;
+}
+
+static int get_field(pdr_ctx* c, double* dst, const void* src, long long n) {
+  if (!dst) return fail(PDR_ERR_ARG, "null host array");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->prec == PDR_FP64) {
+    HIPCHK(hipMemcpy(dst, src, size_t(n) * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(static_cast<size_t>(n));
+    HIPCHK(hipMemcpy(tmp.data(), src, size_t(n) * sizeof(float), hipMemcpyDeviceToHost));
+    for (long long q = 0; q < n; ++q) dst[q] = double(tmp[size_t(q)]);
+  }
+  return PDR_OK;
+}
+
+int pdr_upload_state(pdr_ctx* c, const double* a, const double* b, const double* d) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const long long n = 2 * c->g.mw;
+  int rc = put_field(c, c->u[0], a, n);
+  if (rc == PDR_OK) rc = put_field(c, c->u[1], b, n);
+  if (rc == PDR_OK) rc = put_field(c, c->u[2], d, n);
+  return rc;
+}
+
+int pdr_download_state(pdr_ctx* c, double* a, double* b, double* d) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  const long long n = 2 * c->g.mw;
+  int rc = get_field(c, a, c->u[0], n);
+  if (rc == PDR_OK) rc = get_field(c, b, c->u[1], n);
+  if (rc == PDR_OK) rc = get_field(c, d, c->u[2], n);
+  return rc;
+}
+
+int pdr_upload_rk3(pdr_ctx* c, const double* a, const double* b, const double* d) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const long long n = 2 * c->g.mw;
+  int rc = put_field(c, c->rk[0], a, n);
+  if (rc == PDR_OK) rc = put_field(c, c->rk[1], b, n);
+  if (rc == PDR_OK) rc = put_field(c, c->rk[2], d, n);
+  return rc;
+}
+
+int pdr_download_rk3(pdr_ctx* c, double* a, double* b, double* d) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  const long long n = 2 * c->g.mw;
+  int rc = get_field(c, a, c->rk[0], n);
+  if (rc == PDR_OK) rc = get_field(c, b, c->rk[1], n);
+  if (rc == PDR_OK) rc = get_field(c, d, c->rk[2], n);
+  return rc;
+}
+
+int pdr_set_frame(pdr_ctx* c, const double bmat[9], const double binv[9]) {
+  if (!c || !bmat || !binv) return fail(PDR_ERR_ARG, "pdr_set_frame: null argument");
+  std::memcpy(c->bmat, bmat, sizeof(c->bmat));
+  std::memcpy(c->binv, binv, sizeof(c->binv));
+  return PDR_OK;
+}
+
+int pdr_get_frame(const pdr_ctx* c, double bmat[9], double binv[9]) {
+  if (!c || !bmat || !binv) return fail(PDR_ERR_ARG, "pdr_get_frame: null argument");
+  std::memcpy(bmat, c->bmat, sizeof(c->bmat));
+  std::memcpy(binv, c->binv, sizeof(c->binv));
+  return PDR_OK;
+}
+
+int pdr_set_dt(pdr_ctx* c, double dt) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  c->dt = dt;
+  return PDR_OK;
+}
+
+int pdr_get_dt(const pdr_ctx* c, double* dt) {
+  if (!c || !dt) return fail(PDR_ERR_ARG, "null argument");
+  *dt = c->dt;
+  return PDR_OK;
+}
+
+int pdr_synchronize(pdr_ctx* c) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PDR_OK;
+}
+
+// f_RK3Remesh (src/pf_rk3.f90:3958-4049) + f_RK3RemeshWaveSpace frame updates (4056-4178)
+static int maybe_remesh(pdr_ctx* c, int tstep, int32_t* done) {
+  if (done) *done = 0;
+  if (c->p.f_type == PDR_F_ISOTROPIC || c->p.rk3_timestep != PDR_RK3_TIMESTEP_DECIDE) return PDR_OK;
+  const int b110 = c->p.b110;
+  const int a = tstep - (b110 * c->p.f_remesh_t) - 1;
+  const int b = 2 * b110 * c->p.f_remesh_t;
+  if (a % b != 0) return PDR_OK;  // Fortran mod truncates toward zero like C %
+  const TracerFrame tf = make_tracer_frame(c);
+  HIPCHK(launch_remesh_particles(c, tf.lx, tf.dx));
+  T9(c->binv, 1, 3) = (double(b110) * 0.5) / T9(c->bmat, 3, 3);
+  T9(c->bmat, 1, 3) = -T9(c->bmat, 1, 1) * (-(double(b110) * 0.5));
+  HIPCHK(launch_remesh(c, c->p.ns_deformed == PDR_NS_BF_ROT ? 1 : 0));
+  T9(c->binv, 1, 3) = (-(double(b110) * 0.5)) / T9(c->bmat, 3, 3);
+  if (done) *done = 1;
+  return PDR_OK;
+}
+
+int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt) {
+  (void)time;
+  if (!c || !dt) return fail(PDR_ERR_ARG, "pdr_stage: null argument");
+  if (stage < 1 || stage > 3) return fail(PDR_ERR_ARG, "stage must be 1, 2 or 3");
+  if (stage == 1) {
+    int rc = maybe_remesh(c, tstep, nullptr);
+    if (rc != PDR_OK) return rc;
+  }
+  const Coef cf = make_coef(c);
+  const bool tracers = c->np > 0;
+  const bool need_cfl =
+      stage == 1 && (c->p.rk3_timestep == PDR_RK3_TIMESTEP_CFL ||
+                     (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE && c->p.f_type == PDR_F_ISOTROPIC));
+  HIPCHK(launch_zinv(c, cf));
+  HIPCHK(launch_yinv(c));
+  if (need_cfl) HIPCHK(hipMemsetAsync(c->umax, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(launch_xpass(c, cf, tracers, need_cfl));
+  HIPCHK(launch_yfwd(c));
+  if (stage == 1) {
+    // f_RK3Tstep (src/pf_rk3.f90:3852-3948)
+    if (need_cfl) {
+      unsigned long long bits = 0;
+      HIPCHK(hipMemcpyAsync(&bits, c->umax, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      double u;
+      std::memcpy(&u, &bits, sizeof(u));
+      c->ucfl = u;
+    }
+    const double dx = kTwoPiRef / double(c->g.nx);
+    switch (c->p.rk3_timestep) {
+      case PDR_RK3_TIMESTEP_FIXED: *dt = c->p.rk3_dt; break;
+      case PDR_RK3_TIMESTEP_CFL: *dt = (c->ucfl == 0.0) ? double(0.02f) : c->p.rk3_cfl * dx / c->ucfl; break;
+      default:
+        if (c->p.f_type == PDR_F_ISOTROPIC) {
+          if (c->ucfl == 0.0) {
+            *dt = double(0.02f);
+          } else {
+            *dt = c->p.rk3_cfl * dx / c->ucfl;
+            if (*dt > c->p.force_tf) *dt = c->p.force_tf;
+          }
+        } else {
+          *dt = ((0.5 / c->p.f_deform_s) / double(c->p.f_remesh_t));
+        }
+    }
+  }
+  double beta, gamma;
+  rk_coeffs(stage, &beta, &gamma);
+  const double gdt = gamma * (*dt);
+  if (tracers) HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt));
+  HIPCHK(launch_zfwd(c));
+  HIPCHK(launch_formf(c, cf, beta, gdt));
+  return PDR_OK;
+}
+
+int pdr_remesh(pdr_ctx* c, int32_t tstep, int32_t* done) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  return maybe_remesh(c, tstep, done);
+}
+
+// g_RK3UpdateB (src/pg_rk3.f90:280-385)
+int pdr_update_b(pdr_ctx* c, int32_t tstep, double simtime) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  if (c->p.f_type == PDR_F_ISOTROPIC) return PDR_OK;
+  const int b110 = c->p.b110;
+  double localtime;
+  if (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE) {
+    if (tstep <= (b110 * c->p.f_remesh_t)) {
+      localtime = simtime;
+    } else {
+      const double a = double(tstep - (b110 * c->p.f_remesh_t) - 1);
+      const double b = double(2 * (b110 * c->p.f_remesh_t));
+      const int localtstep = int(a / b);
+      localtime = simtime - (double(localtstep + 1) / c->p.f_deform_s * double(b110));
+    }
+  } else {
+    localtime = simtime;
+  }
+  T9(c->bmat, 1, 3) = -T9(c->bmat, 1, 1) * c->p.f_deform_s * localtime;
+  T9(c->binv, 1, 3) = c->p.f_deform_s * localtime / T9(c->bmat, 3, 3);
+  return PDR_OK;
+}
+
+// g_RK3Calc (src/pg_rk3.f90:82-129)
+int pdr_step(pdr_ctx* c, int32_t tstep, double* simtime) {
+  if (!c || !simtime) return fail(PDR_ERR_ARG, "pdr_step: null argument");
+  const double tstart = *simtime;
+  int rc = pdr_stage(c, 1, tstep, *simtime, &c->dt);
+  if (rc != PDR_OK) return rc;
+  *simtime = tstart + (1.0 / 3.0) * c->dt;
+  pdr_update_b(c, tstep, *simtime);
+  rc = pdr_stage(c, 2, tstep, *simtime, &c->dt);
+  if (rc != PDR_OK) return rc;
+  *simtime = tstart + (3.0 / 4.0) * c->dt;
+  pdr_update_b(c, tstep, *simtime);
+  rc = pdr_stage(c, 3, tstep, *simtime, &c->dt);
+  if (rc != PDR_OK) return rc;
+  *simtime = tstart + c->dt;
+  pdr_update_b(c, tstep, *simtime);
+  return PDR_OK;
+}
+
+int pdr_run(pdr_ctx* c, int32_t tstep0, int32_t nsteps, double* simtime) {
+  if (!c || !simtime || nsteps < 0) return fail(PDR_ERR_ARG, "pdr_run: bad argument");
+  for (int32_t s = 0; s < nsteps; ++s) {
+    int rc = pdr_step(c, tstep0 + s, simtime);
+    if (rc != PDR_OK) return rc;
+  }
+  return PDR_OK;
+}
+
+int pdr_check_finite(pdr_ctx* c) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  int* flag = nullptr;
+  HIPCHK(hipMalloc(&flag, sizeof(int)));
+  int h = 0;
+  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), c->stream);
+  if (e == hipSuccess) e = launch_finite(c, flag);
+  if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(flag);
+  if (e != hipSuccess) return fail(PDR_ERR_HIP, std::string("finite check: ") + hipGetErrorString(e));
+  if (h) return fail(PDR_ERR_NONFINITE, "velocity state contains NaN or Inf");
+  return PDR_OK;
+}
+
+int pdr_particles_upload(pdr_ctx* c, const double* xp, int64_t np) {
+  if (!c || np < 0 || (np > 0 && !xp)) return fail(PDR_ERR_ARG, "pdr_particles_upload: bad argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->xp) { (void)hipFree(c->xp); c->dev_bytes -= 24 * size_t(c->np); c->xp = nullptr; }
+  if (c->uprk) { (void)hipFree(c->uprk); c->dev_bytes -= 24 * size_t(c->np); c->uprk = nullptr; }
+  c->np = 0;
+  if (np == 0) return PDR_OK;
+  int rc = dalloc(c, (void**)&c->xp, 24 * size_t(np));
+  if (rc == PDR_OK) rc = dalloc(c, (void**)&c->uprk, 24 * size_t(np));
+  if (rc == PDR_OK && !c->ur) rc = dalloc(c, &c->ur, 3 * size_t(c->g.nr) * c->rsz);
+  if (rc != PDR_OK) return rc;
+  HIPCHK(hipMemcpy(c->xp, xp, 24 * size_t(np), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->uprk, 0, 24 * size_t(np)));
+  c->np = np;
+  return PDR_OK;
+}
+
+int pdr_particles_download(pdr_ctx* c, double* xp, double* uprk) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  if (c->np == 0) return fail(PDR_ERR_STATE, "no particles set");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (xp) HIPCHK(hipMemcpy(xp, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToHost));
+  if (uprk) HIPCHK(hipMemcpy(uprk, c->uprk, 24 * size_t(c->np), hipMemcpyDeviceToHost));
+  return PDR_OK;
+}
+
+int pdr_debug_prepare(pdr_ctx* c, int32_t field, double* out) {
+  if (!c || !out || field < 0 || field > 5) return fail(PDR_ERR_ARG, "pdr_debug_prepare: bad argument");
+  const Coef cf = make_coef(c);
+  HIPCHK(launch_zinv(c, cf));
+  const size_t off = size_t(field) * size_t(c->g.m1) * c->csz;
+  return get_field(c, out, (const char*)c->w1 + off, 2 * c->g.m1);
+}
+
+int pdr_profile_enable(pdr_ctx* c, int32_t enable) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  c->prof = enable != 0;
+  if (c->prof) {
+    for (int k = 0; k < K_NCLASS; ++k) {
+      c->prof_ms[k] = 0.0;
+      c->prof_cnt[k] = 0;
+    }
+  }
+  return PDR_OK;
+}
+
+int pdr_profile_get(pdr_ctx* c, int32_t n, char* names, int32_t namelen, int64_t* counts, double* total_ms) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (auto& e : c->ev_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) {
+      c->prof_ms[e.first] += ms;
+      c->prof_cnt[e.first] += 1;
+    }
+    c->ev_pool.push_back(e.second.first);
+    c->ev_pool.push_back(e.second.second);
+  }
+  c->ev_pending.clear();
+  static const char* kNames[K_NCLASS] = {"zinv", "yinv", "xpass", "yfwd", "zfwd", "formf", "tracers", "remesh", "misc"};
+  std::string all;
+  for (int k = 0; k < K_NCLASS && k < n; ++k) {
+    if (counts) counts[k] = c->prof_cnt[k];
+    if (total_ms) total_ms[k] = c->prof_ms[k];
+    if (k) all += ",";
+    all += kNames[k];
+  }
+  if (names && namelen > 0) {
+    std::strncpy(names, all.c_str(), size_t(namelen) - 1);
+    names[namelen - 1] = 0;
+  }
+  return PDR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,312 @@
+// pdr_host.cpp -- host-side pieces of the C-ABI that sit either side of the
+// time step: seeded initial conditions, PETSc-binary restart I/O and the
+// wave-space spectrum.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pdr_internal.hpp"
+
+namespace {
+
+// float32 pi widened to double: ONEPI, TWOPI, FOURPI of src/pg_constants.f90:126-131
+const double kOnePi = double(3.14159265358979323846f);
+const double kTwoPi = 2.0 * kOnePi;
+const double kFourPi = 4.0 * kOnePi;
+const int32_t kVecFileClassId = 1211214;  // PETSc VEC_FILE_CLASSID
+
+// Counter-based uniform generator: SplitMix64 (Steele, Lea, Flood, OOPSLA 2014)
+// of (seed, draw index), top 53 bits scaled to [0, 1).
+inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+inline double u01(uint64_t seed, uint64_t idx) {
+  const uint64_t h = splitmix64(splitmix64(seed) ^ (idx * 0xD1B54A32D192ED03ull + 0x8CB92BA72F3D8DD7ull));
+  return double(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+inline double wvn(int k, int kmax) { return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1)); }
+
+struct Mode {
+  double t[3][2];
+};
+
+// One Rogallo mode (src/pf_initialise.f90:161-219) with the type-10 spectrum
+// E(k) = 16 (2/pi)^0.5 (3/2) U0 k^4/kp^5 exp(-2k^2/kp^2) (src/pf_initialise.f90:1081-1083).
+Mode rogallo(double ki, double kj, double kk, double kmag, double k2d, double r1, double r2, double r3,
+             double u0, double kp) {
+  Mode m;
+  std::memset(&m, 0, sizeof(m));
+  if (!(kmag > 0.0) || !(k2d > 0.0)) return m;
+  const double ek = 16.0 * std::sqrt(2.0 / kOnePi) * 1.5 * u0 *
+                    ((kmag * kmag * kmag * kmag) / (kp * kp * kp * kp * kp)) *
+                    std::exp(-(2.0 * kmag * kmag) / (kp * kp));
+  const double factor = std::sqrt(ek / (kFourPi * kmag * kmag));
+  const double al[2] = {factor * std::cos(r1) * std::cos(r3), factor * std::sin(r1) * std::cos(r3)};
+  const double be[2] = {factor * std::cos(r2) * std::sin(r3), factor * std::sin(r2) * std::sin(r3)};
+  const double s2 = std::sqrt(k2d);
+  for (int q = 0; q < 2; ++q) {
+    m.t[0][q] = (al[q] * kmag * kj + be[q] * ki * kk) / (kmag * s2);
+    m.t[1][q] = (be[q] * kj * kk - al[q] * kmag * ki) / (kmag * s2);
+    m.t[2][q] = -(be[q] * s2) / kmag;
+  }
+  return m;
+}
+
+void put(double** u, long long e, const Mode& m, bool conj) {
+  for (int c = 0; c < 3; ++c) {
+    u[c][2 * e] = m.t[c][0];
+    u[c][2 * e + 1] = conj ? -m.t[c][1] : m.t[c][1];
+  }
+}
+
+void be_store(unsigned char* p, uint64_t v, int n) {
+  for (int q = 0; q < n; ++q) p[q] = (unsigned char)((v >> (8 * (n - 1 - q))) & 0xff);
+}
+uint64_t be_load(const unsigned char* p, int n) {
+  uint64_t v = 0;
+  for (int q = 0; q < n; ++q) v = (v << 8) | p[q];
+  return v;
+}
+
+// Neumaier compensated summation
+struct KSum {
+  double s = 0.0, c = 0.0;
+  void add(double x) {
+    const double t = s + x;
+    if (std::fabs(s) >= std::fabs(x))
+      c += (s - t) + x;
+    else
+      c += (x - t) + s;
+    s = t;
+  }
+  double get() const { return s + c; }
+};
+
+}  // namespace
+
+extern "C" {
+
+int pdr_init_field_host(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, double init_u0,
+                        double init_kp, double* u1, double* u2, double* u3) {
+  if (nx < 2 || ny < 2 || nz < 2 || !u1 || !u2 || !u3 || !(init_kp > 0.0))
+    return pdr::set_error(PDR_ERR_ARG, "pdr_init_field_host: bad argument");
+  const int kj = ny / 3, kk = nz / 3;
+  const int nki = nx / 3 + 1, nkj = 2 * kj + 1, nkk = 2 * kk + 1;
+  const long long mw = (long long)nki * nkj * nkk;
+  double* u[3] = {u1, u2, u3};
+  for (int c = 0; c < 3; ++c) std::memset(u[c], 0, sizeof(double) * size_t(2 * mw));
+  auto idx = [&](int i, int j, int k) { return ((long long)i * nkj + j) * nkk + k; };
+  auto draw = [&](long long id, double* r) {
+    for (int q = 0; q < 3; ++q) r[q] = kTwoPi * u01(seed, uint64_t(id) * 3 + uint64_t(q));
+  };
+  // i = 0 plane with conjugate symmetry u(0,-j,-k) = conj u(0,j,k) (src/pf_initialise.f90:150-411)
+  for (int j = 0; j <= kj; ++j)
+    for (int k = 0; k <= kk; ++k) {
+      const double wj = wvn(j, kj), wk = wvn(k, kk);
+      const double kmag = std::sqrt((wj * wj) + (wk * wk));
+      const double k2d = wj * wj;
+      double r[3];
+      draw(2 * idx(0, j, k), r);
+      const Mode m = rogallo(0.0, wj, wk, kmag, k2d, r[0], r[1], r[2], init_u0, init_kp);
+      const int jj = (j != 0) ? nkj - j : j;
+      const int kc = (k != 0) ? nkk - k : k;
+      put(u, idx(0, j, k), m, false);
+      put(u, idx(0, jj, kc), m, true);
+      if (k != 0) {
+        draw(2 * idx(0, j, k) + 1, r);
+        const Mode m2 = rogallo(0.0, wvn(jj, kj), wk, kmag, k2d, r[0], r[1], r[2], init_u0, init_kp);
+        put(u, idx(0, jj, k), m2, false);
+        put(u, idx(0, j, kc), m2, true);
+      }
+    }
+  // i >= 1: no symmetry to enforce (src/pf_initialise.f90:416-505)
+  for (int i = 1; i < nki; ++i)
+    for (int j = 0; j < nkj; ++j)
+      for (int k = 0; k < nkk; ++k) {
+        const double wi = double(i), wj = wvn(j, kj), wk = wvn(k, kk);
+        const double kmag = std::sqrt((wi * wi) + (wj * wj) + (wk * wk));
+        const double k2d = (wi * wi) + (wj * wj);
+        double r[3];
+        draw(2 * idx(i, j, k), r);
+        put(u, idx(i, j, k), rogallo(wi, wj, wk, kmag, k2d, r[0], r[1], r[2], init_u0, init_kp), false);
+      }
+  return PDR_OK;
+}
+
+int pdr_init_field(pdr_ctx* ctx, uint64_t seed, double init_u0, double init_kp) {
+  if (!ctx) return pdr::set_error(PDR_ERR_ARG, "pdr_init_field: null context");
+  const pdr::Geo& g = ctx->g;
+  std::vector<double> u[3];
+  for (int c = 0; c < 3; ++c) u[c].assign(size_t(2 * g.mw), 0.0);
+  int rc = pdr_init_field_host(g.nx, g.ny, g.nz, seed, init_u0, init_kp, u[0].data(), u[1].data(), u[2].data());
+  if (rc != PDR_OK) return rc;
+  rc = pdr_upload_state(ctx, u[0].data(), u[1].data(), u[2].data());
+  if (rc != PDR_OK) return rc;
+  for (int c = 0; c < 3; ++c) std::fill(u[c].begin(), u[c].end(), 0.0);
+  return pdr_upload_rk3(ctx, u[0].data(), u[1].data(), u[2].data());
+}
+
+int pdr_init_particles(pdr_ctx* ctx, uint64_t seed, int64_t np) {
+  if (!ctx || np < 0) return pdr::set_error(PDR_ERR_ARG, "pdr_init_particles: bad argument");
+  const long double lx = kTwoPi, ly = kTwoPi, lz = kTwoPi;
+  const long double dx = (long double)(kTwoPi / double(ctx->g.nx));
+  const long double dy = (long double)(kTwoPi / double(ctx->g.ny));
+  const long double dz = (long double)(kTwoPi / double(ctx->g.nz));
+  std::vector<double> xp(size_t(3 * np));
+  // P_INIT_RANDOM: xp = rn * L - dx/2 evaluated in long double (src/pp_control.f90:657-686)
+  for (int64_t q = 0; q < np; ++q) {
+    const uint64_t b = uint64_t(q) * 3;
+    xp[size_t(3 * q)] = double((long double)u01(seed, b) * lx - dx * 0.5L);
+    xp[size_t(3 * q + 1)] = double((long double)u01(seed, b + 1) * ly - dy * 0.5L);
+    xp[size_t(3 * q + 2)] = double((long double)u01(seed, b + 2) * lz - dz * 0.5L);
+  }
+  return pdr_particles_upload(ctx, xp.data(), np);
+}
+
+int pdr_restart_write_host(const char* path, int64_t n, const double* u1, const double* u2,
+                           const double* u3, int32_t int64_header) {
+  if (!path || n <= 0 || !u1 || !u2 || !u3) return pdr::set_error(PDR_ERR_ARG, "pdr_restart_write_host: bad argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return pdr::set_error(PDR_ERR_IO, std::string("cannot open for writing: ") + path);
+  const int hl = int64_header ? 8 : 4;
+  const double* u[3] = {u1, u2, u3};
+  std::vector<unsigned char> buf(size_t(n) * 8);
+  bool ok = true;
+  for (int c = 0; c < 3 && ok; ++c) {
+    unsigned char h[12];
+    be_store(h, uint64_t(uint32_t(kVecFileClassId)), 4);
+    be_store(h + 4, uint64_t(n), hl);
+    ok = std::fwrite(h, 1, size_t(4 + hl), f) == size_t(4 + hl);
+    for (int64_t q = 0; q < n && ok; ++q) {
+      uint64_t bits;
+      std::memcpy(&bits, &u[c][q], 8);
+      be_store(&buf[size_t(8 * q)], bits, 8);
+    }
+    if (ok) ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  }
+  if (std::fclose(f) != 0) ok = false;
+  return ok ? PDR_OK : pdr::set_error(PDR_ERR_IO, std::string("write failed: ") + path);
+}
+
+int pdr_restart_read_host(const char* path, int64_t n, double* u1, double* u2, double* u3) {
+  if (!path || n <= 0 || !u1 || !u2 || !u3) return pdr::set_error(PDR_ERR_ARG, "pdr_restart_read_host: bad argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return pdr::set_error(PDR_ERR_IO, std::string("cannot open: ") + path);
+  std::fseek(f, 0, SEEK_END);
+  const long long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  int hl = 0;
+  if (size == 3 * (8 + 8 * n)) hl = 4;
+  if (size == 3 * (12 + 8 * n)) hl = 8;
+  if (!hl) {
+    std::fclose(f);
+    return pdr::set_error(PDR_ERR_IO, "restart file size does not match three PETSc Vec records of this grid");
+  }
+  double* u[3] = {u1, u2, u3};
+  std::vector<unsigned char> buf(size_t(n) * 8);
+  bool ok = true;
+  for (int c = 0; c < 3 && ok; ++c) {
+    unsigned char h[12];
+    ok = std::fread(h, 1, size_t(4 + hl), f) == size_t(4 + hl);
+    if (ok) ok = int32_t(be_load(h, 4)) == kVecFileClassId && (int64_t)be_load(h + 4, hl) == n;
+    if (ok) ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+    if (!ok) break;
+    for (int64_t q = 0; q < n; ++q) {
+      const uint64_t bits = be_load(&buf[size_t(8 * q)], 8);
+      std::memcpy(&u[c][q], &bits, 8);
+    }
+  }
+  std::fclose(f);
+  if (!ok) return pdr::set_error(PDR_ERR_IO, "bad PETSc Vec header or short read in restart file");
+  return PDR_OK;
+}
+
+int pdr_restart_write(pdr_ctx* ctx, const char* path, int32_t int64_header) {
+  if (!ctx || !path) return pdr::set_error(PDR_ERR_ARG, "pdr_restart_write: null argument");
+  const long long n = 2 * ctx->g.mw;
+  std::vector<double> u[3];
+  for (int c = 0; c < 3; ++c) u[c].resize(static_cast<size_t>(n));
+  int rc = pdr_download_state(ctx, u[0].data(), u[1].data(), u[2].data());
+  if (rc != PDR_OK) return rc;
+  return pdr_restart_write_host(path, n, u[0].data(), u[1].data(), u[2].data(), int64_header);
+}
+
+int pdr_restart_read(pdr_ctx* ctx, const char* path) {
+  if (!ctx || !path) return pdr::set_error(PDR_ERR_ARG, "pdr_restart_read: null argument");
+  const long long n = 2 * ctx->g.mw;
+  std::vector<double> u[3];
+  for (int c = 0; c < 3; ++c) u[c].resize(static_cast<size_t>(n));
+  int rc = pdr_restart_read_host(path, n, u[0].data(), u[1].data(), u[2].data());
+  if (rc != PDR_OK) return rc;
+  return pdr_upload_state(ctx, u[0].data(), u[1].data(), u[2].data());
+}
+
+int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6]) {
+  if (!ctx || !ek || !dk || !dsk || !tot) return pdr::set_error(PDR_ERR_ARG, "pdr_spectrum: null argument");
+  const pdr::Geo& g = ctx->g;
+  const long long n = 2 * g.mw;
+  std::vector<double> u[3];
+  for (int c = 0; c < 3; ++c) u[c].resize(static_cast<size_t>(n));
+  int rc = pdr_download_state(ctx, u[0].data(), u[1].data(), u[2].data());
+  if (rc != PDR_OK) return rc;
+  const int kkmax = int(std::llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk))));
+  std::vector<KSum> se(static_cast<size_t>(kkmax)), sd(static_cast<size_t>(kkmax)), ss(static_cast<size_t>(kkmax));
+  KSum s11, s22, s33;
+  const double nu = ctx->p.f_nu;
+  for (int i = 0; i < g.nki; ++i) {
+    const double wi = double(i);
+    for (int j = 0; j < g.nkj; ++j) {
+      const double wj = wvn(j, g.kj);
+      for (int k = 0; k < g.nkk; ++k) {
+        const double wk = wvn(k, g.kk);
+        const double km = std::sqrt((wi * wi) + (wj * wj) + (wk * wk));
+        const size_t e = size_t((((long long)i * g.nkj + j) * g.nkk + k) * 2);
+        double p11 = u[0][e] * u[0][e] + u[0][e + 1] * u[0][e + 1];
+        double p22 = u[1][e] * u[1][e] + u[1][e + 1] * u[1][e + 1];
+        double p33 = u[2][e] * u[2][e] + u[2][e + 1] * u[2][e + 1];
+        if (i != 0) {
+          p11 *= 2.0;
+          p22 *= 2.0;
+          p33 *= 2.0;
+        }
+        s11.add(p11);
+        s22.add(p22);
+        s33.add(p33);
+        const double e_ = 0.5 * (p11 + p22 + p33);
+        const int kc = int(std::floor(km + 0.5));
+        for (int kb = kc - 1; kb <= kc + 1; ++kb) {
+          if (kb < 1 || kb > kkmax) continue;
+          if (km >= double(kb) - 0.5 && km < double(kb) + 0.5) {
+            se[size_t(kb - 1)].add(e_);
+            sd[size_t(kb - 1)].add(2.0 * km * km * nu * e_);
+            ss[size_t(kb - 1)].add(4.0 * km * km * km * km * nu * nu * e_);
+          }
+        }
+      }
+    }
+  }
+  KSum fk, fd, fs;
+  for (int q = 0; q < kkmax; ++q) {
+    ek[q] = se[size_t(q)].get();
+    dk[q] = sd[size_t(q)].get();
+    dsk[q] = ss[size_t(q)].get();
+    fk.add(ek[q]);
+    fd.add(dk[q]);
+    fs.add(dsk[q]);
+  }
+  tot[0] = fk.get();
+  tot[1] = fd.get();
+  tot[2] = fs.get();
+  tot[3] = s11.get();
+  tot[4] = s22.get();
+  tot[5] = s33.get();
+  return PDR_OK;
+}
+
+}  // extern "C"

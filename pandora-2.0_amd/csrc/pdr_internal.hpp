@@ -1,0 +1,139 @@
+// pdr_internal.hpp -- shared declarations of the MI355X RK3-step library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pandora_amd.h"
+#include "pdr_fft.hpp"
+
+namespace pdr {
+
+// Domain sizes (g_DomainCalc, src/pg_domain.f90:220-228).
+struct Geo {
+  int nx, ny, nz;
+  int ki, kj, kk;     // KI_MAX, KJ_MAX, KK_MAX
+  int nki, nkj, nkk;  // NODES_KI, NODES_KJ, NODES_KK
+  long long mw;       // modes per wave field      NKI*NKJ*NKK
+  long long m1;       // modes per 1r2w field      NKI*NZ*NKJ
+  long long m2;       // modes per x-pencil field  NZ*NY*NKI
+  long long nr;       // real points               NX*NY*NZ
+};
+
+// Per-stage operator coefficients, evaluated on the host in FP64 from the
+// current frame (bmat, binv, amat and their existence masks), so that one
+// kernel serves the isotropic and the BF_ROT shear forms:
+//   vorticity   w_c = i * sum_n (sum_m G[c][n][m] k_m) u_n   (f_RK3VorticityBFRot,
+//               src/pf_rk3.f90:3786-3843; the isotropic i k x u is the identity frame)
+//   product     NL  = w x (V u), V = masked binv              (src/pf_rk3.f90:1597-1711)
+//   FormF       nlf = Bm NL; visc = -nu (k.M k) u; p1 = M k; p4 = k.M k;
+//               p3 = 2 sum_c (sum_m k_m P[m][c]) u_c; hom = 2 H u
+//               (src/pf_rk3.f90:2864-3209, 2323-2487)
+struct Coef {
+  double G[27];  // [c][n][m]
+  double V[9];   // [r][m] masked binv
+  double Bm[9];  // [i][c] masked bmat
+  double M[9];   // [l][n] sum_j bmat(l,j) bmat(n,j)
+  double P[9];   // [m][c]
+  double H[9];   // [i][c]
+  double nu;
+  int use_nl;    // F_TYPE 1, 2
+  int use_visc;  // F_TYPE 1, 2, 3
+  int has_hom;   // shear
+  int pad;
+};
+
+// Tracer frame transforms (p_RK3Mov2Lab / p_RK3Lab2Mov, src/pp_rk3.f90:989-1043).
+struct TracerFrame {
+  double binv[9];  // masked, [m][n]
+  double bmat[9];  // masked, [m][n]
+  int bf;          // 0: isotropic (no transform)
+  int order;       // P_INTERP_ORDER
+  double dx, dy, dz, lx, ly, lz;
+};
+
+enum KernelClass {
+  K_ZINV = 0,
+  K_YINV,
+  K_XPASS,
+  K_YFWD,
+  K_ZFWD,
+  K_FORMF,
+  K_TRACER,
+  K_REMESH,
+  K_MISC,
+  K_NCLASS
+};
+
+struct Tables {
+  void* twz = nullptr;
+  void* twy = nullptr;
+  void* twx = nullptr;
+};
+
+}  // namespace pdr
+
+struct pdr_ctx {
+  pdr_params p;
+  pdr::Geo g;
+  int prec = 0;     // 0 FP64, 1 FP32-mixed
+  size_t csz = 16;  // bytes per complex element
+  size_t rsz = 8;   // bytes per real element
+  // resident state
+  void* u[3] = {nullptr, nullptr, nullptr};
+  void* rk[3] = {nullptr, nullptr, nullptr};
+  // work buffers
+  void* w1 = nullptr;  // 6 fields x m1 (1r2w layout)
+  void* w2 = nullptr;  // 6 fields x m2 (x-pencil layout); the NL fields of pass 5 alias it
+  void* ur = nullptr;  // 3 x nr real-space velocity for tracers
+  double* xp = nullptr;
+  double* uprk = nullptr;
+  int64_t np = 0;
+  unsigned long long* umax = nullptr;  // CFL reduction (bits of a non-negative double)
+  double* red = nullptr;               // spectrum reduction scratch
+  pdr::Tables tab;
+  pdr::FftPlan pz, py, px;
+  // frame (FP64 on the host)
+  double bmat[9], binv[9], amat[9];
+  int bval[9], aval[9];
+  double dt = 99999.0;
+  double ucfl = 0.0;
+  // launch shapes
+  int tj_z = 1, ti_y = 1;
+  size_t lds_max = 65536;
+  hipStream_t stream = nullptr;
+  size_t dev_bytes = 0;
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  double prof_ms[pdr::K_NCLASS] = {0};
+  long long prof_cnt[pdr::K_NCLASS] = {0};
+};
+
+namespace pdr {
+
+// launchers (pdr_kernels.hip); return hipSuccess or the launch error
+hipError_t launch_zinv(pdr_ctx* c, const Coef& cf);
+hipError_t launch_yinv(pdr_ctx* c);
+hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl);
+hipError_t launch_yfwd(pdr_ctx* c);
+hipError_t launch_zfwd(pdr_ctx* c);
+hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
+hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt);
+hipError_t launch_remesh(pdr_ctx* c, int bfrot);
+hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);
+hipError_t launch_finite(pdr_ctx* c, int* flag);
+
+// profiling hooks (pdr_api.hip)
+void prof_begin(pdr_ctx* c, int cls);
+void prof_end(pdr_ctx* c, int cls);
+
+void fft_plan(FftPlan* p, int n);
+
+// records the text returned by pdr_last_error() and returns code
+int set_error(int code, const std::string& msg);
+
+}  // namespace pdr

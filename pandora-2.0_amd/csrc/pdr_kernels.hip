@@ -1,0 +1,780 @@
+// pdr_kernels.hip -- HIP kernels of the MI355X RK3 step (gfx950).
+//
+// One RK3 stage is five LDS-tiled passes over HBM-resident data plus two
+// elementwise kernels:
+//   1 k_zinv   per (i, j-tile): vorticity + zero-pad in k + inverse z FFT of
+//              u1..u3, w1..w3 -> 1r2w layout [f][i][z][j]
+//              (f_RK3PrepareRealSpace, src/pf_rk3.f90:205-948; pf_fftw.f90:586-624)
+//   2 k_yinv   per (z, i-tile): zero-pad in j + inverse y FFT -> x-pencils [f][z][y][i]
+//              (f_RK3TransformjiFs2Rs, src/pf_rk3.f90:1437-1498; pf_fftw.f90:673-759)
+//   3 k_xpass  per (z, y) row: zero-pad in i, c2r (two rows per complex FFT),
+//              nonlinear product, r2c, truncation to i <= KI; optionally writes the
+//              real-space velocity for the tracers and the CFL maximum
+//              (src/pf_rk3.f90:1069-1124, 1597-1711, 1780-1940; pf_fftw.f90:766-834)
+//   4 k_yfwd   per (z, i-tile): forward y FFT + truncation in j -> 1r2w layout
+//   5 k_zfwd   per (i, j-tile): forward z FFT, x 1/(NX NY NZ), truncation in k
+//              (f_RK3TransformkRs2Fs, src/pf_rk3.f90:2242-2316; pf_fftw.f90:841-879)
+//   k_formf    per mode: FormF + low-storage RK3 update + continuity fix
+//              (src/pf_rk3.f90:2323-2673, 3457-3531)
+//   k_tracers  per tracer: Neville interpolation + RK3 + periodic wrap
+//              (src/pp_interp.f90:313-560, src/pp_rk3.f90:57-303, src/pp_mpi.f90:1011-1057)
+//   k_remesh   per (i, j) line: integer k-index gather of the Rogallo remesh
+//              (src/pf_rk3.f90:4056-4178)
+// All kernels are HBM-bandwidth bound; none is a dense contraction.
+#include "pdr_internal.hpp"
+
+namespace pdr {
+
+__device__ __forceinline__ double wvn(int k, int kmax) {
+  return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1));
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> zero_c() {
+  return mk<T>(T(0), T(0));
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> conj_c(cx<T> a) {
+  return mk<T>(a.x, -a.y);
+}
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// pass 1: vorticity + inverse z FFT
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_zinv(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
+           const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, FftPlan pz,
+           const cx<T>* __restrict__ tw, int tj, int fsel) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nz = g.nz, K = g.kk, nkk = g.nkk, nkj = g.nkj;
+  const int ld = nz + 1;
+  cx<T>* A = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* B = A + tj * ld;
+  const int i = blockIdx.y;
+  const int j0 = blockIdx.x * tj;
+  const double wi = wvn(i, g.ki);
+  const int f_begin = (fsel < 0) ? 0 : fsel;
+  const int f_end = (fsel < 0) ? 6 : fsel + 1;
+  for (int f = f_begin; f < f_end; ++f) {
+    for (int idx = threadIdx.x; idx < tj * nz; idx += blockDim.x) {
+      const int line = idx / nz;
+      const int z = idx - line * nz;
+      const int j = j0 + line;
+      const int k = (z <= K) ? z : ((z >= nz - K) ? z - (nz - nkk) : -1);
+      cx<T> v = zero_c<T>();
+      if (j < nkj && k >= 0) {
+        const long long e = ((long long)i * nkj + j) * nkk + k;
+        if (f == 0) {
+          v = u0[e];
+        } else if (f == 1) {
+          v = u1[e];
+        } else if (f == 2) {
+          v = u2[e];
+        } else {
+          const double kv[3] = {wi, wvn(j, g.kj), wvn(k, K)};
+          const int c = f - 3;
+          const cx<T> un[3] = {u0[e], u1[e], u2[e]};
+          cx<T> acc = zero_c<T>();
+#pragma unroll
+          for (int n = 0; n < 3; ++n) {
+            const double* G = &cf.G[(c * 3 + n) * 3];
+            const T co = T(G[0] * kv[0] + G[1] * kv[1] + G[2] * kv[2]);
+            acc = add(acc, scl(un[n], co));
+          }
+          v = mk<T>(-acc.y, acc.x);  // i * acc
+        }
+      }
+      A[line * ld + z] = v;
+    }
+    __syncthreads();
+    cx<T>* R = fft_lines<T, +1>(A, B, pz, tj, ld, tw);
+    cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * nz * nkj;
+    for (int idx = threadIdx.x; idx < tj * nz; idx += blockDim.x) {
+      const int z = idx / tj;
+      const int line = idx - z * tj;
+      const int j = j0 + line;
+      if (j < nkj) dst[(long long)z * nkj + j] = R[line * ld + z];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: inverse y FFT, 1r2w [f][i][z][j] -> x-pencils [f][z][y][i]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, FftPlan py,
+           const cx<T>* __restrict__ tw, int ti) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ny = g.ny, nz = g.nz, KJ = g.kj, nkj = g.nkj, nki = g.nki;
+  const int ld = ny + 1;
+  cx<T>* A = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* B = A + ti * ld;
+  const int z = blockIdx.y;
+  const int i0 = blockIdx.x * ti;
+  for (int f = 0; f < 6; ++f) {
+    const cx<T>* src = w1 + (long long)f * g.m1;
+    for (int idx = threadIdx.x; idx < ti * ny; idx += blockDim.x) {
+      const int line = idx / ny;
+      const int y = idx - line * ny;
+      const int i = i0 + line;
+      const int j = (y <= KJ) ? y : ((y >= ny - KJ) ? y - (ny - nkj) : -1);
+      cx<T> v = zero_c<T>();
+      if (i < nki && j >= 0) v = src[((long long)i * nz + z) * nkj + j];
+      A[line * ld + y] = v;
+    }
+    __syncthreads();
+    cx<T>* R = fft_lines<T, +1>(A, B, py, ti, ld, tw);
+    cx<T>* dst = w2 + (long long)f * g.m2 + (long long)z * ny * nki;
+    for (int idx = threadIdx.x; idx < ti * ny; idx += blockDim.x) {
+      const int y = idx / ti;
+      const int line = idx - y * ti;
+      const int i = i0 + line;
+      if (i < nki) dst[(long long)y * nki + i] = R[line * ld + y];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 3: x c2r, nonlinear product, x r2c for one (z, y) row.
+// Two real rows share one complex FFT: z = a + i b.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
+            unsigned long long* __restrict__ umax, FftPlan px, const cx<T>* __restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ T red[kThreads];
+  const int nx = g.nx, KI = g.ki, nki = g.nki;
+  const int ld = nx + 1;
+  cx<T>* A = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* B = A + 3 * ld;
+  const int y = blockIdx.x;
+  const int z = blockIdx.y;
+  const long long row = ((long long)z * g.ny + y) * nki;
+  for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
+    const int p = idx / nx;
+    const int m = idx - p * nx;
+    cx<T> X1 = zero_c<T>(), X2 = zero_c<T>();
+    const cx<T>* fa = w2 + (long long)(2 * p) * g.m2 + row;
+    const cx<T>* fb = w2 + (long long)(2 * p + 1) * g.m2 + row;
+    if (m <= KI) {
+      X1 = fa[m];
+      X2 = fb[m];
+      if (m == 0) {  // c2r uses only the real part of the DC term
+        X1.y = T(0);
+        X2.y = T(0);
+      }
+    } else if (m >= nx - KI) {
+      X1 = conj_c(fa[nx - m]);
+      X2 = conj_c(fb[nx - m]);
+    }
+    A[p * ld + m] = mk<T>(X1.x - X2.y, X1.y + X2.x);
+  }
+  __syncthreads();
+  cx<T>* R = fft_lines<T, +1>(A, B, px, 3, ld, tw);
+  cx<T>* O = (R == A) ? B : A;
+  T lmax = T(0);
+  const T V[9] = {T(cf.V[0]), T(cf.V[1]), T(cf.V[2]), T(cf.V[3]), T(cf.V[4]),
+                  T(cf.V[5]), T(cf.V[6]), T(cf.V[7]), T(cf.V[8])};
+  const long long rr = ((long long)z * g.ny + y) * nx;
+  for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+    const cx<T> a = R[x], b = R[ld + x], c = R[2 * ld + x];
+    const T u1 = a.x, u2 = a.y, u3 = b.x;
+    const T o1 = b.y, o2 = c.x, o3 = c.y;
+    const T v1 = V[0] * u1 + V[1] * u2 + V[2] * u3;
+    const T v2 = V[3] * u1 + V[4] * u2 + V[5] * u3;
+    const T v3 = V[6] * u1 + V[7] * u2 + V[8] * u3;
+    const T n1 = o2 * v3 - o3 * v2;
+    const T n2 = o3 * v1 - o1 * v3;
+    const T n3 = o1 * v2 - o2 * v1;
+    if (want_tr) {
+      ur[rr + x] = u1;
+      ur[g.nr + rr + x] = u2;
+      ur[2 * g.nr + rr + x] = u3;
+    }
+    if (want_cfl) {
+      const T s = fabs(u1) + fabs(u2) + fabs(u3);
+      lmax = s > lmax ? s : lmax;
+    }
+    O[x] = mk<T>(n1, n2);
+    O[ld + x] = mk<T>(n3, T(0));
+  }
+  __syncthreads();
+  cx<T>* Q = fft_lines<T, -1>(O, R, px, 2, ld, tw);
+  cx<T>* d0 = w2 + row;
+  cx<T>* d1 = w2 + g.m2 + row;
+  cx<T>* d2 = w2 + 2 * g.m2 + row;
+  for (int m = threadIdx.x; m < nki; m += blockDim.x) {
+    const int mr = (m == 0) ? 0 : nx - m;
+    const cx<T> Z = Q[m], Zc = conj_c(Q[mr]);
+    const cx<T> s = add(Z, Zc), df = sub(Z, Zc);
+    d0[m] = mk<T>(T(0.5) * s.x, T(0.5) * s.y);
+    d1[m] = mk<T>(T(0.5) * df.y, T(-0.5) * df.x);  // (Z - conj Z') / (2i)
+    const cx<T> W = Q[ld + m], Wc = conj_c(Q[ld + mr]);
+    const cx<T> s3 = add(W, Wc);
+    d2[m] = mk<T>(T(0.5) * s3.x, T(0.5) * s3.y);
+  }
+  if (want_cfl) {
+    red[threadIdx.x] = lmax;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + s] ? red[threadIdx.x] : red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double v = double(red[0]);
+      atomicMax(umax, (unsigned long long)__double_as_longlong(v));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 4: forward y FFT + truncation, x-pencils -> 1r2w (NL fields 0..2)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, FftPlan py,
+           const cx<T>* __restrict__ tw, int ti) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ny = g.ny, nz = g.nz, nkj = g.nkj, nki = g.nki;
+  const int ld = ny + 1;
+  cx<T>* A = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* B = A + ti * ld;
+  const int z = blockIdx.y;
+  const int i0 = blockIdx.x * ti;
+  for (int f = 0; f < 3; ++f) {
+    const cx<T>* src = w2 + (long long)f * g.m2 + (long long)z * ny * nki;
+    for (int idx = threadIdx.x; idx < ti * ny; idx += blockDim.x) {
+      const int y = idx / ti;
+      const int line = idx - y * ti;
+      const int i = i0 + line;
+      A[line * ld + y] = (i < nki) ? src[(long long)y * nki + i] : zero_c<T>();
+    }
+    __syncthreads();
+    cx<T>* R = fft_lines<T, -1>(A, B, py, ti, ld, tw);
+    cx<T>* dst = w1 + (long long)f * g.m1;
+    for (int idx = threadIdx.x; idx < ti * nkj; idx += blockDim.x) {
+      const int line = idx / nkj;
+      const int j = idx - line * nkj;
+      const int i = i0 + line;
+      if (i < nki) {
+        const int yy = (j <= g.kj) ? j : ny - nkj + j;
+        dst[((long long)i * nz + z) * nkj + j] = R[line * ld + yy];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 5: forward z FFT, normalisation, truncation -> NL [c][i][j][k]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_zfwd(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ nl, FftPlan pz,
+           const cx<T>* __restrict__ tw, int tj, T norm) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int nz = g.nz, nkj = g.nkj, nkk = g.nkk;
+  const int ld = nz + 1;
+  cx<T>* A = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* B = A + tj * ld;
+  const int i = blockIdx.y;
+  const int j0 = blockIdx.x * tj;
+  for (int f = 0; f < 3; ++f) {
+    const cx<T>* src = w1 + (long long)f * g.m1 + (long long)i * nz * nkj;
+    for (int idx = threadIdx.x; idx < tj * nz; idx += blockDim.x) {
+      const int z = idx / tj;
+      const int line = idx - z * tj;
+      const int j = j0 + line;
+      A[line * ld + z] = (j < nkj) ? src[(long long)z * nkj + j] : zero_c<T>();
+    }
+    __syncthreads();
+    cx<T>* R = fft_lines<T, -1>(A, B, pz, tj, ld, tw);
+    cx<T>* dst = nl + (long long)f * g.mw;
+    for (int idx = threadIdx.x; idx < tj * nkk; idx += blockDim.x) {
+      const int line = idx / nkk;
+      const int k = idx - line * nkk;
+      const int j = j0 + line;
+      if (j < nkj) {
+        const int zz = (k <= g.kk) ? k : nz - nkk + k;
+        dst[((long long)i * nkj + j) * nkk + k] = scl(R[line * ld + zz], norm);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FormF + RK3 + continuity, one thread per mode
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_formf(Geo g, Coef cf, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1, cx<T>* __restrict__ u2,
+            cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
+            const cx<T>* __restrict__ nl, double beta, double gdt) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= g.mw) return;
+  const int k = int(e % g.nkk);
+  const long long ij = e / g.nkk;
+  const int j = int(ij % g.nkj);
+  const int i = int(ij / g.nkj);
+  const double kv[3] = {wvn(i, g.ki), wvn(j, g.kj), wvn(k, g.kk)};
+  const double kmag2 = (kv[0] * kv[0]) + (kv[1] * kv[1]) + (kv[2] * kv[2]);
+  if (!(kmag2 > 0.0)) return;  // the k = 0 mode is never updated (src/pf_rk3.f90:2385)
+  cx<T> u[3] = {u0[e], u1[e], u2[e]};
+  cx<T> r[3] = {r0[e], r1[e], r2[e]};
+  cx<T> n[3];
+  if (cf.use_nl) {
+    const cx<T> a = nl[e], b = nl[g.mw + e], c = nl[2 * g.mw + e];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double* Bm = &cf.Bm[3 * q];
+      n[q] = add(add(scl(a, T(Bm[0])), scl(b, T(Bm[1]))), scl(c, T(Bm[2])));
+    }
+  } else {
+    n[0] = n[1] = n[2] = zero_c<T>();
+  }
+  // k . M k, p1 = M k
+  double p1[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) p1[l] = cf.M[3 * l] * kv[0] + cf.M[3 * l + 1] * kv[1] + cf.M[3 * l + 2] * kv[2];
+  const double p4 = kv[0] * p1[0] + kv[1] * p1[1] + kv[2] * p1[2];
+  // p2 = k . nlf ; p3 = 2 sum_c (sum_m k_m P[m][c]) u_c
+  cx<T> p23 = add(add(scl(n[0], T(kv[0])), scl(n[1], T(kv[1]))), scl(n[2], T(kv[2])));
+  if (cf.has_hom) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double q = 2.0 * (kv[0] * cf.P[c] + kv[1] * cf.P[3 + c] + kv[2] * cf.P[6 + c]);
+      p23 = add(p23, scl(u[c], T(q)));
+    }
+  }
+  const T visc = cf.use_visc ? T(-cf.nu * p4) : T(0);
+  cx<T> tot[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T pc = (p4 > 0.0) ? T(p1[c] / p4) : T(0);
+    cx<T> t = sub(scl(u[c], visc), n[c]);
+    t = add(t, scl(p23, pc));
+    if (cf.has_hom) {
+      const double* H = &cf.H[3 * c];
+      const cx<T> h = add(add(scl(u[0], T(2.0 * H[0])), scl(u[1], T(2.0 * H[1]))), scl(u[2], T(2.0 * H[2])));
+      t = sub(t, h);
+    }
+    tot[c] = t;
+  }
+  const T tb = T(beta), tg = T(gdt);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    r[c] = add(scl(r[c], tb), tot[c]);
+    u[c] = add(u[c], scl(r[c], tg));
+  }
+  // continuity fix u -= k (k.u) / |k|^2 (src/pf_rk3.f90:3504-3531)
+  const cx<T> kd = add(add(scl(u[0], T(kv[0])), scl(u[1], T(kv[1]))), scl(u[2], T(kv[2])));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[c] = sub(u[c], scl(kd, T(kv[c] / kmag2)));
+  u0[e] = u[0];
+  u1[e] = u[1];
+  u2[e] = u[2];
+  r0[e] = r[0];
+  r1[e] = r[1];
+  r2[e] = r[2];
+}
+
+// ---------------------------------------------------------------------------
+// Tracers: molecule search + Neville (y, z, x) + RK3 + wrap, one thread each.
+// Field ur is [c][z][y][x]; positions are FP64 (3, np).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long pmod(long long a, long long n) {
+  long long r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+template <typename T, int P>
+__device__ void interp_p(const Geo& g, const TracerFrame& tf, const T* __restrict__ ur,
+                         const double* xp, double* uf) {
+  const int nhalo = P / 2;
+  // g_DomainInitHalo (src/pg_domain.f90:360-384) and p_InterpFindMolecule (src/pp_interp.f90:406-467)
+  double lxh = double(nhalo) * tf.dx, lyh = double(nhalo) * tf.dy, lzh = double(nhalo) * tf.dz;
+  lxh = lxh - (0.5 * tf.dx);
+  lyh = lyh - (0.5 * tf.dy);
+  lzh = lzh - (0.5 * tf.dz);
+  const double xd1 = xp[0] + lxh + tf.dx + (0.5 * tf.dx);
+  const double xd2 = xp[1] + lyh + tf.dy + (0.5 * tf.dy);
+  const double xd3 = xp[2] + lzh + tf.dz + (0.5 * tf.dz);
+  const int i0 = int(xd1 / tf.dx) - nhalo - nhalo;
+  const int j0 = int(xd2 / tf.dy) - nhalo - nhalo;
+  const int k0 = int(xd3 / tf.dz) - nhalo - nhalo;
+  double ix[P], iy[P], iz[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    ix[m] = double(i0 + m) * tf.dx;
+    iy[m] = double(j0 + m) * tf.dy;
+    iz[m] = double(k0 + m) * tf.dz;
+  }
+  int gx[P], gy[P], gz[P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) {
+    gx[m] = int(pmod(i0 + m, g.nx));
+    gy[m] = int(pmod(j0 + m, g.ny));
+    gz[m] = int(pmod(k0 + m, g.nz));
+  }
+  for (int c = 0; c < 3; ++c) {
+    const T* f = ur + (long long)c * g.nr;
+    double v[P][P][P];  // [x][z][y]
+#pragma unroll
+    for (int a = 0; a < P; ++a)
+#pragma unroll
+      for (int b = 0; b < P; ++b)
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+          v[a][q][b] = double(f[((long long)gz[q] * g.ny + gy[b]) * g.nx + gx[a]]);
+    // Neville along y (last index), then z, then x (src/pp_interp.f90:474-560)
+#pragma unroll
+    for (int col = 1; col <= P - 1; ++col)
+#pragma unroll
+      for (int nv = 1; nv <= P - col; ++nv) {
+        const double d1 = xp[1] - iy[nv + col - 1];
+        const double d2 = iy[nv - 1] - xp[1];
+        const double d3 = iy[nv - 1] - iy[nv + col - 1];
+#pragma unroll
+        for (int q = 0; q < P; ++q)
+#pragma unroll
+          for (int a = 0; a < P; ++a) v[a][q][nv - 1] = ((d1 * v[a][q][nv - 1]) + (d2 * v[a][q][nv])) / d3;
+      }
+#pragma unroll
+    for (int col = 1; col <= P - 1; ++col)
+#pragma unroll
+      for (int nv = 1; nv <= P - col; ++nv) {
+        const double d1 = xp[2] - iz[nv + col - 1];
+        const double d2 = iz[nv - 1] - xp[2];
+        const double d3 = iz[nv - 1] - iz[nv + col - 1];
+#pragma unroll
+        for (int a = 0; a < P; ++a) v[a][nv - 1][0] = ((d1 * v[a][nv - 1][0]) + (d2 * v[a][nv][0])) / d3;
+      }
+#pragma unroll
+    for (int col = 1; col <= P - 1; ++col)
+#pragma unroll
+      for (int nv = 1; nv <= P - col; ++nv) {
+        const double d1 = xp[0] - ix[nv + col - 1];
+        const double d2 = ix[nv - 1] - xp[0];
+        const double d3 = ix[nv - 1] - ix[nv + col - 1];
+        v[nv - 1][0][0] = ((d1 * v[nv - 1][0][0]) + (d2 * v[nv][0][0])) / d3;
+      }
+    uf[c] = v[0][0][0];
+  }
+}
+
+template <typename T, int P>
+__global__ void __launch_bounds__(kThreads)
+    k_tracers(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
+              double* __restrict__ uprk, long long np, double beta, double gdt) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  double x[3] = {xpa[3 * q], xpa[3 * q + 1], xpa[3 * q + 2]};
+  double r[3] = {uprk[3 * q], uprk[3 * q + 1], uprk[3 * q + 2]};
+  double uf[3];
+  interp_p<T, P>(g, tf, ur, x, uf);
+  if (!tf.bf) {
+    // p_RK3SolveParticle tracer branch (src/pp_rk3.f90:103-134)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      r[c] = (beta * r[c]) + uf[c];
+      x[c] = x[c] + (gdt * r[c]);
+    }
+  } else {
+    // p_RK3SolveParticleBF tracer branch (src/pp_rk3.f90:223-288)
+    double xl[3] = {0.0, 0.0, 0.0}, ul[3] = {0.0, 0.0, 0.0}, xm[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        xl[m] = xl[m] + tf.binv[3 * m + n] * x[n];
+        ul[m] = ul[m] + tf.binv[3 * m + n] * uf[n];
+      }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      r[c] = (beta * r[c]) + ul[c];
+      xl[c] = xl[c] + (gdt * r[c]);
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 3; ++n) xm[m] = xm[m] + tf.bmat[3 * m + n] * xl[n];
+    x[0] = xm[0];
+    x[1] = xm[1];
+    x[2] = xm[2];
+  }
+  // p_MPICorrectPosition (src/pp_mpi.f90:1011-1057): one shift, x3 upper test is '>'
+  const double x1min = -0.5 * tf.dx, x1max = x1min + tf.lx;
+  const double x2min = -0.5 * tf.dy, x2max = x2min + tf.ly;
+  const double x3min = -0.5 * tf.dz, x3max = x3min + tf.lz;
+  if (x[0] < x1min) x[0] = x[0] + tf.lx;
+  if (x[0] >= x1max) x[0] = x[0] - tf.lx;
+  if (x[1] < x2min) x[1] = x[1] + tf.ly;
+  if (x[1] >= x2max) x[1] = x[1] - tf.ly;
+  if (x[2] < x3min) x[2] = x[2] + tf.lz;
+  if (x[2] > x3max) x[2] = x[2] - tf.lz;
+  xpa[3 * q] = x[0];
+  xpa[3 * q + 1] = x[1];
+  xpa[3 * q + 2] = x[2];
+  uprk[3 * q] = r[0];
+  uprk[3 * q + 1] = r[1];
+  uprk[3 * q + 2] = r[2];
+}
+
+// ---------------------------------------------------------------------------
+// Remesh: per (i, j) line new[s] = old[s + i] for s + i <= K (signed k indices),
+// BF_ROT: u1 <- u1' + u3'.  In place; one workgroup per line.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_remesh(Geo g, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1, cx<T>* __restrict__ u2, int bfrot) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* buf = reinterpret_cast<cx<T>*>(smem);
+  const int nkk = g.nkk, K = g.kk;
+  const int j = blockIdx.x, i = blockIdx.y;
+  const long long base = ((long long)i * g.nkj + j) * nkk;
+  for (int k = threadIdx.x; k < nkk; k += blockDim.x) {
+    buf[k] = u0[base + k];
+    buf[nkk + k] = u1[base + k];
+    buf[2 * nkk + k] = u2[base + k];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nkk; k += blockDim.x) {
+    const int s = (k <= K) ? k : k - nkk;  // new signed index
+    const int a = s + i;                   // old signed index
+    cx<T> b1 = zero_c<T>(), b2 = zero_c<T>(), b3 = zero_c<T>();
+    if (a <= K) {
+      const int ko = (a >= 0) ? a : a + nkk;
+      b1 = buf[ko];
+      b2 = buf[nkk + ko];
+      b3 = buf[2 * nkk + ko];
+    }
+    u0[base + k] = bfrot ? add(b1, b3) : b1;
+    u1[base + k] = b2;
+    u2[base + k] = b3;
+  }
+}
+
+// p_MPIRemeshPosition (src/pp_mpi.f90:1062-1092)
+__global__ void k_remesh_particles(double* __restrict__ xp, long long np, double lx, double dx) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  double x1 = xp[3 * q] + xp[3 * q + 2];
+  const double x1min = -0.5 * dx, x1max = x1min + lx;
+  if (x1 < x1min) x1 = x1 + lx;
+  if (x1 >= x1max) x1 = x1 - lx;
+  xp[3 * q] = x1;
+}
+
+template <typename T>
+__global__ void k_finite(const T* __restrict__ a, long long n, int* flag) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+       q += (long long)gridDim.x * blockDim.x) {
+    if (!isfinite(a[q])) {
+      atomicOr(flag, 1);
+      return;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+static hipError_t set_lds(const void* fn, size_t bytes) {
+  if (bytes > 65536) return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+  return hipSuccess;
+}
+
+template <typename T>
+static hipError_t zinv_t(pdr_ctx* c, const Coef& cf, int fsel) {
+  const Geo& g = c->g;
+  const int tj = c->tj_z;
+  const size_t lds = 2 * size_t(tj) * (g.nz + 1) * sizeof(cx<T>);
+  hipError_t e = set_lds<T>((const void*)k_zinv<T>, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.nkj + tj - 1) / tj, g.nki);
+  k_zinv<T><<<grid, kThreads, lds, c->stream>>>(g, cf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1],
+                                                (const cx<T>*)c->u[2], (cx<T>*)c->w1, c->pz,
+                                                (const cx<T>*)c->tab.twz, tj, fsel);
+  return hipGetLastError();
+}
+
+hipError_t launch_zinv(pdr_ctx* c, const Coef& cf) {
+  prof_begin(c, K_ZINV);
+  hipError_t e = c->prec ? zinv_t<float>(c, cf, -1) : zinv_t<double>(c, cf, -1);
+  prof_end(c, K_ZINV);
+  return e;
+}
+
+template <typename T>
+static hipError_t yinv_t(pdr_ctx* c) {
+  const Geo& g = c->g;
+  const int ti = c->ti_y;
+  const size_t lds = 2 * size_t(ti) * (g.ny + 1) * sizeof(cx<T>);
+  hipError_t e = set_lds<T>((const void*)k_yinv<T>, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.nki + ti - 1) / ti, g.nz);
+  k_yinv<T><<<grid, kThreads, lds, c->stream>>>(g, (const cx<T>*)c->w1, (cx<T>*)c->w2, c->py,
+                                                (const cx<T>*)c->tab.twy, ti);
+  return hipGetLastError();
+}
+
+hipError_t launch_yinv(pdr_ctx* c) {
+  prof_begin(c, K_YINV);
+  hipError_t e = c->prec ? yinv_t<float>(c) : yinv_t<double>(c);
+  prof_end(c, K_YINV);
+  return e;
+}
+
+template <typename T>
+static hipError_t xpass_t(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+  const Geo& g = c->g;
+  const size_t lds = 6 * size_t(g.nx + 1) * sizeof(cx<T>);
+  hipError_t e = set_lds<T>((const void*)k_xpass<T>, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid(g.ny, g.nz);
+  k_xpass<T><<<grid, kThreads, lds, c->stream>>>(g, cf, (cx<T>*)c->w2, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0,
+                                                 c->umax, c->px, (const cx<T>*)c->tab.twx);
+  return hipGetLastError();
+}
+
+hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+  prof_begin(c, K_XPASS);
+  hipError_t e = c->prec ? xpass_t<float>(c, cf, tr, cfl) : xpass_t<double>(c, cf, tr, cfl);
+  prof_end(c, K_XPASS);
+  return e;
+}
+
+template <typename T>
+static hipError_t yfwd_t(pdr_ctx* c) {
+  const Geo& g = c->g;
+  const int ti = c->ti_y;
+  const size_t lds = 2 * size_t(ti) * (g.ny + 1) * sizeof(cx<T>);
+  hipError_t e = set_lds<T>((const void*)k_yfwd<T>, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.nki + ti - 1) / ti, g.nz);
+  k_yfwd<T><<<grid, kThreads, lds, c->stream>>>(g, (const cx<T>*)c->w2, (cx<T>*)c->w1, c->py,
+                                                (const cx<T>*)c->tab.twy, ti);
+  return hipGetLastError();
+}
+
+hipError_t launch_yfwd(pdr_ctx* c) {
+  prof_begin(c, K_YFWD);
+  hipError_t e = c->prec ? yfwd_t<float>(c) : yfwd_t<double>(c);
+  prof_end(c, K_YFWD);
+  return e;
+}
+
+template <typename T>
+static hipError_t zfwd_t(pdr_ctx* c) {
+  const Geo& g = c->g;
+  const int tj = c->tj_z;
+  const size_t lds = 2 * size_t(tj) * (g.nz + 1) * sizeof(cx<T>);
+  hipError_t e = set_lds<T>((const void*)k_zfwd<T>, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid((g.nkj + tj - 1) / tj, g.nki);
+  const double norm = 1.0 / (double(g.nx) * double(g.ny) * double(g.nz));
+  k_zfwd<T><<<grid, kThreads, lds, c->stream>>>(g, (const cx<T>*)c->w1, (cx<T>*)c->w2, c->pz,
+                                                (const cx<T>*)c->tab.twz, tj, T(norm));
+  return hipGetLastError();
+}
+
+hipError_t launch_zfwd(pdr_ctx* c) {
+  prof_begin(c, K_ZFWD);
+  hipError_t e = c->prec ? zfwd_t<float>(c) : zfwd_t<double>(c);
+  prof_end(c, K_ZFWD);
+  return e;
+}
+
+template <typename T>
+static hipError_t formf_t(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
+  const Geo& g = c->g;
+  const long long nb = (g.mw + kThreads - 1) / kThreads;
+  k_formf<T><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(
+      g, cf, (cx<T>*)c->u[0], (cx<T>*)c->u[1], (cx<T>*)c->u[2], (cx<T>*)c->rk[0], (cx<T>*)c->rk[1],
+      (cx<T>*)c->rk[2], (const cx<T>*)c->w2, beta, gdt);
+  return hipGetLastError();
+}
+
+hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
+  prof_begin(c, K_FORMF);
+  hipError_t e = c->prec ? formf_t<float>(c, cf, beta, gdt) : formf_t<double>(c, cf, beta, gdt);
+  prof_end(c, K_FORMF);
+  return e;
+}
+
+template <typename T>
+static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt) {
+  const long long nb = (c->np + kThreads - 1) / kThreads;
+  const T* ur = (const T*)c->ur;
+  switch (tf.order) {
+    case 2:
+      k_tracers<T, 2><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
+      break;
+    case 6:
+      k_tracers<T, 6><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
+      break;
+    case 8:
+      k_tracers<T, 8><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
+      break;
+    default:
+      k_tracers<T, 4><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt) {
+  if (c->np <= 0) return hipSuccess;
+  prof_begin(c, K_TRACER);
+  hipError_t e = c->prec ? tracers_t<float>(c, tf, beta, gdt) : tracers_t<double>(c, tf, beta, gdt);
+  prof_end(c, K_TRACER);
+  return e;
+}
+
+template <typename T>
+static hipError_t remesh_t(pdr_ctx* c, int bfrot) {
+  const Geo& g = c->g;
+  const size_t lds = 3 * size_t(g.nkk) * sizeof(cx<T>);
+  hipError_t e = set_lds<T>((const void*)k_remesh<T>, lds);
+  if (e != hipSuccess) return e;
+  k_remesh<T><<<dim3(g.nkj, g.nki), kThreads, lds, c->stream>>>(g, (cx<T>*)c->u[0], (cx<T>*)c->u[1],
+                                                              (cx<T>*)c->u[2], bfrot);
+  return hipGetLastError();
+}
+
+hipError_t launch_remesh(pdr_ctx* c, int bfrot) {
+  prof_begin(c, K_REMESH);
+  hipError_t e = c->prec ? remesh_t<float>(c, bfrot) : remesh_t<double>(c, bfrot);
+  prof_end(c, K_REMESH);
+  return e;
+}
+
+hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx) {
+  if (c->np <= 0) return hipSuccess;
+  const long long nb = (c->np + kThreads - 1) / kThreads;
+  k_remesh_particles<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->xp, c->np, lx, dx);
+  return hipGetLastError();
+}
+
+hipError_t launch_finite(pdr_ctx* c, int* flag) {
+  const long long n = 2 * c->g.mw;
+  for (int q = 0; q < 3; ++q) {
+    if (c->prec)
+      k_finite<float><<<dim3(1024), kThreads, 0, c->stream>>>((const float*)c->u[q], n, flag);
+    else
+      k_finite<double><<<dim3(1024), kThreads, 0, c->stream>>>((const double*)c->u[q], n, flag);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pdr

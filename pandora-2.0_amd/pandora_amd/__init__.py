@@ -1,0 +1,338 @@
+"""Python binding of the MI355X-native PANDORA 2.0 RK3 step (C-ABI in include/pandora_amd.h).
+
+The Fortran drop-in is ``pandora-2.0_amd/fortran/pdr_rk3_amd.f90``; this module binds
+the same C entry points with ctypes for tests, benchmarks and scripting.  Method
+names follow the reference interfaces they replace:
+
+    ctx.rk3_calc(tstep, simtime)      g_RK3Calc        (src/pg_rk3.f90:82-129)
+    ctx.stage(stage, tstep, t, dt)    f_RK3Stage1/2/3  (src/pf_rk3.f90:40,121,161)
+    ctx.update_b(tstep, simtime)      g_RK3UpdateB     (src/pg_rk3.f90:280-385)
+
+There is no CPU fallback: if the HIP library is missing or no GPU is visible the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "lib", "libpandora_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "pandora_amd.h")
+
+PDR_FP64 = 0
+PDR_FP32_MIXED = 1
+F_ISOTROPIC, F_SHEAR_DNS, F_SHEAR_RDT_VISCOUS, F_SHEAR_RDT_INVISCID = 1, 2, 3, 4
+RK3_TIMESTEP_DECIDE, RK3_TIMESTEP_FIXED, RK3_TIMESTEP_CFL = 1, 2, 3
+ERRORS = {0: "PDR_OK", -1: "PDR_ERR_ARG", -2: "PDR_ERR_HIP", -3: "PDR_ERR_NOMEM",
+          -4: "PDR_ERR_UNSUPPORTED", -5: "PDR_ERR_IO", -6: "PDR_ERR_STATE", -7: "PDR_ERR_NONFINITE"}
+
+# float32 pi widened to double (src/pg_constants.f90:126)
+ONEPI_REF = float(np.float32(np.pi))
+TWOPI_REF = 2.0 * ONEPI_REF
+
+
+class PdrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("nodes_x", C.c_int32), ("nodes_y", C.c_int32), ("nodes_z", C.c_int32),
+        ("f_type", C.c_int32), ("ns_deformed", C.c_int32), ("dealiasing", C.c_int32),
+        ("rk3_timestep", C.c_int32), ("f_remesh_t", C.c_int32), ("b110", C.c_int32),
+        ("p_interp_order", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("rk3_dt", C.c_double), ("rk3_cfl", C.c_double), ("f_nu", C.c_double),
+        ("f_deform_s", C.c_double), ("force_tf", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run pandora-2.0_amd/build_lib.py)")
+    L = C.CDLL(path)
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+    vp = C.c_void_p
+    i32, i64, d = C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "pdr_version": (C.c_char_p, []),
+        "pdr_last_error": (C.c_char_p, []),
+        "pdr_create": (C.c_int, [C.POINTER(vp), C.POINTER(Params)]),
+        "pdr_destroy": (C.c_int, [vp]),
+        "pdr_sizes": (C.c_int, [vp, np.ctypeslib.ndpointer(dtype=np.int64)]),
+        "pdr_upload_state": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_download_state": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_upload_rk3": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_download_rk3": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_set_frame": (C.c_int, [vp, dp, dp]),
+        "pdr_get_frame": (C.c_int, [vp, dp, dp]),
+        "pdr_set_dt": (C.c_int, [vp, d]),
+        "pdr_get_dt": (C.c_int, [vp, C.POINTER(d)]),
+        "pdr_stage": (C.c_int, [vp, i32, i32, d, C.POINTER(d)]),
+        "pdr_update_b": (C.c_int, [vp, i32, d]),
+        "pdr_remesh": (C.c_int, [vp, i32, C.POINTER(i32)]),
+        "pdr_step": (C.c_int, [vp, i32, C.POINTER(d)]),
+        "pdr_run": (C.c_int, [vp, i32, i32, C.POINTER(d)]),
+        "pdr_synchronize": (C.c_int, [vp]),
+        "pdr_check_finite": (C.c_int, [vp]),
+        "pdr_particles_upload": (C.c_int, [vp, dp, i64]),
+        "pdr_particles_download": (C.c_int, [vp, vp, vp]),
+        "pdr_init_field": (C.c_int, [vp, C.c_uint64, d, d]),
+        "pdr_init_particles": (C.c_int, [vp, C.c_uint64, i64]),
+        "pdr_init_field_host": (C.c_int, [i32, i32, i32, C.c_uint64, d, d, dp, dp, dp]),
+        "pdr_restart_write_host": (C.c_int, [C.c_char_p, i64, dp, dp, dp, i32]),
+        "pdr_restart_read_host": (C.c_int, [C.c_char_p, i64, dp, dp, dp]),
+        "pdr_restart_write": (C.c_int, [vp, C.c_char_p, i32]),
+        "pdr_restart_read": (C.c_int, [vp, C.c_char_p]),
+        "pdr_spectrum": (C.c_int, [vp, dp, dp, dp, dp]),
+        "pdr_debug_prepare": (C.c_int, [vp, i32, dp]),
+        "pdr_profile_enable": (C.c_int, [vp, i32]),
+        "pdr_profile_get": (C.c_int, [vp, i32, C.c_char_p, i32,
+                                      np.ctypeslib.ndpointer(dtype=np.int64), dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols_from_header(path: str = HEADER_PATH):
+    """Names of every function declared in include/pandora_amd.h."""
+    import re
+    txt = open(path).read()
+    return sorted(set(re.findall(r"\b(pdr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _check(rc):
+    if rc != 0:
+        raise PdrError(rc, load_library().pdr_last_error().decode())
+
+
+def wave_shape(nx, ny=None, nz=None):
+    """(NKI, NKJ, NKK) of the stored 2/3-dealiased modes (src/pg_domain.f90:220-228)."""
+    ny = ny or nx
+    nz = nz or nx
+    return (nx // 3 + 1, 2 * (ny // 3) + 1, 2 * (nz // 3) + 1)
+
+
+def init_field_host(n, seed, u0, kp, ny=None, nz=None):
+    """Seeded Rogallo initial field computed on the host (no GPU): three complex128
+    arrays shaped (NKI, NKJ, NKK)."""
+    L = load_library()
+    ny = ny or n
+    nz = nz or n
+    shape = wave_shape(n, ny, nz)
+    out = [np.zeros(shape, np.complex128) for _ in range(3)]
+    _check(L.pdr_init_field_host(n, ny, nz, seed, u0, kp, *[o.view(np.float64).reshape(-1) for o in out]))
+    return out
+
+
+def restart_write_host(path, u1, u2, u3, int64_header=False):
+    L = load_library()
+    arrs = [np.ascontiguousarray(a, np.complex128).view(np.float64).reshape(-1) for a in (u1, u2, u3)]
+    _check(L.pdr_restart_write_host(path.encode(), arrs[0].size, *arrs, 1 if int64_header else 0))
+
+
+def restart_read_host(path, shape):
+    L = load_library()
+    out = [np.zeros(shape, np.complex128) for _ in range(3)]
+    flat = [o.view(np.float64).reshape(-1) for o in out]
+    _check(L.pdr_restart_read_host(path.encode(), flat[0].size, *flat))
+    return out
+
+
+class Pandora:
+    """One resident box on one GPU.
+
+    Wave-space arrays are complex128 numpy arrays shaped (NKI, NKJ, NKK), the C-order
+    view of the reference's Fortran arr(0:1, k, j, i)."""
+
+    def __init__(self, n=None, *, nx=None, ny=None, nz=None, f_type=F_ISOTROPIC, ns_deformed=1,
+                 rk3_timestep=RK3_TIMESTEP_FIXED, rk3_dt=0.005, rk3_cfl=0.75, nu=0.01, deform_s=0.0,
+                 remesh_t=0, b110=1, interp_order=4, precision=PDR_FP64, force_tf=0.0):
+        L = load_library()
+        nx = nx or n
+        ny = ny or n
+        nz = nz or n
+        self.params = Params(nx, ny, nz, f_type, ns_deformed, 1, rk3_timestep, remesh_t, b110,
+                             interp_order, precision, 0, rk3_dt, rk3_cfl, nu, deform_s, force_tf)
+        h = C.c_void_p()
+        _check(L.pdr_create(C.byref(h), C.byref(self.params)))
+        self.h = h
+        s = np.zeros(8, dtype=np.int64)
+        _check(L.pdr_sizes(self.h, s))
+        self.nki, self.nkj, self.nkk, self.mw, self.m1, self.kkmax = (int(v) for v in s[:6])
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.shape = (self.nki, self.nkj, self.nkk)
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "h", None):
+            load_library().pdr_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def L(self):
+        return load_library()
+
+    def sizes(self):
+        s = np.zeros(8, dtype=np.int64)
+        _check(self.L.pdr_sizes(self.h, s))
+        return s
+
+    # ---- state -------------------------------------------------------------
+    def _wave(self, a):
+        a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.shape)
+        return a.view(np.float64).reshape(-1)
+
+    def set_state(self, u1, u2, u3):
+        _check(self.L.pdr_upload_state(self.h, self._wave(u1), self._wave(u2), self._wave(u3)))
+
+    def get_state(self):
+        out = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        _check(self.L.pdr_download_state(self.h, *[o.view(np.float64).reshape(-1) for o in out]))
+        return out
+
+    def set_rk(self, r1, r2, r3):
+        _check(self.L.pdr_upload_rk3(self.h, self._wave(r1), self._wave(r2), self._wave(r3)))
+
+    def get_rk(self):
+        out = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        _check(self.L.pdr_download_rk3(self.h, *[o.view(np.float64).reshape(-1) for o in out]))
+        return out
+
+    def set_frame(self, bmat, binv):
+        _check(self.L.pdr_set_frame(self.h, np.ascontiguousarray(bmat, np.float64).reshape(9),
+                                    np.ascontiguousarray(binv, np.float64).reshape(9)))
+
+    def get_frame(self):
+        b = np.zeros(9)
+        bi = np.zeros(9)
+        _check(self.L.pdr_get_frame(self.h, b, bi))
+        return b, bi
+
+    @property
+    def dt(self):
+        d = C.c_double()
+        _check(self.L.pdr_get_dt(self.h, C.byref(d)))
+        return d.value
+
+    @dt.setter
+    def dt(self, v):
+        _check(self.L.pdr_set_dt(self.h, float(v)))
+
+    # ---- stepping ----------------------------------------------------------
+    def rk3_calc(self, tstep: int, simtime: float) -> float:
+        """g_RK3Calc: one time step; returns the new simtime."""
+        t = C.c_double(simtime)
+        _check(self.L.pdr_step(self.h, tstep, C.byref(t)))
+        return t.value
+
+    step = rk3_calc
+
+    def run(self, tstep0: int, nsteps: int, simtime: float) -> float:
+        t = C.c_double(simtime)
+        _check(self.L.pdr_run(self.h, tstep0, nsteps, C.byref(t)))
+        return t.value
+
+    def stage(self, stage: int, tstep: int, time: float, dt: float) -> float:
+        d = C.c_double(dt)
+        _check(self.L.pdr_stage(self.h, stage, tstep, time, C.byref(d)))
+        return d.value
+
+    def remesh(self, tstep: int) -> bool:
+        """f_RK3Remesh: remesh if this step needs it; returns whether it did."""
+        done = C.c_int32(0)
+        _check(self.L.pdr_remesh(self.h, tstep, C.byref(done)))
+        return bool(done.value)
+
+    def update_b(self, tstep: int, simtime: float):
+        _check(self.L.pdr_update_b(self.h, tstep, simtime))
+
+    def synchronize(self):
+        _check(self.L.pdr_synchronize(self.h))
+
+    def check_finite(self) -> bool:
+        rc = self.L.pdr_check_finite(self.h)
+        if rc == -7:
+            return False
+        _check(rc)
+        return True
+
+    # ---- tracers -----------------------------------------------------------
+    def set_particles(self, xp):
+        xp = np.ascontiguousarray(xp, np.float64).reshape(-1, 3)
+        _check(self.L.pdr_particles_upload(self.h, xp.reshape(-1), xp.shape[0]))
+        self.np = xp.shape[0]
+
+    def init_particles(self, seed: int, np_: int):
+        _check(self.L.pdr_init_particles(self.h, seed, np_))
+        self.np = np_
+
+    def get_particles(self):
+        n = int(self.sizes()[7])
+        xp = np.zeros((n, 3))
+        ur = np.zeros((n, 3))
+        _check(self.L.pdr_particles_download(self.h, xp.ctypes.data, ur.ctypes.data))
+        return xp, ur
+
+    # ---- initial field / restart / statistics -------------------------------
+    def init_field(self, seed: int, u0: float, kp: float):
+        _check(self.L.pdr_init_field(self.h, seed, u0, kp))
+
+    def restart_write(self, path: str, int64_header: bool = False):
+        _check(self.L.pdr_restart_write(self.h, path.encode(), 1 if int64_header else 0))
+
+    def restart_read(self, path: str):
+        _check(self.L.pdr_restart_read(self.h, path.encode()))
+
+    def spectrum(self):
+        ek = np.zeros(self.kkmax)
+        dk = np.zeros(self.kkmax)
+        ds = np.zeros(self.kkmax)
+        tot = np.zeros(6)
+        _check(self.L.pdr_spectrum(self.h, ek, dk, ds, tot))
+        return ek, dk, ds, tot
+
+    def debug_prepare(self, field: int):
+        out = np.zeros((self.nki, self.nz, self.nkj), np.complex128)
+        _check(self.L.pdr_debug_prepare(self.h, field, out.view(np.float64).reshape(-1)))
+        return out
+
+    # ---- profiling ---------------------------------------------------------
+    def profile(self, enable: bool = True):
+        _check(self.L.pdr_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_get(self):
+        n = 9
+        names = C.create_string_buffer(256)
+        cnt = np.zeros(n, dtype=np.int64)
+        ms = np.zeros(n)
+        _check(self.L.pdr_profile_get(self.h, n, names, 256, cnt, ms))
+        keys = names.value.decode().split(",")
+        return {k: (int(cnt[i]), float(ms[i])) for i, k in enumerate(keys)}

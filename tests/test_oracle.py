@@ -1,0 +1,423 @@
+"""Pins for the FP64 CPU oracle (oracle/pdr_oracle.c).
+
+Each test checks the oracle against something it does not share code with: the
+reference's own known answer (src/ptf_rk3.f90:1030-1198), brute-force DFTs and
+direct Fourier sums, closed-form solutions, conservation laws and brute-force
+enumeration.  A plausible slip (wrong sign, index, padding offset, RK
+coefficient, normalisation) fails at least one of them.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+TWOPI = 2.0 * float(np.float32(np.pi))  # float32 pi widened (src/pg_constants.f90:126)
+
+
+def wvn(idx, kmax):
+    idx = np.asarray(idx)
+    return np.where(idx <= kmax, idx, idx - (2 * kmax + 1)).astype(np.float64)
+
+
+def kgrid(o):
+    ki = np.arange(o.nki, dtype=np.float64)
+    kj = wvn(np.arange(o.nkj), o.nkj // 2)
+    kk = wvn(np.arange(o.nkk), o.nkk // 2)
+    return np.meshgrid(ki, kj, kk, indexing="ij")
+
+
+def random_state(o, seed, amp=1.0, hermitian=True, solenoidal=True):
+    rng = np.random.default_rng(seed)
+    u = [amp * (rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)) for _ in range(3)]
+    KI, KJ, KK = kgrid(o)
+    k2 = KI**2 + KJ**2 + KK**2
+    if solenoidal:
+        kd = (KI * u[0] + KJ * u[1] + KK * u[2]) / np.where(k2 > 0, k2, 1.0)
+        u = [u[0] - KI * kd, u[1] - KJ * kd, u[2] - KK * kd]
+    for c in range(3):
+        u[c][0, 0, 0] = 0.0
+    if hermitian:
+        # u(0,-j,-k) = conj u(0,j,k)
+        nkj, nkk = o.nkj, o.nkk
+        jn = (-np.arange(nkj)) % nkj
+        kn = (-np.arange(nkk)) % nkk
+        for c in range(3):
+            p = u[c][0]
+            u[c][0] = 0.5 * (p + np.conj(p[np.ix_(jn, kn)]))
+    return u
+
+
+def direct_realspace(o, u):
+    """u(x) = sum over stored modes with the c2r convention: i>0 modes count
+    2 Re(.), the i=0 plane contributes Re(.) (real part of the DC term)."""
+    n = o.nx
+    x = np.arange(n) * 2 * np.pi / n
+    KI, KJ, KK = kgrid(o)
+    w = np.where(KI > 0, 2.0, 1.0)
+    out = []
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    ph = np.exp(1j * (pts[:, 0:1] * KI.ravel()[None] + pts[:, 1:2] * KJ.ravel()[None]
+                      + pts[:, 2:3] * KK.ravel()[None]))
+    for c in range(3):
+        v = (ph * (w.ravel() * u[c].ravel())[None]).real.sum(1)
+        out.append(v.reshape(n, n, n).transpose(2, 1, 0))  # -> [z][y][x]
+    return out
+
+
+# ---------------------------------------------------------------------------
+def test_fft_against_brute_dft_and_pocketfft(oracle_mod):
+    rng = np.random.default_rng(1)
+    for n in [2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 25, 30, 32, 36, 48, 60, 64, 96]:
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        m = np.arange(n)
+        F = np.exp(-2j * np.pi * (np.outer(m, m) % n) / n)  # reduced phase: exact arguments
+        assert rel(oracle_mod.fft(x, -1), F @ x) < 1e-14, n
+        assert rel(oracle_mod.fft(x, +1), np.conj(F) @ x) < 1e-14, n
+    for n in [256, 512, 768, 1024, 1536]:
+        x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        assert rel(oracle_mod.fft(x, -1), np.fft.fft(x)) < 1e-14, n
+        assert rel(oracle_mod.fft(x, +1), np.fft.ifft(x) * n) < 1e-14, n
+
+
+@pytest.mark.parametrize("n", [12, 15])
+def test_inverse_transform_matches_direct_fourier_sum(oracle_mod, n):
+    """Zero padding, transform signs and the c2r DC convention against a direct sum."""
+    o = oracle_mod.Oracle(n)
+    u = random_state(o, 3, hermitian=False, solenoidal=False)
+    o.set_state(*u)
+    o.set_particles(np.zeros((1, 3)))  # makes the oracle keep the real-space velocity
+    o.prepare_realspace()
+    o.realspace(0)
+    ur = o.tracer_field()
+    ref = direct_realspace(o, u)
+    for c in range(3):
+        assert rel(ur[c], ref[c]) < 1e-13
+
+
+def test_nonlinear_term_matches_direct_galerkin_sum(oracle_mod):
+    """omega x u evaluated by direct sums on the grid, forward-transformed by a
+    direct DFT, x 1/N^3 and truncated, equals the oracle's NL (N = 12)."""
+    n = 12
+    o = oracle_mod.Oracle(n)
+    u = random_state(o, 5)
+    KI, KJ, KK = kgrid(o)
+    om = [1j * (KJ * u[2] - KK * u[1]), 1j * (KK * u[0] - KI * u[2]), 1j * (KI * u[1] - KJ * u[0])]
+    ur = direct_realspace(o, u)
+    wr = direct_realspace(o, om)
+    nl = [ur[2] * wr[1] - ur[1] * wr[2], ur[0] * wr[2] - ur[2] * wr[0], ur[1] * wr[0] - ur[0] * wr[1]]
+    x = np.arange(n) * 2 * np.pi / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    expect = []
+    for c in range(3):
+        e = np.zeros(o.shape, complex)
+        for a in range(o.nki):
+            for b in range(o.nkj):
+                for d in range(o.nkk):
+                    ph = np.exp(-1j * (KI[a, b, d] * X + KJ[a, b, d] * Y + KK[a, b, d] * Z))
+                    e[a, b, d] = (nl[c] * ph).sum() / n**3
+        expect.append(e)
+    o.set_state(*u)
+    o.prepare_realspace()
+    o.realspace(0)
+    got = o.nl_forward()
+    for c in range(3):
+        assert rel(got[c], expect[c]) < 1e-13
+
+
+def _taylor_green_reverse(o):
+    """tf_RK3InitTaylorGreenReverse (src/ptf_rk3.f90:1030-1098)."""
+    u = [np.zeros(o.shape, complex) for _ in range(3)]
+    K, J = o.nkk, o.nkj
+    vals = {(2, 3): (-1 / 8, 1 / 8, -1 / 24), (2, K - 3): (1 / 8, -1 / 8, -1 / 24),
+            (J - 2, 3): (1 / 8, 1 / 8, 1 / 24), (J - 2, K - 3): (-1 / 8, -1 / 8, 1 / 24)}
+    for (j, k), (a, b, c) in vals.items():
+        u[0][1, j, k], u[1][1, j, k], u[2][1, j, k] = a, b, c
+    return u
+
+
+@pytest.mark.parametrize("n", [32, 48, 64])
+def test_reference_known_answer_taylor_green_nonlinear(oracle_mod, n):
+    """The reference's own known answer: L1 norms of the transformed nonlinear
+    term are 3/4, 9/8, 11/8 (tf_RK3CheckNonlinearError, src/ptf_rk3.f90:1157-1181)."""
+    o = oracle_mod.Oracle(n)
+    o.set_state(*_taylor_green_reverse(o))
+    o.prepare_realspace()
+    o.realspace(0)
+    nl = o.nl_forward()
+    l1 = [np.abs(a.view(np.float64)).sum() for a in nl]
+    assert abs(l1[0] - 3 / 4) < 1e-13
+    assert abs(l1[1] - 9 / 8) < 1e-13
+    assert abs(l1[2] - 11 / 8) < 1e-13
+
+
+def test_known_answer_needs_resolution(oracle_mod):
+    """The derivation needs N/3 >= 6: at N = 16 the 2/3 truncation removes modes."""
+    o = oracle_mod.Oracle(16)
+    o.set_state(*_taylor_green_reverse(o))
+    o.prepare_realspace()
+    o.realspace(0)
+    l1 = np.abs(o.nl_forward()[0].view(np.float64)).sum()
+    assert abs(l1 - 3 / 4) > 1e-3
+
+
+def test_single_mode_viscous_decay_closed_form(oracle_mod):
+    """A single Fourier mode is an exact Navier-Stokes solution (its self-interaction
+    is a gradient).  Williamson's 3-stage RK3 advances it by R(z) = 1+z+z^2/2+z^3/6."""
+    n, nu, dt = 24, 0.05, 0.01
+    o = oracle_mod.Oracle(n, nu=nu, rk3_dt=dt)
+    u = [np.zeros(o.shape, complex) for _ in range(3)]
+    k = np.array([1.0, 2.0, 3.0])
+    a = np.array([3.0 + 1j, -1.0 + 0.5j, 0.0])
+    a[2] = -(k[0] * a[0] + k[1] * a[1]) / k[2]
+    for c in range(3):
+        u[c][1, 2, 3] = a[c]
+    o.set_state(*u)
+    t = 0.0
+    for s in range(1, 4):
+        t = o.step(s, t)
+    z = -nu * (k @ k) * dt
+    R = 1 + z + z**2 / 2 + z**3 / 6
+    got = o.get_state()
+    for c in range(3):
+        assert abs(got[c][1, 2, 3] - a[c] * R**3) < 1e-14 * abs(a).max()
+        rest = got[c].copy()
+        rest[1, 2, 3] = 0
+        assert np.abs(rest).max() < 1e-15 * abs(a).max()
+
+
+def _energy_weights(o):
+    KI, _, _ = kgrid(o)
+    return np.where(KI > 0, 2.0, 1.0)
+
+
+def test_truncated_nonlinear_term_conserves_energy(oracle_mod):
+    """sum_k w |u* . NL| = 0 for the alias-free (N not divisible by 3) Galerkin system."""
+    o = oracle_mod.Oracle(32)
+    u = random_state(o, 11)
+    o.set_state(*u)
+    o.prepare_realspace()
+    o.realspace(0)
+    nl = o.nl_forward()
+    w = _energy_weights(o)
+    p = sum((w * (np.conj(u[c]) * nl[c])).real.sum() for c in range(3))
+    scale = sum((w * np.abs(u[c]) * np.abs(nl[c])).sum() for c in range(3))
+    assert abs(p) < 1e-13 * scale
+
+
+def test_step_keeps_divergence_free_and_hermitian(oracle_mod):
+    import pandora_amd as pa
+    o = oracle_mod.Oracle(24, nu=0.01, rk3_dt=0.005)
+    u = pa.init_field_host(24, 7, 2.0 / 3.0, 4.0)
+    o.set_state(*u)
+    t = 0.0
+    for s in range(1, 4):
+        t = o.step(s, t)
+    v = o.get_state()
+    KI, KJ, KK = kgrid(o)
+    div = KI * v[0] + KJ * v[1] + KK * v[2]
+    kabs = np.sqrt(KI**2 + KJ**2 + KK**2)
+    mag = kabs * np.sqrt(sum(np.abs(a) ** 2 for a in v))
+    assert np.abs(div).max() < 1e-14 * mag.max()
+    jn = (-np.arange(o.nkj)) % o.nkj
+    kn = (-np.arange(o.nkk)) % o.nkk
+    for c in range(3):
+        p = v[c][0]
+        assert np.abs(p - np.conj(p[np.ix_(jn, kn)])).max() < 1e-14 * np.abs(p).max()
+
+
+def test_shear_form_with_zero_shear_reduces_to_isotropic(oracle_mod):
+    import pandora_amd as pa
+    n = 20
+    u = pa.init_field_host(n, 3, 2.0 / 3.0, 4.0)
+    a = oracle_mod.Oracle(n, f_type=1, rk3_timestep=2, rk3_dt=0.004, nu=0.01)
+    b = oracle_mod.Oracle(n, f_type=2, rk3_timestep=2, rk3_dt=0.004, nu=0.01, deform_s=0.0)
+    a.set_state(*u)
+    b.set_state(*u)
+    ta = tb = 0.0
+    for s in range(1, 4):
+        ta = a.step(s, ta)
+        tb = b.step(s, tb)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert rel(y, x) < 1e-14
+
+
+def _lab_budget(o, S, nu):
+    """K, production -S<u1 u3>, dissipation nu <|grad u|^2> of the lab-frame velocity
+    binv u, with lab wavenumbers k_lab = bmat^T k (sum over the Hermitian half)."""
+    b, bi = o.get_frame()
+    B = b.reshape(3, 3, order="F")
+    BI = bi.reshape(3, 3, order="F")
+    u = o.get_state()
+    KI, KJ, KK = kgrid(o)
+    w = _energy_weights(o)
+    ul = [BI[m, 0] * u[0] + BI[m, 1] * u[1] + BI[m, 2] * u[2] for m in range(3)]
+    kl = [B[0, j] * KI + B[1, j] * KJ + B[2, j] * KK for j in range(3)]
+    kl2 = kl[0] ** 2 + kl[1] ** 2 + kl[2] ** 2
+    K = 0.5 * sum((w * np.abs(x) ** 2).sum() for x in ul)
+    P = -S * (w * (ul[0] * np.conj(ul[2])).real).sum()
+    eps = nu * sum((w * kl2 * np.abs(x) ** 2).sum() for x in ul)
+    return K, P, eps
+
+
+def test_shear_energy_budget(oracle_mod):
+    """dK/dt = -S <u1 u3> - eps for homogeneous shear dU1/dx3 = S: checks the sign
+    and size of the source, pressure and viscous terms of the BF_ROT form."""
+    import pandora_amd as pa
+    n, S, nu = 32, 5.0, 0.02
+    remesh_t = 100                      # dt = (0.5/S)/F_REMESH_T = 1e-3
+    o = oracle_mod.Oracle(n, f_type=2, rk3_timestep=1, remesh_t=remesh_t, deform_s=S, nu=nu)
+    o.set_state(*pa.init_field_host(n, 5, 2.0 / 3.0, 4.0))
+    t = 0.0
+    for s in range(1, 61):              # St = 0.3: shear production has built up
+        t = o.step(s, t)
+    K0, P0, e0 = _lab_budget(o, S, nu)
+    t1 = o.step(61, t)
+    K1, P1, e1 = _lab_budget(o, S, nu)
+    dt = t1 - t
+    lhs = (K1 - K0) / dt
+    rhs = 0.5 * ((P0 - e0) + (P1 - e1))
+    assert abs(P0) > 0.05 * abs(e0)     # production is not negligible
+    assert abs(lhs - rhs) < 2e-4 * (abs(P0) + abs(e0))
+
+
+def test_remesh_map_enumeration(oracle_mod):
+    """knew = k_signed - i, dropped below -K (src/pf_rk3.f90:4100-4125)."""
+    for K in [4, 10, 21, 32]:
+        nkk = 2 * K + 1
+        for i in range(0, K + 1):
+            targets = []
+            for k in range(nkk):
+                s = k if k <= K else k - nkk
+                t = oracle_mod.remesh_target(K, i, k)
+                if s - i < -K:
+                    assert t == -1
+                else:
+                    assert t == (s - i) % nkk
+                    targets.append(t)
+            assert len(targets) == nkk - i and len(set(targets)) == len(targets)
+
+
+def test_remesh_moves_modes_and_adds_u3(oracle_mod):
+    n, R = 24, 2
+    o = oracle_mod.Oracle(n, f_type=2, rk3_timestep=1, remesh_t=R, deform_s=1.0)
+    u = random_state(o, 9, solenoidal=False)
+    o.set_state(*u)
+    assert not o.remesh(R)            # first remesh is at step R+1
+    assert o.remesh(R + 1)
+    v = o.get_state()
+    K = o.nkk // 2
+    for i in range(o.nki):
+        for k in range(o.nkk):
+            s = k if k <= K else k - o.nkk
+            src = s + i
+            if src > K:
+                assert v[1][i, :, k].tolist() == [0j] * o.nkj
+                continue
+            ko = src % o.nkk
+            assert np.array_equal(v[1][i, :, k], u[1][i, :, ko])
+            assert np.array_equal(v[0][i, :, k], u[0][i, :, ko] + u[2][i, :, ko])
+    b, bi = o.get_frame()
+    assert b[6] == 0.5 and bi[6] == -0.5   # bmat(1,3), binv(1,3) after remesh
+
+
+def test_shear_clock_counts_remeshes(oracle_mod):
+    """bmat(1,3) = -S (t - m/S) with m the number of remeshes so far; the remesh
+    fires at steps R+1, 3R+1, 5R+1, ... (src/pf_rk3.f90:3985-3991)."""
+    S, R, n = 1.0, 3, 16
+    o = oracle_mod.Oracle(n, f_type=2, rk3_timestep=1, remesh_t=R, deform_s=S)
+    t = 0.0
+    m = 0
+    for tstep in range(1, 14):
+        if tstep >= R + 1 and (tstep - R - 1) % (2 * R) == 0:
+            m += 1
+        t = o.step(tstep, t)
+        assert abs(o.dt - 1.0 / (2 * R)) < 1e-16
+        b, bi = o.get_frame()
+        assert abs(b[6] - (-S * (t - m / S))) < 1e-13, tstep
+        assert abs(bi[6] - S * (t - m / S)) < 1e-13, tstep
+        assert -0.5 - 1e-12 <= b[6] <= 0.5 + 1e-12
+
+
+def test_neville_reproduces_polynomials(oracle_mod):
+    """Order-4 Neville interpolation is exact for cubics in each direction
+    (molecule search src/pp_interp.f90:406-467, tableau 474-560)."""
+    n = 16
+    o = oracle_mod.Oracle(n)
+    dx = TWOPI / n
+    g = np.arange(n) * dx
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    f1 = 0.3 + X - 0.5 * Y**2 + 0.1 * Z**3 + 0.2 * X * Y * Z
+    f2 = X**3 - Y**3 + Z**2 * X
+    f3 = 1.0 + 0.0 * X
+    rng = np.random.default_rng(4)
+    xp = rng.uniform(0.3 * TWOPI, 0.65 * TWOPI, size=(200, 3))
+    uf = o.interpolate(f1, f2, f3, xp)
+    x, y, z = xp[:, 0], xp[:, 1], xp[:, 2]
+    e1 = 0.3 + x - 0.5 * y**2 + 0.1 * z**3 + 0.2 * x * y * z
+    e2 = x**3 - y**3 + z**2 * x
+    assert np.abs(uf[:, 0] - e1).max() < 1e-11 * np.abs(e1).max()
+    assert np.abs(uf[:, 1] - e2).max() < 1e-11 * np.abs(e2).max()
+    assert np.abs(uf[:, 2] - 1.0).max() < 1e-14
+    # at a node the value is the nodal value, wherever the molecule sits
+    nodes = np.array([[3, 5, 7], [0, 0, 0], [15, 1, 8]]) * dx
+    un = o.interpolate(f1, f2, f3, nodes)
+    for q, (a, b, c) in enumerate([[3, 5, 7], [0, 0, 0], [15, 1, 8]]):
+        assert abs(un[q, 0] - f1[c, b, a]) < 1e-12 * max(1.0, abs(f1[c, b, a]))
+
+
+def test_correct_position_boundaries(oracle_mod):
+    """[-dx/2, L-dx/2) in x1, x2 and the reference's '>' test in x3 (src/pp_mpi.f90:1011-1057)."""
+    n = 16
+    o = oracle_mod.Oracle(n)
+    dx = TWOPI / n
+    lo, hi = -0.5 * dx, -0.5 * dx + TWOPI
+    xp = np.array([[hi, 0.0, 0.0], [0.0, hi, 0.0], [0.0, 0.0, hi], [lo - 1e-9, lo - 1e-9, lo - 1e-9]])
+    out = o.correct_position(xp)
+    assert out[0, 0] == hi - TWOPI
+    assert out[1, 1] == hi - TWOPI
+    assert out[2, 2] == hi               # x3 == max is kept
+    assert np.allclose(out[3], lo - 1e-9 + TWOPI)
+
+
+def test_tracers_in_uniform_flow(oracle_mod):
+    """The k = 0 mode is a uniform flow that is never updated; RK3 moves tracers
+    by exactly U dt per step (sum of the stage increments is 1)."""
+    n, dt = 16, 0.01
+    o = oracle_mod.Oracle(n, rk3_dt=dt, nu=0.01)
+    u = [np.zeros(o.shape, complex) for _ in range(3)]
+    U = np.array([0.7, -0.4, 0.25])
+    for c in range(3):
+        u[c][0, 0, 0] = U[c]
+    o.set_state(*u)
+    rng = np.random.default_rng(2)
+    dx = TWOPI / n
+    x0 = rng.uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(50, 3))
+    o.set_particles(x0)
+    t = 0.0
+    nst = 20
+    for s in range(1, nst + 1):
+        t = o.step(s, t)
+    xp, _ = o.get_particles()
+    exp = x0 + nst * dt * U
+    lo = -0.5 * dx
+    exp = lo + np.mod(exp - lo, TWOPI)
+    d = np.abs(xp - exp)
+    d = np.minimum(d, TWOPI - d)
+    assert d.max() < 1e-12
+
+
+def test_spectrum_totals(oracle_mod):
+    import pandora_amd as pa
+    o = oracle_mod.Oracle(32, nu=0.01)
+    u = pa.init_field_host(32, 1, 2.0 / 3.0, 4.0)
+    o.set_state(*u)
+    ek, dk, ds, tot = o.spectrum()
+    KI, KJ, KK = kgrid(o)
+    w = _energy_weights(o)
+    kabs = np.sqrt(KI**2 + KJ**2 + KK**2)
+    inside = (kabs >= 0.5) & (kabs < o.kkmax + 0.5)
+    e = 0.5 * sum(w * np.abs(a) ** 2 for a in u)
+    assert abs(ek.sum() - e[inside].sum()) < 1e-13 * e.sum()
+    assert abs(dk.sum() - (2 * 0.01 * kabs**2 * e)[inside].sum()) < 1e-13 * dk.sum()
+    assert abs(0.5 * (tot[3] + tot[4] + tot[5]) - e.sum()) < 1e-13 * e.sum()
