@@ -157,6 +157,7 @@ def main():
                 "unit": "GB/s", "frac": b / avg / HBM_PEAK, "traffic": None,
                 "bytes_per_launch": b, "avg_launch_ms": avg * 1e3}
     bstep = model_bytes_per_step(g, 8, 4, ntr)
+    copy_gbps = pa.copy_bandwidth(1 << 30, 20)
     cpu = None
     if not args.no_cpu:
         print("[bench] CPU baseline (FP64 oracle, one stage)", file=sys.stderr, flush=True)
@@ -183,6 +184,7 @@ def main():
         "step_model": {"bytes_per_step": bstep, "achieved_GBps": bstep * sps32 / 1e9,
                        "frac_of_peak": bstep * sps32 / HBM_PEAK},
         "kernels": kernels,
+        "measured_copy_GBps": copy_gbps,
         "roofline": roof,
         "cpu_baseline": cpu,
         "finite": bool(ok32),

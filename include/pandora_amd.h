@@ -217,6 +217,11 @@ int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6
  * current state; the state is not modified. */
 int pdr_debug_prepare(pdr_ctx* ctx, int32_t field, double* out);
 
+/* Measured device copy bandwidth (read + write bytes per second, GB/s) of a
+ * plain 16-byte-vector copy of `bytes` bytes, timed over `iters` launches.
+ * No reference counterpart: the roofline reference for the kernels. */
+int pdr_copy_bandwidth(int64_t bytes, int32_t iters, double* gbps);
+
 /* Per-kernel timing with HIP events on the context's stream.  enable != 0
  * starts collection; pdr_profile_get returns, for up to n kernel classes,
  * names (comma-separated into names[namelen]), launch counts and total ms. */

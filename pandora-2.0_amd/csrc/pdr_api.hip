@@ -651,6 +651,35 @@ int pdr_debug_prepare(pdr_ctx* c, int32_t field, double* out) {
   return get_field(c, out, (const char*)c->w1 + off, 2 * c->g.m1);
 }
 
+int pdr_copy_bandwidth(int64_t bytes, int32_t iters, double* gbps) {
+  if (bytes < 16 || iters < 1 || !gbps) return fail(PDR_ERR_ARG, "pdr_copy_bandwidth: bad argument");
+  bytes &= ~int64_t(15);
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, size_t(bytes));
+  if (e == hipSuccess) e = hipMalloc(&b, size_t(bytes));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMemsetAsync(a, 0, size_t(bytes), s);
+  if (e == hipSuccess) e = launch_copy(a, b, bytes, s);  // warm-up
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int q = 0; q < iters && e == hipSuccess; ++q) e = launch_copy(a, b, bytes, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e != hipSuccess) return fail(PDR_ERR_HIP, std::string("copy bandwidth: ") + hipGetErrorString(e));
+  *gbps = 2.0 * double(bytes) * iters / (double(ms) * 1e-3) / 1e9;
+  return PDR_OK;
+}
+
 int pdr_profile_enable(pdr_ctx* c, int32_t enable) {
   if (!c) return fail(PDR_ERR_ARG, "null context");
   c->prof = enable != 0;

@@ -126,6 +126,7 @@ hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);
 hipError_t launch_finite(pdr_ctx* c, int* flag);
+hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s);
 
 // profiling hooks (pdr_api.hip)
 void prof_begin(pdr_ctx* c, int cls);

@@ -585,6 +585,19 @@ __global__ void k_finite(const T* __restrict__ a, long long n, int* flag) {
   }
 }
 
+// Plain streaming copy (16-byte vectors): the measured HBM reference for the
+// roofline and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(kThreads) k_copy(const float4* __restrict__ a, float4* __restrict__ b, long long n) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+    b[q] = a[q];
+}
+
+hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s) {
+  const long long n = bytes / 16;
+  k_copy<<<dim3(8192), kThreads, 0, s>>>((const float4*)a, (float4*)b, n);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------

@@ -1,0 +1,298 @@
+!---------------------------------------------------------------------------------
+! module pdr_rk3_amd
+!> ISO_C_BINDING shim that keeps the reference's time-step interfaces
+!! (g_RK3Init / g_RK3Calc / g_RK3UpdateB in src/pg_rk3.f90:46-385 and
+!! f_RK3Stage1/2/3 in src/pf_rk3.f90:40-200) and runs them on the MI355X
+!! library declared in include/pandora_amd.h.  The Fortran caller keeps its
+!! TIMELOOP (src/pg_control.f90:177-194) and its pf_arrays-layout arrays
+!! arr(0:1, 0:NKK-1, 0:NKJ-1, 0:NKI-1); the whole box lives on one GPU.
+!!
+!! Differences from the reference module: run parameters are handed over once
+!! with pdr_Configure (the reference reads module globals of g_parameters), and
+!! state moves between host arrays and the device with pdr_PutVelocity /
+!! pdr_GetVelocity (the reference keeps PETSc Vecs).  ierr is 0 on success and
+!! the library's negative error code otherwise; pdr_ErrorMessage returns the text.
+module pdr_rk3_amd
+  use iso_c_binding
+  implicit none
+  private
+
+  !> Mirror of struct pdr_params (include/pandora_amd.h)
+  type, bind(C), public :: pdr_params_t
+    integer(c_int32_t) :: nodes_x = 0, nodes_y = 0, nodes_z = 0
+    integer(c_int32_t) :: f_type = 1
+    integer(c_int32_t) :: ns_deformed = 1
+    integer(c_int32_t) :: dealiasing = 1
+    integer(c_int32_t) :: rk3_timestep = 2
+    integer(c_int32_t) :: f_remesh_t = 0
+    integer(c_int32_t) :: b110 = 1
+    integer(c_int32_t) :: p_interp_order = 4
+    integer(c_int32_t) :: precision = 0
+    integer(c_int32_t) :: reserved = 0
+    real(c_double)     :: rk3_dt = 0.01d0
+    real(c_double)     :: rk3_cfl = 0.75d0
+    real(c_double)     :: f_nu = 0.0d0
+    real(c_double)     :: f_deform_s = 0.0d0
+    real(c_double)     :: force_tf = 0.0d0
+  end type pdr_params_t
+
+  integer(c_int32_t), parameter, public :: PDR_FP64 = 0, PDR_FP32_MIXED = 1
+
+  type(c_ptr), save         :: ctx = c_null_ptr
+  type(pdr_params_t), save  :: params
+
+  public :: pdr_Configure, g_RK3Init, g_RK3Calc, g_RK3UpdateB, g_RK3Finalise
+  public :: f_RK3Stage1, f_RK3Stage2, f_RK3Stage3, f_RK3Remesh
+  public :: pdr_PutVelocity, pdr_GetVelocity, pdr_PutParticles, pdr_GetParticles
+  public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_Spectrum, pdr_ErrorMessage
+  public :: pdr_GetFrame
+
+  interface
+    integer(c_int) function c_pdr_create(out, p) bind(C, name="pdr_create")
+      import :: c_int, c_ptr, pdr_params_t
+      type(c_ptr), intent(out) :: out
+      type(pdr_params_t), intent(in) :: p
+    end function
+    integer(c_int) function c_pdr_destroy(c) bind(C, name="pdr_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c
+    end function
+    integer(c_int) function c_pdr_step(c, tstep, simtime) bind(C, name="pdr_step")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: tstep
+      real(c_double), intent(inout) :: simtime
+    end function
+    integer(c_int) function c_pdr_stage(c, stage, tstep, time, dt) bind(C, name="pdr_stage")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: stage, tstep
+      real(c_double), value :: time
+      real(c_double), intent(inout) :: dt
+    end function
+    integer(c_int) function c_pdr_update_b(c, tstep, simtime) bind(C, name="pdr_update_b")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: tstep
+      real(c_double), value :: simtime
+    end function
+    integer(c_int) function c_pdr_remesh(c, tstep, done) bind(C, name="pdr_remesh")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: tstep
+      integer(c_int32_t), intent(out) :: done
+    end function
+    integer(c_int) function c_pdr_upload_state(c, u1, u2, u3) bind(C, name="pdr_upload_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: u1(*), u2(*), u3(*)
+    end function
+    integer(c_int) function c_pdr_download_state(c, u1, u2, u3) bind(C, name="pdr_download_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: u1(*), u2(*), u3(*)
+    end function
+    integer(c_int) function c_pdr_particles_upload(c, xp, np) bind(C, name="pdr_particles_upload")
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: c
+      real(c_double), intent(in) :: xp(*)
+      integer(c_int64_t), value :: np
+    end function
+    integer(c_int) function c_pdr_particles_download(c, xp, uprk) bind(C, name="pdr_particles_download")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: xp(*)
+      type(c_ptr), value :: uprk
+    end function
+    integer(c_int) function c_pdr_get_frame(c, bmat, binv) bind(C, name="pdr_get_frame")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: bmat(9), binv(9)
+    end function
+    integer(c_int) function c_pdr_init_field(c, seed, u0, kp) bind(C, name="pdr_init_field")
+      import :: c_int, c_ptr, c_int64_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int64_t), value :: seed
+      real(c_double), value :: u0, kp
+    end function
+    integer(c_int) function c_pdr_restart_write(c, path, i64) bind(C, name="pdr_restart_write")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: c
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: i64
+    end function
+    integer(c_int) function c_pdr_restart_read(c, path) bind(C, name="pdr_restart_read")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), value :: c
+      character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function c_pdr_spectrum(c, ek, dk, dsk, tot) bind(C, name="pdr_spectrum")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: ek(*), dk(*), dsk(*), tot(6)
+    end function
+    type(c_ptr) function c_pdr_last_error() bind(C, name="pdr_last_error")
+      import :: c_ptr
+    end function
+  end interface
+
+contains
+
+  !> Hand the run parameters (the g_parameters subset the step uses) to the shim.
+  subroutine pdr_Configure ( p, ierr )
+    type(pdr_params_t), intent(in) :: p
+    integer, intent(inout)         :: ierr
+    params = p
+    ierr = 0
+  end subroutine pdr_Configure
+
+  !> g_RK3Init (src/pg_rk3.f90:46-70): checks the parameters (CFL < 1 as
+  !! g_RK3Check) and creates the resident device state.
+  subroutine g_RK3Init ( ierr )
+    integer, intent(inout) :: ierr
+    if ( c_associated(ctx) ) ierr = int(c_pdr_destroy(ctx))
+    ierr = int(c_pdr_create(ctx, params))
+  end subroutine g_RK3Init
+
+  !> g_RK3Calc (src/pg_rk3.f90:82-129): three stages with the shear-frame update.
+  subroutine g_RK3Calc ( tstep, simtime, ierr )
+    integer, intent(in)               :: tstep
+    real(kind=c_double), intent(inout) :: simtime
+    integer, intent(inout)            :: ierr
+    ierr = int(c_pdr_step(ctx, int(tstep, c_int32_t), simtime))
+  end subroutine g_RK3Calc
+
+  !> g_RK3UpdateB (src/pg_rk3.f90:280-385); rk3dt is unused, as in the reference
+  subroutine g_RK3UpdateB ( tstep, rk3dt, simtime, ierr )
+    integer, intent(in)             :: tstep
+    real(kind=c_double), intent(in) :: rk3dt
+    real(kind=c_double), intent(in) :: simtime
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_update_b(ctx, int(tstep, c_int32_t), simtime))
+  end subroutine g_RK3UpdateB
+
+  !> f_RK3Stage1 (src/pf_rk3.f90:40-115): remesh check, stage 1, sets dt
+  subroutine f_RK3Stage1 ( tstep, time, dt, ierr )
+    integer, intent(in)                :: tstep
+    real(kind=c_double), intent(in)    :: time
+    real(kind=c_double), intent(inout) :: dt
+    integer, intent(inout)             :: ierr
+    ierr = int(c_pdr_stage(ctx, 1_c_int32_t, int(tstep, c_int32_t), time, dt))
+  end subroutine f_RK3Stage1
+
+  !> f_RK3Stage2 (src/pf_rk3.f90:121-155)
+  subroutine f_RK3Stage2 ( tstep, time, dt, ierr )
+    integer, intent(in)                :: tstep
+    real(kind=c_double), intent(in)    :: time
+    real(kind=c_double), intent(inout) :: dt
+    integer, intent(inout)             :: ierr
+    ierr = int(c_pdr_stage(ctx, 2_c_int32_t, int(tstep, c_int32_t), time, dt))
+  end subroutine f_RK3Stage2
+
+  !> f_RK3Stage3 (src/pf_rk3.f90:161-200)
+  subroutine f_RK3Stage3 ( tstep, time, dt, ierr )
+    integer, intent(in)                :: tstep
+    real(kind=c_double), intent(in)    :: time
+    real(kind=c_double), intent(inout) :: dt
+    integer, intent(inout)             :: ierr
+    ierr = int(c_pdr_stage(ctx, 3_c_int32_t, int(tstep, c_int32_t), time, dt))
+  end subroutine f_RK3Stage3
+
+  !> f_RK3Remesh (src/pf_rk3.f90:3958-4049)
+  subroutine f_RK3Remesh ( tstep, ierr )
+    integer, intent(in)    :: tstep
+    integer, intent(inout) :: ierr
+    integer(c_int32_t)     :: done
+    ierr = int(c_pdr_remesh(ctx, int(tstep, c_int32_t), done))
+  end subroutine f_RK3Remesh
+
+  !> Release the device state (ProgramFinalise, src/pg_control.f90:200-259)
+  subroutine g_RK3Finalise ( ierr )
+    integer, intent(inout) :: ierr
+    ierr = 0
+    if ( c_associated(ctx) ) ierr = int(c_pdr_destroy(ctx))
+    ctx = c_null_ptr
+  end subroutine g_RK3Finalise
+
+  !> Copy u1_3w, u2_3w, u3_3w (pf_arrays layout, FP64) to the device
+  subroutine pdr_PutVelocity ( u1, u2, u3, ierr )
+    real(kind=c_double), intent(in) :: u1(0:,0:,0:,0:), u2(0:,0:,0:,0:), u3(0:,0:,0:,0:)
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_upload_state(ctx, u1, u2, u3))
+  end subroutine pdr_PutVelocity
+
+  !> Copy the velocity back into pf_arrays-layout host arrays
+  subroutine pdr_GetVelocity ( u1, u2, u3, ierr )
+    real(kind=c_double), intent(out) :: u1(0:,0:,0:,0:), u2(0:,0:,0:,0:), u3(0:,0:,0:,0:)
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_download_state(ctx, u1, u2, u3))
+  end subroutine pdr_GetVelocity
+
+  !> Tracer positions particle_xp(1:3, np) (src/pp_arrays.f90:25-34)
+  subroutine pdr_PutParticles ( xp, np, ierr )
+    real(kind=c_double), intent(in) :: xp(:,:)
+    integer(c_int64_t), intent(in)  :: np
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_particles_upload(ctx, xp, np))
+  end subroutine pdr_PutParticles
+
+  subroutine pdr_GetParticles ( xp, ierr )
+    real(kind=c_double), intent(out) :: xp(:,:)
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_particles_download(ctx, xp, c_null_ptr))
+  end subroutine pdr_GetParticles
+
+  !> bmat, binv (src/pg_domain.f90:78-87), Fortran column-major
+  subroutine pdr_GetFrame ( bmat, binv, ierr )
+    real(kind=c_double), intent(out) :: bmat(3,3), binv(3,3)
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_get_frame(ctx, bmat, binv))
+  end subroutine pdr_GetFrame
+
+  !> Seeded Rogallo initial field, type-10 spectrum (f_InitialiseFluid)
+  subroutine pdr_InitField ( seed, u0, kp, ierr )
+    integer(c_int64_t), intent(in)  :: seed
+    real(kind=c_double), intent(in) :: u0, kp
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_init_field(ctx, seed, u0, kp))
+  end subroutine pdr_InitField
+
+  !> g_RestartPetscFluid(WRITE_FILE) (src/pg_restart.f90:97-150)
+  subroutine pdr_RestartWrite ( path, int64header, ierr )
+    character(len=*), intent(in) :: path
+    logical, intent(in)          :: int64header
+    integer, intent(inout)       :: ierr
+    integer(c_int32_t)           :: flag
+    flag = 0
+    if ( int64header ) flag = 1
+    ierr = int(c_pdr_restart_write(ctx, trim(path)//c_null_char, flag))
+  end subroutine pdr_RestartWrite
+
+  !> g_RestartPetscFluid(READ_FILE) (src/pg_restart.f90:57-96)
+  subroutine pdr_RestartRead ( path, ierr )
+    character(len=*), intent(in) :: path
+    integer, intent(inout)       :: ierr
+    ierr = int(c_pdr_restart_read(ctx, trim(path)//c_null_char))
+  end subroutine pdr_RestartRead
+
+  !> E(k), D(k) shell sums and totals (f_FluidstatsFsPrimaryQuants)
+  subroutine pdr_Spectrum ( ek, dk, dsk, tot, ierr )
+    real(kind=c_double), intent(out) :: ek(:), dk(:), dsk(:), tot(6)
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_spectrum(ctx, ek, dk, dsk, tot))
+  end subroutine pdr_Spectrum
+
+  !> Text of the last library error
+  function pdr_ErrorMessage () result ( msg )
+    character(len=256)          :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: i
+    msg = ' '
+    call c_f_pointer(c_pdr_last_error(), p, [256])
+    do i = 1, 256
+      if ( p(i) == c_null_char ) exit
+      msg(i:i) = p(i)
+    end do
+  end function pdr_ErrorMessage
+
+end module pdr_rk3_amd

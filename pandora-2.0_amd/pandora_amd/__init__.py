@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_spectrum": (C.c_int, [vp, dp, dp, dp, dp]),
         "pdr_debug_prepare": (C.c_int, [vp, i32, dp]),
         "pdr_profile_enable": (C.c_int, [vp, i32]),
+        "pdr_copy_bandwidth": (C.c_int, [i64, i32, C.POINTER(d)]),
         "pdr_profile_get": (C.c_int, [vp, i32, C.c_char_p, i32,
                                       np.ctypeslib.ndpointer(dtype=np.int64), dp]),
     }
@@ -152,6 +153,13 @@ def restart_read_host(path, shape):
     flat = [o.view(np.float64).reshape(-1) for o in out]
     _check(L.pdr_restart_read_host(path.encode(), flat[0].size, *flat))
     return out
+
+
+def copy_bandwidth(nbytes=1 << 30, iters=20):
+    """Measured HBM copy bandwidth in GB/s (bytes read + written per second)."""
+    g = C.c_double()
+    _check(load_library().pdr_copy_bandwidth(nbytes, iters, C.byref(g)))
+    return g.value
 
 
 class Pandora:

@@ -79,9 +79,10 @@ def test_isotropic_step_parity_fp64(oracle_mod, n):
 
 @pytest.mark.parametrize("n", [32, 48])
 def test_shear_step_parity_fp64_with_remesh(oracle_mod, n):
-    """Rogallo frame, RK3_TIMESTEP = DECIDE, F_REMESH_T = 3: remeshes at steps 4 and 10."""
+    """Rogallo frame, RK3_TIMESTEP = DECIDE, F_REMESH_T = 3, S = 20 (dt = 0.5/(S F_REMESH_T)
+    = 1/120): remeshes at steps 4 and 10."""
     note(f"shear BF_ROT FP64 N={n}, 10 steps with remesh")
-    o, g = _pair(oracle_mod, n, f_type=2, rk3_timestep=1, deform_s=1.0, remesh_t=3, nu=0.005)
+    o, g = _pair(oracle_mod, n, f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=3, nu=0.005)
     u = pa.init_field_host(n, 200 + n, 2.0 / 3.0, 6.0)
     o.set_state(*u)
     g.set_state(*u)
@@ -136,7 +137,7 @@ def test_remesh_is_bit_exact(oracle_mod, prec):
 def test_tracers_parity_shear_fp64(oracle_mod):
     note("tracers + shear FP64 N=32, 2000 tracers, 8 steps (remesh at 4)")
     n = 32
-    o, g = _pair(oracle_mod, n, f_type=2, rk3_timestep=1, deform_s=1.0, remesh_t=3, nu=0.005)
+    o, g = _pair(oracle_mod, n, f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=3, nu=0.005)
     u = pa.init_field_host(n, 7, 2.0 / 3.0, 6.0)
     o.set_state(*u)
     g.set_state(*u)
@@ -265,7 +266,7 @@ def test_error_paths_on_device():
     g.set_state(*u)
     assert g.check_finite() is True
     with pytest.raises(pa.PdrError) as e:
-        pa.Pandora(6000, precision=pa.PDR_FP64)
+        pa.Pandora(2560, precision=pa.PDR_FP32_MIXED)  # fits the LDS, exceeds 288 GB
     assert e.value.code == -3
 
 
