@@ -2,6 +2,7 @@
 // orchestration, shear-frame clock and remesh decision (all FP64 on the host).
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -217,6 +218,7 @@ static int upload_tw(pdr_ctx* c, int n, void** dst) {
 static int dalloc(pdr_ctx* c, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
+    (void)hipGetLastError();  // clear it: later launch checks read hipGetLastError()
     *p = nullptr;
     return fail(PDR_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " +
                                    hipGetErrorString(e));
@@ -295,9 +297,11 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   g.nki = g.ki + 1;
   g.nkj = 2 * g.kj + 1;
   g.nkk = 2 * g.kk + 1;
+  g.ldj = (g.nkj + 15) / 16 * 16;
+  g.ldi = (g.nki + 15) / 16 * 16;
   g.mw = (long long)g.nki * g.nkj * g.nkk;
-  g.m1 = (long long)g.nki * g.nz * g.nkj;
-  g.m2 = (long long)g.nz * g.ny * g.nki;
+  g.m1 = (long long)g.nki * g.nz * g.ldj;
+  g.m2 = (long long)g.nz * g.ny * g.ldi;
   g.nr = (long long)nx * ny * nz;
   frame_init(c);
   c->dt = 99999.0;
@@ -324,6 +328,12 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
       2 * size_t(ny + 1) * c->csz > c->lds_max) {
     delete c;
     return fail(PDR_ERR_UNSUPPORTED, "grid too large for the LDS of this device");
+  }
+  {
+    const char* env = std::getenv("PDR_GENERIC_KERNELS");
+    const bool generic = env && env[0] && env[0] != '0';
+    c->spec = !generic && spec_supported(nx) && spec_supported(ny) && spec_supported(nz) &&
+              spec_lds_need(c->prec, nx, ny, nz) <= c->lds_max;
   }
   fft_plan(&c->pz, nz);
   fft_plan(&c->py, ny);
@@ -647,8 +657,15 @@ int pdr_debug_prepare(pdr_ctx* c, int32_t field, double* out) {
   if (!c || !out || field < 0 || field > 5) return fail(PDR_ERR_ARG, "pdr_debug_prepare: bad argument");
   const Coef cf = make_coef(c);
   HIPCHK(launch_zinv(c, cf));
-  const size_t off = size_t(field) * size_t(c->g.m1) * c->csz;
-  return get_field(c, out, (const char*)c->w1 + off, 2 * c->g.m1);
+  const Geo& g = c->g;
+  const size_t off = size_t(field) * size_t(g.m1) * c->csz;
+  std::vector<double> pad(size_t(2 * g.m1));
+  int rc = get_field(c, pad.data(), (const char*)c->w1 + off, 2 * g.m1);
+  if (rc != PDR_OK) return rc;
+  // drop the padding of the j stride: (0:1, 0:NKJ-1, 0:NZ-1, 0:NKI-1)
+  for (long long line = 0; line < (long long)g.nki * g.nz; ++line)
+    std::memcpy(out + 2 * line * g.nkj, pad.data() + 2 * line * g.ldj, sizeof(double) * 2 * size_t(g.nkj));
+  return PDR_OK;
 }
 
 int pdr_copy_bandwidth(int64_t bytes, int32_t iters, double* gbps) {

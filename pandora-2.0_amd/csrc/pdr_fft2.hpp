@@ -1,0 +1,120 @@
+// pdr_fft2.hpp -- compile-time specialised batched FFTs in LDS for CDNA4.
+//
+// Same Stockham algorithm and conventions as pdr_fft.hpp, but the length N, the
+// number of lines and the radix plan are template constants, so every index
+// division is folded, the pass loop is unrolled and the butterflies of a pass are
+// held in registers: each pass reads its inputs into registers, waits at a
+// barrier, then writes in place (one LDS buffer instead of two).  Twiddles are
+// staged in LDS once per workgroup.
+#pragma once
+#include "pdr_fft.hpp"
+
+namespace pdr {
+
+constexpr int kBlock2 = 256;
+
+// Radix plan: 8s, then one 4 or 2, then 3s, then 5s.
+constexpr int plan_radix(int n, int p) {
+  int m = n, q = 0;
+  while (m % 8 == 0) {
+    if (q == p) return 8;
+    m /= 8;
+    ++q;
+  }
+  if (m % 4 == 0) {
+    if (q == p) return 4;
+    m /= 4;
+    ++q;
+  }
+  if (m % 2 == 0) {
+    if (q == p) return 2;
+    m /= 2;
+    ++q;
+  }
+  while (m % 3 == 0) {
+    if (q == p) return 3;
+    m /= 3;
+    ++q;
+  }
+  while (m % 5 == 0) {
+    if (q == p) return 5;
+    m /= 5;
+    ++q;
+  }
+  return 0;
+}
+
+constexpr int plan_npass(int n) {
+  int p = 0;
+  while (plan_radix(n, p) != 0) ++p;
+  return p;
+}
+
+template <typename T, int N, int LINES, int LD, int R, int NS, int SIGN>
+__device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
+  constexpr int NBL = N / R;
+  constexpr int TOTAL = NBL * LINES;
+  constexpr int NB = (TOTAL + kBlock2 - 1) / kBlock2;
+  constexpr int TWS = N / (NS * R);
+  cx<T> v[NB][R];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int t = int(threadIdx.x) + kBlock2 * b;
+    if (TOTAL % kBlock2 == 0 || t < TOTAL) {
+      const int line = t / NBL;
+      const int j = t - line * NBL;
+      const cx<T>* s = buf + line * LD + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = s[r * NBL];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int t = int(threadIdx.x) + kBlock2 * b;
+    if (TOTAL % kBlock2 == 0 || t < TOTAL) {
+      const int line = t / NBL;
+      const int j = t - line * NBL;
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx<T> w = tw[k * r * TWS];
+          if (SIGN > 0) w.y = -w.y;
+          v[b][r] = mul(v[b][r], w);
+        }
+      }
+      Dft<T, R, SIGN>::run(v[b]);
+      cx<T>* d = buf + line * LD + (j - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) d[r * NS] = v[b][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int LINES, int LD, int SIGN, int P = 0, int NS = 1>
+__device__ __forceinline__ void fft_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
+  if constexpr (P < plan_npass(N)) {
+    constexpr int R = plan_radix(N, P);
+    pass_ip<T, N, LINES, LD, R, NS, SIGN>(buf, tw);
+    fft_ip<T, N, LINES, LD, SIGN, P + 1, NS * R>(buf, tw);
+  }
+}
+
+// Copy the length-N twiddle table into LDS (the caller synchronises).
+template <typename T, int N>
+__device__ __forceinline__ void load_tw(cx<T>* __restrict__ dst, const cx<T>* __restrict__ src) {
+  for (int m = threadIdx.x; m < N; m += kBlock2) dst[m] = src[m];
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2);
+// renumber them so that every XCD walks a contiguous range of tiles.
+__device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nblk) {
+  constexpr unsigned kXcd = 8;
+  const unsigned full = (nblk / kXcd) * kXcd;
+  if (bid >= full) return bid;
+  return (bid % kXcd) * (full / kXcd) + bid / kXcd;
+}
+
+}  // namespace pdr

@@ -16,9 +16,11 @@ struct Geo {
   int nx, ny, nz;
   int ki, kj, kk;     // KI_MAX, KJ_MAX, KK_MAX
   int nki, nkj, nkk;  // NODES_KI, NODES_KJ, NODES_KK
+  int ldj;            // j stride of the 1r2w buffer (NKJ padded to 16)
+  int ldi;            // i stride of the x-pencil buffer (NKI padded to 16)
   long long mw;       // modes per wave field      NKI*NKJ*NKK
-  long long m1;       // modes per 1r2w field      NKI*NZ*NKJ
-  long long m2;       // modes per x-pencil field  NZ*NY*NKI
+  long long m1;       // elements per 1r2w field   NKI*NZ*ldj
+  long long m2;       // elements per x-pencil     NZ*NY*ldi
   long long nr;       // real points               NX*NY*NZ
 };
 
@@ -87,7 +89,7 @@ struct pdr_ctx {
   // work buffers
   void* w1 = nullptr;  // 6 fields x m1 (1r2w layout)
   void* w2 = nullptr;  // 6 fields x m2 (x-pencil layout); the NL fields of pass 5 alias it
-  void* ur = nullptr;  // 3 x nr real-space velocity for tracers
+  void* ur = nullptr;  // real-space velocity for tracers, [z][y][x][3]
   double* xp = nullptr;
   double* uprk = nullptr;
   int64_t np = 0;
@@ -101,6 +103,7 @@ struct pdr_ctx {
   double dt = 99999.0;
   double ucfl = 0.0;
   // launch shapes
+  bool spec = false;  // length-specialised kernels (pdr_kernels2.hip)
   int tj_z = 1, ti_y = 1;
   size_t lds_max = 65536;
   hipStream_t stream = nullptr;
@@ -133,6 +136,15 @@ void prof_begin(pdr_ctx* c, int cls);
 void prof_end(pdr_ctx* c, int cls);
 
 void fft_plan(FftPlan* p, int n);
+
+// length-specialised kernels (pdr_kernels2.hip)
+bool spec_supported(int n);
+size_t spec_lds_need(int prec, int nx, int ny, int nz);
+hipError_t launch2_zinv(pdr_ctx* c, const Coef& cf);
+hipError_t launch2_yinv(pdr_ctx* c);
+hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl);
+hipError_t launch2_yfwd(pdr_ctx* c);
+hipError_t launch2_zfwd(pdr_ctx* c);
 
 // records the text returned by pdr_last_error() and returns code
 int set_error(int code, const std::string& msg);

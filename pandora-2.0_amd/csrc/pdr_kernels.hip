@@ -92,12 +92,12 @@ __global__ void __launch_bounds__(kThreads)
     }
     __syncthreads();
     cx<T>* R = fft_lines<T, +1>(A, B, pz, tj, ld, tw);
-    cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * nz * nkj;
+    cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * nz * g.ldj;
     for (int idx = threadIdx.x; idx < tj * nz; idx += blockDim.x) {
       const int z = idx / tj;
       const int line = idx - z * tj;
       const int j = j0 + line;
-      if (j < nkj) dst[(long long)z * nkj + j] = R[line * ld + z];
+      if (j < nkj) dst[(long long)z * g.ldj + j] = R[line * ld + z];
     }
     __syncthreads();
   }
@@ -125,17 +125,17 @@ __global__ void __launch_bounds__(kThreads)
       const int i = i0 + line;
       const int j = (y <= KJ) ? y : ((y >= ny - KJ) ? y - (ny - nkj) : -1);
       cx<T> v = zero_c<T>();
-      if (i < nki && j >= 0) v = src[((long long)i * nz + z) * nkj + j];
+      if (i < nki && j >= 0) v = src[((long long)i * nz + z) * g.ldj + j];
       A[line * ld + y] = v;
     }
     __syncthreads();
     cx<T>* R = fft_lines<T, +1>(A, B, py, ti, ld, tw);
-    cx<T>* dst = w2 + (long long)f * g.m2 + (long long)z * ny * nki;
+    cx<T>* dst = w2 + (long long)f * g.m2 + (long long)z * ny * g.ldi;
     for (int idx = threadIdx.x; idx < ti * ny; idx += blockDim.x) {
       const int y = idx / ti;
       const int line = idx - y * ti;
       const int i = i0 + line;
-      if (i < nki) dst[(long long)y * nki + i] = R[line * ld + y];
+      if (i < nki) dst[(long long)y * g.ldi + i] = R[line * ld + y];
     }
     __syncthreads();
   }
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kThreads)
   cx<T>* B = A + 3 * ld;
   const int y = blockIdx.x;
   const int z = blockIdx.y;
-  const long long row = ((long long)z * g.ny + y) * nki;
+  const long long row = ((long long)z * g.ny + y) * g.ldi;
   for (int idx = threadIdx.x; idx < 3 * nx; idx += blockDim.x) {
     const int p = idx / nx;
     const int m = idx - p * nx;
@@ -195,9 +195,10 @@ __global__ void __launch_bounds__(kThreads)
     const T n2 = o3 * v1 - o1 * v3;
     const T n3 = o1 * v2 - o2 * v1;
     if (want_tr) {
-      ur[rr + x] = u1;
-      ur[g.nr + rr + x] = u2;
-      ur[2 * g.nr + rr + x] = u3;
+      T* d = ur + (rr + x) * 3;
+      d[0] = u1;
+      d[1] = u2;
+      d[2] = u3;
     }
     if (want_cfl) {
       const T s = fabs(u1) + fabs(u2) + fabs(u3);
@@ -250,12 +251,12 @@ __global__ void __launch_bounds__(kThreads)
   const int z = blockIdx.y;
   const int i0 = blockIdx.x * ti;
   for (int f = 0; f < 3; ++f) {
-    const cx<T>* src = w2 + (long long)f * g.m2 + (long long)z * ny * nki;
+    const cx<T>* src = w2 + (long long)f * g.m2 + (long long)z * ny * g.ldi;
     for (int idx = threadIdx.x; idx < ti * ny; idx += blockDim.x) {
       const int y = idx / ti;
       const int line = idx - y * ti;
       const int i = i0 + line;
-      A[line * ld + y] = (i < nki) ? src[(long long)y * nki + i] : zero_c<T>();
+      A[line * ld + y] = (i < nki) ? src[(long long)y * g.ldi + i] : zero_c<T>();
     }
     __syncthreads();
     cx<T>* R = fft_lines<T, -1>(A, B, py, ti, ld, tw);
@@ -266,7 +267,7 @@ __global__ void __launch_bounds__(kThreads)
       const int i = i0 + line;
       if (i < nki) {
         const int yy = (j <= g.kj) ? j : ny - nkj + j;
-        dst[((long long)i * nz + z) * nkj + j] = R[line * ld + yy];
+        dst[((long long)i * nz + z) * g.ldj + j] = R[line * ld + yy];
       }
     }
     __syncthreads();
@@ -288,12 +289,12 @@ __global__ void __launch_bounds__(kThreads)
   const int i = blockIdx.y;
   const int j0 = blockIdx.x * tj;
   for (int f = 0; f < 3; ++f) {
-    const cx<T>* src = w1 + (long long)f * g.m1 + (long long)i * nz * nkj;
+    const cx<T>* src = w1 + (long long)f * g.m1 + (long long)i * nz * g.ldj;
     for (int idx = threadIdx.x; idx < tj * nz; idx += blockDim.x) {
       const int z = idx / tj;
       const int line = idx - z * tj;
       const int j = j0 + line;
-      A[line * ld + z] = (j < nkj) ? src[(long long)z * nkj + j] : zero_c<T>();
+      A[line * ld + z] = (j < nkj) ? src[(long long)z * g.ldj + j] : zero_c<T>();
     }
     __syncthreads();
     cx<T>* R = fft_lines<T, -1>(A, B, pz, tj, ld, tw);
@@ -426,15 +427,15 @@ __device__ void interp_p(const Geo& g, const TracerFrame& tf, const T* __restric
     gz[m] = int(pmod(k0 + m, g.nz));
   }
   for (int c = 0; c < 3; ++c) {
-    const T* f = ur + (long long)c * g.nr;
-    double v[P][P][P];  // [x][z][y]
+    const T* f = ur + c;  // field is [z][y][x][3]
+    double v[P][P][P];    // [x][z][y]
 #pragma unroll
     for (int a = 0; a < P; ++a)
 #pragma unroll
       for (int b = 0; b < P; ++b)
 #pragma unroll
         for (int q = 0; q < P; ++q)
-          v[a][q][b] = double(f[((long long)gz[q] * g.ny + gy[b]) * g.nx + gx[a]]);
+          v[a][q][b] = double(f[(((long long)gz[q] * g.ny + gy[b]) * g.nx + gx[a]) * 3]);
     // Neville along y (last index), then z, then x (src/pp_interp.f90:474-560)
 #pragma unroll
     for (int col = 1; col <= P - 1; ++col)
@@ -623,7 +624,7 @@ static hipError_t zinv_t(pdr_ctx* c, const Coef& cf, int fsel) {
 
 hipError_t launch_zinv(pdr_ctx* c, const Coef& cf) {
   prof_begin(c, K_ZINV);
-  hipError_t e = c->prec ? zinv_t<float>(c, cf, -1) : zinv_t<double>(c, cf, -1);
+  hipError_t e = c->spec ? launch2_zinv(c, cf) : c->prec ? zinv_t<float>(c, cf, -1) : zinv_t<double>(c, cf, -1);
   prof_end(c, K_ZINV);
   return e;
 }
@@ -643,7 +644,7 @@ static hipError_t yinv_t(pdr_ctx* c) {
 
 hipError_t launch_yinv(pdr_ctx* c) {
   prof_begin(c, K_YINV);
-  hipError_t e = c->prec ? yinv_t<float>(c) : yinv_t<double>(c);
+  hipError_t e = c->spec ? launch2_yinv(c) : c->prec ? yinv_t<float>(c) : yinv_t<double>(c);
   prof_end(c, K_YINV);
   return e;
 }
@@ -662,7 +663,7 @@ static hipError_t xpass_t(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
 
 hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
   prof_begin(c, K_XPASS);
-  hipError_t e = c->prec ? xpass_t<float>(c, cf, tr, cfl) : xpass_t<double>(c, cf, tr, cfl);
+  hipError_t e = c->spec ? launch2_xpass(c, cf, tr, cfl) : c->prec ? xpass_t<float>(c, cf, tr, cfl) : xpass_t<double>(c, cf, tr, cfl);
   prof_end(c, K_XPASS);
   return e;
 }
@@ -682,7 +683,7 @@ static hipError_t yfwd_t(pdr_ctx* c) {
 
 hipError_t launch_yfwd(pdr_ctx* c) {
   prof_begin(c, K_YFWD);
-  hipError_t e = c->prec ? yfwd_t<float>(c) : yfwd_t<double>(c);
+  hipError_t e = c->spec ? launch2_yfwd(c) : c->prec ? yfwd_t<float>(c) : yfwd_t<double>(c);
   prof_end(c, K_YFWD);
   return e;
 }
@@ -703,7 +704,7 @@ static hipError_t zfwd_t(pdr_ctx* c) {
 
 hipError_t launch_zfwd(pdr_ctx* c) {
   prof_begin(c, K_ZFWD);
-  hipError_t e = c->prec ? zfwd_t<float>(c) : zfwd_t<double>(c);
+  hipError_t e = c->spec ? launch2_zfwd(c) : c->prec ? zfwd_t<float>(c) : zfwd_t<double>(c);
   prof_end(c, K_ZFWD);
   return e;
 }

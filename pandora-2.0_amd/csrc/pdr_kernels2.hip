@@ -1,0 +1,534 @@
+// pdr_kernels2.hip -- length-specialised pass kernels (the default path for
+// grid sizes in PDR_SIZES).  Same work split and data layouts as pdr_kernels.hip
+// (pass 1 k_zinv ... pass 5 k_zfwd); see that file for the reference routines
+// each pass replaces.  Differences: compile-time transform length and tile
+// height, in-place LDS transforms, twiddles in LDS, 128-byte tile chunks on
+// 16-element-padded strides, XCD-aware tile order, several x rows per block.
+#include "pdr_fft2.hpp"
+#include "pdr_internal.hpp"
+
+namespace pdr {
+
+namespace {
+
+__device__ __forceinline__ double wvn2(int k, int kmax) {
+  return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1));
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> czero() {
+  return mk<T>(T(0), T(0));
+}
+
+// lines per tile: 128-byte chunks (16 FP32 / 8 FP64 complex) while the LDS of a
+// tile stays near 64 KB, so that two workgroups fit the 160 KB of a CU
+template <typename T>
+constexpr int tile_lines(int n) {
+  return sizeof(T) == 4 ? (n <= 512 ? 16 : (n <= 1024 ? 8 : 4)) : (n <= 512 ? 8 : (n <= 1024 ? 4 : 2));
+}
+template <typename T>
+constexpr int xrows(int n) {
+  return sizeof(T) == 4 ? (n <= 512 ? 4 : 2) : 2;
+}
+
+// ---------------------------------------------------------------------------
+// pass 1: vorticity + inverse z FFT, (i, j-tile) per block
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2)
+    k2_zinv(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
+            const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
+  constexpr int L = tile_lines<T>(N);
+  constexpr int LD = N + 1;
+  constexpr int K = N / 3;
+  constexpr int NKK = 2 * K + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* buf = tw + N;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = int(bid % unsigned(ntile));
+  const int i = int(bid / unsigned(ntile));
+  const int j0 = tile * L;
+  const int nkj = g.nkj;
+  load_tw<T, N>(tw, twg);
+  const double wi = double(i);
+  bool need[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+      const double* G = &cf.G[(c * 3 + n) * 3];
+      need[c][n] = (G[0] != 0.0) || (G[1] != 0.0) || (G[2] != 0.0);
+    }
+  const cx<T>* uu[3] = {u0, u1, u2};
+  for (int f = 0; f < 6; ++f) {
+    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+      const int line = idx / N;
+      const int z = idx - line * N;
+      const int j = j0 + line;
+      const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : -1);
+      cx<T> v = czero<T>();
+      if (j < nkj && k >= 0) {
+        const long long e = ((long long)i * nkj + j) * NKK + k;
+        if (f < 3) {
+          v = uu[f][e];
+        } else {
+          const int c = f - 3;
+          const double kv[3] = {wi, wvn2(j, g.kj), wvn2(k, K)};
+          cx<T> acc = czero<T>();
+#pragma unroll
+          for (int n = 0; n < 3; ++n) {
+            if (need[c][n]) {
+              const double* G = &cf.G[(c * 3 + n) * 3];
+              acc = add(acc, scl(uu[n][e], T(G[0] * kv[0] + G[1] * kv[1] + G[2] * kv[2])));
+            }
+          }
+          v = mk<T>(-acc.y, acc.x);
+        }
+      }
+      buf[line * LD + z] = v;
+    }
+    __syncthreads();
+    fft_ip<T, N, L, LD, +1>(buf, tw);
+    cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
+    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+      const int z = idx / L;
+      const int line = idx - z * L;
+      if (j0 + line < nkj) dst[(long long)z * g.ldj + line] = buf[line * LD + z];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 2: inverse y FFT, (z, i-tile) per block
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2)
+    k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
+            int ntile) {
+  constexpr int L = tile_lines<T>(N);
+  constexpr int LD = N + 1;
+  constexpr int K = N / 3;
+  constexpr int NKJ = 2 * K + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* buf = tw + N;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = int(bid % unsigned(ntile));
+  const int z = int(bid / unsigned(ntile));
+  const int i0 = tile * L;
+  const int nki = g.nki;
+  load_tw<T, N>(tw, twg);
+  for (int f = 0; f < 6; ++f) {
+    const cx<T>* src = w1 + (long long)f * g.m1 + ((long long)i0 * g.nz + z) * g.ldj;
+    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+      const int line = idx / N;
+      const int y = idx - line * N;
+      const int j = (y <= K) ? y : ((y >= N - K) ? y - (N - NKJ) : -1);
+      cx<T> v = czero<T>();
+      if (i0 + line < nki && j >= 0) v = src[(long long)line * g.nz * g.ldj + j];
+      buf[line * LD + y] = v;
+    }
+    __syncthreads();
+    fft_ip<T, N, L, LD, +1>(buf, tw);
+    cx<T>* dst = w2 + (long long)f * g.m2 + (long long)z * N * g.ldi + i0;
+    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+      const int y = idx / L;
+      const int line = idx - y * L;
+      if (i0 + line < nki) dst[(long long)y * g.ldi + line] = buf[line * LD + y];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 3: RB rows (z, y0..y0+RB-1) per block: c2r, product, r2c
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2)
+    k2_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
+             unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile) {
+  constexpr int RB = xrows<T>(N);
+  constexpr int LD = N + 1;
+  constexpr int KI = N / 3;
+  constexpr int NKI = KI + 1;
+  constexpr int IT = (RB * N + kBlock2 - 1) / kBlock2;  // product items per thread
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ T red[kBlock2];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* buf = tw + N;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = int(bid % unsigned(ntile));
+  const int z = int(bid / unsigned(ntile));
+  const int y0 = tile * RB;
+  const int ny = g.ny;
+  load_tw<T, N>(tw, twg);
+  const long long row0 = ((long long)z * ny + y0) * g.ldi;
+  for (int idx = threadIdx.x; idx < 3 * RB * N; idx += kBlock2) {
+    const int line = idx / N;
+    const int m = idx - line * N;
+    const int r = line / 3;
+    const int p = line - 3 * r;
+    cx<T> X1 = czero<T>(), X2 = czero<T>();
+    if (y0 + r < ny) {
+      const cx<T>* fa = w2 + (long long)(2 * p) * g.m2 + row0 + (long long)r * g.ldi;
+      const cx<T>* fb = fa + g.m2;
+      if (m <= KI) {
+        X1 = fa[m];
+        X2 = fb[m];
+        if (m == 0) {  // c2r uses only the real part of the DC term
+          X1.y = T(0);
+          X2.y = T(0);
+        }
+      } else if (m >= N - KI) {
+        X1 = fa[N - m];
+        X2 = fb[N - m];
+        X1.y = -X1.y;
+        X2.y = -X2.y;
+      }
+    }
+    buf[line * LD + m] = mk<T>(X1.x - X2.y, X1.y + X2.x);
+  }
+  __syncthreads();
+  fft_ip<T, N, 3 * RB, LD, +1>(buf, tw);
+  const T V[9] = {T(cf.V[0]), T(cf.V[1]), T(cf.V[2]), T(cf.V[3]), T(cf.V[4]),
+                  T(cf.V[5]), T(cf.V[6]), T(cf.V[7]), T(cf.V[8])};
+  T nl[IT][3];
+  T lmax = T(0);
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int idx = int(threadIdx.x) + kBlock2 * q;
+    if ((RB * N) % kBlock2 == 0 || idx < RB * N) {
+      const int r = idx / N;
+      const int x = idx - r * N;
+      const cx<T> a = buf[(3 * r) * LD + x], b = buf[(3 * r + 1) * LD + x], c = buf[(3 * r + 2) * LD + x];
+      const T u1 = a.x, u2 = a.y, u3 = b.x, o1 = b.y, o2 = c.x, o3 = c.y;
+      const T v1 = V[0] * u1 + V[1] * u2 + V[2] * u3;
+      const T v2 = V[3] * u1 + V[4] * u2 + V[5] * u3;
+      const T v3 = V[6] * u1 + V[7] * u2 + V[8] * u3;
+      nl[q][0] = o2 * v3 - o3 * v2;
+      nl[q][1] = o3 * v1 - o1 * v3;
+      nl[q][2] = o1 * v2 - o2 * v1;
+      if (want_tr && y0 + r < ny) {
+        T* d = ur + (((long long)z * ny + y0 + r) * N + x) * 3;
+        d[0] = u1;
+        d[1] = u2;
+        d[2] = u3;
+      }
+      if (want_cfl) {
+        const T s = fabs(u1) + fabs(u2) + fabs(u3);
+        lmax = s > lmax ? s : lmax;
+      }
+    }
+  }
+  __syncthreads();
+  // forward lines: r -> n1 + i n2 of row r; RB + r/2 -> n3 of rows (r, r+1)
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const int idx = int(threadIdx.x) + kBlock2 * q;
+    if ((RB * N) % kBlock2 == 0 || idx < RB * N) {
+      const int r = idx / N;
+      const int x = idx - r * N;
+      buf[r * LD + x] = mk<T>(nl[q][0], nl[q][1]);
+      cx<T>* p3 = &buf[(RB + r / 2) * LD + x];
+      if (r & 1)
+        p3->y = nl[q][2];
+      else
+        p3->x = nl[q][2];
+    }
+  }
+  __syncthreads();
+  fft_ip<T, N, 3 * RB / 2, LD, -1>(buf, tw);
+  for (int idx = threadIdx.x; idx < RB * NKI; idx += kBlock2) {
+    const int r = idx / NKI;
+    const int m = idx - r * NKI;
+    if (y0 + r >= ny) continue;
+    const int mr = (m == 0) ? 0 : N - m;
+    const cx<T> Z = buf[r * LD + m], Zr = buf[r * LD + mr];
+    const cx<T> W = buf[(RB + r / 2) * LD + m], Wr = buf[(RB + r / 2) * LD + mr];
+    cx<T>* d0 = w2 + row0 + (long long)r * g.ldi + m;
+    d0[0] = mk<T>(T(0.5) * (Z.x + Zr.x), T(0.5) * (Z.y - Zr.y));
+    d0[g.m2] = mk<T>(T(0.5) * (Z.y + Zr.y), T(-0.5) * (Z.x - Zr.x));
+    if (r & 1)
+      d0[2 * g.m2] = mk<T>(T(0.5) * (W.y + Wr.y), T(-0.5) * (W.x - Wr.x));
+    else
+      d0[2 * g.m2] = mk<T>(T(0.5) * (W.x + Wr.x), T(0.5) * (W.y - Wr.y));
+  }
+  if (want_cfl) {
+    red[threadIdx.x] = lmax;
+    __syncthreads();
+    for (int s = kBlock2 / 2; s > 0; s >>= 1) {
+      if (int(threadIdx.x) < s) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + s] ? red[threadIdx.x] : red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(umax, (unsigned long long)__double_as_longlong(double(red[0])));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 4: forward y FFT + truncation, (z, i-tile) per block
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2)
+    k2_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
+            int ntile) {
+  constexpr int L = tile_lines<T>(N);
+  constexpr int LD = N + 1;
+  constexpr int K = N / 3;
+  constexpr int NKJ = 2 * K + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* buf = tw + N;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = int(bid % unsigned(ntile));
+  const int z = int(bid / unsigned(ntile));
+  const int i0 = tile * L;
+  const int nki = g.nki;
+  load_tw<T, N>(tw, twg);
+  for (int f = 0; f < 3; ++f) {
+    const cx<T>* src = w2 + (long long)f * g.m2 + (long long)z * N * g.ldi + i0;
+    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+      const int y = idx / L;
+      const int line = idx - y * L;
+      buf[line * LD + y] = (i0 + line < nki) ? src[(long long)y * g.ldi + line] : czero<T>();
+    }
+    __syncthreads();
+    fft_ip<T, N, L, LD, -1>(buf, tw);
+    cx<T>* dst = w1 + (long long)f * g.m1 + ((long long)i0 * g.nz + z) * g.ldj;
+    for (int idx = threadIdx.x; idx < L * NKJ; idx += kBlock2) {
+      const int line = idx / NKJ;
+      const int j = idx - line * NKJ;
+      if (i0 + line < nki) {
+        const int yy = (j <= K) ? j : N - NKJ + j;
+        dst[(long long)line * g.nz * g.ldj + j] = buf[line * LD + yy];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass 5: forward z FFT, normalisation, truncation -> NL [c][i][j][k]
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2)
+    k2_zfwd(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ nl, const cx<T>* __restrict__ twg,
+            int ntile, T norm) {
+  constexpr int L = tile_lines<T>(N);
+  constexpr int LD = N + 1;
+  constexpr int K = N / 3;
+  constexpr int NKK = 2 * K + 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* buf = tw + N;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = int(bid % unsigned(ntile));
+  const int i = int(bid / unsigned(ntile));
+  const int j0 = tile * L;
+  const int nkj = g.nkj;
+  load_tw<T, N>(tw, twg);
+  for (int f = 0; f < 3; ++f) {
+    const cx<T>* src = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
+    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+      const int z = idx / L;
+      const int line = idx - z * L;
+      buf[line * LD + z] = (j0 + line < nkj) ? src[(long long)z * g.ldj + line] : czero<T>();
+    }
+    __syncthreads();
+    fft_ip<T, N, L, LD, -1>(buf, tw);
+    cx<T>* dst = nl + (long long)f * g.mw + ((long long)i * nkj + j0) * NKK;
+    const int nl_lines = (nkj - j0) < L ? (nkj - j0) : L;
+    for (int idx = threadIdx.x; idx < nl_lines * NKK; idx += kBlock2) {
+      const int line = idx / NKK;
+      const int k = idx - line * NKK;
+      const int zz = (k <= K) ? k : N - NKK + k;
+      dst[idx] = scl(buf[line * LD + zz], norm);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int N>
+size_t lds_lines() {
+  return size_t(N + tile_lines<T>(N) * (N + 1)) * sizeof(cx<T>);
+}
+template <typename T, int N>
+size_t lds_x() {
+  return size_t(N + 3 * xrows<T>(N) * (N + 1)) * sizeof(cx<T>);
+}
+
+hipError_t big_lds(const void* fn, size_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes));
+}
+
+template <typename T, int N>
+hipError_t zinv_n(pdr_ctx* c, const Coef& cf) {
+  const Geo& g = c->g;
+  constexpr int L = tile_lines<T>(N);
+  const int nt = (g.nkj + L - 1) / L;
+  const size_t lds = lds_lines<T, N>();
+  hipError_t e = big_lds((const void*)k2_zinv<T, N>, lds);
+  if (e != hipSuccess) return e;
+  k2_zinv<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
+      g, cf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1], (const cx<T>*)c->u[2], (cx<T>*)c->w1,
+      (const cx<T>*)c->tab.twz, nt);
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t yinv_n(pdr_ctx* c) {
+  const Geo& g = c->g;
+  constexpr int L = tile_lines<T>(N);
+  const int nt = (g.nki + L - 1) / L;
+  const size_t lds = lds_lines<T, N>();
+  hipError_t e = big_lds((const void*)k2_yinv<T, N>, lds);
+  if (e != hipSuccess) return e;
+  k2_yinv<T, N><<<dim3(unsigned(nt * g.nz)), kBlock2, lds, c->stream>>>(
+      g, (const cx<T>*)c->w1, (cx<T>*)c->w2, (const cx<T>*)c->tab.twy, nt);
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t xpass_n(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+  const Geo& g = c->g;
+  constexpr int RB = xrows<T>(N);
+  const int nt = (g.ny + RB - 1) / RB;
+  const size_t lds = lds_x<T, N>();
+  hipError_t e = big_lds((const void*)k2_xpass<T, N>, lds);
+  if (e != hipSuccess) return e;
+  k2_xpass<T, N><<<dim3(unsigned(nt * g.nz)), kBlock2, lds, c->stream>>>(
+      g, cf, (cx<T>*)c->w2, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0, c->umax, (const cx<T>*)c->tab.twx, nt);
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t yfwd_n(pdr_ctx* c) {
+  const Geo& g = c->g;
+  constexpr int L = tile_lines<T>(N);
+  const int nt = (g.nki + L - 1) / L;
+  const size_t lds = lds_lines<T, N>();
+  hipError_t e = big_lds((const void*)k2_yfwd<T, N>, lds);
+  if (e != hipSuccess) return e;
+  k2_yfwd<T, N><<<dim3(unsigned(nt * g.nz)), kBlock2, lds, c->stream>>>(
+      g, (const cx<T>*)c->w2, (cx<T>*)c->w1, (const cx<T>*)c->tab.twy, nt);
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t zfwd_n(pdr_ctx* c) {
+  const Geo& g = c->g;
+  constexpr int L = tile_lines<T>(N);
+  const int nt = (g.nkj + L - 1) / L;
+  const size_t lds = lds_lines<T, N>();
+  hipError_t e = big_lds((const void*)k2_zfwd<T, N>, lds);
+  if (e != hipSuccess) return e;
+  const double norm = 1.0 / (double(g.nx) * double(g.ny) * double(g.nz));
+  k2_zfwd<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
+      g, (const cx<T>*)c->w1, (cx<T>*)c->w2, (const cx<T>*)c->tab.twz, nt, T(norm));
+  return hipGetLastError();
+}
+
+}  // namespace
+
+#define PDR_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)
+
+bool spec_supported(int n) {
+  switch (n) {
+#define PDR_CASE(n) case n:
+    PDR_SIZES(PDR_CASE)
+#undef PDR_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+size_t spec_lds_need(int prec, int nx, int ny, int nz) {
+  size_t m = 0;
+  auto upd = [&](size_t v) { m = v > m ? v : m; };
+#define PDR_LDS(T, n, fn)            \
+  switch (n) {                       \
+    case 32: upd(fn<T, 32>()); break;     \
+    case 48: upd(fn<T, 48>()); break;     \
+    case 64: upd(fn<T, 64>()); break;     \
+    case 96: upd(fn<T, 96>()); break;     \
+    case 128: upd(fn<T, 128>()); break;   \
+    case 192: upd(fn<T, 192>()); break;   \
+    case 256: upd(fn<T, 256>()); break;   \
+    case 384: upd(fn<T, 384>()); break;   \
+    case 512: upd(fn<T, 512>()); break;   \
+    case 768: upd(fn<T, 768>()); break;   \
+    case 1024: upd(fn<T, 1024>()); break; \
+    case 1536: upd(fn<T, 1536>()); break; \
+    default: break;                  \
+  }
+  if (prec) {
+    PDR_LDS(float, nz, lds_lines)
+    PDR_LDS(float, ny, lds_lines)
+    PDR_LDS(float, nx, lds_x)
+  } else {
+    PDR_LDS(double, nz, lds_lines)
+    PDR_LDS(double, ny, lds_lines)
+    PDR_LDS(double, nx, lds_x)
+  }
+#undef PDR_LDS
+  return m;
+}
+
+#define PDR_DISPATCH(n, call_f, call_d) \
+  case n:                               \
+    return c->prec ? call_f : call_d;
+
+hipError_t launch2_zinv(pdr_ctx* c, const Coef& cf) {
+  switch (c->g.nz) {
+#define PDR_CASE(n) PDR_DISPATCH(n, (zinv_n<float, n>(c, cf)), (zinv_n<double, n>(c, cf)))
+    PDR_SIZES(PDR_CASE)
+#undef PDR_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch2_yinv(pdr_ctx* c) {
+  switch (c->g.ny) {
+#define PDR_CASE(n) PDR_DISPATCH(n, (yinv_n<float, n>(c)), (yinv_n<double, n>(c)))
+    PDR_SIZES(PDR_CASE)
+#undef PDR_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+  switch (c->g.nx) {
+#define PDR_CASE(n) PDR_DISPATCH(n, (xpass_n<float, n>(c, cf, tr, cfl)), (xpass_n<double, n>(c, cf, tr, cfl)))
+    PDR_SIZES(PDR_CASE)
+#undef PDR_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch2_yfwd(pdr_ctx* c) {
+  switch (c->g.ny) {
+#define PDR_CASE(n) PDR_DISPATCH(n, (yfwd_n<float, n>(c)), (yfwd_n<double, n>(c)))
+    PDR_SIZES(PDR_CASE)
+#undef PDR_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch2_zfwd(pdr_ctx* c) {
+  switch (c->g.nz) {
+#define PDR_CASE(n) PDR_DISPATCH(n, (zfwd_n<float, n>(c)), (zfwd_n<double, n>(c)))
+    PDR_SIZES(PDR_CASE)
+#undef PDR_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace pdr

@@ -322,3 +322,39 @@ def test_full_size_512_shear_fp32_with_tracers():
     k0 = 0.5 * (e0[3] + e0[4] + e0[5])
     k1 = 0.5 * (e1[3] + e1[4] + e1[5])
     assert 0.9 * k0 < k1 < 1.1 * k0
+
+
+@pytest.mark.parametrize("f_type", [1, 2])
+def test_generic_kernel_path_parity(oracle_mod, monkeypatch, f_type):
+    """The runtime-radix kernels (used for lengths without a specialised kernel)
+    stay at parity: forced with PDR_GENERIC_KERNELS=1."""
+    note(f"generic kernels FP64 N=48 f_type={f_type}, 6 steps")
+    monkeypatch.setenv("PDR_GENERIC_KERNELS", "1")
+    n = 48
+    kw = dict(f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=2, nu=0.005) if f_type == 2 else dict(nu=0.01)
+    o, g = _pair(oracle_mod, n, **kw)
+    u = pa.init_field_host(n, 300 + f_type, 2.0 / 3.0, 4.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    to = tg = 0.0
+    for s in range(1, 7):
+        to = o.step(s, to)
+        tg = g.rk3_calc(s, tg)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert rel(a, b) < 1e-12
+
+
+def test_radix5_length_uses_generic_path(oracle_mod):
+    """N = 40 = 2^3 * 5 has no specialised kernel: the generic path runs it."""
+    note("N=40 (radix 5) isotropic FP64, 4 steps")
+    n = 40
+    o, g = _pair(oracle_mod, n, nu=0.01)
+    u = pa.init_field_host(n, 41, 2.0 / 3.0, 4.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    to = tg = 0.0
+    for s in range(1, 5):
+        to = o.step(s, to)
+        tg = g.rk3_calc(s, tg)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert rel(a, b) < 1e-12
