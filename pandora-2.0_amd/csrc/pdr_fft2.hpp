@@ -50,12 +50,39 @@ constexpr int plan_npass(int n) {
   return p;
 }
 
+// LDS layout of a tile of lines: line l, element o at l * LD + o + (o >> PS),
+// i.e. one pad element per 256 bytes inside a line (PS = log2(256 / sizeof(cx<T>)))
+// and a line stride LD = N + (N >> PS) + 1.  Without the pad the stride-R writes
+// of the first Stockham passes (NS = 1, 8) fall on a few banks (8-way conflicts
+// for radix 8); with it every access pattern of these kernels stays within ~1.5x
+// of the conflict-free rate on 32- and 64-bank LDS, and the odd line stride
+// keeps the line-fastest (transposing) accesses spread as well.
+template <typename T>
+constexpr int lds_pad_shift() {
+  return sizeof(T) == 4 ? 5 : 4;
+}
+template <typename T>
+constexpr int lds_ld(int n) {
+  return n + (n >> lds_pad_shift<T>()) + 1;
+}
+template <typename T>
+__device__ __forceinline__ int lds_off(int o) {
+  return o + (o >> lds_pad_shift<T>());
+}
+
+// One radix-R Stockham pass over LINES lines of length N held in LDS (line
+// stride LD, padded layout above).  The per-element offsets o(r) = B + r*S are
+// turned into a base plus compile-time constants wherever the pad positions
+// allow it (all passes of power-of-two lengths), which keeps the address
+// registers of the butterflies few.
 template <typename T, int N, int LINES, int LD, int R, int NS, int SIGN>
 __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
   constexpr int NBL = N / R;
   constexpr int TOTAL = NBL * LINES;
   constexpr int NB = (TOTAL + kBlock2 - 1) / kBlock2;
   constexpr int TWS = N / (NS * R);
+  constexpr int PS = lds_pad_shift<T>();
+  constexpr int P2 = 1 << PS;
   cx<T> v[NB][R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
@@ -63,9 +90,14 @@ __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __
     if (TOTAL % kBlock2 == 0 || t < TOTAL) {
       const int line = t / NBL;
       const int j = t - line * NBL;
-      const cx<T>* s = buf + line * LD + j;
+      if constexpr (NBL % P2 == 0) {
+        const cx<T>* s = buf + line * LD + lds_off<T>(j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[b][r] = s[r * NBL];
+        for (int r = 0; r < R; ++r) v[b][r] = s[r * (NBL + (NBL >> PS))];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = buf[line * LD + lds_off<T>(j + r * NBL)];
+      }
     }
   }
   __syncthreads();
@@ -85,9 +117,23 @@ __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __
         }
       }
       Dft<T, R, SIGN>::run(v[b]);
-      cx<T>* d = buf + line * LD + (j - k) * R + k;
+      const int A = (j - k) * R;  // a multiple of NS * R
+      if constexpr (NS % P2 == 0) {
+        cx<T>* d = buf + line * LD + lds_off<T>(A + k);
 #pragma unroll
-      for (int r = 0; r < R; ++r) d[r * NS] = v[b][r];
+        for (int r = 0; r < R; ++r) d[r * (NS + (NS >> PS))] = v[b][r];
+      } else if constexpr (P2 % NS == 0 && (NS * R) % P2 == 0) {
+        cx<T>* d = buf + line * LD + lds_off<T>(A) + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r * NS + ((r * NS) >> PS)] = v[b][r];
+      } else if constexpr (P2 % (NS * R) == 0) {
+        cx<T>* d = buf + line * LD + lds_off<T>(A) + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) d[r * NS] = v[b][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[line * LD + lds_off<T>(A + k + r * NS)] = v[b][r];
+      }
     }
   }
   __syncthreads();

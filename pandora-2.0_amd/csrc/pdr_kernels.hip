@@ -389,17 +389,24 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // ---------------------------------------------------------------------------
-// Tracers: molecule search + Neville (y, z, x) + RK3 + wrap, one thread each.
-// Field ur is [c][z][y][x]; positions are FP64 (3, np).
+// Tracers: molecule search + Neville (y, z, x) + RK3 + wrap.  Four lanes per
+// tracer: lane c < 3 gathers and interpolates velocity component c, lane 3
+// only takes part in the exchange; every lane of the quad then applies the
+// same RK3 update and lane c stores component c.  The molecule is walked one
+// z plane at a time (P x P points of one component, Neville along y for each
+// x column), which keeps the registers per lane low and the loads of a plane
+// independent.  Field ur is [z][y][x][3]; positions are FP64 (3, np).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ long long pmod(long long a, long long n) {
   long long r = a % n;
   return r < 0 ? r + n : r;
 }
 
+// One component of the interpolated velocity at x (f = ur + c, point stride 3).
+// The operation sequence per (x, z) column, per x column and for the final x
+// line is that of p_InterpNeville (src/pp_interp.f90:474-560).
 template <typename T, int P>
-__device__ void interp_p(const Geo& g, const TracerFrame& tf, const T* __restrict__ ur,
-                         const double* xp, double* uf) {
+__device__ double interp_c(const Geo& g, const TracerFrame& tf, const T* __restrict__ f, const double* xp) {
   const int nhalo = P / 2;
   // g_DomainInitHalo (src/pg_domain.f90:360-384) and p_InterpFindMolecule (src/pp_interp.f90:406-467)
   double lxh = double(nhalo) * tf.dx, lyh = double(nhalo) * tf.dy, lzh = double(nhalo) * tf.dz;
@@ -419,75 +426,85 @@ __device__ void interp_p(const Geo& g, const TracerFrame& tf, const T* __restric
     iy[m] = double(j0 + m) * tf.dy;
     iz[m] = double(k0 + m) * tf.dz;
   }
-  int gx[P], gy[P], gz[P];
+  int gx3[P];
 #pragma unroll
-  for (int m = 0; m < P; ++m) {
-    gx[m] = int(pmod(i0 + m, g.nx));
-    gy[m] = int(pmod(j0 + m, g.ny));
-    gz[m] = int(pmod(k0 + m, g.nz));
+  for (int m = 0; m < P; ++m) gx3[m] = 3 * int(pmod(i0 + m, g.nx));
+  double vz[P][P];  // [x][z] after the y pass
+#pragma unroll
+  for (int q = 0; q < P; ++q) {
+    const long long zoff = pmod(k0 + q, g.nz) * g.ny;
+    T w[P][P];  // [x][y]
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+      const T* row = f + (zoff + pmod(j0 + b, g.ny)) * g.nx * 3;
+#pragma unroll
+      for (int a = 0; a < P; ++a) w[a][b] = row[gx3[a]];
+    }
+#pragma unroll
+    for (int a = 0; a < P; ++a) {
+      double v[P];
+#pragma unroll
+      for (int b = 0; b < P; ++b) v[b] = double(w[a][b]);
+#pragma unroll
+      for (int col = 1; col <= P - 1; ++col)
+#pragma unroll
+        for (int nv = 1; nv <= P - col; ++nv) {
+          const double d1 = xp[1] - iy[nv + col - 1];
+          const double d2 = iy[nv - 1] - xp[1];
+          const double d3 = iy[nv - 1] - iy[nv + col - 1];
+          v[nv - 1] = ((d1 * v[nv - 1]) + (d2 * v[nv])) / d3;
+        }
+      vz[a][q] = v[0];
+    }
   }
-  for (int c = 0; c < 3; ++c) {
-    const T* f = ur + c;  // field is [z][y][x][3]
-    double v[P][P][P];    // [x][z][y]
 #pragma unroll
-    for (int a = 0; a < P; ++a)
+  for (int col = 1; col <= P - 1; ++col)
 #pragma unroll
-      for (int b = 0; b < P; ++b)
+    for (int nv = 1; nv <= P - col; ++nv) {
+      const double d1 = xp[2] - iz[nv + col - 1];
+      const double d2 = iz[nv - 1] - xp[2];
+      const double d3 = iz[nv - 1] - iz[nv + col - 1];
 #pragma unroll
-        for (int q = 0; q < P; ++q)
-          v[a][q][b] = double(f[(((long long)gz[q] * g.ny + gy[b]) * g.nx + gx[a]) * 3]);
-    // Neville along y (last index), then z, then x (src/pp_interp.f90:474-560)
+      for (int a = 0; a < P; ++a) vz[a][nv - 1] = ((d1 * vz[a][nv - 1]) + (d2 * vz[a][nv])) / d3;
+    }
+  double vx[P];
 #pragma unroll
-    for (int col = 1; col <= P - 1; ++col)
+  for (int a = 0; a < P; ++a) vx[a] = vz[a][0];
 #pragma unroll
-      for (int nv = 1; nv <= P - col; ++nv) {
-        const double d1 = xp[1] - iy[nv + col - 1];
-        const double d2 = iy[nv - 1] - xp[1];
-        const double d3 = iy[nv - 1] - iy[nv + col - 1];
+  for (int col = 1; col <= P - 1; ++col)
 #pragma unroll
-        for (int q = 0; q < P; ++q)
-#pragma unroll
-          for (int a = 0; a < P; ++a) v[a][q][nv - 1] = ((d1 * v[a][q][nv - 1]) + (d2 * v[a][q][nv])) / d3;
-      }
-#pragma unroll
-    for (int col = 1; col <= P - 1; ++col)
-#pragma unroll
-      for (int nv = 1; nv <= P - col; ++nv) {
-        const double d1 = xp[2] - iz[nv + col - 1];
-        const double d2 = iz[nv - 1] - xp[2];
-        const double d3 = iz[nv - 1] - iz[nv + col - 1];
-#pragma unroll
-        for (int a = 0; a < P; ++a) v[a][nv - 1][0] = ((d1 * v[a][nv - 1][0]) + (d2 * v[a][nv][0])) / d3;
-      }
-#pragma unroll
-    for (int col = 1; col <= P - 1; ++col)
-#pragma unroll
-      for (int nv = 1; nv <= P - col; ++nv) {
-        const double d1 = xp[0] - ix[nv + col - 1];
-        const double d2 = ix[nv - 1] - xp[0];
-        const double d3 = ix[nv - 1] - ix[nv + col - 1];
-        v[nv - 1][0][0] = ((d1 * v[nv - 1][0][0]) + (d2 * v[nv][0][0])) / d3;
-      }
-    uf[c] = v[0][0][0];
-  }
+    for (int nv = 1; nv <= P - col; ++nv) {
+      const double d1 = xp[0] - ix[nv + col - 1];
+      const double d2 = ix[nv - 1] - xp[0];
+      const double d3 = ix[nv - 1] - ix[nv + col - 1];
+      vx[nv - 1] = ((d1 * vx[nv - 1]) + (d2 * vx[nv])) / d3;
+    }
+  return vx[0];
 }
 
 template <typename T, int P>
 __global__ void __launch_bounds__(kThreads)
     k_tracers(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
               double* __restrict__ uprk, long long np, double beta, double gdt) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= np) return;
-  double x[3] = {xpa[3 * q], xpa[3 * q + 1], xpa[3 * q + 2]};
-  double r[3] = {uprk[3 * q], uprk[3 * q + 1], uprk[3 * q + 2]};
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long q = gt >> 2;
+  const int c = int(gt & 3);
+  const bool live = q < np;
+  const long long qq = live ? q : np - 1;  // np >= 1: dead lanes read a valid tracer
+  double x[3] = {xpa[3 * qq], xpa[3 * qq + 1], xpa[3 * qq + 2]};
+  const double mine = (c < 3) ? interp_c<T, P>(g, tf, ur + c, x) : 0.0;
+  double r[3] = {uprk[3 * qq], uprk[3 * qq + 1], uprk[3 * qq + 2]};
+  const int lane0 = int(threadIdx.x & 63u) & ~3;  // first lane of the quad within the wave
   double uf[3];
-  interp_p<T, P>(g, tf, ur, x, uf);
+  uf[0] = __shfl(mine, lane0 + 0);
+  uf[1] = __shfl(mine, lane0 + 1);
+  uf[2] = __shfl(mine, lane0 + 2);
   if (!tf.bf) {
     // p_RK3SolveParticle tracer branch (src/pp_rk3.f90:103-134)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      r[c] = (beta * r[c]) + uf[c];
-      x[c] = x[c] + (gdt * r[c]);
+    for (int m = 0; m < 3; ++m) {
+      r[m] = (beta * r[m]) + uf[m];
+      x[m] = x[m] + (gdt * r[m]);
     }
   } else {
     // p_RK3SolveParticleBF tracer branch (src/pp_rk3.f90:223-288)
@@ -500,9 +517,9 @@ __global__ void __launch_bounds__(kThreads)
         ul[m] = ul[m] + tf.binv[3 * m + n] * uf[n];
       }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      r[c] = (beta * r[c]) + ul[c];
-      xl[c] = xl[c] + (gdt * r[c]);
+    for (int m = 0; m < 3; ++m) {
+      r[m] = (beta * r[m]) + ul[m];
+      xl[m] = xl[m] + (gdt * r[m]);
     }
 #pragma unroll
     for (int m = 0; m < 3; ++m)
@@ -522,12 +539,10 @@ __global__ void __launch_bounds__(kThreads)
   if (x[1] >= x2max) x[1] = x[1] - tf.ly;
   if (x[2] < x3min) x[2] = x[2] + tf.lz;
   if (x[2] > x3max) x[2] = x[2] - tf.lz;
-  xpa[3 * q] = x[0];
-  xpa[3 * q + 1] = x[1];
-  xpa[3 * q + 2] = x[2];
-  uprk[3 * q] = r[0];
-  uprk[3 * q + 1] = r[1];
-  uprk[3 * q + 2] = r[2];
+  if (live && c < 3) {
+    xpa[3 * q + c] = (c == 0) ? x[0] : ((c == 1) ? x[1] : x[2]);
+    uprk[3 * q + c] = (c == 0) ? r[0] : ((c == 1) ? r[1] : r[2]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -728,7 +743,7 @@ hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
 
 template <typename T>
 static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt) {
-  const long long nb = (c->np + kThreads - 1) / kThreads;
+  const long long nb = (4 * c->np + kThreads - 1) / kThreads;  // four lanes per tracer
   const T* ur = (const T*)c->ur;
   switch (tf.order) {
     case 2:

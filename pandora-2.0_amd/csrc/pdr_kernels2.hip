@@ -31,17 +31,29 @@ constexpr int xrows(int n) {
   return sizeof(T) == 4 ? (n <= 512 ? 4 : 2) : 2;
 }
 
+// Line kernels issue all global loads of a field (QE elements per thread)
+// before the LDS stores of that field, so that one HBM latency is exposed per
+// field rather than one per element.  (Loading field f+1 during the transform
+// of field f was measured slower: it needs more registers than the two waves
+// per SIMD allow and spills.)
+
+// The LDS of a line tile (~64-78 KB) admits two workgroups per CU, i.e. two
+// waves per SIMD; tell the register allocator so that it stays within the
+// 256 registers per lane that two waves leave.
+#define PDR_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
+
 // ---------------------------------------------------------------------------
 // pass 1: vorticity + inverse z FFT, (i, j-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2)
+__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
     k2_zinv(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
             const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
   constexpr int L = tile_lines<T>(N);
-  constexpr int LD = N + 1;
+  constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKK = 2 * K + 1;
+  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -60,33 +72,74 @@ __global__ void __launch_bounds__(kBlock2)
       const double* G = &cf.G[(c * 3 + n) * 3];
       need[c][n] = (G[0] != 0.0) || (G[1] != 0.0) || (G[2] != 0.0);
     }
-  const cx<T>* uu[3] = {u0, u1, u2};
-  for (int f = 0; f < 6; ++f) {
-    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+  const long long base = ((long long)i * nkj + j0) * NKK;
+  const cx<T>* uu[3] = {u0 + base, u1 + base, u2 + base};
+  // velocity fields: QE elements per thread, loaded in one batch
+  cx<T> pre[QE];
+  auto load_u = [&](int f) {
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
       const int line = idx / N;
       const int z = idx - line * N;
-      const int j = j0 + line;
       const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : -1);
-      cx<T> v = czero<T>();
-      if (j < nkj && k >= 0) {
-        const long long e = ((long long)i * nkj + j) * NKK + k;
-        if (f < 3) {
-          v = uu[f][e];
-        } else {
-          const int c = f - 3;
-          const double kv[3] = {wi, wvn2(j, g.kj), wvn2(k, K)};
+      pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && j0 + line < nkj && k >= 0) ? uu[f][line * NKK + k]
+                                                                                    : czero<T>();
+    }
+  };
+  // vorticity fields need two or three components per element: computed and
+  // stored to LDS in chunks of VC elements to bound the registers in flight
+  constexpr int VC0 = sizeof(T) == 4 ? 8 : 4;
+  constexpr int VC = QE < VC0 ? QE : VC0;
+  auto fill_w = [&](int c) {
+#pragma unroll 1
+    for (int q0 = 0; q0 < QE; q0 += VC) {
+      cx<T> a[VC][3];
+#pragma unroll
+      for (int qq = 0; qq < VC; ++qq) {
+        const int idx = int(threadIdx.x) + kBlock2 * (q0 + qq);
+        const int line = idx / N;
+        const int z = idx - line * N;
+        const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : -1);
+        const bool ok = (QE % VC == 0 || q0 + qq < QE) && (L * N % kBlock2 == 0 || idx < L * N) &&
+                        j0 + line < nkj && k >= 0;
+#pragma unroll
+        for (int n = 0; n < 3; ++n) a[qq][n] = (ok && need[c][n]) ? uu[n][line * NKK + k] : czero<T>();
+      }
+#pragma unroll
+      for (int qq = 0; qq < VC; ++qq) {
+        const int idx = int(threadIdx.x) + kBlock2 * (q0 + qq);
+        if ((QE % VC == 0 || q0 + qq < QE) && (L * N % kBlock2 == 0 || idx < L * N)) {
+          const int line = idx / N;
+          const int z = idx - line * N;
+          const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : 0);
+          const double kv[3] = {wi, wvn2(j0 + line, g.kj), wvn2(k, K)};
           cx<T> acc = czero<T>();
 #pragma unroll
           for (int n = 0; n < 3; ++n) {
             if (need[c][n]) {
               const double* G = &cf.G[(c * 3 + n) * 3];
-              acc = add(acc, scl(uu[n][e], T(G[0] * kv[0] + G[1] * kv[1] + G[2] * kv[2])));
+              acc = add(acc, scl(a[qq][n], T(G[0] * kv[0] + G[1] * kv[1] + G[2] * kv[2])));
             }
           }
-          v = mk<T>(-acc.y, acc.x);
+          buf[line * LD + lds_off<T>(z)] = mk<T>(-acc.y, acc.x);
         }
       }
-      buf[line * LD + z] = v;
+    }
+  };
+  for (int f = 0; f < 6; ++f) {
+    if (f < 3) {
+      load_u(f);
+#pragma unroll
+      for (int q = 0; q < QE; ++q) {
+        const int idx = int(threadIdx.x) + kBlock2 * q;
+        if (L * N % kBlock2 == 0 || idx < L * N) {
+          const int line = idx / N;
+          buf[line * LD + lds_off<T>(idx - line * N)] = pre[q];
+        }
+      }
+    } else {
+      fill_w(f - 3);
     }
     __syncthreads();
     fft_ip<T, N, L, LD, +1>(buf, tw);
@@ -94,7 +147,7 @@ __global__ void __launch_bounds__(kBlock2)
     for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
       const int z = idx / L;
       const int line = idx - z * L;
-      if (j0 + line < nkj) dst[(long long)z * g.ldj + line] = buf[line * LD + z];
+      if (j0 + line < nkj) dst[z * g.ldj + line] = buf[line * LD + lds_off<T>(z)];
     }
     __syncthreads();
   }
@@ -104,13 +157,14 @@ __global__ void __launch_bounds__(kBlock2)
 // pass 2: inverse y FFT, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2)
+__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
     k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
             int ntile) {
   constexpr int L = tile_lines<T>(N);
-  constexpr int LD = N + 1;
+  constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKJ = 2 * K + 1;
+  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -119,16 +173,31 @@ __global__ void __launch_bounds__(kBlock2)
   const int z = int(bid / unsigned(ntile));
   const int i0 = tile * L;
   const int nki = g.nki;
+  const int lstride = g.nz * g.ldj;
   load_tw<T, N>(tw, twg);
-  for (int f = 0; f < 6; ++f) {
+  cx<T> pre[QE];
+  auto load = [&](int f) {
     const cx<T>* src = w1 + (long long)f * g.m1 + ((long long)i0 * g.nz + z) * g.ldj;
-    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
       const int line = idx / N;
       const int y = idx - line * N;
       const int j = (y <= K) ? y : ((y >= N - K) ? y - (N - NKJ) : -1);
       cx<T> v = czero<T>();
-      if (i0 + line < nki && j >= 0) v = src[(long long)line * g.nz * g.ldj + j];
-      buf[line * LD + y] = v;
+      if ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki && j >= 0) v = src[line * lstride + j];
+      pre[q] = v;
+    }
+  };
+  for (int f = 0; f < 6; ++f) {
+    load(f);
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
+      if (L * N % kBlock2 == 0 || idx < L * N) {
+        const int line = idx / N;
+        buf[line * LD + lds_off<T>(idx - line * N)] = pre[q];
+      }
     }
     __syncthreads();
     fft_ip<T, N, L, LD, +1>(buf, tw);
@@ -136,7 +205,7 @@ __global__ void __launch_bounds__(kBlock2)
     for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
       const int y = idx / L;
       const int line = idx - y * L;
-      if (i0 + line < nki) dst[(long long)y * g.ldi + line] = buf[line * LD + y];
+      if (i0 + line < nki) dst[y * g.ldi + line] = buf[line * LD + lds_off<T>(y)];
     }
     __syncthreads();
   }
@@ -146,11 +215,11 @@ __global__ void __launch_bounds__(kBlock2)
 // pass 3: RB rows (z, y0..y0+RB-1) per block: c2r, product, r2c
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2)
+__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
     k2_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
              unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile) {
   constexpr int RB = xrows<T>(N);
-  constexpr int LD = N + 1;
+  constexpr int LD = lds_ld<T>(N);
   constexpr int KI = N / 3;
   constexpr int NKI = KI + 1;
   constexpr int IT = (RB * N + kBlock2 - 1) / kBlock2;  // product items per thread
@@ -164,31 +233,51 @@ __global__ void __launch_bounds__(kBlock2)
   const int y0 = tile * RB;
   const int ny = g.ny;
   load_tw<T, N>(tw, twg);
-  const long long row0 = ((long long)z * ny + y0) * g.ldi;
-  for (int idx = threadIdx.x; idx < 3 * RB * N; idx += kBlock2) {
-    const int line = idx / N;
-    const int m = idx - line * N;
-    const int r = line / 3;
-    const int p = line - 3 * r;
-    cx<T> X1 = czero<T>(), X2 = czero<T>();
-    if (y0 + r < ny) {
-      const cx<T>* fa = w2 + (long long)(2 * p) * g.m2 + row0 + (long long)r * g.ldi;
-      const cx<T>* fb = fa + g.m2;
-      if (m <= KI) {
-        X1 = fa[m];
-        X2 = fb[m];
-        if (m == 0) {  // c2r uses only the real part of the DC term
-          X1.y = T(0);
-          X2.y = T(0);
+  cx<T>* rowp = w2 + ((long long)z * ny + y0) * g.ldi;
+  // fill: the 3 * RB lines pack pairs of real inputs (p = 0: u1 + i u2,
+  // 1: u3 + i w1, 2: w2 + i w3); all loads of a thread are issued before its
+  // LDS stores
+  constexpr int QX = (3 * RB * N + kBlock2 - 1) / kBlock2;
+  constexpr int QC0 = sizeof(T) == 4 ? 24 : 12;  // elements per load batch
+  constexpr int QC = QX < QC0 ? QX : QC0;
+#pragma unroll 1
+  for (int q0 = 0; q0 < QX; q0 += QC) {
+    cx<T> X1[QC], X2[QC];
+#pragma unroll
+    for (int qq = 0; qq < QC; ++qq) {
+      const int idx = int(threadIdx.x) + kBlock2 * (q0 + qq);
+      const int line = idx / N;
+      const int m = idx - line * N;
+      const int r = line / 3;
+      const int p = line - 3 * r;
+      X1[qq] = czero<T>();
+      X2[qq] = czero<T>();
+      if ((QX % QC == 0 || q0 + qq < QX) && (3 * RB * N % kBlock2 == 0 || idx < 3 * RB * N) && y0 + r < ny) {
+        const cx<T>* fa = rowp + (long long)(2 * p) * g.m2 + r * g.ldi;
+        const cx<T>* fb = fa + g.m2;
+        if (m <= KI) {
+          X1[qq] = fa[m];
+          X2[qq] = fb[m];
+          if (m == 0) {  // c2r uses only the real part of the DC term
+            X1[qq].y = T(0);
+            X2[qq].y = T(0);
+          }
+        } else if (m >= N - KI) {
+          X1[qq] = fa[N - m];
+          X2[qq] = fb[N - m];
+          X1[qq].y = -X1[qq].y;
+          X2[qq].y = -X2[qq].y;
         }
-      } else if (m >= N - KI) {
-        X1 = fa[N - m];
-        X2 = fb[N - m];
-        X1.y = -X1.y;
-        X2.y = -X2.y;
       }
     }
-    buf[line * LD + m] = mk<T>(X1.x - X2.y, X1.y + X2.x);
+#pragma unroll
+    for (int qq = 0; qq < QC; ++qq) {
+      const int idx = int(threadIdx.x) + kBlock2 * (q0 + qq);
+      if ((QX % QC == 0 || q0 + qq < QX) && (3 * RB * N % kBlock2 == 0 || idx < 3 * RB * N)) {
+        const int line = idx / N;
+        buf[line * LD + lds_off<T>(idx - line * N)] = mk<T>(X1[qq].x - X2[qq].y, X1[qq].y + X2[qq].x);
+      }
+    }
   }
   __syncthreads();
   fft_ip<T, N, 3 * RB, LD, +1>(buf, tw);
@@ -196,13 +285,15 @@ __global__ void __launch_bounds__(kBlock2)
                   T(cf.V[5]), T(cf.V[6]), T(cf.V[7]), T(cf.V[8])};
   T nl[IT][3];
   T lmax = T(0);
+  T* urow = ur + ((long long)z * ny + y0) * N * 3;
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
     const int idx = int(threadIdx.x) + kBlock2 * q;
     if ((RB * N) % kBlock2 == 0 || idx < RB * N) {
       const int r = idx / N;
       const int x = idx - r * N;
-      const cx<T> a = buf[(3 * r) * LD + x], b = buf[(3 * r + 1) * LD + x], c = buf[(3 * r + 2) * LD + x];
+      const cx<T> a = buf[(3 * r) * LD + lds_off<T>(x)], b = buf[(3 * r + 1) * LD + lds_off<T>(x)],
+                  c = buf[(3 * r + 2) * LD + lds_off<T>(x)];
       const T u1 = a.x, u2 = a.y, u3 = b.x, o1 = b.y, o2 = c.x, o3 = c.y;
       const T v1 = V[0] * u1 + V[1] * u2 + V[2] * u3;
       const T v2 = V[3] * u1 + V[4] * u2 + V[5] * u3;
@@ -211,7 +302,7 @@ __global__ void __launch_bounds__(kBlock2)
       nl[q][1] = o3 * v1 - o1 * v3;
       nl[q][2] = o1 * v2 - o2 * v1;
       if (want_tr && y0 + r < ny) {
-        T* d = ur + (((long long)z * ny + y0 + r) * N + x) * 3;
+        T* d = urow + idx * 3;
         d[0] = u1;
         d[1] = u2;
         d[2] = u3;
@@ -230,8 +321,8 @@ __global__ void __launch_bounds__(kBlock2)
     if ((RB * N) % kBlock2 == 0 || idx < RB * N) {
       const int r = idx / N;
       const int x = idx - r * N;
-      buf[r * LD + x] = mk<T>(nl[q][0], nl[q][1]);
-      cx<T>* p3 = &buf[(RB + r / 2) * LD + x];
+      buf[r * LD + lds_off<T>(x)] = mk<T>(nl[q][0], nl[q][1]);
+      cx<T>* p3 = &buf[(RB + r / 2) * LD + lds_off<T>(x)];
       if (r & 1)
         p3->y = nl[q][2];
       else
@@ -245,9 +336,9 @@ __global__ void __launch_bounds__(kBlock2)
     const int m = idx - r * NKI;
     if (y0 + r >= ny) continue;
     const int mr = (m == 0) ? 0 : N - m;
-    const cx<T> Z = buf[r * LD + m], Zr = buf[r * LD + mr];
-    const cx<T> W = buf[(RB + r / 2) * LD + m], Wr = buf[(RB + r / 2) * LD + mr];
-    cx<T>* d0 = w2 + row0 + (long long)r * g.ldi + m;
+    const cx<T> Z = buf[r * LD + lds_off<T>(m)], Zr = buf[r * LD + lds_off<T>(mr)];
+    const cx<T> W = buf[(RB + r / 2) * LD + lds_off<T>(m)], Wr = buf[(RB + r / 2) * LD + lds_off<T>(mr)];
+    cx<T>* d0 = rowp + r * g.ldi + m;
     d0[0] = mk<T>(T(0.5) * (Z.x + Zr.x), T(0.5) * (Z.y - Zr.y));
     d0[g.m2] = mk<T>(T(0.5) * (Z.y + Zr.y), T(-0.5) * (Z.x - Zr.x));
     if (r & 1)
@@ -270,13 +361,14 @@ __global__ void __launch_bounds__(kBlock2)
 // pass 4: forward y FFT + truncation, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2)
+__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
     k2_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
             int ntile) {
   constexpr int L = tile_lines<T>(N);
-  constexpr int LD = N + 1;
+  constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKJ = 2 * K + 1;
+  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -285,13 +377,28 @@ __global__ void __launch_bounds__(kBlock2)
   const int z = int(bid / unsigned(ntile));
   const int i0 = tile * L;
   const int nki = g.nki;
+  const int lstride = g.nz * g.ldj;
   load_tw<T, N>(tw, twg);
-  for (int f = 0; f < 3; ++f) {
+  cx<T> pre[QE];
+  auto load = [&](int f) {
     const cx<T>* src = w2 + (long long)f * g.m2 + (long long)z * N * g.ldi + i0;
-    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
       const int y = idx / L;
       const int line = idx - y * L;
-      buf[line * LD + y] = (i0 + line < nki) ? src[(long long)y * g.ldi + line] : czero<T>();
+      pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki) ? src[y * g.ldi + line] : czero<T>();
+    }
+  };
+  for (int f = 0; f < 3; ++f) {
+    load(f);
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
+      if (L * N % kBlock2 == 0 || idx < L * N) {
+        const int y = idx / L;
+        buf[(idx - y * L) * LD + lds_off<T>(y)] = pre[q];
+      }
     }
     __syncthreads();
     fft_ip<T, N, L, LD, -1>(buf, tw);
@@ -301,7 +408,7 @@ __global__ void __launch_bounds__(kBlock2)
       const int j = idx - line * NKJ;
       if (i0 + line < nki) {
         const int yy = (j <= K) ? j : N - NKJ + j;
-        dst[(long long)line * g.nz * g.ldj + j] = buf[line * LD + yy];
+        dst[line * lstride + j] = buf[line * LD + lds_off<T>(yy)];
       }
     }
     __syncthreads();
@@ -312,13 +419,14 @@ __global__ void __launch_bounds__(kBlock2)
 // pass 5: forward z FFT, normalisation, truncation -> NL [c][i][j][k]
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2)
+__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
     k2_zfwd(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ nl, const cx<T>* __restrict__ twg,
             int ntile, T norm) {
   constexpr int L = tile_lines<T>(N);
-  constexpr int LD = N + 1;
+  constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKK = 2 * K + 1;
+  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -328,12 +436,26 @@ __global__ void __launch_bounds__(kBlock2)
   const int j0 = tile * L;
   const int nkj = g.nkj;
   load_tw<T, N>(tw, twg);
-  for (int f = 0; f < 3; ++f) {
+  cx<T> pre[QE];
+  auto load = [&](int f) {
     const cx<T>* src = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
-    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
       const int z = idx / L;
       const int line = idx - z * L;
-      buf[line * LD + z] = (j0 + line < nkj) ? src[(long long)z * g.ldj + line] : czero<T>();
+      pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && j0 + line < nkj) ? src[z * g.ldj + line] : czero<T>();
+    }
+  };
+  for (int f = 0; f < 3; ++f) {
+    load(f);
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
+      if (L * N % kBlock2 == 0 || idx < L * N) {
+        const int z = idx / L;
+        buf[(idx - z * L) * LD + lds_off<T>(z)] = pre[q];
+      }
     }
     __syncthreads();
     fft_ip<T, N, L, LD, -1>(buf, tw);
@@ -343,7 +465,7 @@ __global__ void __launch_bounds__(kBlock2)
       const int line = idx / NKK;
       const int k = idx - line * NKK;
       const int zz = (k <= K) ? k : N - NKK + k;
-      dst[idx] = scl(buf[line * LD + zz], norm);
+      dst[idx] = scl(buf[line * LD + lds_off<T>(zz)], norm);
     }
     __syncthreads();
   }
@@ -351,11 +473,11 @@ __global__ void __launch_bounds__(kBlock2)
 
 template <typename T, int N>
 size_t lds_lines() {
-  return size_t(N + tile_lines<T>(N) * (N + 1)) * sizeof(cx<T>);
+  return size_t(N + tile_lines<T>(N) * lds_ld<T>(N)) * sizeof(cx<T>);
 }
 template <typename T, int N>
 size_t lds_x() {
-  return size_t(N + 3 * xrows<T>(N) * (N + 1)) * sizeof(cx<T>);
+  return size_t(N + 3 * xrows<T>(N) * lds_ld<T>(N)) * sizeof(cx<T>);
 }
 
 hipError_t big_lds(const void* fn, size_t bytes) {

@@ -19,7 +19,9 @@ import os
 import numpy as np
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_DIR, "lib", "libpandora_amd.so")
+# PDR_LIBRARY selects another build of the library (e.g. one compiled with
+# different tuning macros); the default is the in-tree build.
+LIB_PATH = os.environ.get("PDR_LIBRARY") or os.path.join(_PKG_DIR, "lib", "libpandora_amd.so")
 HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "pandora_amd.h")
 
 PDR_FP64 = 0
