@@ -20,33 +20,61 @@ __device__ __forceinline__ cx<T> czero() {
   return mk<T>(T(0), T(0));
 }
 
-// lines per tile: 128-byte chunks (16 FP32 / 8 FP64 complex) while the LDS of a
-// tile stays near 64 KB, so that two workgroups fit the 160 KB of a CU
+// Tile shapes.  Lines per tile of the y/z kernels and rows per block of the x
+// kernel; the build may override the FP32 / FP64 values for lengths <= 512
+// (PDR_LINES_F32, PDR_LINES_F64, PDR_XROWS_F32, PDR_XROWS_F64) to tune.
+#ifndef PDR_LINES_F32
+#define PDR_LINES_F32 16
+#endif
+#ifndef PDR_LINES_F64
+#define PDR_LINES_F64 8
+#endif
+#ifndef PDR_XROWS_F32
+#define PDR_XROWS_F32 2
+#endif
+#ifndef PDR_XROWS_F64
+#define PDR_XROWS_F64 2
+#endif
 template <typename T>
 constexpr int tile_lines(int n) {
-  return sizeof(T) == 4 ? (n <= 512 ? 16 : (n <= 1024 ? 8 : 4)) : (n <= 512 ? 8 : (n <= 1024 ? 4 : 2));
+  return sizeof(T) == 4 ? (n <= 512 ? PDR_LINES_F32 : (n <= 1024 ? 8 : 4))
+                        : (n <= 512 ? PDR_LINES_F64 : (n <= 1024 ? 4 : 2));
 }
 template <typename T>
 constexpr int xrows(int n) {
-  return sizeof(T) == 4 ? (n <= 512 ? 4 : 2) : 2;
+  return sizeof(T) == 4 ? (n <= 512 ? PDR_XROWS_F32 : 2) : (n <= 512 ? PDR_XROWS_F64 : 2);
 }
+
+// LDS bytes of a workgroup, and the waves per SIMD that the 160 KB of LDS of a
+// CU admit (each 256-thread workgroup puts one wave on each of the 4 SIMDs).
+// The kernels declare that occupancy to the register allocator, which then
+// keeps to the 512 / waves registers per lane that it leaves.
+constexpr int kLdsPerCu = 160 * 1024;
+template <typename T>
+constexpr int lds_lines_bytes(int n) {
+  return (n + tile_lines<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
+}
+template <typename T>
+constexpr int lds_x_bytes(int n) {
+  return (n + 3 * xrows<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>)) + kBlock2 * int(sizeof(T));
+}
+constexpr int waves_for(int lds) {
+  return kLdsPerCu / lds < 1 ? 1 : (kLdsPerCu / lds > 8 ? 8 : kLdsPerCu / lds);
+}
+#define PDR_LINE_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_lines_bytes<T>(N)))))
+#define PDR_X_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_x_bytes<T>(N)))))
 
 // Line kernels issue all global loads of a field (QE elements per thread)
 // before the LDS stores of that field, so that one HBM latency is exposed per
 // field rather than one per element.  (Loading field f+1 during the transform
-// of field f was measured slower: it needs more registers than the two waves
-// per SIMD allow and spills.)
-
-// The LDS of a line tile (~64-78 KB) admits two workgroups per CU, i.e. two
-// waves per SIMD; tell the register allocator so that it stays within the
-// 256 registers per lane that two waves leave.
-#define PDR_TWO_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
+// of field f was measured slower: it needs more registers than the occupancy
+// leaves and spills.)
 
 // ---------------------------------------------------------------------------
 // pass 1: vorticity + inverse z FFT, (i, j-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
     k2_zinv(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
             const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
   constexpr int L = tile_lines<T>(N);
@@ -157,7 +185,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
 // pass 2: inverse y FFT, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
     k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
             int ntile) {
   constexpr int L = tile_lines<T>(N);
@@ -215,7 +243,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
 // pass 3: RB rows (z, y0..y0+RB-1) per block: c2r, product, r2c
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
     k2_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
              unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile) {
   constexpr int RB = xrows<T>(N);
@@ -361,7 +389,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
 // pass 4: forward y FFT + truncation, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
     k2_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
             int ntile) {
   constexpr int L = tile_lines<T>(N);
@@ -419,7 +447,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
 // pass 5: forward z FFT, normalisation, truncation -> NL [c][i][j][k]
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
     k2_zfwd(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ nl, const cx<T>* __restrict__ twg,
             int ntile, T norm) {
   constexpr int L = tile_lines<T>(N);
@@ -473,11 +501,11 @@ __global__ void __launch_bounds__(kBlock2) PDR_TWO_WAVES
 
 template <typename T, int N>
 size_t lds_lines() {
-  return size_t(N + tile_lines<T>(N) * lds_ld<T>(N)) * sizeof(cx<T>);
+  return size_t(lds_lines_bytes<T>(N));
 }
 template <typename T, int N>
 size_t lds_x() {
-  return size_t(N + 3 * xrows<T>(N) * lds_ld<T>(N)) * sizeof(cx<T>);
+  return size_t(lds_x_bytes<T>(N) - kBlock2 * int(sizeof(T)));  // dynamic part
 }
 
 hipError_t big_lds(const void* fn, size_t bytes) {
