@@ -68,12 +68,15 @@ typedef struct pdr_params {
   int32_t b110;                      /* B110 (box stretch in x1; 1 for all configs)     */
   int32_t p_interp_order;            /* P_INTERP_ORDER: even, 2..8 (reference default 4) */
   int32_t precision;                 /* PDR_FP64 or PDR_FP32_MIXED                      */
-  int32_t reserved;                  /* must be 0                                       */
+  int32_t force_stop;                /* FORCE_STOP: OU forcing while tstep < FORCE_STOP */
   double  rk3_dt;                    /* RK3_DT  (FIXED)                                 */
   double  rk3_cfl;                   /* RK3_CFL (CFL, DECIDE-isotropic)                 */
   double  f_nu;                      /* F_NU                                            */
   double  f_deform_s;                /* F_DEFORM_S: amat(1,3) = dU1/dx3                 */
-  double  force_tf;                  /* FORCE_TF: dt cap in DECIDE-isotropic mode       */
+  double  force_tf;                  /* FORCE_TF: OU time scale; dt cap (DECIDE-iso)    */
+  double  force_kf;                  /* FORCE_KF: forced modes 0 < |k| <= FORCE_KF      */
+  double  force_sigma;               /* FORCE_SIGMA: OU amplitude                       */
+  int64_t force_seed;                /* seed of the forcing draws (counter-based RNG)   */
 } pdr_params;
 
 /* ---- library / context -------------------------------------------------- */
@@ -116,6 +119,18 @@ int pdr_download_rk3(pdr_ctx* ctx, double* r1, double* r2, double* r3);
  * variables, src/pg_domain.f90:78-87). */
 int pdr_set_frame(pdr_ctx* ctx, const double bmat[9], const double binv[9]);
 int pdr_get_frame(const pdr_ctx* ctx, double bmat[9], double binv[9]);
+
+/* Ornstein-Uhlenbeck forcing state fluid_f1..3 (src/pf_force.f90:27-33): three
+ * arrays in the reference's order f(0:1, -NK:NK, -NK:NK, 0:NK) (re/im, k, j, i;
+ * Fortran column-major), each 2 (2NK+1)^2 (NK+1) doubles, NK = aint(FORCE_KF).
+ * The state starts at zero (f_ForceAlloc, src/pf_force.f90:275-303) and is
+ * advanced on the device once per stage (f_ForceCalc, src/pf_force.f90:106-268,
+ * called from f_RK3SolveNavierStokes, src/pf_rk3.f90:2063) while tstep <
+ * FORCE_STOP; it is applied in the isotropic FormF (src/pf_rk3.f90:2421-2467).
+ * nk must equal the context's NK.  Contexts without forcing (FORCE_STOP <= 0
+ * or F_TYPE != 1) return PDR_ERR_STATE. */
+int pdr_set_forcing(pdr_ctx* ctx, int32_t nk, const double* f1, const double* f2, const double* f3);
+int pdr_get_forcing(pdr_ctx* ctx, int32_t nk, double* f1, double* f2, double* f3);
 
 /* sim_dt, the SAVEd step width of g_RK3Calc (src/pg_rk3.f90:92). */
 int pdr_set_dt(pdr_ctx* ctx, double dt);

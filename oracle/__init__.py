@@ -27,6 +27,8 @@ class OrcParams(C.Structure):
         ("rk3_dt", C.c_double), ("rk3_cfl", C.c_double), ("f_nu", C.c_double),
         ("f_deform_s", C.c_double), ("f_remesh_t", C.c_int32), ("b110", C.c_int32),
         ("force_tf", C.c_double), ("p_interp_order", C.c_int32), ("nthreads", C.c_int32),
+        ("force_stop", C.c_int32), ("pad0", C.c_int32), ("force_kf", C.c_double),
+        ("force_sigma", C.c_double), ("force_seed", C.c_int64),
     ]
 
 
@@ -73,6 +75,12 @@ def lib():
         L.orc_correct_position.argtypes = [vp, dp, C.c_int64]
         L.orc_fft.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_spectrum.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_force_nk.restype = C.c_int
+        L.orc_force_nk.argtypes = [vp]
+        L.orc_set_forcing.argtypes = [vp, dp, dp, dp]
+        L.orc_get_forcing.argtypes = [vp, dp, dp, dp]
+        L.orc_force_calc.argtypes = [vp, C.c_int, C.c_double]
+        L.orc_normal_pair.argtypes = [C.c_uint64, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -86,6 +94,13 @@ def fft(x: np.ndarray, sign: int) -> np.ndarray:
     return out.view(np.complex128)
 
 
+def normal_pair(seed: int, key: int):
+    """The counter-based N(0,1) pair shared by the oracle and the library."""
+    a, b = C.c_double(), C.c_double()
+    lib().orc_normal_pair(seed, key, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
 def remesh_target(kmax: int, i: int, k: int) -> int:
     return lib().orc_remesh_target(kmax, i, k)
 
@@ -96,12 +111,14 @@ class Oracle:
 
     def __init__(self, n=None, *, nx=None, ny=None, nz=None, f_type=1, ns_deformed=1,
                  rk3_timestep=2, rk3_dt=0.005, rk3_cfl=0.75, nu=0.01, deform_s=0.0,
-                 remesh_t=0, b110=1, force_tf=0.0, interp_order=4, nthreads=0):
+                 remesh_t=0, b110=1, force_tf=0.0, interp_order=4, nthreads=0, force_stop=0,
+                 force_kf=0.0, force_sigma=0.0, force_seed=0):
         nx = nx or n
         ny = ny or n
         nz = nz or n
         self.p = OrcParams(nx, ny, nz, f_type, ns_deformed, rk3_timestep, rk3_dt, rk3_cfl,
-                           nu, deform_s, remesh_t, b110, force_tf, interp_order, nthreads)
+                           nu, deform_s, remesh_t, b110, force_tf, interp_order, nthreads,
+                           force_stop, 0, force_kf, force_sigma, force_seed)
         self.h = lib().orc_create(C.byref(self.p))
         s = np.zeros(6, dtype=np.int64)
         lib().orc_sizes(self.h, s)
@@ -125,6 +142,30 @@ class Oracle:
     def _wave(self, a):
         a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.shape)
         return a.view(np.float64).reshape(-1)
+
+    # ---- forcing -----------------------------------------------------------
+    @property
+    def force_nk(self) -> int:
+        """NK = aint(FORCE_KF), or -1 without forcing."""
+        return lib().orc_force_nk(self.h)
+
+    def _fshape(self):
+        w = 2 * self.force_nk + 1
+        return (self.force_nk + 1, w, w)   # C order (i, j, k) = Fortran f(0:1, k, j, i)
+
+    def set_forcing(self, f1, f2, f3):
+        a = [np.ascontiguousarray(f, dtype=np.complex128).reshape(self._fshape()).view(np.float64).reshape(-1)
+             for f in (f1, f2, f3)]
+        lib().orc_set_forcing(self.h, *a)
+
+    def get_forcing(self):
+        out = [np.zeros(self._fshape(), dtype=np.complex128) for _ in range(3)]
+        v = [o.view(np.float64).reshape(-1) for o in out]
+        lib().orc_get_forcing(self.h, *v)
+        return out
+
+    def force_calc(self, tstep: int, dt: float):
+        lib().orc_force_calc(self.h, tstep, dt)
 
     def set_state(self, u1, u2, u3):
         lib().orc_set_state(self.h, self._wave(u1), self._wave(u2), self._wave(u3))

@@ -139,6 +139,10 @@ struct orc {
   double* uprk;
   fftp fz, fy, fx;
   int nthreads;
+  int forcing;          /* F_TYPE 1 and FORCE_STOP > 0 */
+  int force_nk;         /* FORCE_NK = aint(FORCE_KF), src/pf_force.f90:289 */
+  double* force;        /* [c][i][j+NK][k+NK][re,im] */
+  uint64_t force_calls; /* f_ForceCalc calls that drew (draw stream index) */
 };
 
 #define T9(a, m, n) ((a)[((m) - 1) + 3 * ((n) - 1)])
@@ -194,10 +198,129 @@ orc* orc_create(const orc_params* p) {
 #else
   o->nthreads = 1;
 #endif
+  o->forcing = (p->f_type == 1 && p->force_stop > 0);
+  o->force_nk = o->forcing ? (int)trunc(p->force_kf) : 0;
+  if (o->forcing) {
+    const long w = 2 * o->force_nk + 1;
+    o->force = (double*)calloc((size_t)(3 * (o->force_nk + 1) * w * w * 2), sizeof(double));
+  }
   return o;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Forcing: f_ForceCalc, src/pf_force.f90:106-268                             */
+/* ------------------------------------------------------------------------- */
+static uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+static double u01(uint64_t seed, uint64_t idx) {
+  const uint64_t h = splitmix64(splitmix64(seed) ^ (idx * 0xD1B54A32D192ED03ull + 0x8CB92BA72F3D8DD7ull));
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+/* g_RandomNormal (src/pg_random.f90:112-146), mean 0, variance 1, on
+ * counter-based uniforms: attempt a reads uniforms key*64 + 2a, 2a + 1. */
+void orc_normal_pair(uint64_t seed, uint64_t key, double* r1, double* r2) {
+  *r1 = 0.0; *r2 = 0.0;
+  for (int a = 0; a < 32; a++) {
+    const double x1 = 2.0 * u01(seed, key * 64u + (uint64_t)(2 * a)) - 1.0;
+    const double x2 = 2.0 * u01(seed, key * 64u + (uint64_t)(2 * a + 1)) - 1.0;
+    const double w = x1 * x1 + x2 * x2;
+    if (w < 1.0 && w > 0.0) {
+      const double f = sqrt((-2.0 * log(w)) / w);
+      *r1 = x1 * f; *r2 = x2 * f;
+      return;
+    }
+  }
+}
+
+int orc_force_nk(const orc* o) { return o->forcing ? o->force_nk : -1; }
+
+static double* fat(const orc* o, int c, int i, int j, int k) {
+  const long w = 2 * o->force_nk + 1;
+  return o->force + 2 * ((((long)c * (o->force_nk + 1) + i) * w + (j + o->force_nk)) * w + (k + o->force_nk));
+}
+
+void orc_set_forcing(orc* o, const double* f1, const double* f2, const double* f3) {
+  const double* f[3] = {f1, f2, f3};
+  const int nk = o->force_nk;
+  for (int c = 0; c < 3; c++)
+    for (int i = 0; i <= nk; i++)
+      for (int j = -nk; j <= nk; j++)
+        for (int k = -nk; k <= nk; k++) {
+          const long h = 2 * (((long)i * (2 * nk + 1) + (j + nk)) * (2 * nk + 1) + (k + nk));
+          double* d = fat(o, c, i, j, k);
+          d[0] = f[c][h]; d[1] = f[c][h + 1];
+        }
+}
+
+void orc_get_forcing(const orc* o, double* f1, double* f2, double* f3) {
+  double* f[3] = {f1, f2, f3};
+  const int nk = o->force_nk;
+  for (int c = 0; c < 3; c++)
+    for (int i = 0; i <= nk; i++)
+      for (int j = -nk; j <= nk; j++)
+        for (int k = -nk; k <= nk; k++) {
+          const long h = 2 * (((long)i * (2 * nk + 1) + (j + nk)) * (2 * nk + 1) + (k + nk));
+          const double* d = fat(o, c, i, j, k);
+          f[c][h] = d[0]; f[c][h + 1] = d[1];
+        }
+}
+
+/* one OU update of the three components of mode (i, j, k) with draw keys key..key+2 */
+static void ou_mode(orc* o, int i, int j, int k, double om, double sd, uint64_t key) {
+  for (int c = 0; c < 3; c++) {
+    double re, im;
+    orc_normal_pair((uint64_t)o->p.force_seed, key + (uint64_t)c, &re, &im);
+    double* p = fat(o, c, i, j, k);
+    p[0] = p[0] * om + re * sd;
+    p[1] = p[1] * om + im * sd;
+  }
+}
+static void conj_copy(orc* o, int i, int j, int k, int si, int sj, int sk) {
+  for (int c = 0; c < 3; c++) {
+    const double* s = fat(o, c, si, sj, sk);
+    double* d = fat(o, c, i, j, k);
+    d[0] = s[0]; d[1] = -s[1];
+  }
+}
+
+void orc_force_calc(orc* o, int tstep, double dt) {
+  if (!o->forcing) return;
+  const int nk = o->force_nk;
+  const long w = 2 * nk + 1;
+  if (tstep >= o->p.force_stop) {  /* IFDECAY */
+    memset(o->force, 0, sizeof(double) * (size_t)(3 * (nk + 1) * w * w * 2));
+    return;
+  }
+  const double a = dt / o->p.force_tf;
+  const double om = 1.0 - a;
+  const double sd = sqrt(2.0 * (o->p.force_sigma * o->p.force_sigma) * a);
+  const uint64_t key0 = o->force_calls << 24;
+  /* i = 0 plane, conjugate symmetric (src/pf_force.f90:143-221) */
+  for (int j = 0; j <= nk; j++)
+    for (int k = 0; k <= nk; k++) {
+      const uint64_t t = (uint64_t)(j * (nk + 1) + k);
+      ou_mode(o, 0, j, k, om, sd, key0 + 6 * t);
+      conj_copy(o, 0, -j, -k, 0, j, k);
+      if (j > 0) {
+        ou_mode(o, 0, -j, k, om, sd, key0 + 6 * t + 3);
+        conj_copy(o, 0, j, -k, 0, -j, k);
+      }
+    }
+  /* i >= 1 (src/pf_force.f90:223-258) */
+  const uint64_t n0 = (uint64_t)((nk + 1) * (nk + 1));
+  uint64_t u = 0;
+  for (int i = 1; i <= nk; i++)
+    for (int j = -nk; j <= nk; j++)
+      for (int k = -nk; k <= nk; k++, u++) ou_mode(o, i, j, k, om, sd, key0 + 6 * n0 + 3 * u);
+  o->force_calls++;
+}
+
 void orc_destroy(orc* o) {
+  free(o->force);
   if (!o) return;
   for (int c = 0; c < 3; c++) { free(o->u[c]); free(o->rk[c]); free(o->ur[c]); }
   for (int f = 0; f < 6; f++) free(o->w[f]);
@@ -676,8 +799,9 @@ static inline void rk_update(double* rk[3], double* u[3], long e, const double t
 }
 
 /* f_RK3SolveNavierStokes, src/pf_rk3.f90:1996-2142 with f_RK3FormFIsotropic
- * (2323-2487) and f_RK3FormFBFRot (2494-2673).  Forcing is zero (decaying
- * runs, FORCE_STOP = 0); the two-way coupling term is zero. */
+ * (2323-2487) and f_RK3FormFBFRot (2494-2673).  The isotropic form adds the
+ * OU forcing; the shear forms carry none (F_TYPE 2..4 do not call
+ * f_RK3BFRotForcing); the two-way coupling term is zero. */
 static void solve_ns(orc* o, double dt, int stage) {
   double beta = 0.0, gamma = THIRD;
   if (stage == 2) { beta = -FIVE_OVER_NINE; gamma = FIFTEEN_OVER_SIXTEEN; }
@@ -713,13 +837,27 @@ static void solve_ns(orc* o, double dt, int stage) {
           if (o->p.f_type == 1) {
             double kdn0 = (wi * nn[0][0]) + (wj * nn[1][0]) + (wk * nn[2][0]);
             double kdn1 = (wi * nn[0][1]) + (wj * nn[1][1]) + (wk * nn[2][1]);
+            /* forcing on 0 < |k| <= FORCE_KF (src/pf_rk3.f90:2421-2467) */
+            double fo[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            const int nk = o->force_nk;
+            const int jj = (int)wj, kq = (int)wk;
+            if (o->forcing && i <= nk && jj >= -nk && jj <= nk && kq >= -nk && kq <= nk &&
+                sqrt(kmag2) <= o->p.force_kf) {
+              const double* f[3] = {fat(o, 0, i, jj, kq), fat(o, 1, i, jj, kq), fat(o, 2, i, jj, kq)};
+              const double kb0 = (wi * f[0][0] + wj * f[1][0]) + wk * f[2][0];
+              const double kb1 = (wi * f[0][1] + wj * f[1][1]) + wk * f[2][1];
+              for (int c = 0; c < 3; c++) {
+                fo[c][0] = f[c][0] - ((kv[c] / kmag2) * kb0);
+                fo[c][1] = f[c][1] - ((kv[c] / kmag2) * kb1);
+              }
+            }
             for (int c = 0; c < 3; c++) {
               double visc0 = -(o->p.f_nu * kmag2 * uu[c][0]);
               double visc1 = -(o->p.f_nu * kmag2 * uu[c][1]);
               double pr0 = (kv[c] / kmag2) * kdn0;
               double pr1 = (kv[c] / kmag2) * kdn1;
-              tot[c][0] = ((visc0 - nn[c][0]) + pr0) + 0.0;
-              tot[c][1] = ((visc1 - nn[c][1]) + pr1) + 0.0;
+              tot[c][0] = (((visc0 - nn[c][0]) + pr0) + fo[c][0]) + 0.0;
+              tot[c][1] = (((visc1 - nn[c][1]) + pr1) + fo[c][1]) + 0.0;
             }
           } else {
             double nlf[3][2], pf[3][2], hom[3][2], visc[3][2];
@@ -963,6 +1101,7 @@ void orc_stage(orc* o, int stage, int tstep, double time, double* dt) {
   orc_prepare_realspace(o);
   orc_realspace(o, stage, tstep, dt);
   if (o->np > 0) tracers_stage(o, stage, *dt);
+  orc_force_calc(o, tstep, *dt);  /* src/pf_rk3.f90:2063 */
   solve_ns(o, *dt, stage);
 }
 

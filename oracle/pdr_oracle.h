@@ -40,12 +40,25 @@ typedef struct orc_params {
   double  force_tf;
   int32_t p_interp_order;  /* even orders (4 is the reference default) */
   int32_t nthreads;        /* 0 = OpenMP default */
+  int32_t force_stop;      /* FORCE_STOP: OU forcing (F_TYPE 1) while tstep < FORCE_STOP */
+  int32_t pad0;
+  double  force_kf;        /* FORCE_KF */
+  double  force_sigma;     /* FORCE_SIGMA */
+  int64_t force_seed;      /* seed of the counter-based forcing draws */
 } orc_params;
 
 typedef struct orc orc;
 
 orc*   orc_create(const orc_params* p);
 void   orc_destroy(orc* o);
+/* forcing state fluid_f1..3 in the reference order f(0:1, -NK:NK, -NK:NK, 0:NK) */
+int    orc_force_nk(const orc* o);
+void   orc_set_forcing(orc* o, const double* f1, const double* f2, const double* f3);
+void   orc_get_forcing(const orc* o, double* f1, double* f2, double* f3);
+/* one f_ForceCalc call (src/pf_force.f90:106-268); orc_stage makes it itself */
+void   orc_force_calc(orc* o, int tstep, double dt);
+/* counter-based N(0,1) pair (same definition as the library's) */
+void   orc_normal_pair(uint64_t seed, uint64_t key, double* r1, double* r2);
 void   orc_sizes(const orc* o, int64_t* out6);   /* NKI,NKJ,NKK, Mw, M1, kkmax */
 
 void   orc_set_state(orc* o, const double* u1, const double* u2, const double* u3);

@@ -247,6 +247,7 @@ int pdr_destroy(pdr_ctx* c) {
   if (c->uprk) (void)hipFree(c->uprk);
   if (c->umax) (void)hipFree(c->umax);
   if (c->red) (void)hipFree(c->red);
+  if (c->force) (void)hipFree(c->force);
   if (c->tab.twz) (void)hipFree(c->tab.twz);
   if (c->tab.twy) (void)hipFree(c->tab.twy);
   if (c->tab.twx) (void)hipFree(c->tab.twx);
@@ -258,6 +259,13 @@ int pdr_destroy(pdr_ctx* c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PDR_OK;
+}
+
+// bytes of the forcing state: 3 components x (NK+1)(2NK+1)^2 complex FP64
+static size_t force_bytes(const pdr_ctx* c) {
+  if (!c->forcing) return 0;
+  const size_t w = size_t(2 * c->force_nk + 1);
+  return 3 * size_t(c->force_nk + 1) * w * w * 16;
 }
 
 int pdr_create(pdr_ctx** out, const pdr_params* p) {
@@ -281,6 +289,9 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   if (p->precision != PDR_FP64 && p->precision != PDR_FP32_MIXED)
     return fail(PDR_ERR_ARG, "precision must be PDR_FP64 or PDR_FP32_MIXED");
   if (p->b110 != 1) return fail(PDR_ERR_UNSUPPORTED, "B110 must be 1");
+  const bool forcing = p->f_type == PDR_F_ISOTROPIC && p->force_stop > 0;
+  if (forcing && !(p->force_tf > 0.0 && p->force_kf > 0.0 && p->force_kf <= 64.0 && p->force_sigma >= 0.0))
+    return fail(PDR_ERR_ARG, "forcing needs FORCE_TF > 0, 0 < FORCE_KF <= 64 and FORCE_SIGMA >= 0");
 
   pdr_ctx* c = new pdr_ctx();
   c->p = *p;
@@ -305,6 +316,9 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   g.nr = (long long)nx * ny * nz;
   frame_init(c);
   c->dt = 99999.0;
+  // f_ForceAlloc: FORCE_NK = aint(FORCE_KF) (src/pf_force.f90:289)
+  c->forcing = forcing;
+  c->force_nk = forcing ? int(std::trunc(p->force_kf)) : 0;
 
   int dev = 0;
   hipError_t he = hipGetDevice(&dev);
@@ -354,6 +368,12 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   const size_t w2n = std::max<size_t>(6 * size_t(g.m2), 3 * size_t(g.mw));
   if (rc == PDR_OK) rc = dalloc(c, &c->w2, w2n * c->csz);
   if (rc == PDR_OK) rc = dalloc(c, (void**)&c->umax, 256);
+  const size_t fbytes = force_bytes(c);
+  if (rc == PDR_OK && fbytes) {
+    rc = dalloc(c, (void**)&c->force, fbytes);
+    if (rc == PDR_OK && hipMemsetAsync(c->force, 0, fbytes, c->stream) != hipSuccess)
+      rc = fail(PDR_ERR_HIP, "zeroing forcing failed");
+  }
   if (rc == PDR_OK) rc = upload_tw(c, nz, &c->tab.twz);
   if (rc == PDR_OK) rc = upload_tw(c, ny, &c->tab.twy);
   if (rc == PDR_OK) rc = upload_tw(c, nx, &c->tab.twx);
@@ -552,6 +572,7 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   const double gdt = gamma * (*dt);
   if (tracers) HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt));
   HIPCHK(launch_zfwd(c));
+  if (c->forcing) HIPCHK(launch_force(c, tstep, *dt));  // f_ForceCalc (src/pf_rk3.f90:2063)
   HIPCHK(launch_formf(c, cf, beta, gdt));
   return PDR_OK;
 }
@@ -734,6 +755,56 @@ int pdr_profile_get(pdr_ctx* c, int32_t n, char* names, int32_t namelen, int64_t
     std::strncpy(names, all.c_str(), size_t(namelen) - 1);
     names[namelen - 1] = 0;
   }
+  return PDR_OK;
+}
+
+
+// Forcing state in the reference's array order f(0:1, -NK:NK (k), -NK:NK (j), 0:NK (i))
+// per component; the device holds [i][j+NK][k+NK][c].
+static int force_check(pdr_ctx* c, int32_t nk, const void* a, const void* b, const void* d) {
+  if (!c || !a || !b || !d) return fail(PDR_ERR_ARG, "forcing: null argument");
+  if (!c->forcing) return fail(PDR_ERR_STATE, "forcing is off (F_TYPE != 1 or FORCE_STOP <= 0)");
+  if (nk != c->force_nk) return fail(PDR_ERR_ARG, "forcing: nk does not match aint(FORCE_KF)");
+  return PDR_OK;
+}
+
+int pdr_set_forcing(pdr_ctx* c, int32_t nk, const double* f1, const double* f2, const double* f3) {
+  int rc = force_check(c, nk, f1, f2, f3);
+  if (rc != PDR_OK) return rc;
+  const int w = 2 * nk + 1;
+  std::vector<double> dev(force_bytes(c) / 8);
+  const double* src[3] = {f1, f2, f3};
+  for (int i = 0; i <= nk; ++i)
+    for (int j = 0; j < w; ++j)
+      for (int k = 0; k < w; ++k)
+        for (int q = 0; q < 3; ++q) {
+          const size_t h = 2 * ((size_t(i) * w + j) * w + k);
+          const size_t d = 2 * (((size_t(i) * w + j) * w + k) * 3 + q);
+          dev[d] = src[q][h];
+          dev[d + 1] = src[q][h + 1];
+        }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->force, dev.data(), dev.size() * 8, hipMemcpyHostToDevice));
+  return PDR_OK;
+}
+
+int pdr_get_forcing(pdr_ctx* c, int32_t nk, double* f1, double* f2, double* f3) {
+  int rc = force_check(c, nk, f1, f2, f3);
+  if (rc != PDR_OK) return rc;
+  const int w = 2 * nk + 1;
+  std::vector<double> dev(force_bytes(c) / 8);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(dev.data(), c->force, dev.size() * 8, hipMemcpyDeviceToHost));
+  double* dst[3] = {f1, f2, f3};
+  for (int i = 0; i <= nk; ++i)
+    for (int j = 0; j < w; ++j)
+      for (int k = 0; k < w; ++k)
+        for (int q = 0; q < 3; ++q) {
+          const size_t h = 2 * ((size_t(i) * w + j) * w + k);
+          const size_t d = 2 * (((size_t(i) * w + j) * w + k) * 3 + q);
+          dst[q][h] = dev[d];
+          dst[q][h + 1] = dev[d + 1];
+        }
   return PDR_OK;
 }
 

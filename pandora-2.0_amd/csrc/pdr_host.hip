@@ -1,4 +1,4 @@
-// pdr_host.cpp -- host-side pieces of the C-ABI that sit either side of the
+// pdr_host.hip -- host-side pieces of the C-ABI that sit either side of the
 // time step: seeded initial conditions, PETSc-binary restart I/O and the
 // wave-space spectrum.
 #include <cmath>
@@ -8,27 +8,17 @@
 #include <vector>
 
 #include "pdr_internal.hpp"
+#include "pdr_rng.hpp"
 
 namespace {
+
+using pdr::u01;
 
 // float32 pi widened to double: ONEPI, TWOPI, FOURPI of src/pg_constants.f90:126-131
 const double kOnePi = double(3.14159265358979323846f);
 const double kTwoPi = 2.0 * kOnePi;
 const double kFourPi = 4.0 * kOnePi;
 const int32_t kVecFileClassId = 1211214;  // PETSc VEC_FILE_CLASSID
-
-// Counter-based uniform generator: SplitMix64 (Steele, Lea, Flood, OOPSLA 2014)
-// of (seed, draw index), top 53 bits scaled to [0, 1).
-inline uint64_t splitmix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-inline double u01(uint64_t seed, uint64_t idx) {
-  const uint64_t h = splitmix64(splitmix64(seed) ^ (idx * 0xD1B54A32D192ED03ull + 0x8CB92BA72F3D8DD7ull));
-  return double(h >> 11) * (1.0 / 9007199254740992.0);
-}
 
 inline double wvn(int k, int kmax) { return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1)); }
 

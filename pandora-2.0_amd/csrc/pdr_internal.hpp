@@ -95,6 +95,10 @@ struct pdr_ctx {
   int64_t np = 0;
   unsigned long long* umax = nullptr;  // CFL reduction (bits of a non-negative double)
   double* red = nullptr;               // spectrum reduction scratch
+  double* force = nullptr;             // OU forcing state, [i][j+NK][k+NK][c] complex FP64
+  int force_nk = 0;                    // NK = aint(FORCE_KF)
+  bool forcing = false;                // F_TYPE 1 with FORCE_STOP > 0
+  uint64_t force_calls = 0;            // f_ForceCalc calls so far (draw stream index)
   pdr::Tables tab;
   pdr::FftPlan pz, py, px;
   // frame (FP64 on the host)
@@ -125,6 +129,7 @@ hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl);
 hipError_t launch_yfwd(pdr_ctx* c);
 hipError_t launch_zfwd(pdr_ctx* c);
 hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
+hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
 hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt);
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);

@@ -22,6 +22,7 @@
 //              (src/pf_rk3.f90:4056-4178)
 // All kernels are HBM-bandwidth bound; none is a dense contraction.
 #include "pdr_internal.hpp"
+#include "pdr_rng.hpp"
 
 namespace pdr {
 
@@ -319,7 +320,8 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_formf(Geo g, Coef cf, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1, cx<T>* __restrict__ u2,
             cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
-            const cx<T>* __restrict__ nl, double beta, double gdt) {
+            const cx<T>* __restrict__ nl, const double2* __restrict__ frc, int fnk, double fkf, double beta,
+            double gdt) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= g.mw) return;
   const int k = int(e % g.nkk);
@@ -370,6 +372,22 @@ __global__ void __launch_bounds__(kThreads)
     }
     tot[c] = t;
   }
+  // forcing on 0 < |k| <= FORCE_KF, made solenoidal (src/pf_rk3.f90:2421-2467);
+  // frc is [i][j+NK][k+NK][c] (isotropic frame only, fnk = 0 otherwise)
+  if (fnk > 0) {
+    const int jj = int(kv[1]), kq = int(kv[2]);
+    if (i <= fnk && jj >= -fnk && jj <= fnk && kq >= -fnk && kq <= fnk && sqrt(kmag2) <= fkf) {
+      const int w = 2 * fnk + 1;
+      const double2* f = frc + ((long long)(i * w + jj + fnk) * w + kq + fnk) * 3;
+      const double kbr = (kv[0] * f[0].x + kv[1] * f[1].x) + kv[2] * f[2].x;
+      const double kbi = (kv[0] * f[0].y + kv[1] * f[1].y) + kv[2] * f[2].y;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double q = kv[c] / kmag2;
+        tot[c] = add(tot[c], mk<T>(T(f[c].x - q * kbr), T(f[c].y - q * kbi)));
+      }
+    }
+  }
   const T tb = T(beta), tg = T(gdt);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -386,6 +404,62 @@ __global__ void __launch_bounds__(kThreads)
   r0[e] = r[0];
   r1[e] = r[1];
   r2[e] = r[2];
+}
+
+// ---------------------------------------------------------------------------
+// Ornstein-Uhlenbeck forcing update, f_ForceCalc (src/pf_force.f90:106-268).
+// F is [i][j+NK][k+NK][c] complex, i = 0..NK, j, k = -NK..NK.  Thread t < (NK+1)^2
+// runs one (j, k) iteration of the i = 0 loop (j, k = 0..NK) with its
+// conjugate-symmetric copies, in the reference's statement order (for k = 0
+// the (0, -j, 0) update reads the copy just written, as there); the other
+// threads update one i >= 1 mode each.  The iterations touch disjoint modes.
+// Normal pairs come from normal_pair(seed, call * 2^24 + slot); draws of one
+// call never repeat another call's.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+    k_force_ou(double2* __restrict__ F, int nk, double om, double sd, unsigned long long seed,
+               unsigned long long call) {
+  const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+  const int w = 2 * nk + 1;
+  const int n0 = (nk + 1) * (nk + 1);
+  const int n1 = nk * w * w;
+  if (t >= n0 + n1) return;
+  auto at = [&](int i, int j, int k) { return F + ((i * w + j + nk) * w + k + nk) * 3; };
+  auto ou = [&](double2* p, unsigned long long key) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      double re, im;
+      pdr::normal_pair(seed, key + unsigned(c), &re, &im);
+      p[c].x = p[c].x * om + re * sd;
+      p[c].y = p[c].y * om + im * sd;
+    }
+  };
+  const unsigned long long key0 = call << 24;
+  if (t < n0) {
+    const int j = t / (nk + 1), k = t - (t / (nk + 1)) * (nk + 1);
+    double2* p = at(0, j, k);
+    ou(p, key0 + 6ull * unsigned(t));
+    double2* m = at(0, -j, -k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c] = make_double2(p[c].x, -p[c].y);
+    if (j > 0) {
+      double2* q = at(0, -j, k);
+      ou(q, key0 + 6ull * unsigned(t) + 3ull);
+      double2* r = at(0, j, -k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r[c] = make_double2(q[c].x, -q[c].y);
+    }
+  } else {
+    const int u = t - n0;
+    const int i = 1 + u / (w * w);
+    const int rem = u - (i - 1) * w * w;
+    ou(at(i, rem / w - nk, rem % w - nk), key0 + 6ull * unsigned(n0) + 3ull * unsigned(u));
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_zero_d2(double2* __restrict__ p, int n) {
+  const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t < n) p[t] = make_double2(0.0, 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -730,7 +804,28 @@ static hipError_t formf_t(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
   const long long nb = (g.mw + kThreads - 1) / kThreads;
   k_formf<T><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(
       g, cf, (cx<T>*)c->u[0], (cx<T>*)c->u[1], (cx<T>*)c->u[2], (cx<T>*)c->rk[0], (cx<T>*)c->rk[1],
-      (cx<T>*)c->rk[2], (const cx<T>*)c->w2, beta, gdt);
+      (cx<T>*)c->rk[2], (const cx<T>*)c->w2, (const double2*)c->force, c->forcing ? c->force_nk : 0,
+      c->p.force_kf, beta, gdt);
+  return hipGetLastError();
+}
+
+hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt) {
+  const int nk = c->force_nk, w = 2 * nk + 1;
+  prof_begin(c, K_MISC);
+  if (tstep >= c->p.force_stop) {
+    // IFDECAY branch: the forcing arrays are zeroed, no draws
+    const int n = (nk + 1) * w * w * 3;
+    k_zero_d2<<<dim3(unsigned((n + kThreads - 1) / kThreads)), kThreads, 0, c->stream>>>((double2*)c->force, n);
+  } else {
+    const double a = dt / c->p.force_tf;  // dt_over_TF
+    const double sd = sqrt(2.0 * (c->p.force_sigma * c->p.force_sigma) * a);
+    const int nt = (nk + 1) * (nk + 1) + nk * w * w;
+    k_force_ou<<<dim3(unsigned((nt + kThreads - 1) / kThreads)), kThreads, 0, c->stream>>>(
+        (double2*)c->force, nk, 1.0 - a, sd, (unsigned long long)c->p.force_seed,
+        (unsigned long long)c->force_calls);
+    ++c->force_calls;
+  }
+  prof_end(c, K_MISC);
   return hipGetLastError();
 }
 

@@ -28,12 +28,15 @@ module pdr_rk3_amd
     integer(c_int32_t) :: b110 = 1
     integer(c_int32_t) :: p_interp_order = 4
     integer(c_int32_t) :: precision = 0
-    integer(c_int32_t) :: reserved = 0
+    integer(c_int32_t) :: force_stop = 0
     real(c_double)     :: rk3_dt = 0.01d0
     real(c_double)     :: rk3_cfl = 0.75d0
     real(c_double)     :: f_nu = 0.0d0
     real(c_double)     :: f_deform_s = 0.0d0
     real(c_double)     :: force_tf = 0.0d0
+    real(c_double)     :: force_kf = 0.0d0
+    real(c_double)     :: force_sigma = 0.0d0
+    integer(c_int64_t) :: force_seed = 0
   end type pdr_params_t
 
   integer(c_int32_t), parameter, public :: PDR_FP64 = 0, PDR_FP32_MIXED = 1
@@ -45,7 +48,7 @@ module pdr_rk3_amd
   public :: f_RK3Stage1, f_RK3Stage2, f_RK3Stage3, f_RK3Remesh
   public :: pdr_PutVelocity, pdr_GetVelocity, pdr_PutParticles, pdr_GetParticles
   public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_Spectrum, pdr_ErrorMessage
-  public :: pdr_GetFrame
+  public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing
 
   interface
     integer(c_int) function c_pdr_create(out, p) bind(C, name="pdr_create")
@@ -108,6 +111,18 @@ module pdr_rk3_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: c
       real(c_double), intent(out) :: bmat(9), binv(9)
+    end function
+    integer(c_int) function c_pdr_set_forcing(c, nk, f1, f2, f3) bind(C, name="pdr_set_forcing")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: nk
+      real(c_double), intent(in) :: f1(*), f2(*), f3(*)
+    end function
+    integer(c_int) function c_pdr_get_forcing(c, nk, f1, f2, f3) bind(C, name="pdr_get_forcing")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: nk
+      real(c_double), intent(out) :: f1(*), f2(*), f3(*)
     end function
     integer(c_int) function c_pdr_init_field(c, seed, u0, kp) bind(C, name="pdr_init_field")
       import :: c_int, c_ptr, c_int64_t, c_double
@@ -248,6 +263,23 @@ contains
     integer, intent(inout)           :: ierr
     ierr = int(c_pdr_get_frame(ctx, bmat, binv))
   end subroutine pdr_GetFrame
+
+  !> OU forcing arrays fluid_f1..3(0:1,-NK:NK,-NK:NK,0:NK) (src/pf_force.f90:27-33)
+  subroutine pdr_PutForcing ( nk, f1, f2, f3, ierr )
+    integer, intent(in)             :: nk
+    real(kind=c_double), intent(in) :: f1(0:1,-nk:nk,-nk:nk,0:nk), f2(0:1,-nk:nk,-nk:nk,0:nk), &
+                                       f3(0:1,-nk:nk,-nk:nk,0:nk)
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_set_forcing(ctx, int(nk, c_int32_t), f1, f2, f3))
+  end subroutine pdr_PutForcing
+
+  subroutine pdr_GetForcing ( nk, f1, f2, f3, ierr )
+    integer, intent(in)              :: nk
+    real(kind=c_double), intent(out) :: f1(0:1,-nk:nk,-nk:nk,0:nk), f2(0:1,-nk:nk,-nk:nk,0:nk), &
+                                        f3(0:1,-nk:nk,-nk:nk,0:nk)
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_get_forcing(ctx, int(nk, c_int32_t), f1, f2, f3))
+  end subroutine pdr_GetForcing
 
   !> Seeded Rogallo initial field, type-10 spectrum (f_InitialiseFluid)
   subroutine pdr_InitField ( seed, u0, kp, ierr )

@@ -47,9 +47,10 @@ class Params(C.Structure):
         ("nodes_x", C.c_int32), ("nodes_y", C.c_int32), ("nodes_z", C.c_int32),
         ("f_type", C.c_int32), ("ns_deformed", C.c_int32), ("dealiasing", C.c_int32),
         ("rk3_timestep", C.c_int32), ("f_remesh_t", C.c_int32), ("b110", C.c_int32),
-        ("p_interp_order", C.c_int32), ("precision", C.c_int32), ("reserved", C.c_int32),
+        ("p_interp_order", C.c_int32), ("precision", C.c_int32), ("force_stop", C.c_int32),
         ("rk3_dt", C.c_double), ("rk3_cfl", C.c_double), ("f_nu", C.c_double),
-        ("f_deform_s", C.c_double), ("force_tf", C.c_double),
+        ("f_deform_s", C.c_double), ("force_tf", C.c_double), ("force_kf", C.c_double),
+        ("force_sigma", C.c_double), ("force_seed", C.c_int64),
     ]
 
 
@@ -77,6 +78,8 @@ def load_library(path: str = LIB_PATH):
         "pdr_download_state": (C.c_int, [vp, dp, dp, dp]),
         "pdr_upload_rk3": (C.c_int, [vp, dp, dp, dp]),
         "pdr_download_rk3": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_set_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
+        "pdr_get_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_set_frame": (C.c_int, [vp, dp, dp]),
         "pdr_get_frame": (C.c_int, [vp, dp, dp]),
         "pdr_set_dt": (C.c_int, [vp, d]),
@@ -172,13 +175,16 @@ class Pandora:
 
     def __init__(self, n=None, *, nx=None, ny=None, nz=None, f_type=F_ISOTROPIC, ns_deformed=1,
                  rk3_timestep=RK3_TIMESTEP_FIXED, rk3_dt=0.005, rk3_cfl=0.75, nu=0.01, deform_s=0.0,
-                 remesh_t=0, b110=1, interp_order=4, precision=PDR_FP64, force_tf=0.0):
+                 remesh_t=0, b110=1, interp_order=4, precision=PDR_FP64, force_tf=0.0, force_stop=0,
+                 force_kf=0.0, force_sigma=0.0, force_seed=0):
         L = load_library()
         nx = nx or n
         ny = ny or n
         nz = nz or n
         self.params = Params(nx, ny, nz, f_type, ns_deformed, 1, rk3_timestep, remesh_t, b110,
-                             interp_order, precision, 0, rk3_dt, rk3_cfl, nu, deform_s, force_tf)
+                             interp_order, precision, force_stop, rk3_dt, rk3_cfl, nu, deform_s,
+                             force_tf, force_kf, force_sigma, force_seed)
+        self.force_nk = int(force_kf) if (f_type == F_ISOTROPIC and force_stop > 0) else -1
         h = C.c_void_p()
         _check(L.pdr_create(C.byref(h), C.byref(self.params)))
         self.h = h
@@ -236,6 +242,22 @@ class Pandora:
         _check(self.L.pdr_download_rk3(self.h, *[o.view(np.float64).reshape(-1) for o in out]))
         return out
 
+
+    # ---- forcing (fluid_f1..3, src/pf_force.f90:27-33) -----------------------
+    def _fshape(self):
+        w = 2 * self.force_nk + 1
+        return (self.force_nk + 1, w, w)   # C order (i, j, k) = Fortran f(0:1, k, j, i)
+
+    def set_forcing(self, f1, f2, f3):
+        a = [np.ascontiguousarray(f, dtype=np.complex128).reshape(self._fshape()).view(np.float64).reshape(-1)
+             for f in (f1, f2, f3)]
+        _check(self.L.pdr_set_forcing(self.h, self.force_nk, *a))
+
+    def get_forcing(self):
+        out = [np.zeros(self._fshape(), dtype=np.complex128) for _ in range(3)]
+        v = [o.view(np.float64).reshape(-1) for o in out]
+        _check(self.L.pdr_get_forcing(self.h, self.force_nk, *v))
+        return out
     def set_frame(self, bmat, binv):
         _check(self.L.pdr_set_frame(self.h, np.ascontiguousarray(bmat, np.float64).reshape(9),
                                     np.ascontiguousarray(binv, np.float64).reshape(9)))

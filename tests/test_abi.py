@@ -30,7 +30,7 @@ def test_library_exports_every_header_symbol():
 
 def _params(**kw):
     base = dict(nodes_x=32, nodes_y=32, nodes_z=32, f_type=1, ns_deformed=1, dealiasing=1,
-                rk3_timestep=2, f_remesh_t=0, b110=1, p_interp_order=4, precision=0, reserved=0,
+                rk3_timestep=2, f_remesh_t=0, b110=1, p_interp_order=4, precision=0, force_stop=0,
                 rk3_dt=0.005, rk3_cfl=0.75, f_nu=0.01, f_deform_s=0.0, force_tf=0.0)
     base.update(kw)
     return pa.Params(**base)
@@ -43,6 +43,8 @@ def _params(**kw):
     (dict(f_type=2, ns_deformed=3), -4),                  # Rogallo forms other than BF_ROT
     (dict(dealiasing=2), -4),
     (dict(rk3_cfl=1.0), -1),                              # g_RK3Check abort condition
+    (dict(force_stop=10, force_tf=0.0, force_kf=2.5), -1),  # OU time scale must be > 0
+    (dict(force_stop=10, force_tf=0.2, force_kf=0.0), -1),  # no forced band
     (dict(f_type=2, rk3_timestep=1, f_remesh_t=0, f_deform_s=1.0), -1),
     (dict(p_interp_order=3), -4),
     (dict(precision=2), -1),
