@@ -217,11 +217,35 @@ int pdr_restart_write_host(const char* path, int64_t n, const double* u1, const 
                            const double* u3, int32_t int64_header);
 int pdr_restart_read_host(const char* path, int64_t n, double* u1, double* u2, double* u3);
 
-/* Wave-space statistics of the current velocity: f_FluidstatsFsPrimaryQuants
- * (src/pf_fluidstats.f90:1423-1576) before mode-density normalisation.
- * ek, dk, dsk have kkmax entries (shell sums of energy, 2 nu k^2 E and
- * 4 nu^2 k^4 E); tot[6] = {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}.
- * Sums are FP64 with compensated (Kahan-Neumaier) accumulation. */
+/* Wave-space statistics of the current velocity, computed on the device:
+ * f_FluidstatsFs (src/pf_fluidstats.f90:1293-1361).  Modes are taken to the
+ * laboratory frame as in f_FluidstatsFsPrimaryQuantsBFRot (1586-1792; the
+ * identity for isotropic runs), binned by |k_lab| in [kk-1/2, kk+1/2),
+ * kk = 1..kkmax, and summed in FP64 with compensated accumulation; the mode
+ * density (696-774, 1977-2049), the normalised spectra and totals (1547-1575)
+ * and the secondary quantities (f_FluidstatsFsSecondaryQuants, 2491-2558)
+ * follow on the host.  Field names are the reference's module variables. */
+typedef struct pdr_stats_t {
+  double tke;                                   /* 1/2 (u1u1 + u2u2 + u3u3)      */
+  double u1u1, u2u2, u3u3, u1u2, u1u3, u2u3;    /* fs_u1u1 ... Reynolds sums     */
+  double f_ken, f_dis, f_disskew;               /* un-normalised totals          */
+  double fn_ken, fn_ken1, fn_ken2, fn_ken3;     /* normalised kinetic energies   */
+  double fn_dis, fn_disskew;                    /* normalised dissipation (eps)  */
+  double fn_int, fn_int1, fn_int2, fn_int3;     /* r.m.s. velocities u'          */
+  double tn_tur, ln_tur;                        /* t_eps, l_eps                  */
+  double ln_tay, rn_tay;                        /* Taylor scale, Re_lambda       */
+  double ln_int, ln_int1, ln_int2, ln_int3;     /* integral length scales        */
+  double tn_edy, ln_kol, tn_kol;                /* eddy turnover, eta, tau_eta   */
+  double disskew, c_epsilon;                    /* dissipation skewness, C_eps   */
+} pdr_stats_t;
+
+/* spec (may be NULL): 7 * kkmax doubles, the normalised spectra spec_ken,
+ * spec_ken1..3, spec_dis, spec_disskew and dens_mod, bin kk at index kk-1. */
+int pdr_stats(pdr_ctx* ctx, pdr_stats_t* out, double* spec);
+
+/* Shell sums before mode-density normalisation from the same device pass:
+ * ek, dk, dsk have kkmax entries (energy, 2 nu k^2 E, 4 nu^2 k^4 E);
+ * tot[6] = {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}. */
 int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6]);
 
 /* ---- diagnostics ---------------------------------------------------------- */

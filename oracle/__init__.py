@@ -75,6 +75,11 @@ def lib():
         L.orc_correct_position.argtypes = [vp, dp, C.c_int64]
         L.orc_fft.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_spectrum.argtypes = [vp, dp, dp, dp, dp]
+        L.orc_kkmax.restype = C.c_int
+        L.orc_kkmax.argtypes = [vp]
+        L.orc_stats_raw.argtypes = [vp, dp, dp]
+        L.orc_stats_finish.argtypes = [C.c_int, C.c_double, dp, dp, dp, dp]
+        L.orc_stats.argtypes = [vp, dp, dp]
         L.orc_force_nk.restype = C.c_int
         L.orc_force_nk.argtypes = [vp]
         L.orc_set_forcing.argtypes = [vp, dp, dp, dp]
@@ -92,6 +97,23 @@ def fft(x: np.ndarray, sign: int) -> np.ndarray:
     out = np.zeros_like(inp)
     lib().orc_fft(len(x), sign, inp, out)
     return out.view(np.complex128)
+
+
+# quantities of f_FluidstatsFs, in the order the oracle and the library return them
+STATS_NAMES = ("tke", "u1u1", "u2u2", "u3u3", "u1u2", "u1u3", "u2u3", "f_ken", "f_dis", "f_disskew",
+               "fn_ken", "fn_ken1", "fn_ken2", "fn_ken3", "fn_dis", "fn_disskew", "fn_int", "fn_int1",
+               "fn_int2", "fn_int3", "tn_tur", "ln_tur", "ln_tay", "rn_tay", "ln_int", "ln_int1",
+               "ln_int2", "ln_int3", "tn_edy", "ln_kol", "tn_kol", "disskew", "c_epsilon")
+SPEC_NAMES = ("ken", "ken1", "ken2", "ken3", "dis", "disskew", "dens_mod")
+
+
+def stats_finish(kkmax: int, nu: float, raw: np.ndarray, uu: np.ndarray):
+    """Normalisation and secondary quantities from raw shell sums."""
+    spec = np.zeros(7 * kkmax)
+    q = np.zeros(33)
+    lib().orc_stats_finish(kkmax, nu, np.ascontiguousarray(raw, np.float64), np.ascontiguousarray(uu, np.float64),
+                           spec, q)
+    return dict(zip(SPEC_NAMES, spec.reshape(7, kkmax))), dict(zip(STATS_NAMES, q))
 
 
 def normal_pair(seed: int, key: int):
@@ -142,6 +164,20 @@ class Oracle:
     def _wave(self, a):
         a = np.ascontiguousarray(a, dtype=np.complex128).reshape(self.shape)
         return a.view(np.float64).reshape(-1)
+
+    # ---- statistics (f_FluidstatsFs) -----------------------------------------
+    def stats_raw(self):
+        raw = np.zeros(7 * self.kkmax)
+        uu = np.zeros(6)
+        lib().orc_stats_raw(self.h, raw, uu)
+        return raw, uu
+
+    def stats(self):
+        """(spectra dict of kkmax arrays, dict of the 33 scalar quantities)"""
+        spec = np.zeros(7 * self.kkmax)
+        q = np.zeros(33)
+        lib().orc_stats(self.h, spec, q)
+        return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
 
     # ---- forcing -----------------------------------------------------------
     @property

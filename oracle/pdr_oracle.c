@@ -1142,42 +1142,155 @@ void orc_step(orc* o, int tstep, double* simtime) {
 /* ------------------------------------------------------------------------- */
 /* Wave-space statistics: src/pf_fluidstats.f90:1423-1576                     */
 /* ------------------------------------------------------------------------- */
-void orc_spectrum(const orc* o, double* ek, double* dk, double* dsk, double* tot) {
+/* ------------------------------------------------------------------------- */
+/* Wave-space statistics: f_FluidstatsFs (src/pf_fluidstats.f90:1293-1361)    */
+/* ------------------------------------------------------------------------- */
+/* lab-frame wavenumber and velocity of a mode: k_lab(n) = sum_m bmat(m,n) k(m),
+ * u_lab(m) = sum_n binv(m,n) u(n) over existing entries
+ * (f_FluidstatsFsPrimaryQuantsBFRot, src/pf_fluidstats.f90:1633-1642); the
+ * isotropic frame is the identity, which reproduces f_FluidstatsFsPrimaryQuants. */
+static void lab_frame(const orc* o, const double* kv, const double um[3][2], double* kl, double ul[3][2]) {
+  for (int n = 0; n < 3; n++) { kl[n] = 0.0; ul[n][0] = ul[n][1] = 0.0; }
+  for (int m = 1; m <= 3; m++)
+    for (int n = 1; n <= 3; n++)
+      if (T9(o->bval, m, n)) {
+        kl[n - 1] = kl[n - 1] + T9(o->bmat, m, n) * kv[m - 1];
+        ul[m - 1][0] = ul[m - 1][0] + T9(o->binv, m, n) * um[n - 1][0];
+        ul[m - 1][1] = ul[m - 1][1] + T9(o->binv, m, n) * um[n - 1][1];
+      }
+}
+
+/* bin kk in 1..kkmax with kk - 1/2 <= |k| < kk + 1/2, or 0 (src/pf_fluidstats.f90:1530-1545) */
+static int shell_of(double km, int kkmax) {
+  const int kc = (int)floor(km + 0.5);
+  for (int kk = kc - 1; kk <= kc + 1; kk++) {
+    if (kk < 1 || kk > kkmax) continue;
+    const double up_ = (double)kk + 0.5, lo_ = (double)kk - 0.5;
+    if (km >= lo_ && km < up_) return kk;
+  }
+  return 0;
+}
+
+/* raw[7*kkmax]: shell sums ken, ken1, ken2, ken3, dis, disskew, then the mode
+ * count (1 for i = 0, 2 otherwise); uu[6]: u1u1 u2u2 u3u3 u1u2 u1u3 u2u3 */
+void orc_stats_raw(const orc* o, double* raw, double* uu) {
   const int kkmax = o->kkmax;
-  for (int q = 0; q < kkmax; q++) ek[q] = dk[q] = dsk[q] = 0.0;
-  double s11 = 0, s22 = 0, s33 = 0;
+  for (int q = 0; q < 7 * kkmax; q++) raw[q] = 0.0;
+  for (int q = 0; q < 6; q++) uu[q] = 0.0;
   const double nu = o->p.f_nu;
   for (int i = 0; i < o->nki; i++) {
     const double wi = wvn(i, o->ki);
     for (int j = 0; j < o->nkj; j++) {
       const double wj = wvn(j, o->kj);
       for (int k = 0; k < o->nkk; k++) {
-        const double wk = wvn(k, o->kk);
-        const double km = sqrt((wi * wi) + (wj * wj) + (wk * wk));
+        const double kv[3] = {wi, wj, wvn(k, o->kk)};
         const long e = (((long)i * o->nkj + j) * o->nkk + k) * 2;
-        const double* a = &o->u[0][e];
-        const double* b = &o->u[1][e];
-        const double* c = &o->u[2][e];
-        double p11 = a[0] * a[0] + a[1] * a[1];
-        double p22 = b[0] * b[0] + b[1] * b[1];
-        double p33 = c[0] * c[0] + c[1] * c[1];
-        if (i != 0) { p11 = p11 * 2.0; p22 = p22 * 2.0; p33 = p33 * 2.0; }
-        s11 += p11; s22 += p22; s33 += p33;
-        const double e_ = 0.5 * (p11 + p22 + p33);
+        double um[3][2], ul[3][2], kl[3];
+        for (int c = 0; c < 3; c++) { um[c][0] = o->u[c][e]; um[c][1] = o->u[c][e + 1]; }
+        lab_frame(o, kv, um, kl, ul);
+        const double km = sqrt((kl[0] * kl[0]) + (kl[1] * kl[1]) + (kl[2] * kl[2]));
+        double ph[6];
+        ph[0] = ul[0][0] * ul[0][0] + ul[0][1] * ul[0][1];
+        ph[1] = ul[1][0] * ul[1][0] + ul[1][1] * ul[1][1];
+        ph[2] = ul[2][0] * ul[2][0] + ul[2][1] * ul[2][1];
+        ph[3] = ul[0][0] * ul[1][0] + ul[0][1] * ul[1][1];
+        ph[4] = ul[0][0] * ul[2][0] + ul[0][1] * ul[2][1];
+        ph[5] = ul[1][0] * ul[2][0] + ul[1][1] * ul[2][1];
+        if (i != 0) for (int q = 0; q < 6; q++) ph[q] = ph[q] * 2.0;
+        for (int q = 0; q < 6; q++) uu[q] += ph[q];
+        const double e_ = 0.5 * (ph[0] + ph[1] + ph[2]);
         const double d_ = 2.0 * km * km * nu * e_;
         const double ds = 4.0 * km * km * km * km * nu * nu * e_;
-        /* the reference scans every bin kk = 1..kkmax with [kk-1/2, kk+1/2);
-         * only the bins next to nint(|k|) can match */
-        const int kc = (int)floor(km + 0.5);
-        for (int kk = kc - 1; kk <= kc + 1; kk++) {
-          if (kk < 1 || kk > kkmax) continue;
-          const double up_ = (double)kk + 0.5, lo_ = (double)kk - 0.5;
-          if (km >= lo_ && km < up_) { ek[kk - 1] += e_; dk[kk - 1] += d_; dsk[kk - 1] += ds; }
+        const int kk = shell_of(km, kkmax);
+        if (kk) {
+          double* r = raw + (kk - 1);
+          r[0] += e_;
+          r[kkmax] += 0.5 * ph[0];
+          r[2 * kkmax] += 0.5 * ph[1];
+          r[3 * kkmax] += 0.5 * ph[2];
+          r[4 * kkmax] += d_;
+          r[5 * kkmax] += ds;
+          r[6 * kkmax] += (i != 0) ? 2.0 : 1.0;
         }
       }
     }
   }
+}
+
+/* From the raw sums: mode density (f_FluidstatsModeDensity[BFRot],
+ * src/pf_fluidstats.f90:696-774, 1977-2049), the normalised spectra and totals
+ * (1547-1575) and the secondary quantities (f_FluidstatsFsSecondaryQuants,
+ * 2491-2558).  spec[7*kkmax] = spec_ken, ken1, ken2, ken3, dis, disskew,
+ * dens_mod; q[33] in the order of pdr_stats_t. */
+void orc_stats_finish(int kkmax, double nu, const double* raw, const double* uu, double* spec, double* q) {
+  double f_ken = 0, f_dis = 0, f_ds = 0, fn[6] = {0, 0, 0, 0, 0, 0};
+  for (int kk = 1; kk <= kkmax; kk++) {
+    const double up_ = (double)kk + 0.5, lo_ = (double)kk - 0.5;
+    const double dens = raw[6 * kkmax + kk - 1] / (4.0 / 3.0 * ONEPI_REF * (pow(up_, 3) - pow(lo_, 3)));
+    spec[6 * kkmax + kk - 1] = dens;
+    for (int a = 0; a < 6; a++) spec[a * kkmax + kk - 1] = 0.0;
+    if (dens != 0.0) {
+      f_ken = f_ken + raw[kk - 1];
+      f_dis = f_dis + raw[4 * kkmax + kk - 1];
+      f_ds = f_ds + raw[5 * kkmax + kk - 1];
+      for (int a = 0; a < 6; a++) {
+        spec[a * kkmax + kk - 1] = raw[a * kkmax + kk - 1] / dens;
+        fn[a] = fn[a] + spec[a * kkmax + kk - 1];
+      }
+    }
+  }
+  q[0] = 0.5 * (uu[0] + uu[1] + uu[2]);
+  for (int a = 0; a < 6; a++) q[1 + a] = uu[a];
+  q[7] = f_ken; q[8] = f_dis; q[9] = f_ds;
+  for (int a = 0; a < 6; a++) q[10 + a] = fn[a];
+  const double fn_ken = fn[0], fn_dis = fn[4], fn_ds = fn[5];
+  const double fint = sqrt(2.0 * fn_ken / 3.0);
+  q[16] = fint;
+  q[17] = sqrt(2.0 * fn[1]);
+  q[18] = sqrt(2.0 * fn[2]);
+  q[19] = sqrt(2.0 * fn[3]);
+  q[20] = fn_ken / fn_dis;
+  q[21] = pow(fn_ken, 3.0 / 2.0) / fn_dis;
+  q[22] = sqrt((15.0 * nu * (fint * fint)) / fn_dis);
+  q[23] = q[22] * fint / nu;
+  double li[4] = {0, 0, 0, 0};
+  for (int kk = 1; kk <= kkmax; kk++) {
+    const double f = ONEPI_REF / 2.0 / (fint * fint);
+    li[0] = li[0] + (f * spec[kk - 1] / (double)kk);
+    li[1] = li[1] + (f * spec[kkmax + kk - 1] / (double)kk);
+    li[2] = li[2] + (f * spec[2 * kkmax + kk - 1] / (double)kk);
+    li[3] = li[3] + (f * spec[3 * kkmax + kk - 1] / (double)kk);
+  }
+  for (int a = 0; a < 4; a++) q[24 + a] = li[a];
+  q[28] = li[0] / fint;
+  q[29] = pow(nu * nu * nu / fn_dis, 1.0 / 4.0);
+  q[30] = sqrt(nu / fn_dis);
+  q[31] = fn_ds * pow(15.0 / fn_dis, 3.0 / 2.0) * sqrt(nu) / 35.0;
+  q[32] = fn_dis * li[0] / (fint * fint * fint);
+}
+
+int orc_kkmax(const orc* o) { return o->kkmax; }
+
+void orc_stats(const orc* o, double* spec, double* q) {
+  const int kkmax = o->kkmax;
+  double* raw = (double*)malloc(sizeof(double) * 7 * (size_t)kkmax);
+  double uu[6];
+  orc_stats_raw(o, raw, uu);
+  orc_stats_finish(kkmax, o->p.f_nu, raw, uu, spec, q);
+  free(raw);
+}
+
+/* shell sums before normalisation (ek, dk, dsk) and {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3} */
+void orc_spectrum(const orc* o, double* ek, double* dk, double* dsk, double* tot) {
+  const int kkmax = o->kkmax;
+  double* raw = (double*)malloc(sizeof(double) * 7 * (size_t)kkmax);
+  double uu[6];
+  orc_stats_raw(o, raw, uu);
   double fk = 0, fd = 0, fs = 0;
-  for (int q = 0; q < kkmax; q++) { fk += ek[q]; fd += dk[q]; fs += dsk[q]; }
-  tot[0] = fk; tot[1] = fd; tot[2] = fs; tot[3] = s11; tot[4] = s22; tot[5] = s33;
+  for (int q = 0; q < kkmax; q++) {
+    ek[q] = raw[q]; dk[q] = raw[4 * kkmax + q]; dsk[q] = raw[5 * kkmax + q];
+    if (raw[6 * kkmax + q] != 0.0) { fk += ek[q]; fd += dk[q]; fs += dsk[q]; }
+  }
+  tot[0] = fk; tot[1] = fd; tot[2] = fs; tot[3] = uu[0]; tot[4] = uu[1]; tot[5] = uu[2];
+  free(raw);
 }

@@ -104,6 +104,12 @@ void   orc_fft(int n, int sign, const double* in, double* out);
  * disskew shells and totals.  ek/dk/dsk have kkmax entries; tot[6] =
  * {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}. Mode density is not applied. */
 void   orc_spectrum(const orc* o, double* ek, double* dk, double* dsk, double* tot);
+/* f_FluidstatsFs: raw shell sums (7*kkmax) + Reynolds sums (6); finishing step
+ * (mode density, normalised spectra, secondary quantities: spec 7*kkmax, q 33) */
+int    orc_kkmax(const orc* o);
+void   orc_stats_raw(const orc* o, double* raw, double* uu);
+void   orc_stats_finish(int kkmax, double nu, const double* raw, const double* uu, double* spec, double* q);
+void   orc_stats(const orc* o, double* spec, double* q);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,6 @@ int pdr::set_error(int code, const std::string& msg) { return fail(code, msg); }
     if (e_ != hipSuccess) return fail(PDR_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-#define T9(a, m, n) ((a)[((m)-1) + 3 * ((n)-1)])
 
 // Reference RK3 coefficients (src/pg_constants.f90:56-62, src/pf_rk3.f90:3478-3488)
 static void rk_coeffs(int stage, double* beta, double* gamma) {
@@ -248,6 +247,7 @@ int pdr_destroy(pdr_ctx* c) {
   if (c->umax) (void)hipFree(c->umax);
   if (c->red) (void)hipFree(c->red);
   if (c->force) (void)hipFree(c->force);
+  if (c->stat_buf) (void)hipFree(c->stat_buf);
   if (c->tab.twz) (void)hipFree(c->tab.twz);
   if (c->tab.twy) (void)hipFree(c->tab.twy);
   if (c->tab.twx) (void)hipFree(c->tab.twx);

@@ -1,6 +1,5 @@
 // pdr_host.hip -- host-side pieces of the C-ABI that sit either side of the
-// time step: seeded initial conditions, PETSc-binary restart I/O and the
-// wave-space spectrum.
+// time step: seeded initial conditions and PETSc-binary restart I/O.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -65,18 +64,6 @@ uint64_t be_load(const unsigned char* p, int n) {
 }
 
 // Neumaier compensated summation
-struct KSum {
-  double s = 0.0, c = 0.0;
-  void add(double x) {
-    const double t = s + x;
-    if (std::fabs(s) >= std::fabs(x))
-      c += (s - t) + x;
-    else
-      c += (x - t) + s;
-    s = t;
-  }
-  double get() const { return s + c; }
-};
 
 }  // namespace
 
@@ -235,68 +222,6 @@ int pdr_restart_read(pdr_ctx* ctx, const char* path) {
   int rc = pdr_restart_read_host(path, n, u[0].data(), u[1].data(), u[2].data());
   if (rc != PDR_OK) return rc;
   return pdr_upload_state(ctx, u[0].data(), u[1].data(), u[2].data());
-}
-
-int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6]) {
-  if (!ctx || !ek || !dk || !dsk || !tot) return pdr::set_error(PDR_ERR_ARG, "pdr_spectrum: null argument");
-  const pdr::Geo& g = ctx->g;
-  const long long n = 2 * g.mw;
-  std::vector<double> u[3];
-  for (int c = 0; c < 3; ++c) u[c].resize(static_cast<size_t>(n));
-  int rc = pdr_download_state(ctx, u[0].data(), u[1].data(), u[2].data());
-  if (rc != PDR_OK) return rc;
-  const int kkmax = int(std::llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk))));
-  std::vector<KSum> se(static_cast<size_t>(kkmax)), sd(static_cast<size_t>(kkmax)), ss(static_cast<size_t>(kkmax));
-  KSum s11, s22, s33;
-  const double nu = ctx->p.f_nu;
-  for (int i = 0; i < g.nki; ++i) {
-    const double wi = double(i);
-    for (int j = 0; j < g.nkj; ++j) {
-      const double wj = wvn(j, g.kj);
-      for (int k = 0; k < g.nkk; ++k) {
-        const double wk = wvn(k, g.kk);
-        const double km = std::sqrt((wi * wi) + (wj * wj) + (wk * wk));
-        const size_t e = size_t((((long long)i * g.nkj + j) * g.nkk + k) * 2);
-        double p11 = u[0][e] * u[0][e] + u[0][e + 1] * u[0][e + 1];
-        double p22 = u[1][e] * u[1][e] + u[1][e + 1] * u[1][e + 1];
-        double p33 = u[2][e] * u[2][e] + u[2][e + 1] * u[2][e + 1];
-        if (i != 0) {
-          p11 *= 2.0;
-          p22 *= 2.0;
-          p33 *= 2.0;
-        }
-        s11.add(p11);
-        s22.add(p22);
-        s33.add(p33);
-        const double e_ = 0.5 * (p11 + p22 + p33);
-        const int kc = int(std::floor(km + 0.5));
-        for (int kb = kc - 1; kb <= kc + 1; ++kb) {
-          if (kb < 1 || kb > kkmax) continue;
-          if (km >= double(kb) - 0.5 && km < double(kb) + 0.5) {
-            se[size_t(kb - 1)].add(e_);
-            sd[size_t(kb - 1)].add(2.0 * km * km * nu * e_);
-            ss[size_t(kb - 1)].add(4.0 * km * km * km * km * nu * nu * e_);
-          }
-        }
-      }
-    }
-  }
-  KSum fk, fd, fs;
-  for (int q = 0; q < kkmax; ++q) {
-    ek[q] = se[size_t(q)].get();
-    dk[q] = sd[size_t(q)].get();
-    dsk[q] = ss[size_t(q)].get();
-    fk.add(ek[q]);
-    fd.add(dk[q]);
-    fs.add(dsk[q]);
-  }
-  tot[0] = fk.get();
-  tot[1] = fd.get();
-  tot[2] = fs.get();
-  tot[3] = s11.get();
-  tot[4] = s22.get();
-  tot[5] = s33.get();
-  return PDR_OK;
 }
 
 }  // extern "C"

@@ -9,6 +9,9 @@
 #include "pandora_amd.h"
 #include "pdr_fft.hpp"
 
+// Fortran-order 3x3 tensor access a(m, n), 1-based, as the reference's bmat/binv/amat
+#define T9(a, m, n) ((a)[((m)-1) + 3 * ((n)-1)])
+
 namespace pdr {
 
 // Domain sizes (g_DomainCalc, src/pg_domain.f90:220-228).
@@ -95,6 +98,8 @@ struct pdr_ctx {
   int64_t np = 0;
   unsigned long long* umax = nullptr;  // CFL reduction (bits of a non-negative double)
   double* red = nullptr;               // spectrum reduction scratch
+  void* stat_buf = nullptr;            // statistics partials (allocated on first use)
+  size_t stat_bytes = 0;
   double* force = nullptr;             // OU forcing state, [i][j+NK][k+NK][c] complex FP64
   int force_nk = 0;                    // NK = aint(FORCE_KF)
   bool forcing = false;                // F_TYPE 1 with FORCE_STOP > 0

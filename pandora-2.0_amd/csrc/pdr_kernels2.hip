@@ -58,7 +58,7 @@ template <typename T>
 constexpr int lds_x_bytes(int n) {
   return (n + 3 * xrows<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>)) + kBlock2 * int(sizeof(T));
 }
-constexpr int waves_for(int lds) {
+[[maybe_unused]] constexpr int waves_for(int lds) {
   return kLdsPerCu / lds < 1 ? 1 : (kLdsPerCu / lds > 8 ? 8 : kLdsPerCu / lds);
 }
 #define PDR_LINE_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_lines_bytes<T>(N)))))

@@ -1,0 +1,326 @@
+// pdr_stats.hip -- wave-space statistics of the velocity on the device:
+// f_FluidstatsFs (src/pf_fluidstats.f90:1293-1361) with the primary quantities
+// of f_FluidstatsFsPrimaryQuants / ...BFRot (1423-1576, 1586-1792), the mode
+// density f_FluidstatsModeDensity / ...BFRot (696-774, 1977-2049) and the
+// secondary quantities f_FluidstatsFsSecondaryQuants (2491-2558).
+//
+// One pass over the state: every mode is taken to the laboratory frame
+// (k_lab(n) = sum_m bmat(m,n) k(m), u_lab(m) = sum_n binv(m,n) u(n) over the
+// existing entries; the identity for isotropic runs), binned by |k_lab| into
+// [kk - 1/2, kk + 1/2), and its energy, component energies, dissipation and
+// dissipation-skewness terms and its mode count (1 on i = 0, 2 elsewhere) are
+// added to a per-workgroup LDS histogram.  The LDS adds are FP64 atomics that
+// return the old value, so each thread recovers the exact rounding error of its
+// add (TwoSum) and adds it to a second histogram: the per-workgroup sums are
+// compensated.  Workgroup partials are then reduced in workgroup order with
+// Neumaier summation, and the host finishes the kkmax-sized arrays in FP64.
+#include <cmath>
+#include <vector>
+
+#include "pdr_internal.hpp"
+
+namespace pdr {
+namespace {
+
+constexpr int kStatThreads = 256;
+constexpr int kStatBlocks = 1024;
+
+struct StatsFrame {
+  double bm[9];  // masked bmat, [m][n]
+  double bi[9];  // masked binv, [m][n]
+  double nu;
+  int kkmax;
+};
+
+__device__ __forceinline__ double wvn_s(int k, int kmax) {
+  return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1));
+}
+
+__device__ __forceinline__ void two_sum_add(double* hi, double* lo, double v) {
+  const double old = atomicAdd(hi, v);
+  const double t = old + v;  // the value the atomic stored
+  const double bv = t - old;
+  const double err = (old - (t - bv)) + (v - bv);
+  if (err != 0.0) atomicAdd(lo, err);
+}
+
+// LDS: hist[13][kkmax] = hi sums of 6 quantities, their lo parts, mode counts;
+// then 12 doubles x kStatThreads for the Reynolds-stress tree reduction.
+// Partial of workgroup b: part[b][13 * kkmax + 12].
+template <typename T>
+__global__ void __launch_bounds__(kStatThreads)
+    k_stats(Geo g, StatsFrame sf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
+            const cx<T>* __restrict__ u2, double* __restrict__ part) {
+  // no contraction into FMAs: each mode's |k_lab| and terms are then rounded
+  // exactly as in a plain FP64 evaluation, so shell membership near a bin
+  // edge cannot depend on the compiler
+#pragma clang fp contract(off)
+  extern __shared__ double sh[];
+  const int kkmax = sf.kkmax;
+  const int nh = 13 * kkmax;
+  double* red = sh + nh;
+  for (int q = threadIdx.x; q < nh; q += kStatThreads) sh[q] = 0.0;
+  __syncthreads();
+  double uu[6] = {0, 0, 0, 0, 0, 0}, uc[6] = {0, 0, 0, 0, 0, 0};
+  const long long stride = (long long)gridDim.x * kStatThreads;
+  for (long long e = (long long)blockIdx.x * kStatThreads + threadIdx.x; e < g.mw; e += stride) {
+    const int k = int(e % g.nkk);
+    const long long ij = e / g.nkk;
+    const int j = int(ij % g.nkj);
+    const int i = int(ij / g.nkj);
+    const double kv[3] = {double(i), wvn_s(j, g.kj), wvn_s(k, g.kk)};
+    const cx<T> a = u0[e], b = u1[e], c = u2[e];
+    const double um[3][2] = {{double(a.x), double(a.y)}, {double(b.x), double(b.y)}, {double(c.x), double(c.y)}};
+    double kl[3] = {0, 0, 0}, ul[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        kl[n] = kl[n] + sf.bm[3 * m + n] * kv[m];
+        ul[m][0] = ul[m][0] + sf.bi[3 * m + n] * um[n][0];
+        ul[m][1] = ul[m][1] + sf.bi[3 * m + n] * um[n][1];
+      }
+    const double km = sqrt((kl[0] * kl[0]) + (kl[1] * kl[1]) + (kl[2] * kl[2]));
+    double ph[6];
+    ph[0] = ul[0][0] * ul[0][0] + ul[0][1] * ul[0][1];
+    ph[1] = ul[1][0] * ul[1][0] + ul[1][1] * ul[1][1];
+    ph[2] = ul[2][0] * ul[2][0] + ul[2][1] * ul[2][1];
+    ph[3] = ul[0][0] * ul[1][0] + ul[0][1] * ul[1][1];
+    ph[4] = ul[0][0] * ul[2][0] + ul[0][1] * ul[2][1];
+    ph[5] = ul[1][0] * ul[2][0] + ul[1][1] * ul[2][1];
+    if (i != 0) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) ph[q] = ph[q] * 2.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {  // Neumaier per thread
+      const double t = uu[q] + ph[q];
+      uc[q] += (fabs(uu[q]) >= fabs(ph[q])) ? (uu[q] - t) + ph[q] : (ph[q] - t) + uu[q];
+      uu[q] = t;
+    }
+    const double ek = 0.5 * (ph[0] + ph[1] + ph[2]);
+    const double dk = 2.0 * km * km * sf.nu * ek;
+    const double dsk = 4.0 * km * km * km * km * sf.nu * sf.nu * ek;
+    int kk = 0;
+    const int kc = int(floor(km + 0.5));
+    for (int kb = kc - 1; kb <= kc + 1; ++kb)
+      if (kb >= 1 && kb <= kkmax && km >= double(kb) - 0.5 && km < double(kb) + 0.5) kk = kb;
+    if (kk) {
+      const int o = kk - 1;
+      two_sum_add(&sh[o], &sh[6 * kkmax + o], ek);
+      two_sum_add(&sh[kkmax + o], &sh[7 * kkmax + o], 0.5 * ph[0]);
+      two_sum_add(&sh[2 * kkmax + o], &sh[8 * kkmax + o], 0.5 * ph[1]);
+      two_sum_add(&sh[3 * kkmax + o], &sh[9 * kkmax + o], 0.5 * ph[2]);
+      two_sum_add(&sh[4 * kkmax + o], &sh[10 * kkmax + o], dk);
+      two_sum_add(&sh[5 * kkmax + o], &sh[11 * kkmax + o], dsk);
+      atomicAdd(&sh[12 * kkmax + o], (i != 0) ? 2.0 : 1.0);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    red[q * kStatThreads + threadIdx.x] = uu[q];
+    red[(6 + q) * kStatThreads + threadIdx.x] = uc[q];
+  }
+  __syncthreads();
+  for (int s = kStatThreads / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) {
+#pragma unroll
+      for (int q = 0; q < 12; ++q) red[q * kStatThreads + threadIdx.x] += red[q * kStatThreads + threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  double* out = part + (size_t)blockIdx.x * size_t(nh + 12);
+  for (int q = threadIdx.x; q < nh; q += kStatThreads) out[q] = sh[q];
+  if (threadIdx.x < 12) out[nh + threadIdx.x] = red[threadIdx.x * kStatThreads];
+}
+
+// out[q] = Neumaier sum over workgroups b = 0, 1, ... of part[b][q]
+__global__ void __launch_bounds__(kStatThreads)
+    k_stats_reduce(const double* __restrict__ part, int nblk, int width, double* __restrict__ out) {
+  const int q = int(blockIdx.x * kStatThreads + threadIdx.x);
+  if (q >= width) return;
+  double s = 0.0, c = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    const double v = part[(size_t)b * width + q];
+    const double t = s + v;
+    c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+    s = t;
+  }
+  out[q] = s + c;
+}
+
+template <typename T>
+hipError_t stats_t(pdr_ctx* c, const StatsFrame& sf, int nblk, double* part, double* out) {
+  const int width = 13 * sf.kkmax + 12;
+  const size_t lds = size_t(width - 12 + 12 * kStatThreads) * sizeof(double);
+  if (lds > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_stats<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    if (e != hipSuccess) return e;
+  }
+  k_stats<T><<<dim3(unsigned(nblk)), kStatThreads, lds, c->stream>>>(
+      c->g, sf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1], (const cx<T>*)c->u[2], part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_stats_reduce<<<dim3(unsigned((width + kStatThreads - 1) / kStatThreads)), kStatThreads, 0, c->stream>>>(
+      part, nblk, width, out);
+  return hipGetLastError();
+}
+
+int kkmax_of(const Geo& g) {
+  return int(std::llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk))));
+}
+
+// raw[7 * kkmax] (ken, ken1, ken2, ken3, dis, disskew, count) and uu[6]
+int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6]) {
+  const int kkmax = kkmax_of(c->g);
+  const size_t lds = size_t(13 * kkmax + 12 * kStatThreads) * sizeof(double);
+  if (lds > c->lds_max) return set_error(PDR_ERR_UNSUPPORTED, "statistics: spectrum too long for the LDS");
+  StatsFrame sf;
+  for (int m = 1; m <= 3; ++m)
+    for (int n = 1; n <= 3; ++n) {
+      sf.bm[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->bmat, m, n) : 0.0;
+      sf.bi[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->binv, m, n) : 0.0;
+    }
+  sf.nu = c->p.f_nu;
+  sf.kkmax = kkmax;
+  const long long want = (c->g.mw + 4LL * kStatThreads - 1) / (4LL * kStatThreads);
+  const int nblk = int(want < kStatBlocks ? (want < 1 ? 1 : want) : kStatBlocks);
+  const int width = 13 * kkmax + 12;
+  const size_t need = (size_t(nblk) + 1) * size_t(width) * sizeof(double);
+  if (c->stat_bytes < need) {
+    if (c->stat_buf) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(c->stat_buf);
+      c->dev_bytes -= c->stat_bytes;
+      c->stat_buf = nullptr;
+      c->stat_bytes = 0;
+    }
+    if (hipMalloc(&c->stat_buf, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(PDR_ERR_NOMEM, "statistics: device allocation failed");
+    }
+    c->stat_bytes = need;
+    c->dev_bytes += need;
+  }
+  double* part = (double*)c->stat_buf;
+  double* out = part + size_t(nblk) * width;
+  hipError_t e = c->prec ? stats_t<float>(c, sf, nblk, part, out) : stats_t<double>(c, sf, nblk, part, out);
+  if (e != hipSuccess) return set_error(PDR_ERR_HIP, std::string("statistics kernels: ") + hipGetErrorString(e));
+  std::vector<double> h(static_cast<size_t>(width));
+  e = hipMemcpyAsync(h.data(), out, size_t(width) * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_error(PDR_ERR_HIP, std::string("statistics copy: ") + hipGetErrorString(e));
+  raw.assign(size_t(7) * kkmax, 0.0);
+  for (int a = 0; a < 6; ++a)
+    for (int q = 0; q < kkmax; ++q)
+      raw[size_t(a * kkmax + q)] = h[size_t(a * kkmax + q)] + h[size_t((6 + a) * kkmax + q)];
+  for (int q = 0; q < kkmax; ++q) raw[size_t(6 * kkmax + q)] = h[size_t(12 * kkmax + q)];
+  for (int a = 0; a < 6; ++a) uu[a] = h[size_t(13 * kkmax + a)] + h[size_t(13 * kkmax + 6 + a)];
+  return PDR_OK;
+}
+
+// float32 pi widened to double (ONEPI, src/pg_constants.f90:126)
+const double kOnePiRef = double(3.14159265358979323846f);
+
+}  // namespace
+}  // namespace pdr
+
+extern "C" {
+
+int pdr_stats(pdr_ctx* ctx, pdr_stats_t* q, double* spec) {
+  if (!ctx || !q) return pdr::set_error(PDR_ERR_ARG, "pdr_stats: null argument");
+  std::vector<double> raw;
+  double uu[6];
+  int rc = pdr::stats_raw(ctx, raw, uu);
+  if (rc != PDR_OK) return rc;
+  const int kkmax = pdr::kkmax_of(ctx->g);
+  const double nu = ctx->p.f_nu;
+  std::vector<double> sp(size_t(7) * kkmax, 0.0);
+  // normalisation by the mode density (src/pf_fluidstats.f90:1547-1575, 767-769)
+  double f_ken = 0, f_dis = 0, f_ds = 0, fn[6] = {0, 0, 0, 0, 0, 0};
+  for (int kk = 1; kk <= kkmax; ++kk) {
+    const double up = double(kk) + 0.5, lo = double(kk) - 0.5;
+    const double dens = raw[size_t(6 * kkmax + kk - 1)] / (4.0 / 3.0 * pdr::kOnePiRef * (up * up * up - lo * lo * lo));
+    sp[size_t(6 * kkmax + kk - 1)] = dens;
+    if (dens != 0.0) {
+      f_ken = f_ken + raw[size_t(kk - 1)];
+      f_dis = f_dis + raw[size_t(4 * kkmax + kk - 1)];
+      f_ds = f_ds + raw[size_t(5 * kkmax + kk - 1)];
+      for (int a = 0; a < 6; ++a) {
+        sp[size_t(a * kkmax + kk - 1)] = raw[size_t(a * kkmax + kk - 1)] / dens;
+        fn[a] = fn[a] + sp[size_t(a * kkmax + kk - 1)];
+      }
+    }
+  }
+  q->tke = 0.5 * (uu[0] + uu[1] + uu[2]);
+  q->u1u1 = uu[0];
+  q->u2u2 = uu[1];
+  q->u3u3 = uu[2];
+  q->u1u2 = uu[3];
+  q->u1u3 = uu[4];
+  q->u2u3 = uu[5];
+  q->f_ken = f_ken;
+  q->f_dis = f_dis;
+  q->f_disskew = f_ds;
+  q->fn_ken = fn[0];
+  q->fn_ken1 = fn[1];
+  q->fn_ken2 = fn[2];
+  q->fn_ken3 = fn[3];
+  q->fn_dis = fn[4];
+  q->fn_disskew = fn[5];
+  // f_FluidstatsFsSecondaryQuants (src/pf_fluidstats.f90:2511-2556)
+  const double fint = std::sqrt(2.0 * fn[0] / 3.0);
+  q->fn_int = fint;
+  q->fn_int1 = std::sqrt(2.0 * fn[1]);
+  q->fn_int2 = std::sqrt(2.0 * fn[2]);
+  q->fn_int3 = std::sqrt(2.0 * fn[3]);
+  q->tn_tur = fn[0] / fn[4];
+  q->ln_tur = std::pow(fn[0], 3.0 / 2.0) / fn[4];
+  q->ln_tay = std::sqrt((15.0 * nu * (fint * fint)) / fn[4]);
+  q->rn_tay = q->ln_tay * fint / nu;
+  double li[4] = {0, 0, 0, 0};
+  for (int kk = 1; kk <= kkmax; ++kk) {
+    const double f = pdr::kOnePiRef / 2.0 / (fint * fint);
+    for (int a = 0; a < 4; ++a) li[a] = li[a] + (f * sp[size_t(a * kkmax + kk - 1)] / double(kk));
+  }
+  q->ln_int = li[0];
+  q->ln_int1 = li[1];
+  q->ln_int2 = li[2];
+  q->ln_int3 = li[3];
+  q->tn_edy = li[0] / fint;
+  q->ln_kol = std::pow(nu * nu * nu / fn[4], 1.0 / 4.0);
+  q->tn_kol = std::sqrt(nu / fn[4]);
+  q->disskew = fn[5] * std::pow(15.0 / fn[4], 3.0 / 2.0) * std::sqrt(nu) / 35.0;
+  q->c_epsilon = fn[4] * li[0] / (fint * fint * fint);
+  if (spec) std::copy(sp.begin(), sp.end(), spec);
+  return PDR_OK;
+}
+
+int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6]) {
+  if (!ctx || !ek || !dk || !dsk || !tot) return pdr::set_error(PDR_ERR_ARG, "pdr_spectrum: null argument");
+  std::vector<double> raw;
+  double uu[6];
+  int rc = pdr::stats_raw(ctx, raw, uu);
+  if (rc != PDR_OK) return rc;
+  const int kkmax = pdr::kkmax_of(ctx->g);
+  double fk = 0, fd = 0, fs = 0;
+  for (int q = 0; q < kkmax; ++q) {
+    ek[q] = raw[size_t(q)];
+    dk[q] = raw[size_t(4 * kkmax + q)];
+    dsk[q] = raw[size_t(5 * kkmax + q)];
+    if (raw[size_t(6 * kkmax + q)] != 0.0) {
+      fk += ek[q];
+      fd += dk[q];
+      fs += dsk[q];
+    }
+  }
+  tot[0] = fk;
+  tot[1] = fd;
+  tot[2] = fs;
+  tot[3] = uu[0];
+  tot[4] = uu[1];
+  tot[5] = uu[2];
+  return PDR_OK;
+}
+
+}  // extern "C"

@@ -42,6 +42,14 @@ class PdrError(RuntimeError):
         self.code = code
 
 
+# fields of pdr_stats_t and the rows of its spectrum array, in order
+STATS_NAMES = ("tke", "u1u1", "u2u2", "u3u3", "u1u2", "u1u3", "u2u3", "f_ken", "f_dis", "f_disskew",
+               "fn_ken", "fn_ken1", "fn_ken2", "fn_ken3", "fn_dis", "fn_disskew", "fn_int", "fn_int1",
+               "fn_int2", "fn_int3", "tn_tur", "ln_tur", "ln_tay", "rn_tay", "ln_int", "ln_int1",
+               "ln_int2", "ln_int3", "tn_edy", "ln_kol", "tn_kol", "disskew", "c_epsilon")
+SPEC_NAMES = ("ken", "ken1", "ken2", "ken3", "dis", "disskew", "dens_mod")
+
+
 class Params(C.Structure):
     _fields_ = [
         ("nodes_x", C.c_int32), ("nodes_y", C.c_int32), ("nodes_z", C.c_int32),
@@ -78,6 +86,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_download_state": (C.c_int, [vp, dp, dp, dp]),
         "pdr_upload_rk3": (C.c_int, [vp, dp, dp, dp]),
         "pdr_download_rk3": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_stats": (C.c_int, [vp, dp, dp]),
         "pdr_set_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_get_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_set_frame": (C.c_int, [vp, dp, dp]),
@@ -342,6 +351,14 @@ class Pandora:
 
     def restart_read(self, path: str):
         _check(self.L.pdr_restart_read(self.h, path.encode()))
+
+    def stats(self):
+        """f_FluidstatsFs on the device: (spectra dict of kkmax arrays, dict of
+        the pdr_stats_t quantities)."""
+        q = np.zeros(len(STATS_NAMES))
+        spec = np.zeros(7 * self.kkmax)
+        _check(self.L.pdr_stats(self.h, q, spec))
+        return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
 
     def spectrum(self):
         ek = np.zeros(self.kkmax)

@@ -1,0 +1,155 @@
+"""Pins for the oracle's wave-space statistics (f_FluidstatsFs,
+src/pf_fluidstats.f90:1293-1361, 1423-1792, 1977-2049, 2491-2558).
+
+Checks that share no code with the oracle: Parseval against a real-space field
+made with numpy's FFT (Reynolds stresses, TKE), the dissipation against
+nu <|curl u|^2> in real space, the mode density against a brute-force count of
+the full (Hermitian-completed) lattice, lab-frame binning against a count over
+the transformed lattice, and algebraic identities of the secondary quantities
+(Re_lambda^2 = 20 k^2 / (3 nu eps), eta^4 = nu^3 / eps, tau_eta^2 = nu / eps).
+"""
+import numpy as np
+import pytest
+
+ONEPI = float(np.float32(np.pi))
+
+
+def _wvn(idx, kmax):
+    idx = np.asarray(idx)
+    return np.where(idx <= kmax, idx, idx - (2 * kmax + 1)).astype(np.float64)
+
+
+def _random_state(o, seed):
+    """Random solenoidal field with the i = 0 plane Hermitian and u(0) = 0."""
+    rng = np.random.default_rng(seed)
+    ki = np.arange(o.nki, dtype=float)
+    kj = _wvn(np.arange(o.nkj), o.nkj // 2)
+    kk = _wvn(np.arange(o.nkk), o.nkk // 2)
+    KI, KJ, KK = np.meshgrid(ki, kj, kk, indexing="ij")
+    k2 = KI**2 + KJ**2 + KK**2
+    amp = np.exp(-k2 / 30.0)
+    u = [amp * (rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)) for _ in range(3)]
+    kd = (KI * u[0] + KJ * u[1] + KK * u[2]) / np.where(k2 > 0, k2, 1.0)
+    u = [u[0] - KI * kd, u[1] - KJ * kd, u[2] - KK * kd]
+    nj, nk = o.nkj, o.nkk
+    for c in range(3):
+        a = u[c]
+        a[0, 0, 0] = 0.0
+        for j in range(nj):
+            for k in range(nk):
+                jm, km = (-j) % nj, (-k) % nk
+                if (j, k) < (jm, km):
+                    a[0, jm, km] = np.conj(a[0, j, k])
+                elif (j, k) == (jm, km):
+                    a[0, j, k] = a[0, j, k].real
+    return u
+
+
+def _real_field(o, a):
+    """u(x) = sum_k a(k) exp(i k.x) on the N^3 grid, with numpy (independent FFT)."""
+    n = o.nx
+    K = o.nki - 1
+    full = np.zeros((n, n, n), complex)  # [x-freq][y-freq][z-freq]
+    for i in range(o.nki):
+        for j in range(o.nkj):
+            jy = int(_wvn(j, o.nkj // 2))
+            for k in range(o.nkk):
+                kz = int(_wvn(k, o.nkk // 2))
+                full[i % n, jy % n, kz % n] = a[i, j, k]
+                if i > 0:
+                    full[(-i) % n, (-jy) % n, (-kz) % n] = np.conj(a[i, j, k])
+    r = np.fft.ifftn(full) * n**3
+    assert np.abs(r.imag).max() < 1e-10 * max(np.abs(r.real).max(), 1e-300)
+    return r.real, full
+
+
+@pytest.fixture(scope="module")
+def iso(oracle_mod):
+    o = oracle_mod.Oracle(16, nu=0.013)
+    u = _random_state(o, 3)
+    o.set_state(*u)
+    return o, u
+
+
+def test_parseval_reynolds_stresses(iso):
+    o, u = iso
+    sp, q = o.stats()
+    r = [_real_field(o, a)[0] for a in u]
+    want = [np.mean(r[0] * r[0]), np.mean(r[1] * r[1]), np.mean(r[2] * r[2]),
+            np.mean(r[0] * r[1]), np.mean(r[0] * r[2]), np.mean(r[1] * r[2])]
+    got = [q[n] for n in ("u1u1", "u2u2", "u3u3", "u1u2", "u1u3", "u2u3")]
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-14)
+    assert abs(q["tke"] - 0.5 * (want[0] + want[1] + want[2])) < 1e-13 * q["tke"]
+    assert abs(q["f_ken"] - q["tke"]) < 1e-13 * q["tke"]  # every k != 0 mode is binned
+
+
+def test_dissipation_real_space(iso):
+    o, u = iso
+    nu = 0.013
+    n = o.nx
+    _, fulls = zip(*[_real_field(o, a) for a in u])
+    f = np.fft.fftfreq(n, 1.0 / n)
+    KX, KY, KZ = np.meshgrid(f, f, f, indexing="ij")
+    F1, F2, F3 = fulls
+    w = [1j * (KY * F3 - KZ * F2), 1j * (KZ * F1 - KX * F3), 1j * (KX * F2 - KY * F1)]
+    wr = [np.fft.ifftn(a).real * n**3 for a in w]
+    eps = nu * np.mean(wr[0] ** 2 + wr[1] ** 2 + wr[2] ** 2)
+    sp, q = o.stats()
+    assert abs(q["f_dis"] - eps) < 1e-12 * eps
+
+
+def test_mode_density_brute_force(iso):
+    o, _ = iso
+    sp, q = o.stats()
+    K = o.nki - 1
+    r = np.arange(-K, K + 1)
+    X, Y, Z = np.meshgrid(r, r, r, indexing="ij")
+    km = np.sqrt(X**2 + Y**2 + Z**2).ravel()
+    kkmax = o.kkmax
+    for kk in range(1, kkmax + 1):
+        cnt = np.count_nonzero((km >= kk - 0.5) & (km < kk + 0.5))
+        vol = 4.0 / 3.0 * ONEPI * ((kk + 0.5) ** 3 - (kk - 0.5) ** 3)
+        assert sp["dens_mod"][kk - 1] == pytest.approx(cnt / vol, rel=1e-15, abs=0)
+
+
+def test_secondary_identities(iso):
+    o, _ = iso
+    nu = 0.013
+    sp, q = o.stats()
+    k, eps = q["fn_ken"], q["fn_dis"]
+    assert q["rn_tay"] ** 2 == pytest.approx(20.0 * k * k / (3.0 * nu * eps), rel=1e-13)
+    assert q["ln_kol"] ** 4 == pytest.approx(nu**3 / eps, rel=1e-13)
+    assert q["tn_kol"] ** 2 == pytest.approx(nu / eps, rel=1e-13)
+    assert q["fn_ken"] == pytest.approx(sp["ken"].sum(), rel=1e-14)
+    # Pope 6.225: L = pi / (2 u'^2) int E(k) / k dk, u'^2 = 2k/3
+    L = ONEPI / (2.0 * (2.0 * k / 3.0)) * np.sum(sp["ken"] / np.arange(1, o.kkmax + 1))
+    assert q["ln_int"] == pytest.approx(L, rel=1e-13)
+    # disskew = S_eps * (15/eps)^{3/2} sqrt(nu) / 35 with S_eps = 4 nu^2 sum k^4 E
+    ds = q["fn_disskew"] * (15.0 / eps) ** 1.5 * np.sqrt(nu) / 35.0
+    assert q["disskew"] == pytest.approx(ds, rel=1e-13)
+
+
+def test_sheared_frame_bins_by_lab_wavenumber(oracle_mod):
+    o = oracle_mod.Oracle(16, f_type=2, rk3_timestep=2, deform_s=1.0, nu=0.01)
+    u = _random_state(o, 9)
+    o.set_state(*u)
+    b, bi = o.get_frame()
+    b[6], bi[6] = -0.7, 0.7   # bmat(1,3), binv(1,3): a sheared Rogallo frame
+    o.set_frame(b, bi)
+    sp, q = o.stats()
+    K = o.nki - 1
+    # counts over the compact storage with weight 2 for i > 0, |k_lab| with
+    # k_lab = bmat^T k (k_lab3 = k3 + bmat(1,3) k1)
+    ki = np.arange(0, K + 1)
+    kj = np.arange(-K, K + 1)
+    I, J, Kz = np.meshgrid(ki, kj, kj, indexing="ij")
+    kl = np.sqrt(I**2.0 + J**2.0 + (Kz + (-0.7) * I) ** 2.0).ravel()
+    wgt = np.where(I.ravel() > 0, 2, 1)
+    for kk in range(1, o.kkmax + 1):
+        m = (kl >= kk - 0.5) & (kl < kk + 0.5)
+        vol = 4.0 / 3.0 * ONEPI * ((kk + 0.5) ** 3 - (kk - 0.5) ** 3)
+        assert sp["dens_mod"][kk - 1] == pytest.approx(wgt[m].sum() / vol, rel=1e-15, abs=0)
+    # lab velocity u_lab1 = u1 + binv(1,3) u3 enters u1u1
+    w = np.where(np.arange(o.nki)[:, None, None] > 0, 2.0, 1.0)
+    ul1 = u[0] + 0.7 * u[2]
+    assert q["u1u1"] == pytest.approx(np.sum(w * np.abs(ul1) ** 2), rel=1e-13)
