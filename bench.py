@@ -100,7 +100,107 @@ def cpu_baseline(n, ntr):
                       f"FP64 C oracle, {cores} OpenMP threads: {stage_s:.2f} s"}
 
 
+# ---------------------------------------------------------------------------
+# rocprofv3 summaries (kernel trace + FETCH_SIZE / WRITE_SIZE counter runs of
+# this script), written to profiles/; the traffic JSON feeds roofline.traffic.
+# ---------------------------------------------------------------------------
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "rk3_step_512_shear_fp32_traffic.json")
+KERNEL_CLASS = {"zinv": "zinv", "yinv": "yinv", "xpass": "xpass", "yfwd": "yfwd", "zfwd": "zfwd",
+                "formf": "formf", "tracers": "tracers", "copy": "copy"}
+
+
+def _kernel_key(name):
+    import re
+    m = re.search(r"(k2?_[a-z0-9_]+)", name)
+    if not m:
+        return None, name[:40]
+    base = m.group(1)
+    cls = KERNEL_CLASS.get(base.split("_", 1)[1])
+    return cls, base
+
+
+def rocpd_summary(trace_db, fetch_db, write_db, out_json, copy_bytes=1 << 30):
+    """Markdown tables of a kernel trace and of per-dispatch FETCH_SIZE /
+    WRITE_SIZE (KiB), calibrated by the copy kernel k_copy (copy_bytes read and
+    written per launch, bench.py's bandwidth probe)."""
+    import collections
+    import sqlite3
+
+    rows = sqlite3.connect(trace_db).execute("select name, duration from kernels").fetchall()
+    agg = collections.OrderedDict()
+    for name, dur in rows:
+        cls, base = _kernel_key(name)
+        a = agg.setdefault(base, [0, 0.0, 1e30, 0.0])
+        a[0] += 1
+        a[1] += dur
+        a[2] = min(a[2], dur)
+        a[3] = max(a[3], dur)
+    tot = sum(a[1] for a in agg.values())
+    print("| kernel | calls | total ms | avg ms | min ms | max ms | % |")
+    print("|---|---|---|---|---|---|---|")
+    for base, a in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {base} | {a[0]} | {a[1] / 1e6:.3f} | {a[1] / a[0] / 1e6:.4f} | {a[2] / 1e6:.4f} | "
+              f"{a[3] / 1e6:.4f} | {100 * a[1] / tot:.1f} |")
+
+    def per_dispatch(db, counter):
+        con = sqlite3.connect(db)
+        vals = collections.defaultdict(lambda: collections.defaultdict(float))
+        for kn, cn, did, v in con.execute(
+                "select kernel_name, counter_name, dispatch_id, value from counters_collection"):
+            if cn == counter:
+                vals[_kernel_key(kn)[1]][did] += float(v)
+        return {k: sum(d.values()) / len(d) for k, d in vals.items()}
+
+    fetch = per_dispatch(fetch_db, "FETCH_SIZE")
+    write = per_dispatch(write_db, "WRITE_SIZE")
+    ff = copy_bytes / (fetch["k_copy"] * 1024.0)
+    wf = copy_bytes / (write["k_copy"] * 1024.0)
+    print()
+    print(f"calibration (k_copy, {copy_bytes} B in and out): FETCH_SIZE x{ff:.3f}, WRITE_SIZE x{wf:.3f}")
+    print()
+    print("| kernel | FETCH_SIZE KiB | read GB | WRITE_SIZE KiB | write GB | traffic GB |")
+    print("|---|---|---|---|---|---|")
+    kern = {}
+    for base in sorted(set(fetch) | set(write)):
+        cls, _ = _kernel_key(base)
+        fk, wk = fetch.get(base, 0.0), write.get(base, 0.0)
+        rb, wb = fk * 1024 * ff, wk * 1024 * wf
+        print(f"| {base} | {fk:,.0f} | {rb / 1e9:.3f} | {wk:,.0f} | {wb / 1e9:.3f} | {(rb + wb) / 1e9:.3f} |")
+        if cls and cls != "copy":
+            kern[cls] = {"kernel": base, "fetch_kib": fk, "write_kib": wk, "read_bytes": rb,
+                         "write_bytes": wb, "traffic_bytes": rb + wb}
+    if out_json:
+        with open(out_json, "w") as f:
+            json.dump({"workload": {"n": 512, "tracers": 1 << 20, "precision": "fp32-mixed"},
+                       "calibration": {"fetch_factor": ff, "write_factor": wf, "copy_bytes": copy_bytes},
+                       "kernels": kern}, f, indent=1)
+
+
+def measured_traffic(kernel, n, ntr):
+    """HBM bytes per launch of `kernel` from the committed counter summary, if
+    it was measured on this workload."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            d = json.load(f)
+    except OSError:
+        return None
+    w = d.get("workload", {})
+    if w.get("n") != n or w.get("tracers") != ntr or kernel not in d.get("kernels", {}):
+        return None
+    return d["kernels"][kernel]["traffic_bytes"]
+
+
 def main():
+    if "--rocpd-summary" in sys.argv:
+        sp = argparse.ArgumentParser()
+        sp.add_argument("--rocpd-summary", action="store_true")
+        sp.add_argument("--trace", required=True)
+        sp.add_argument("--fetch", required=True)
+        sp.add_argument("--write", required=True)
+        sp.add_argument("--json", default=None)
+        a = sp.parse_args()
+        rocpd_summary(a.trace, a.fetch, a.write, a.json)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -153,8 +253,11 @@ def main():
         cnt, ms = prof[dom]
         avg = ms / cnt * 1e-3
         b = kernel_bytes(g, dom, 8, 4, ntr)
+        traffic = measured_traffic(dom, n, ntr)
         roof = {"bound": "hbm", "kernel": dom, "achieved": b / avg / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": b / avg / HBM_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": b / avg / HBM_PEAK, "traffic": traffic,
+                "traffic_source": (os.path.relpath(TRAFFIC_JSON, ROOT) + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                   "copy-calibrated)") if traffic else None,
                 "bytes_per_launch": b, "avg_launch_ms": avg * 1e3}
     bstep = model_bytes_per_step(g, 8, 4, ntr)
     copy_gbps = pa.copy_bandwidth(1 << 30, 20)

@@ -263,49 +263,47 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
   load_tw<T, N>(tw, twg);
   cx<T>* rowp = w2 + ((long long)z * ny + y0) * g.ldi;
   // fill: the 3 * RB lines pack pairs of real inputs (p = 0: u1 + i u2,
-  // 1: u3 + i w1, 2: w2 + i w3); all loads of a thread are issued before its
-  // LDS stores
-  constexpr int QX = (3 * RB * N + kBlock2 - 1) / kBlock2;
-  constexpr int QC0 = sizeof(T) == 4 ? 24 : 12;  // elements per load batch
-  constexpr int QC = QX < QC0 ? QX : QC0;
-#pragma unroll 1
-  for (int q0 = 0; q0 < QX; q0 += QC) {
-    cx<T> X1[QC], X2[QC];
+  // 1: u3 + i w1, 2: w2 + i w3).  Each stored mode m' = 0..KI of a pair (a, b) is
+  // loaded once and written to both m' and its Hermitian image N - m'
+  // (conj(a) + i conj(b)); c2r uses only the real part of the DC term; the band
+  // KI < m < N - KI is zero.  All loads of a thread precede its LDS stores.
+  constexpr int NZB = N - 2 * KI - 1;
+  constexpr int EL = 3 * RB * NKI;
+  constexpr int QL = (EL + kBlock2 - 1) / kBlock2;
+  cx<T> A[QL], B[QL];
 #pragma unroll
-    for (int qq = 0; qq < QC; ++qq) {
-      const int idx = int(threadIdx.x) + kBlock2 * (q0 + qq);
-      const int line = idx / N;
-      const int m = idx - line * N;
-      const int r = line / 3;
-      const int p = line - 3 * r;
-      X1[qq] = czero<T>();
-      X2[qq] = czero<T>();
-      if ((QX % QC == 0 || q0 + qq < QX) && (3 * RB * N % kBlock2 == 0 || idx < 3 * RB * N) && y0 + r < ny) {
-        const cx<T>* fa = rowp + (long long)(2 * p) * g.m2 + r * g.ldi;
-        const cx<T>* fb = fa + g.m2;
-        if (m <= KI) {
-          X1[qq] = fa[m];
-          X2[qq] = fb[m];
-          if (m == 0) {  // c2r uses only the real part of the DC term
-            X1[qq].y = T(0);
-            X2[qq].y = T(0);
-          }
-        } else if (m >= N - KI) {
-          X1[qq] = fa[N - m];
-          X2[qq] = fb[N - m];
-          X1[qq].y = -X1[qq].y;
-          X2[qq].y = -X2[qq].y;
-        }
+  for (int q = 0; q < QL; ++q) {
+    const int idx = int(threadIdx.x) + kBlock2 * q;
+    const int line = idx / NKI;
+    const int m = idx - line * NKI;
+    const int r = line / 3;
+    const int p = line - 3 * r;
+    A[q] = czero<T>();
+    B[q] = czero<T>();
+    if ((EL % kBlock2 == 0 || idx < EL) && y0 + r < ny) {
+      const cx<T>* fa = rowp + (long long)(2 * p) * g.m2 + r * g.ldi;
+      A[q] = fa[m];
+      B[q] = fa[g.m2 + m];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < QL; ++q) {
+    const int idx = int(threadIdx.x) + kBlock2 * q;
+    if (EL % kBlock2 == 0 || idx < EL) {
+      const int line = idx / NKI;
+      const int m = idx - line * NKI;
+      cx<T>* row = buf + line * LD;
+      if (m == 0) {
+        row[0] = mk<T>(A[q].x, B[q].x);
+      } else {
+        row[lds_off<T>(m)] = mk<T>(A[q].x - B[q].y, A[q].y + B[q].x);
+        row[lds_off<T>(N - m)] = mk<T>(A[q].x + B[q].y, B[q].x - A[q].y);
       }
     }
-#pragma unroll
-    for (int qq = 0; qq < QC; ++qq) {
-      const int idx = int(threadIdx.x) + kBlock2 * (q0 + qq);
-      if ((QX % QC == 0 || q0 + qq < QX) && (3 * RB * N % kBlock2 == 0 || idx < 3 * RB * N)) {
-        const int line = idx / N;
-        buf[line * LD + lds_off<T>(idx - line * N)] = mk<T>(X1[qq].x - X2[qq].y, X1[qq].y + X2[qq].x);
-      }
-    }
+  }
+  for (int idx = threadIdx.x; idx < 3 * RB * NZB; idx += kBlock2) {
+    const int line = idx / NZB;
+    buf[line * LD + lds_off<T>(KI + 1 + idx - line * NZB)] = czero<T>();
   }
   __syncthreads();
   fft_ip<T, N, 3 * RB, LD, +1>(buf, tw);

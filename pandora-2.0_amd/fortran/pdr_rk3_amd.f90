@@ -48,7 +48,7 @@ module pdr_rk3_amd
   public :: f_RK3Stage1, f_RK3Stage2, f_RK3Stage3, f_RK3Remesh
   public :: pdr_PutVelocity, pdr_GetVelocity, pdr_PutParticles, pdr_GetParticles
   public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_Spectrum, pdr_ErrorMessage
-  public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing
+  public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing, pdr_Stats
 
   interface
     integer(c_int) function c_pdr_create(out, p) bind(C, name="pdr_create")
@@ -145,6 +145,11 @@ module pdr_rk3_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: c
       real(c_double), intent(out) :: ek(*), dk(*), dsk(*), tot(6)
+    end function
+    integer(c_int) function c_pdr_stats(c, q, spec) bind(C, name="pdr_stats")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: q(33), spec(*)
     end function
     type(c_ptr) function c_pdr_last_error() bind(C, name="pdr_last_error")
       import :: c_ptr
@@ -313,6 +318,17 @@ contains
     integer, intent(inout)           :: ierr
     ierr = int(c_pdr_spectrum(ctx, ek, dk, dsk, tot))
   end subroutine pdr_Spectrum
+
+  !> f_FluidstatsFs on the device: q(1:33) in the order of pdr_stats_t (tke,
+  !! fs_u1u1..fs_u2u3, f_ken, f_dis, f_disskew, fn_ken.., fn_int.., tn_tur,
+  !! ln_tur, ln_tay, rn_tay, ln_int.., tn_edy, ln_kol, tn_kol, disskew,
+  !! C_epsilon); spec(kkmax, 7) = spec_ken, spec_ken1..3, spec_dis,
+  !! spec_disskew, dens_mod
+  subroutine pdr_Stats ( q, spec, ierr )
+    real(kind=c_double), intent(out) :: q(33), spec(:,:)
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_stats(ctx, q, spec))
+  end subroutine pdr_Stats
 
   !> Text of the last library error
   function pdr_ErrorMessage () result ( msg )
