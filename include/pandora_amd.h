@@ -243,6 +243,26 @@ typedef struct pdr_stats_t {
  * spec_ken1..3, spec_dis, spec_disskew and dens_mod, bin kk at index kk-1. */
 int pdr_stats(pdr_ctx* ctx, pdr_stats_t* out, double* spec);
 
+/* Real-space statistics of the current velocity: f_FluidstatsRs2D and
+ * f_FluidstatsRsCollect (src/pf_fluidstats.f90:827-951, 962-1146), which the
+ * reference computes on the real-space planes of stage 1 (src/pf_rk3.f90:
+ * 1075-1078, 1124-1127).  Velocities are laboratory-frame (binv u; u for
+ * isotropic runs), vorticities those of the real-space planes.  Moments are
+ * normalised by NX NY NZ as there, including rs_rmsvort1, which the reference
+ * leaves unnormalised (src/pf_fluidstats.f90:1068) and this entry point does
+ * likewise.  One extra pass (z, y and x transforms of the state); the two-point
+ * correlation rs_lint11 of TS_CORR steps is not computed. */
+typedef struct pdr_rs_stats_t {
+  double rs_u1u1, rs_u2u2, rs_u3u3, rs_u1u2, rs_u1u3, rs_u2u3;
+  double rs_skew1, rs_skew2, rs_skew3, rs_flat1, rs_flat2, rs_flat3;
+  double rs_rmsvort1, rs_rmsvort2, rs_rmsvort3;
+  double rs_skewvort1, rs_skewvort2, rs_skewvort3;
+  double rs_flatvort1, rs_flatvort2, rs_flatvort3;
+  double omegamax1, omegamax2, omegamax3;
+  double ucfl;                                  /* max(|u1|+|u2|+|u3|), lab frame */
+} pdr_rs_stats_t;
+int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out);
+
 /* Shell sums before mode-density normalisation from the same device pass:
  * ek, dk, dsk have kkmax entries (energy, 2 nu k^2 E, 4 nu^2 k^4 E);
  * tot[6] = {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}. */

@@ -80,6 +80,9 @@ def lib():
         L.orc_stats_raw.argtypes = [vp, dp, dp]
         L.orc_stats_finish.argtypes = [C.c_int, C.c_double, dp, dp, dp, dp]
         L.orc_stats.argtypes = [vp, dp, dp]
+        L.orc_rs_raw.argtypes = [vp, dp]
+        L.orc_rs_finish.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
+        L.orc_rs_stats.argtypes = [vp, dp]
         L.orc_force_nk.restype = C.c_int
         L.orc_force_nk.argtypes = [vp]
         L.orc_set_forcing.argtypes = [vp, dp, dp, dp]
@@ -105,6 +108,11 @@ STATS_NAMES = ("tke", "u1u1", "u2u2", "u3u3", "u1u2", "u1u3", "u2u3", "f_ken", "
                "fn_int2", "fn_int3", "tn_tur", "ln_tur", "ln_tay", "rn_tay", "ln_int", "ln_int1",
                "ln_int2", "ln_int3", "tn_edy", "ln_kol", "tn_kol", "disskew", "c_epsilon")
 SPEC_NAMES = ("ken", "ken1", "ken2", "ken3", "dis", "disskew", "dens_mod")
+# f_FluidstatsRsCollect quantities, in the order the oracle and the library return them
+RS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "rs_skew1", "rs_skew2",
+            "rs_skew3", "rs_flat1", "rs_flat2", "rs_flat3", "rs_rmsvort1", "rs_rmsvort2", "rs_rmsvort3",
+            "rs_skewvort1", "rs_skewvort2", "rs_skewvort3", "rs_flatvort1", "rs_flatvort2",
+            "rs_flatvort3", "omegamax1", "omegamax2", "omegamax3", "ucfl")
 
 
 def stats_finish(kkmax: int, nu: float, raw: np.ndarray, uu: np.ndarray):
@@ -178,6 +186,17 @@ class Oracle:
         q = np.zeros(33)
         lib().orc_stats(self.h, spec, q)
         return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
+
+    def rs_raw(self):
+        raw = np.zeros(25)
+        lib().orc_rs_raw(self.h, raw)
+        return raw
+
+    def rs_stats(self):
+        """f_FluidstatsRs2D/RsCollect of the current state (dict of RS_NAMES)."""
+        out = np.zeros(25)
+        lib().orc_rs_stats(self.h, out)
+        return dict(zip(RS_NAMES, out))
 
     # ---- forcing -----------------------------------------------------------
     @property

@@ -525,6 +525,57 @@ static inline void nl_bfrot(const orc* o, const double* u, const double* om, dou
 /* forward x/y transforms with truncation.                                   */
 /* src/pf_rk3.f90:960-1237, 1245-1498, 1780-1940; src/pf_fftw.f90:632-834   */
 /* ------------------------------------------------------------------------- */
+/* u1..u6_2r of plane z from the 1r2w arrays: y transforms for each i
+ * (src/pf_rk3.f90:1437-1498, pf_fftw.f90:673-759), zero pad i to NX/2+1 and c2r
+ * in x (imaginary part of the DC term ignored).  r holds 6 planes [f][y][x];
+ * in/out hold 2 max(NX,NY) doubles, pc 2 NY NKI. */
+static void plane_fields(const orc* o, int z, double* in, double* out, double* pc, double* r) {
+  const int nx = o->nx, ny = o->ny, nz = o->nz, nki = o->nki, nkj = o->nkj, KJ = o->kj;
+  const long plane = (long)nx * ny;
+  for (int f = 0; f < 6; f++) {
+    /* y transforms for each i (src/pf_rk3.f90:1437-1498, pf_fftw.f90:673-759) */
+    for (int i = 0; i < nki; i++) {
+      for (int y = 0; y < 2 * ny; y++) in[y] = 0.0;
+      const double* src = &o->w[f][(((long)i * nz + z) * nkj) * 2];
+      for (int j = 0; j <= KJ; j++) { in[2 * j] = src[2 * j]; in[2 * j + 1] = src[2 * j + 1]; }
+      for (int j = KJ + 1; j < nkj; j++) {
+        int y = ny - nkj + j;
+        in[2 * y] = src[2 * j]; in[2 * y + 1] = src[2 * j + 1];
+      }
+      fft_exec(&o->fy, in, out, +1);
+      for (int y = 0; y < ny; y++) {
+        pc[2 * ((long)y * nki + i)] = out[2 * y];
+        pc[2 * ((long)y * nki + i) + 1] = out[2 * y + 1];
+      }
+    }
+    /* zero pad i to NX/2+1, c2r in x (imaginary part of the DC term ignored) */
+    for (int y = 0; y < ny; y++) {
+      for (int x = 0; x < 2 * nx; x++) in[x] = 0.0;
+      for (int i = 0; i < nki; i++) {
+        double re = pc[2 * ((long)y * nki + i)], im = pc[2 * ((long)y * nki + i) + 1];
+        if (i == 0) { in[0] = re; in[1] = 0.0; continue; }
+        in[2 * i] = re; in[2 * i + 1] = im;
+        in[2 * (nx - i)] = re; in[2 * (nx - i) + 1] = -im;
+      }
+      fft_exec(&o->fx, in, out, +1);
+      double* rr = &r[f * plane + (long)y * nx];
+      for (int x = 0; x < nx; x++) rr[x] = out[2 * x];
+    }
+  }
+}
+
+/* u_lab(m) = sum_n binv(m,n) u(n) over the existing entries
+ * (f_FluidstatsRs2D, src/pf_fluidstats.f90:853-866) */
+static void lab_velocity(const orc* o, double u1, double u2, double u3, double* ul) {
+  const double um[3] = {u1, u2, u3};
+  if (o->p.f_type == 1) { ul[0] = u1; ul[1] = u2; ul[2] = u3; return; }
+  for (int m = 1; m <= 3; m++) {
+    ul[m - 1] = 0.0;
+    for (int n = 1; n <= 3; n++)
+      if (T9(o->bval, m, n)) ul[m - 1] = ul[m - 1] + T9(o->binv, m, n) * um[n - 1];
+  }
+}
+
 void orc_realspace(orc* o, int stage, int tstep, double* dt) {
   (void)tstep;
   const int nx = o->nx, ny = o->ny, nz = o->nz, nki = o->nki, nkj = o->nkj, KJ = o->kj;
@@ -543,40 +594,15 @@ void orc_realspace(orc* o, int stage, int tstep, double* dt) {
     double* r = (double*)malloc(sizeof(double) * 6 * (size_t)plane);       /* u1..u6_2r */
 #pragma omp for schedule(dynamic)
     for (int z = 0; z < nz; z++) {
-      for (int f = 0; f < 6; f++) {
-        /* y transforms for each i (src/pf_rk3.f90:1437-1498, pf_fftw.f90:673-759) */
-        for (int i = 0; i < nki; i++) {
-          for (int y = 0; y < 2 * ny; y++) in[y] = 0.0;
-          const double* src = &o->w[f][(((long)i * nz + z) * nkj) * 2];
-          for (int j = 0; j <= KJ; j++) { in[2 * j] = src[2 * j]; in[2 * j + 1] = src[2 * j + 1]; }
-          for (int j = KJ + 1; j < nkj; j++) {
-            int y = ny - nkj + j;
-            in[2 * y] = src[2 * j]; in[2 * y + 1] = src[2 * j + 1];
-          }
-          fft_exec(&o->fy, in, out, +1);
-          for (int y = 0; y < ny; y++) {
-            pc[2 * ((long)y * nki + i)] = out[2 * y];
-            pc[2 * ((long)y * nki + i) + 1] = out[2 * y + 1];
-          }
-        }
-        /* zero pad i to NX/2+1, c2r in x (imaginary part of the DC term ignored) */
-        for (int y = 0; y < ny; y++) {
-          for (int x = 0; x < 2 * nx; x++) in[x] = 0.0;
-          for (int i = 0; i < nki; i++) {
-            double re = pc[2 * ((long)y * nki + i)], im = pc[2 * ((long)y * nki + i) + 1];
-            if (i == 0) { in[0] = re; in[1] = 0.0; continue; }
-            in[2 * i] = re; in[2 * i + 1] = im;
-            in[2 * (nx - i)] = re; in[2 * (nx - i) + 1] = -im;
-          }
-          fft_exec(&o->fx, in, out, +1);
-          double* rr = &r[f * plane + (long)y * nx];
-          for (int x = 0; x < nx; x++) rr[x] = out[2 * x];
-        }
-      }
-      /* ucfl (src/pf_fluidstats.f90:914-917): computed at stage 1 */
+      plane_fields(o, z, in, out, pc, r);
+      /* ucfl (src/pf_fluidstats.f90:914-917): computed at stage 1 from the
+       * laboratory-frame velocity u_lab = binv u_mov (853-866; identity for
+       * isotropic runs) */
       if (stage == 1) {
         for (long q = 0; q < plane; q++) {
-          double uu = fabs(r[q]) + fabs(r[plane + q]) + fabs(r[2 * plane + q]);
+          double ul[3];
+          lab_velocity(o, r[q], r[plane + q], r[2 * plane + q], ul);
+          double uu = fabs(ul[0]) + fabs(ul[1]) + fabs(ul[2]);
           if (uu > umax_all) umax_all = uu;
         }
       }
@@ -638,6 +664,77 @@ void orc_realspace(orc* o, int stage, int tstep, double* dt) {
         break;
     }
   }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Real-space statistics: f_FluidstatsRs2D + f_FluidstatsRsCollect            */
+/* (src/pf_fluidstats.f90:827-951, 962-1146) of the current state.            */
+/* raw[25]: sums v1v1 v2v2 v3v3 v1v2 v1v3 v2v3, v^3 (3), v^4 (3), w^2 (3),     */
+/* w^3 (3), w^4 (3); maxima |v1|+|v2|+|v3|, |w1|, |w2|, |w3|.                  */
+/* ------------------------------------------------------------------------- */
+void orc_rs_raw(orc* o, double* raw) {
+  orc_prepare_realspace(o);
+  const int nx = o->nx, ny = o->ny, nz = o->nz;
+  const long plane = (long)nx * ny;
+  for (int q = 0; q < 25; q++) raw[q] = 0.0;
+  double* in = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+  double* out = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+  double* pc = (double*)malloc(sizeof(double) * 2 * (size_t)ny * o->nki);
+  double* r = (double*)malloc(sizeof(double) * 6 * (size_t)plane);
+  for (int z = 0; z < nz; z++) {
+    plane_fields(o, z, in, out, pc, r);
+    for (long q = 0; q < plane; q++) {
+      double v[3];
+      lab_velocity(o, r[q], r[plane + q], r[2 * plane + q], v);
+      const double w[3] = {r[3 * plane + q], r[4 * plane + q], r[5 * plane + q]};
+      raw[0] += v[0] * v[0]; raw[1] += v[1] * v[1]; raw[2] += v[2] * v[2];
+      raw[3] += v[0] * v[1]; raw[4] += v[0] * v[2]; raw[5] += v[1] * v[2];
+      for (int c = 0; c < 3; c++) {
+        raw[6 + c] += v[c] * v[c] * v[c];
+        raw[9 + c] += v[c] * v[c] * v[c] * v[c];
+        raw[12 + c] += w[c] * w[c];
+        raw[15 + c] += w[c] * w[c] * w[c];
+        raw[18 + c] += w[c] * w[c] * w[c] * w[c];
+        if (fabs(w[c]) > raw[22 + c]) raw[22 + c] = fabs(w[c]);
+      }
+      const double u = fabs(v[0]) + fabs(v[1]) + fabs(v[2]);
+      if (u > raw[21]) raw[21] = u;
+    }
+  }
+  free(in); free(out); free(pc); free(r);
+}
+
+/* f_FluidstatsRsCollect normalisation (src/pf_fluidstats.f90:992-1116),
+ * including its rs_rmsvort1 that is not divided by the number of points.
+ * out[25]: rs_u1u1 rs_u2u2 rs_u3u3 rs_u1u2 rs_u1u3 rs_u2u3, rs_skew1..3,
+ * rs_flat1..3, rs_rmsvort1..3, rs_skewvort1..3, rs_flatvort1..3,
+ * omegamax1..3, ucfl */
+void orc_rs_finish(int nx, int ny, int nz, const double* raw, double* out) {
+  const double n3 = ((double)nx * (double)ny) * (double)nz;
+  for (int q = 0; q < 6; q++) out[q] = raw[q] / n3;
+  for (int c = 0; c < 3; c++) {
+    out[6 + c] = (raw[6 + c] / n3) / pow(out[c], 1.5);
+    out[9 + c] = (raw[9 + c] / n3) / (out[c] * out[c]);
+  }
+  double rms[3], sk[3], fl[3];
+  rms[0] = raw[12];
+  rms[1] = raw[13] / n3;
+  rms[2] = raw[14] / n3;
+  for (int c = 0; c < 3; c++) { sk[c] = raw[15 + c] / n3; fl[c] = raw[18 + c] / n3; }
+  for (int c = 0; c < 3; c++) {
+    rms[c] = pow(rms[c], 0.5);
+    out[12 + c] = rms[c];
+    out[15 + c] = sk[c] / (rms[c] * rms[c] * rms[c]);
+    out[18 + c] = fl[c] / ((rms[c] * rms[c]) * (rms[c] * rms[c]));
+  }
+  out[21] = raw[22]; out[22] = raw[23]; out[23] = raw[24];
+  out[24] = raw[21];
+}
+
+void orc_rs_stats(orc* o, double* out) {
+  double raw[25];
+  orc_rs_raw(o, raw);
+  orc_rs_finish(o->nx, o->ny, o->nz, raw, out);
 }
 
 /* ------------------------------------------------------------------------- */

@@ -110,6 +110,11 @@ int    orc_kkmax(const orc* o);
 void   orc_stats_raw(const orc* o, double* raw, double* uu);
 void   orc_stats_finish(int kkmax, double nu, const double* raw, const double* uu, double* spec, double* q);
 void   orc_stats(const orc* o, double* spec, double* q);
+/* f_FluidstatsRs2D/RsCollect of the current state: raw[25] sums and maxima,
+ * finished out[25] in the order of pdr_rs_stats_t (overwrites the 1r2w arrays) */
+void   orc_rs_raw(orc* o, double* raw);
+void   orc_rs_finish(int nx, int ny, int nz, const double* raw, double* out);
+void   orc_rs_stats(orc* o, double* out);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,6 @@
 // pdr_api.hip -- C-ABI of the MI355X RK3 step: context, state transfer, stage
 // orchestration, shear-frame clock and remesh decision (all FP64 on the host).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -115,7 +116,7 @@ static void frame_init(pdr_ctx* c) {
 }
 
 // Operator tensors from the current frame (see Coef).
-static Coef make_coef(const pdr_ctx* c) {
+Coef pdr::make_coef(const pdr_ctx* c) {
   Coef f;
   std::memset(&f, 0, sizeof(f));
   const double* B = c->bmat;
@@ -268,6 +269,9 @@ static size_t force_bytes(const pdr_ctx* c) {
   return 3 * size_t(c->force_nk + 1) * w * w * 16;
 }
 
+// x-pencil chunk (MB) used when the whole-box buffers do not fit
+constexpr double kZChunkMB = 2048.0;
+
 int pdr_create(pdr_ctx** out, const pdr_params* p) {
   if (!out || !p) return fail(PDR_ERR_ARG, "pdr_create: null argument");
   *out = nullptr;
@@ -312,6 +316,7 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   g.ldi = (g.nki + 15) / 16 * 16;
   g.mw = (long long)g.nki * g.nkj * g.nkk;
   g.m1 = (long long)g.nki * g.nz * g.ldj;
+  g.zc = g.nz;
   g.m2 = (long long)g.nz * g.ny * g.ldi;
   g.nr = (long long)nx * ny * nz;
   frame_init(c);
@@ -349,6 +354,34 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     c->spec = !generic && spec_supported(nx) && spec_supported(ny) && spec_supported(nz) &&
               spec_lds_need(c->prec, nx, ny, nz) <= c->lds_max;
   }
+  // The x-pencil buffer may hold a chunk of zc planes instead of the whole box
+  // (specialised kernels only): the y/x passes then run chunk by chunk.  It
+  // costs time (measured at 512^3 FP32: 73 steps/s whole, 66 with 192 MB
+  // chunks, 55 with 64 MB), so it is used only when the whole-box buffers
+  // would not fit in device memory, or when PDR_ZCHUNK_MB asks for it.
+  if (c->spec) {
+    const char* env = std::getenv("PDR_ZCHUNK_MB");
+    double mb = env && env[0] ? std::atof(env) : 0.0;
+    const double plane = 6.0 * double(ny) * double(g.ldi) * double(c->csz);
+    if (!(env && env[0])) {
+      size_t free_b = 0, total_b = 0;
+      if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+        const double need = double(g.mw) * 6.0 * double(c->csz) + 6.0 * double(g.m1) * double(c->csz) +
+                            plane * double(nz) + 3.0 * double(g.nr) * double(c->rsz);
+        if (need > 0.9 * double(free_b)) mb = kZChunkMB;
+      }
+    }
+    int zc = mb > 0.0 ? int(mb * 1048576.0 / plane) : nz;
+    zc = std::max(1, std::min(zc, nz));
+    g.zc = zc;
+    g.m2 = (long long)zc * g.ny * g.ldi;
+    // split the fields of the y passes over workgroups while a chunk is small
+    const int tiles = (g.nki + 15) / 16;
+    c->fpb_y = 6;
+    while (c->fpb_y > 1 && tiles * zc * (6 / c->fpb_y) < 1024) c->fpb_y = (c->fpb_y == 6) ? 3 : (c->fpb_y == 3 ? 2 : 1);
+    c->fpb_yf = 3;
+    while (c->fpb_yf > 1 && tiles * zc * (3 / c->fpb_yf) < 1024) c->fpb_yf = 1;
+  }
   fft_plan(&c->pz, nz);
   fft_plan(&c->py, ny);
   fft_plan(&c->px, nx);
@@ -365,8 +398,8 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     if (rc == PDR_OK) rc = dalloc(c, &c->rk[q], wbytes);
   }
   if (rc == PDR_OK) rc = dalloc(c, &c->w1, 6 * size_t(g.m1) * c->csz);
-  const size_t w2n = std::max<size_t>(6 * size_t(g.m2), 3 * size_t(g.mw));
-  if (rc == PDR_OK) rc = dalloc(c, &c->w2, w2n * c->csz);
+  if (rc == PDR_OK) rc = dalloc(c, &c->w2, 6 * size_t(g.m2) * c->csz);
+  if (rc == PDR_OK) c->nl = static_cast<char*>(c->w1) + 3 * size_t(g.m1) * c->csz;
   if (rc == PDR_OK) rc = dalloc(c, (void**)&c->umax, 256);
   const size_t fbytes = force_bytes(c);
   if (rc == PDR_OK && fbytes) {
@@ -536,10 +569,13 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
       stage == 1 && (c->p.rk3_timestep == PDR_RK3_TIMESTEP_CFL ||
                      (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE && c->p.f_type == PDR_F_ISOTROPIC));
   HIPCHK(launch_zinv(c, cf));
-  HIPCHK(launch_yinv(c));
   if (need_cfl) HIPCHK(hipMemsetAsync(c->umax, 0, sizeof(unsigned long long), c->stream));
-  HIPCHK(launch_xpass(c, cf, tracers, need_cfl));
-  HIPCHK(launch_yfwd(c));
+  for (int z0 = 0; z0 < c->g.nz; z0 += c->g.zc) {
+    const int nzc = std::min(c->g.zc, c->g.nz - z0);
+    HIPCHK(launch_yinv(c, z0, nzc));
+    HIPCHK(launch_xpass(c, cf, tracers, need_cfl, z0, nzc, nullptr));
+    HIPCHK(launch_yfwd(c, z0, nzc));
+  }
   if (stage == 1) {
     // f_RK3Tstep (src/pf_rk3.f90:3852-3948)
     if (need_cfl) {

@@ -148,6 +148,26 @@ __device__ __forceinline__ void fft_ip(cx<T>* __restrict__ buf, const cx<T>* __r
   }
 }
 
+// Global store of one complex element; NT selects a non-temporal (streaming)
+// store, which does not keep the line in the caches.
+template <bool NT, typename T>
+__device__ __forceinline__ void put(cx<T>* p, cx<T> v) {
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+  } else {
+    *p = v;
+  }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void put1(T* p, T v) {
+  if constexpr (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // Copy the length-N twiddle table into LDS (the caller synchronises).
 template <typename T, int N>
 __device__ __forceinline__ void load_tw(cx<T>* __restrict__ dst, const cx<T>* __restrict__ src) {

@@ -23,7 +23,8 @@ struct Geo {
   int ldi;            // i stride of the x-pencil buffer (NKI padded to 16)
   long long mw;       // modes per wave field      NKI*NKJ*NKK
   long long m1;       // elements per 1r2w field   NKI*NZ*ldj
-  long long m2;       // elements per x-pencil     NZ*NY*ldi
+  long long m2;       // elements per x-pencil field of the chunk buffer  zc*NY*ldi
+  int zc;             // z planes per chunk of the x-pencil buffer (NZ when unchunked)
   long long nr;       // real points               NX*NY*NZ
 };
 
@@ -91,7 +92,10 @@ struct pdr_ctx {
   void* rk[3] = {nullptr, nullptr, nullptr};
   // work buffers
   void* w1 = nullptr;  // 6 fields x m1 (1r2w layout)
-  void* w2 = nullptr;  // 6 fields x m2 (x-pencil layout); the NL fields of pass 5 alias it
+  void* w2 = nullptr;  // 6 fields x m2 (x-pencil layout, zc planes of z)
+  void* nl = nullptr;  // NL [c][i][j][k] written by pass 5 (aliases w1 fields 3..5)
+  int fpb_y = 6;       // fields per workgroup of the chunked inverse / forward y passes
+  int fpb_yf = 3;
   void* ur = nullptr;  // real-space velocity for tracers, [z][y][x][3]
   double* xp = nullptr;
   double* uprk = nullptr;
@@ -100,6 +104,8 @@ struct pdr_ctx {
   double* red = nullptr;               // spectrum reduction scratch
   void* stat_buf = nullptr;            // statistics partials (allocated on first use)
   size_t stat_bytes = 0;
+  void* rs_buf = nullptr;              // real-space moment partials (allocated on first use)
+  size_t rs_bytes = 0;
   double* force = nullptr;             // OU forcing state, [i][j+NK][k+NK][c] complex FP64
   int force_nk = 0;                    // NK = aint(FORCE_KF)
   bool forcing = false;                // F_TYPE 1 with FORCE_STOP > 0
@@ -129,9 +135,10 @@ namespace pdr {
 
 // launchers (pdr_kernels.hip); return hipSuccess or the launch error
 hipError_t launch_zinv(pdr_ctx* c, const Coef& cf);
-hipError_t launch_yinv(pdr_ctx* c);
-hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl);
-hipError_t launch_yfwd(pdr_ctx* c);
+hipError_t launch_yinv(pdr_ctx* c, int z0, int nzc);
+hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl, int z0, int nzc,
+                        double* rsp = nullptr);
+hipError_t launch_yfwd(pdr_ctx* c, int z0, int nzc);
 hipError_t launch_zfwd(pdr_ctx* c);
 hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
 hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
@@ -151,9 +158,11 @@ void fft_plan(FftPlan* p, int n);
 bool spec_supported(int n);
 size_t spec_lds_need(int prec, int nx, int ny, int nz);
 hipError_t launch2_zinv(pdr_ctx* c, const Coef& cf);
-hipError_t launch2_yinv(pdr_ctx* c);
-hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl);
-hipError_t launch2_yfwd(pdr_ctx* c);
+hipError_t launch2_yinv(pdr_ctx* c, int z0, int nzc);
+hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl, int z0, int nzc, double* rsp);
+int xpass_tiles(const pdr_ctx* c);  // x-pass workgroups per z plane (specialised path)
+Coef make_coef(const pdr_ctx* c);
+hipError_t launch2_yfwd(pdr_ctx* c, int z0, int nzc);
 hipError_t launch2_zfwd(pdr_ctx* c);
 
 // records the text returned by pdr_last_error() and returns code

@@ -23,6 +23,7 @@
 // All kernels are HBM-bandwidth bound; none is a dense contraction.
 #include "pdr_internal.hpp"
 #include "pdr_rng.hpp"
+#include "pdr_rsstats.hpp"
 
 namespace pdr {
 
@@ -149,7 +150,8 @@ __global__ void __launch_bounds__(kThreads)
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
-            unsigned long long* __restrict__ umax, FftPlan px, const cx<T>* __restrict__ tw) {
+            unsigned long long* __restrict__ umax, FftPlan px, const cx<T>* __restrict__ tw,
+            double* __restrict__ rsp) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[kThreads];
   const int nx = g.nx, KI = g.ki, nki = g.nki;
@@ -182,6 +184,8 @@ __global__ void __launch_bounds__(kThreads)
   cx<T>* R = fft_lines<T, +1>(A, B, px, 3, ld, tw);
   cx<T>* O = (R == A) ? B : A;
   T lmax = T(0);
+  RsAcc acc;
+  if (rsp) rs_init(acc);
   const T V[9] = {T(cf.V[0]), T(cf.V[1]), T(cf.V[2]), T(cf.V[3]), T(cf.V[4]),
                   T(cf.V[5]), T(cf.V[6]), T(cf.V[7]), T(cf.V[8])};
   const long long rr = ((long long)z * g.ny + y) * nx;
@@ -202,9 +206,10 @@ __global__ void __launch_bounds__(kThreads)
       d[2] = u3;
     }
     if (want_cfl) {
-      const T s = fabs(u1) + fabs(u2) + fabs(u3);
+      const T s = fabs(v1) + fabs(v2) + fabs(v3);  // lab-frame velocity (src/pf_fluidstats.f90:853-866, 914)
       lmax = s > lmax ? s : lmax;
     }
+    if (rsp) rs_add(acc, double(v1), double(v2), double(v3), double(o1), double(o2), double(o3));
     O[x] = mk<T>(n1, n2);
     O[ld + x] = mk<T>(n3, T(0));
   }
@@ -234,6 +239,10 @@ __global__ void __launch_bounds__(kThreads)
       const double v = double(red[0]);
       atomicMax(umax, (unsigned long long)__double_as_longlong(v));
     }
+  }
+  if (rsp) {
+    __shared__ double rs_lds[kRsWidth * (kThreads / 64)];
+    rs_block_out(acc, rs_lds, rsp + ((long long)z * g.ny + y) * kRsWidth);
   }
 }
 
@@ -731,28 +740,31 @@ static hipError_t yinv_t(pdr_ctx* c) {
   return hipGetLastError();
 }
 
-hipError_t launch_yinv(pdr_ctx* c) {
+// The generic y/x kernels work on the whole box (the buffer is not chunked
+// for them: zc = NZ); the specialised ones take a chunk of z planes.
+hipError_t launch_yinv(pdr_ctx* c, int z0, int nzc) {
   prof_begin(c, K_YINV);
-  hipError_t e = c->spec ? launch2_yinv(c) : c->prec ? yinv_t<float>(c) : yinv_t<double>(c);
+  hipError_t e = c->spec ? launch2_yinv(c, z0, nzc) : c->prec ? yinv_t<float>(c) : yinv_t<double>(c);
   prof_end(c, K_YINV);
   return e;
 }
 
 template <typename T>
-static hipError_t xpass_t(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+static hipError_t xpass_t(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, double* rsp) {
   const Geo& g = c->g;
   const size_t lds = 6 * size_t(g.nx + 1) * sizeof(cx<T>);
   hipError_t e = set_lds<T>((const void*)k_xpass<T>, lds);
   if (e != hipSuccess) return e;
   dim3 grid(g.ny, g.nz);
   k_xpass<T><<<grid, kThreads, lds, c->stream>>>(g, cf, (cx<T>*)c->w2, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0,
-                                                 c->umax, c->px, (const cx<T>*)c->tab.twx);
+                                                 c->umax, c->px, (const cx<T>*)c->tab.twx, rsp);
   return hipGetLastError();
 }
 
-hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, int z0, int nzc, double* rsp) {
   prof_begin(c, K_XPASS);
-  hipError_t e = c->spec ? launch2_xpass(c, cf, tr, cfl) : c->prec ? xpass_t<float>(c, cf, tr, cfl) : xpass_t<double>(c, cf, tr, cfl);
+  hipError_t e = c->spec ? launch2_xpass(c, cf, tr, cfl, z0, nzc, rsp)
+                 : c->prec ? xpass_t<float>(c, cf, tr, cfl, rsp) : xpass_t<double>(c, cf, tr, cfl, rsp);
   prof_end(c, K_XPASS);
   return e;
 }
@@ -770,9 +782,9 @@ static hipError_t yfwd_t(pdr_ctx* c) {
   return hipGetLastError();
 }
 
-hipError_t launch_yfwd(pdr_ctx* c) {
+hipError_t launch_yfwd(pdr_ctx* c, int z0, int nzc) {
   prof_begin(c, K_YFWD);
-  hipError_t e = c->spec ? launch2_yfwd(c) : c->prec ? yfwd_t<float>(c) : yfwd_t<double>(c);
+  hipError_t e = c->spec ? launch2_yfwd(c, z0, nzc) : c->prec ? yfwd_t<float>(c) : yfwd_t<double>(c);
   prof_end(c, K_YFWD);
   return e;
 }
@@ -786,7 +798,7 @@ static hipError_t zfwd_t(pdr_ctx* c) {
   if (e != hipSuccess) return e;
   dim3 grid((g.nkj + tj - 1) / tj, g.nki);
   const double norm = 1.0 / (double(g.nx) * double(g.ny) * double(g.nz));
-  k_zfwd<T><<<grid, kThreads, lds, c->stream>>>(g, (const cx<T>*)c->w1, (cx<T>*)c->w2, c->pz,
+  k_zfwd<T><<<grid, kThreads, lds, c->stream>>>(g, (const cx<T>*)c->w1, (cx<T>*)c->nl, c->pz,
                                                 (const cx<T>*)c->tab.twz, tj, T(norm));
   return hipGetLastError();
 }
@@ -804,7 +816,7 @@ static hipError_t formf_t(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
   const long long nb = (g.mw + kThreads - 1) / kThreads;
   k_formf<T><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(
       g, cf, (cx<T>*)c->u[0], (cx<T>*)c->u[1], (cx<T>*)c->u[2], (cx<T>*)c->rk[0], (cx<T>*)c->rk[1],
-      (cx<T>*)c->rk[2], (const cx<T>*)c->w2, (const double2*)c->force, c->forcing ? c->force_nk : 0,
+      (cx<T>*)c->rk[2], (const cx<T>*)c->nl, (const double2*)c->force, c->forcing ? c->force_nk : 0,
       c->p.force_kf, beta, gdt);
   return hipGetLastError();
 }

@@ -6,6 +6,7 @@
 // 16-element-padded strides, XCD-aware tile order, several x rows per block.
 #include "pdr_fft2.hpp"
 #include "pdr_internal.hpp"
+#include "pdr_rsstats.hpp"
 
 namespace pdr {
 
@@ -40,6 +41,14 @@ constexpr int tile_lines(int n) {
   return sizeof(T) == 4 ? (n <= 512 ? PDR_LINES_F32 : (n <= 1024 ? 8 : 4))
                         : (n <= 512 ? PDR_LINES_F64 : (n <= 1024 ? 4 : 2));
 }
+// forward y/z kernels (3 fields per tile) may use their own FP32 tile height
+#ifndef PDR_LINES_FWD_F32
+#define PDR_LINES_FWD_F32 PDR_LINES_F32
+#endif
+template <typename T>
+constexpr int tile_lines_fwd(int n) {
+  return (sizeof(T) == 4 && n <= 512) ? PDR_LINES_FWD_F32 : tile_lines<T>(n);
+}
 template <typename T>
 constexpr int xrows(int n) {
   return sizeof(T) == 4 ? (n <= 512 ? PDR_XROWS_F32 : 2) : (n <= 512 ? PDR_XROWS_F64 : 2);
@@ -47,21 +56,38 @@ constexpr int xrows(int n) {
 
 // LDS bytes of a workgroup, and the waves per SIMD that the 160 KB of LDS of a
 // CU admit (each 256-thread workgroup puts one wave on each of the 4 SIMDs).
-// The kernels declare that occupancy to the register allocator, which then
-// keeps to the 512 / waves registers per lane that it leaves.
+// The kernels declare that occupancy, capped at two, to the register
+// allocator: it then keeps within 256 registers per lane (without the hint it
+// takes more, and one wave per SIMD), and still uses fewer where it can.
 constexpr int kLdsPerCu = 160 * 1024;
 template <typename T>
 constexpr int lds_lines_bytes(int n) {
   return (n + tile_lines<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
 }
 template <typename T>
+constexpr int lds_fwd_bytes(int n) {
+  return (n + tile_lines_fwd<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
+}
+template <typename T>
 constexpr int lds_x_bytes(int n) {
   return (n + 3 * xrows<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>)) + kBlock2 * int(sizeof(T));
 }
+// Streaming (non-temporal) stores per kernel, bit mask: 1 zinv, 2 yinv,
+// 4 xpass, 8 yfwd, 16 zfwd.
+#ifndef PDR_NT_WRITES
+#define PDR_NT_WRITES 0
+#endif
+constexpr bool kNtZinv = (PDR_NT_WRITES & 1) != 0;
+constexpr bool kNtYinv = (PDR_NT_WRITES & 2) != 0;
+constexpr bool kNtXpass = (PDR_NT_WRITES & 4) != 0;
+constexpr bool kNtYfwd = (PDR_NT_WRITES & 8) != 0;
+constexpr bool kNtZfwd = (PDR_NT_WRITES & 16) != 0;
+
 [[maybe_unused]] constexpr int waves_for(int lds) {
-  return kLdsPerCu / lds < 1 ? 1 : (kLdsPerCu / lds > 8 ? 8 : kLdsPerCu / lds);
+  return kLdsPerCu / lds < 2 ? 1 : 2;
 }
 #define PDR_LINE_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_lines_bytes<T>(N)))))
+#define PDR_FWD_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_fwd_bytes<T>(N)))))
 #define PDR_X_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_x_bytes<T>(N)))))
 
 // Line kernels issue all global loads of a field (QE elements per thread)
@@ -175,7 +201,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
     for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
       const int z = idx / L;
       const int line = idx - z * L;
-      if (j0 + line < nkj) dst[z * g.ldj + line] = buf[line * LD + lds_off<T>(z)];
+      if (j0 + line < nkj) put<kNtZinv>(&dst[z * g.ldj + line], buf[line * LD + lds_off<T>(z)]);
     }
     __syncthreads();
   }
@@ -187,7 +213,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
 template <typename T, int N>
 __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
     k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
-            int ntile) {
+            int ntile, int z0, int nzc, int fpb) {
   constexpr int L = tile_lines<T>(N);
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
@@ -198,7 +224,10 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
   cx<T>* buf = tw + N;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
-  const int z = int(bid / unsigned(ntile));
+  const int zg = int(bid / unsigned(ntile));
+  const int zl = zg % nzc;        // plane within the chunk
+  const int fg = zg / nzc;        // field group
+  const int z = z0 + zl;
   const int i0 = tile * L;
   const int nki = g.nki;
   const int lstride = g.nz * g.ldj;
@@ -217,7 +246,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
       pre[q] = v;
     }
   };
-  for (int f = 0; f < 6; ++f) {
+  for (int f = fg * fpb; f < (fg + 1) * fpb; ++f) {
     load(f);
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
@@ -229,11 +258,11 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
     }
     __syncthreads();
     fft_ip<T, N, L, LD, +1>(buf, tw);
-    cx<T>* dst = w2 + (long long)f * g.m2 + (long long)z * N * g.ldi + i0;
+    cx<T>* dst = w2 + (long long)f * g.m2 + (long long)zl * N * g.ldi + i0;
     for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
       const int y = idx / L;
       const int line = idx - y * L;
-      if (i0 + line < nki) dst[y * g.ldi + line] = buf[line * LD + lds_off<T>(y)];
+      if (i0 + line < nki) put<kNtYinv>(&dst[y * g.ldi + line], buf[line * LD + lds_off<T>(y)]);
     }
     __syncthreads();
   }
@@ -242,10 +271,11 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
 // ---------------------------------------------------------------------------
 // pass 3: RB rows (z, y0..y0+RB-1) per block: c2r, product, r2c
 // ---------------------------------------------------------------------------
-template <typename T, int N>
+template <typename T, int N, bool RS>
 __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
     k2_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
-             unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile) {
+             unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile, int z0,
+             double* __restrict__ rsp) {
   constexpr int RB = xrows<T>(N);
   constexpr int LD = lds_ld<T>(N);
   constexpr int KI = N / 3;
@@ -257,11 +287,12 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
   cx<T>* buf = tw + N;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
-  const int z = int(bid / unsigned(ntile));
+  const int zl = int(bid / unsigned(ntile));  // plane within the chunk
+  const int z = z0 + zl;
   const int y0 = tile * RB;
   const int ny = g.ny;
   load_tw<T, N>(tw, twg);
-  cx<T>* rowp = w2 + ((long long)z * ny + y0) * g.ldi;
+  cx<T>* rowp = w2 + ((long long)zl * ny + y0) * g.ldi;
   // fill: the 3 * RB lines pack pairs of real inputs (p = 0: u1 + i u2,
   // 1: u3 + i w1, 2: w2 + i w3).  Each stored mode m' = 0..KI of a pair (a, b) is
   // loaded once and written to both m' and its Hermitian image N - m'
@@ -312,6 +343,8 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
   T nl[IT][3];
   T lmax = T(0);
   T* urow = ur + ((long long)z * ny + y0) * N * 3;
+  RsAcc acc;
+  if (RS) rs_init(acc);
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
     const int idx = int(threadIdx.x) + kBlock2 * q;
@@ -329,15 +362,21 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
       nl[q][2] = o1 * v2 - o2 * v1;
       if (want_tr && y0 + r < ny) {
         T* d = urow + idx * 3;
-        d[0] = u1;
-        d[1] = u2;
-        d[2] = u3;
+        put1<kNtXpass>(d, u1);
+        put1<kNtXpass>(d + 1, u2);
+        put1<kNtXpass>(d + 2, u3);
       }
       if (want_cfl) {
-        const T s = fabs(u1) + fabs(u2) + fabs(u3);
+        const T s = fabs(v1) + fabs(v2) + fabs(v3);  // lab-frame velocity (src/pf_fluidstats.f90:853-866, 914)
         lmax = s > lmax ? s : lmax;
       }
+      if (RS && y0 + r < ny) rs_add(acc, double(v1), double(v2), double(v3), double(o1), double(o2), double(o3));
     }
+  }
+  if (RS) {
+    // partial of this workgroup at its place in the whole box: (z, tile)
+    __shared__ double rs_lds[kRsWidth * (kBlock2 / 64)];
+    rs_block_out(acc, rs_lds, rsp + ((long long)z * ntile + tile) * kRsWidth);
   }
   __syncthreads();
   // forward lines: r -> n1 + i n2 of row r; RB + r/2 -> n3 of rows (r, r+1)
@@ -365,12 +404,12 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
     const cx<T> Z = buf[r * LD + lds_off<T>(m)], Zr = buf[r * LD + lds_off<T>(mr)];
     const cx<T> W = buf[(RB + r / 2) * LD + lds_off<T>(m)], Wr = buf[(RB + r / 2) * LD + lds_off<T>(mr)];
     cx<T>* d0 = rowp + r * g.ldi + m;
-    d0[0] = mk<T>(T(0.5) * (Z.x + Zr.x), T(0.5) * (Z.y - Zr.y));
-    d0[g.m2] = mk<T>(T(0.5) * (Z.y + Zr.y), T(-0.5) * (Z.x - Zr.x));
+    put<kNtXpass>(d0, mk<T>(T(0.5) * (Z.x + Zr.x), T(0.5) * (Z.y - Zr.y)));
+    put<kNtXpass>(d0 + g.m2, mk<T>(T(0.5) * (Z.y + Zr.y), T(-0.5) * (Z.x - Zr.x)));
     if (r & 1)
-      d0[2 * g.m2] = mk<T>(T(0.5) * (W.y + Wr.y), T(-0.5) * (W.x - Wr.x));
+      put<kNtXpass>(d0 + 2 * g.m2, mk<T>(T(0.5) * (W.y + Wr.y), T(-0.5) * (W.x - Wr.x)));
     else
-      d0[2 * g.m2] = mk<T>(T(0.5) * (W.x + Wr.x), T(0.5) * (W.y - Wr.y));
+      put<kNtXpass>(d0 + 2 * g.m2, mk<T>(T(0.5) * (W.x + Wr.x), T(0.5) * (W.y - Wr.y)));
   }
   if (want_cfl) {
     red[threadIdx.x] = lmax;
@@ -387,10 +426,10 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
 // pass 4: forward y FFT + truncation, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
     k2_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
-            int ntile) {
-  constexpr int L = tile_lines<T>(N);
+            int ntile, int z0, int nzc, int fpb) {
+  constexpr int L = tile_lines_fwd<T>(N);
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKJ = 2 * K + 1;
@@ -400,14 +439,17 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
   cx<T>* buf = tw + N;
   const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
-  const int z = int(bid / unsigned(ntile));
+  const int zg = int(bid / unsigned(ntile));
+  const int zl = zg % nzc;
+  const int fg = zg / nzc;
+  const int z = z0 + zl;
   const int i0 = tile * L;
   const int nki = g.nki;
   const int lstride = g.nz * g.ldj;
   load_tw<T, N>(tw, twg);
   cx<T> pre[QE];
   auto load = [&](int f) {
-    const cx<T>* src = w2 + (long long)f * g.m2 + (long long)z * N * g.ldi + i0;
+    const cx<T>* src = w2 + (long long)f * g.m2 + (long long)zl * N * g.ldi + i0;
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
       const int idx = int(threadIdx.x) + kBlock2 * q;
@@ -416,7 +458,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
       pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki) ? src[y * g.ldi + line] : czero<T>();
     }
   };
-  for (int f = 0; f < 3; ++f) {
+  for (int f = fg * fpb; f < (fg + 1) * fpb; ++f) {
     load(f);
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
@@ -434,7 +476,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
       const int j = idx - line * NKJ;
       if (i0 + line < nki) {
         const int yy = (j <= K) ? j : N - NKJ + j;
-        dst[line * lstride + j] = buf[line * LD + lds_off<T>(yy)];
+        put<kNtYfwd>(&dst[line * lstride + j], buf[line * LD + lds_off<T>(yy)]);
       }
     }
     __syncthreads();
@@ -445,10 +487,10 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
 // pass 5: forward z FFT, normalisation, truncation -> NL [c][i][j][k]
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
     k2_zfwd(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ nl, const cx<T>* __restrict__ twg,
             int ntile, T norm) {
-  constexpr int L = tile_lines<T>(N);
+  constexpr int L = tile_lines_fwd<T>(N);
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKK = 2 * K + 1;
@@ -491,7 +533,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
       const int line = idx / NKK;
       const int k = idx - line * NKK;
       const int zz = (k <= K) ? k : N - NKK + k;
-      dst[idx] = scl(buf[line * LD + lds_off<T>(zz)], norm);
+      put<kNtZfwd>(&dst[idx], scl(buf[line * LD + lds_off<T>(zz)], norm));
     }
     __syncthreads();
   }
@@ -500,6 +542,10 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
 template <typename T, int N>
 size_t lds_lines() {
   return size_t(lds_lines_bytes<T>(N));
+}
+template <typename T, int N>
+size_t lds_fwd() {
+  return size_t(lds_fwd_bytes<T>(N));
 }
 template <typename T, int N>
 size_t lds_x() {
@@ -526,61 +572,80 @@ hipError_t zinv_n(pdr_ctx* c, const Coef& cf) {
 }
 
 template <typename T, int N>
-hipError_t yinv_n(pdr_ctx* c) {
+hipError_t yinv_n(pdr_ctx* c, int z0, int nzc) {
   const Geo& g = c->g;
   constexpr int L = tile_lines<T>(N);
   const int nt = (g.nki + L - 1) / L;
   const size_t lds = lds_lines<T, N>();
   hipError_t e = big_lds((const void*)k2_yinv<T, N>, lds);
   if (e != hipSuccess) return e;
-  k2_yinv<T, N><<<dim3(unsigned(nt * g.nz)), kBlock2, lds, c->stream>>>(
-      g, (const cx<T>*)c->w1, (cx<T>*)c->w2, (const cx<T>*)c->tab.twy, nt);
+  k2_yinv<T, N><<<dim3(unsigned(nt * nzc * (6 / c->fpb_y))), kBlock2, lds, c->stream>>>(
+      g, (const cx<T>*)c->w1, (cx<T>*)c->w2, (const cx<T>*)c->tab.twy, nt, z0, nzc, c->fpb_y);
   return hipGetLastError();
 }
 
 template <typename T, int N>
-hipError_t xpass_n(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+hipError_t xpass_n(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, int z0, int nzc, double* rsp) {
   const Geo& g = c->g;
   constexpr int RB = xrows<T>(N);
   const int nt = (g.ny + RB - 1) / RB;
   const size_t lds = lds_x<T, N>();
-  hipError_t e = big_lds((const void*)k2_xpass<T, N>, lds);
+  if (rsp) {
+    hipError_t e = big_lds((const void*)k2_xpass<T, N, true>, lds);
+    if (e != hipSuccess) return e;
+    k2_xpass<T, N, true><<<dim3(unsigned(nt * nzc)), kBlock2, lds, c->stream>>>(
+        g, cf, (cx<T>*)c->w2, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0, c->umax, (const cx<T>*)c->tab.twx, nt, z0, rsp);
+    return hipGetLastError();
+  }
+  hipError_t e = big_lds((const void*)k2_xpass<T, N, false>, lds);
   if (e != hipSuccess) return e;
-  k2_xpass<T, N><<<dim3(unsigned(nt * g.nz)), kBlock2, lds, c->stream>>>(
-      g, cf, (cx<T>*)c->w2, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0, c->umax, (const cx<T>*)c->tab.twx, nt);
+  k2_xpass<T, N, false><<<dim3(unsigned(nt * nzc)), kBlock2, lds, c->stream>>>(
+      g, cf, (cx<T>*)c->w2, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0, c->umax, (const cx<T>*)c->tab.twx, nt, z0, rsp);
   return hipGetLastError();
 }
 
 template <typename T, int N>
-hipError_t yfwd_n(pdr_ctx* c) {
+hipError_t yfwd_n(pdr_ctx* c, int z0, int nzc) {
   const Geo& g = c->g;
-  constexpr int L = tile_lines<T>(N);
+  constexpr int L = tile_lines_fwd<T>(N);
   const int nt = (g.nki + L - 1) / L;
-  const size_t lds = lds_lines<T, N>();
+  const size_t lds = lds_fwd<T, N>();
   hipError_t e = big_lds((const void*)k2_yfwd<T, N>, lds);
   if (e != hipSuccess) return e;
-  k2_yfwd<T, N><<<dim3(unsigned(nt * g.nz)), kBlock2, lds, c->stream>>>(
-      g, (const cx<T>*)c->w2, (cx<T>*)c->w1, (const cx<T>*)c->tab.twy, nt);
+  k2_yfwd<T, N><<<dim3(unsigned(nt * nzc * (3 / c->fpb_yf))), kBlock2, lds, c->stream>>>(
+      g, (const cx<T>*)c->w2, (cx<T>*)c->w1, (const cx<T>*)c->tab.twy, nt, z0, nzc, c->fpb_yf);
   return hipGetLastError();
 }
 
 template <typename T, int N>
 hipError_t zfwd_n(pdr_ctx* c) {
   const Geo& g = c->g;
-  constexpr int L = tile_lines<T>(N);
+  constexpr int L = tile_lines_fwd<T>(N);
   const int nt = (g.nkj + L - 1) / L;
-  const size_t lds = lds_lines<T, N>();
+  const size_t lds = lds_fwd<T, N>();
   hipError_t e = big_lds((const void*)k2_zfwd<T, N>, lds);
   if (e != hipSuccess) return e;
   const double norm = 1.0 / (double(g.nx) * double(g.ny) * double(g.nz));
   k2_zfwd<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
-      g, (const cx<T>*)c->w1, (cx<T>*)c->w2, (const cx<T>*)c->tab.twz, nt, T(norm));
+      g, (const cx<T>*)c->w1, (cx<T>*)c->nl, (const cx<T>*)c->tab.twz, nt, T(norm));
   return hipGetLastError();
 }
 
 }  // namespace
 
 #define PDR_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)
+
+int xpass_tiles(const pdr_ctx* c) {
+  switch (c->g.nx) {
+#define PDR_XT(n) \
+  case n:         \
+    return c->prec ? (c->g.ny + xrows<float>(n) - 1) / xrows<float>(n) : (c->g.ny + xrows<double>(n) - 1) / xrows<double>(n);
+    PDR_SIZES(PDR_XT)
+#undef PDR_XT
+    default:
+      return c->g.ny;
+  }
+}
 
 bool spec_supported(int n) {
   switch (n) {
@@ -615,10 +680,14 @@ size_t spec_lds_need(int prec, int nx, int ny, int nz) {
   if (prec) {
     PDR_LDS(float, nz, lds_lines)
     PDR_LDS(float, ny, lds_lines)
+    PDR_LDS(float, nz, lds_fwd)
+    PDR_LDS(float, ny, lds_fwd)
     PDR_LDS(float, nx, lds_x)
   } else {
     PDR_LDS(double, nz, lds_lines)
     PDR_LDS(double, ny, lds_lines)
+    PDR_LDS(double, nz, lds_fwd)
+    PDR_LDS(double, ny, lds_fwd)
     PDR_LDS(double, nx, lds_x)
   }
 #undef PDR_LDS
@@ -639,9 +708,9 @@ hipError_t launch2_zinv(pdr_ctx* c, const Coef& cf) {
   }
 }
 
-hipError_t launch2_yinv(pdr_ctx* c) {
+hipError_t launch2_yinv(pdr_ctx* c, int z0, int nzc) {
   switch (c->g.ny) {
-#define PDR_CASE(n) PDR_DISPATCH(n, (yinv_n<float, n>(c)), (yinv_n<double, n>(c)))
+#define PDR_CASE(n) PDR_DISPATCH(n, (yinv_n<float, n>(c, z0, nzc)), (yinv_n<double, n>(c, z0, nzc)))
     PDR_SIZES(PDR_CASE)
 #undef PDR_CASE
     default:
@@ -649,9 +718,10 @@ hipError_t launch2_yinv(pdr_ctx* c) {
   }
 }
 
-hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
+hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, int z0, int nzc, double* rsp) {
   switch (c->g.nx) {
-#define PDR_CASE(n) PDR_DISPATCH(n, (xpass_n<float, n>(c, cf, tr, cfl)), (xpass_n<double, n>(c, cf, tr, cfl)))
+#define PDR_CASE(n) \
+  PDR_DISPATCH(n, (xpass_n<float, n>(c, cf, tr, cfl, z0, nzc, rsp)), (xpass_n<double, n>(c, cf, tr, cfl, z0, nzc, rsp)))
     PDR_SIZES(PDR_CASE)
 #undef PDR_CASE
     default:
@@ -659,9 +729,9 @@ hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl) {
   }
 }
 
-hipError_t launch2_yfwd(pdr_ctx* c) {
+hipError_t launch2_yfwd(pdr_ctx* c, int z0, int nzc) {
   switch (c->g.ny) {
-#define PDR_CASE(n) PDR_DISPATCH(n, (yfwd_n<float, n>(c)), (yfwd_n<double, n>(c)))
+#define PDR_CASE(n) PDR_DISPATCH(n, (yfwd_n<float, n>(c, z0, nzc)), (yfwd_n<double, n>(c, z0, nzc)))
     PDR_SIZES(PDR_CASE)
 #undef PDR_CASE
     default:

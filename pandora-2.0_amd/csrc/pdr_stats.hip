@@ -14,10 +14,12 @@
 // add (TwoSum) and adds it to a second histogram: the per-workgroup sums are
 // compensated.  Workgroup partials are then reduced in workgroup order with
 // Neumaier summation, and the host finishes the kkmax-sized arrays in FP64.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "pdr_internal.hpp"
+#include "pdr_rsstats.hpp"
 
 namespace pdr {
 namespace {
@@ -219,6 +221,36 @@ int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6]) {
   return PDR_OK;
 }
 
+// Column q of the real-space partials: a sum (q < kRsSums, Neumaier per thread,
+// then a fixed-order tree) or a maximum, over nblk workgroup partials.
+__global__ void __launch_bounds__(kStatThreads)
+    k_rs_reduce(const double* __restrict__ part, long long nblk, double* __restrict__ out) {
+  __shared__ double sh[kStatThreads];
+  const int q = int(blockIdx.x);
+  const bool is_max = q >= kRsSums;
+  double s = 0.0, c = 0.0;
+  for (long long b = threadIdx.x; b < nblk; b += kStatThreads) {
+    const double v = part[b * kRsWidth + q];
+    if (is_max) {
+      s = v > s ? v : s;
+    } else {
+      const double t = s + v;
+      c += (fabs(s) >= fabs(v)) ? (s - t) + v : (v - t) + s;
+      s = t;
+    }
+  }
+  sh[threadIdx.x] = is_max ? s : s + c;
+  __syncthreads();
+  for (int w = kStatThreads / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+      const double a = sh[threadIdx.x], b = sh[threadIdx.x + w];
+      sh[threadIdx.x] = is_max ? (b > a ? b : a) : a + b;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = sh[0];
+}
+
 // float32 pi widened to double (ONEPI, src/pg_constants.f90:126)
 const double kOnePiRef = double(3.14159265358979323846f);
 
@@ -293,6 +325,87 @@ int pdr_stats(pdr_ctx* ctx, pdr_stats_t* q, double* spec) {
   q->disskew = fn[5] * std::pow(15.0 / fn[4], 3.0 / 2.0) * std::sqrt(nu) / 35.0;
   q->c_epsilon = fn[4] * li[0] / (fint * fint * fint);
   if (spec) std::copy(sp.begin(), sp.end(), spec);
+  return PDR_OK;
+}
+
+int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out) {
+  if (!ctx || !out) return pdr::set_error(PDR_ERR_ARG, "pdr_rs_stats: null argument");
+  pdr_ctx* c = ctx;
+  const pdr::Geo& g = c->g;
+  const long long nblk = c->spec ? (long long)pdr::xpass_tiles(c) * g.nz : (long long)g.ny * g.nz;
+  const size_t need = size_t(nblk + 1) * pdr::kRsWidth * sizeof(double);
+  if (c->rs_bytes < need) {
+    if (c->rs_buf) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(c->rs_buf);
+      c->dev_bytes -= c->rs_bytes;
+      c->rs_buf = nullptr;
+      c->rs_bytes = 0;
+    }
+    if (hipMalloc(&c->rs_buf, need) != hipSuccess) {
+      (void)hipGetLastError();
+      return pdr::set_error(PDR_ERR_NOMEM, "pdr_rs_stats: device allocation failed");
+    }
+    c->rs_bytes = need;
+    c->dev_bytes += need;
+  }
+  double* part = (double*)c->rs_buf;
+  double* dres = part + size_t(nblk) * pdr::kRsWidth;
+  // the real-space planes of the current state: pass 1, then the y and x
+  // passes with the moments accumulated in the x pass (its outputs are scratch)
+  const pdr::Coef cf = pdr::make_coef(c);
+  hipError_t e = pdr::launch_zinv(c, cf);
+  for (int z0 = 0; e == hipSuccess && z0 < g.nz; z0 += g.zc) {
+    const int nzc = std::min(g.zc, g.nz - z0);
+    e = pdr::launch_yinv(c, z0, nzc);
+    if (e == hipSuccess) e = pdr::launch_xpass(c, cf, false, false, z0, nzc, part);
+  }
+  if (e == hipSuccess) {
+    pdr::k_rs_reduce<<<dim3(unsigned(pdr::kRsWidth)), pdr::kStatThreads, 0, c->stream>>>(part, nblk, dres);
+    e = hipGetLastError();
+  }
+  double raw[pdr::kRsWidth];
+  if (e == hipSuccess) e = hipMemcpyAsync(raw, dres, sizeof(raw), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return pdr::set_error(PDR_ERR_HIP, std::string("pdr_rs_stats: ") + hipGetErrorString(e));
+  // f_FluidstatsRsCollect (src/pf_fluidstats.f90:992-1116); rs_rmsvort1 is not
+  // divided by the number of points, as there
+  const double n3 = (double(g.nx) * double(g.ny)) * double(g.nz);
+  double uu[6], sk[3], fl[3], rms[3], skv[3], flv[3];
+  for (int q = 0; q < 6; ++q) uu[q] = raw[q] / n3;
+  for (int q = 0; q < 3; ++q) {
+    sk[q] = (raw[6 + q] / n3) / std::pow(uu[q], 1.5);
+    fl[q] = (raw[9 + q] / n3) / (uu[q] * uu[q]);
+    rms[q] = (q == 0) ? raw[12] : raw[12 + q] / n3;
+    rms[q] = std::pow(rms[q], 0.5);
+    skv[q] = (raw[15 + q] / n3) / (rms[q] * rms[q] * rms[q]);
+    flv[q] = (raw[18 + q] / n3) / ((rms[q] * rms[q]) * (rms[q] * rms[q]));
+  }
+  out->rs_u1u1 = uu[0];
+  out->rs_u2u2 = uu[1];
+  out->rs_u3u3 = uu[2];
+  out->rs_u1u2 = uu[3];
+  out->rs_u1u3 = uu[4];
+  out->rs_u2u3 = uu[5];
+  out->rs_skew1 = sk[0];
+  out->rs_skew2 = sk[1];
+  out->rs_skew3 = sk[2];
+  out->rs_flat1 = fl[0];
+  out->rs_flat2 = fl[1];
+  out->rs_flat3 = fl[2];
+  out->rs_rmsvort1 = rms[0];
+  out->rs_rmsvort2 = rms[1];
+  out->rs_rmsvort3 = rms[2];
+  out->rs_skewvort1 = skv[0];
+  out->rs_skewvort2 = skv[1];
+  out->rs_skewvort3 = skv[2];
+  out->rs_flatvort1 = flv[0];
+  out->rs_flatvort2 = flv[1];
+  out->rs_flatvort3 = flv[2];
+  out->omegamax1 = raw[22];
+  out->omegamax2 = raw[23];
+  out->omegamax3 = raw[24];
+  out->ucfl = raw[21];
   return PDR_OK;
 }
 

@@ -48,6 +48,11 @@ STATS_NAMES = ("tke", "u1u1", "u2u2", "u3u3", "u1u2", "u1u3", "u2u3", "f_ken", "
                "fn_int2", "fn_int3", "tn_tur", "ln_tur", "ln_tay", "rn_tay", "ln_int", "ln_int1",
                "ln_int2", "ln_int3", "tn_edy", "ln_kol", "tn_kol", "disskew", "c_epsilon")
 SPEC_NAMES = ("ken", "ken1", "ken2", "ken3", "dis", "disskew", "dens_mod")
+# fields of pdr_rs_stats_t, in order
+RS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "rs_skew1", "rs_skew2",
+            "rs_skew3", "rs_flat1", "rs_flat2", "rs_flat3", "rs_rmsvort1", "rs_rmsvort2", "rs_rmsvort3",
+            "rs_skewvort1", "rs_skewvort2", "rs_skewvort3", "rs_flatvort1", "rs_flatvort2",
+            "rs_flatvort3", "omegamax1", "omegamax2", "omegamax3", "ucfl")
 
 
 class Params(C.Structure):
@@ -87,6 +92,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_upload_rk3": (C.c_int, [vp, dp, dp, dp]),
         "pdr_download_rk3": (C.c_int, [vp, dp, dp, dp]),
         "pdr_stats": (C.c_int, [vp, dp, dp]),
+        "pdr_rs_stats": (C.c_int, [vp, dp]),
         "pdr_set_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_get_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_set_frame": (C.c_int, [vp, dp, dp]),
@@ -359,6 +365,12 @@ class Pandora:
         spec = np.zeros(7 * self.kkmax)
         _check(self.L.pdr_stats(self.h, q, spec))
         return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
+
+    def rs_stats(self):
+        """f_FluidstatsRs2D/RsCollect of the current state on the device (dict of RS_NAMES)."""
+        out = np.zeros(len(RS_NAMES))
+        _check(self.L.pdr_rs_stats(self.h, out))
+        return dict(zip(RS_NAMES, out))
 
     def spectrum(self):
         ek = np.zeros(self.kkmax)

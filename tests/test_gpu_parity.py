@@ -358,3 +358,33 @@ def test_radix5_length_uses_generic_path(oracle_mod):
         tg = g.rk3_calc(s, tg)
     for a, b in zip(g.get_state(), o.get_state()):
         assert rel(a, b) < 1e-12
+
+
+@pytest.mark.parametrize("prec", [pa.PDR_FP64, pa.PDR_FP32_MIXED])
+def test_chunked_plane_pass_parity(oracle_mod, monkeypatch, prec):
+    """The y/x passes run chunk by chunk over z with a small x-pencil buffer
+    (PDR_ZCHUNK_MB = 0.75 MB: 5 planes per chunk at 48^3 FP64, the last chunk
+    partial); shear with remesh and tracers stays at parity."""
+    note(f"chunked plane pass N=48 prec={prec}, shear + tracers, 6 steps")
+    monkeypatch.setenv("PDR_ZCHUNK_MB", "0.75")
+    n = 48
+    kw = dict(f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=2, nu=0.005)
+    o, g = _pair(oracle_mod, n, precision=prec, **kw)
+    u = pa.init_field_host(n, 77, 2.0 / 3.0, 4.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    rng = np.random.default_rng(8)
+    dx = TWOPI / n
+    xp = rng.uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(500, 3))
+    o.set_particles(xp)
+    g.set_particles(xp)
+    to = tg = 0.0
+    for s in range(1, 7):
+        to = o.step(s, to)
+        tg = g.rk3_calc(s, tg)
+    tol = 1e-12 if prec == pa.PDR_FP64 else 1e-5
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert rel(a, b) < tol
+    xo = o.get_particles()[0]
+    xg = g.get_particles()[0]
+    assert np.abs(xg - xo).max() < (1e-11 if prec == pa.PDR_FP64 else 1e-5)

@@ -82,3 +82,72 @@ def test_stats_large_grid_runs():
     assert np.isfinite(list(q.values())).all()
     assert q["f_ken"] == pytest.approx(q["tke"], rel=1e-12)
     assert 0.5 < q["tke"] < 1.5
+
+
+def _compare_rs(g, o):
+    qg, qo = g.rs_stats(), o.rs_stats()
+    for name, v in qo.items():
+        assert qg[name] == pytest.approx(v, rel=1e-11, abs=1e-14), name
+
+
+@pytest.mark.parametrize("n", [32, 48])
+def test_rs_stats_isotropic_fp64(oracle_mod, n):
+    note(f"real-space statistics, isotropic FP64 N={n}")
+    g = pa.Pandora(n, nu=0.01)
+    o = oracle_mod.Oracle(n, nu=0.01)
+    u = pa.init_field_host(n, 50 + n, 2.0 / 3.0, 4.0)
+    g.set_state(*u)
+    o.set_state(*u)
+    _compare_rs(g, o)
+
+
+def test_rs_stats_sheared_chunked(oracle_mod, monkeypatch):
+    note("real-space statistics, shear N=48 sheared frame, chunked plane pass, and the generic path")
+    n = 48
+    kw = dict(f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=3, nu=0.005)
+    u = pa.init_field_host(n, 14, 2.0 / 3.0, 6.0)
+    o = oracle_mod.Oracle(n, **kw)
+    o.set_state(*u)
+    b, bi = o.get_frame()
+    b[6], bi[6] = -0.4, 0.4
+    o.set_frame(b, bi)
+    for env in ({"PDR_ZCHUNK_MB": "0.75"}, {"PDR_GENERIC_KERNELS": "1"}):
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        g = pa.Pandora(n, **kw)
+        g.set_state(*u)
+        g.set_frame(b, bi)
+        _compare_rs(g, o)
+        g.close()
+        for k in env:
+            monkeypatch.delenv(k)
+
+
+def test_rs_stats_fp32_state(oracle_mod):
+    note("real-space statistics from an FP32-mixed state N=64")
+    n = 64
+    g = pa.Pandora(n, nu=0.01, precision=pa.PDR_FP32_MIXED)
+    o = oracle_mod.Oracle(n, nu=0.01)
+    g.set_state(*pa.init_field_host(n, 6, 2.0 / 3.0, 4.0))
+    o.set_state(*g.get_state())
+    qg, qo = g.rs_stats(), o.rs_stats()
+    for name, v in qo.items():  # FP32 transforms: single-precision agreement
+        assert qg[name] == pytest.approx(v, rel=2e-5, abs=1e-9), name
+
+
+def test_cfl_dt_sheared_frame(oracle_mod):
+    note("CFL time step with shear uses the laboratory velocity (N=48)")
+    n = 48
+    kw = dict(f_type=2, rk3_timestep=3, rk3_cfl=0.5, deform_s=5.0, remesh_t=100, nu=0.005)
+    g = pa.Pandora(n, **kw)
+    o = oracle_mod.Oracle(n, **kw)
+    u = pa.init_field_host(n, 19, 2.0 / 3.0, 6.0)
+    g.set_state(*u)
+    o.set_state(*u)
+    tg = to = 0.0
+    for s in range(1, 4):
+        tg = g.rk3_calc(s, tg)
+        to = o.step(s, to)
+        assert g.dt == pytest.approx(o.dt, rel=1e-13)
+    assert tg == pytest.approx(to, rel=1e-13)
+    assert g.rs_stats()["ucfl"] == pytest.approx(o.rs_stats()["ucfl"], rel=1e-12)

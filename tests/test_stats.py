@@ -7,6 +7,10 @@ nu <|curl u|^2> in real space, the mode density against a brute-force count of
 the full (Hermitian-completed) lattice, lab-frame binning against a count over
 the transformed lattice, and algebraic identities of the secondary quantities
 (Re_lambda^2 = 20 k^2 / (3 nu eps), eta^4 = nu^3 / eps, tau_eta^2 = nu / eps).
+Real-space statistics (f_FluidstatsRs2D / RsCollect, src/pf_fluidstats.f90:
+827-1146): closed forms for single Fourier modes (<cos^2> = 1/2, flatness 3/2,
+maxima on the grid, the reference's unnormalised rs_rmsvort1) and Parseval
+against the wave-space Reynolds stresses, in the isotropic and a sheared frame.
 """
 import numpy as np
 import pytest
@@ -153,3 +157,63 @@ def test_sheared_frame_bins_by_lab_wavenumber(oracle_mod):
     w = np.where(np.arange(o.nki)[:, None, None] > 0, 2.0, 1.0)
     ul1 = u[0] + 0.7 * u[2]
     assert q["u1u1"] == pytest.approx(np.sum(w * np.abs(ul1) ** 2), rel=1e-13)
+
+
+# ---- real-space statistics (f_FluidstatsRs2D / RsCollect) --------------------
+
+def _single_mode(o, comp, axis, amp):
+    """u_comp = amp cos(x_axis) as compact wave arrays (i = 0 plane explicit)."""
+    u = [np.zeros(o.shape, complex) for _ in range(3)]
+    if axis == 0:
+        u[comp][1, 0, 0] = amp / 2
+    elif axis == 1:
+        u[comp][0, 1, 0] = amp / 2
+        u[comp][0, o.nkj - 1, 0] = amp / 2
+    else:
+        u[comp][0, 0, 1] = amp / 2
+        u[comp][0, 0, o.nkk - 1] = amp / 2
+    return u
+
+
+def test_rs_stats_single_mode(oracle_mod):
+    n, A = 16, 0.8
+    o = oracle_mod.Oracle(n)
+    o.set_state(*_single_mode(o, 1, 2, A))       # u2 = A cos z: w1 = A sin z
+    q = o.rs_stats()
+    assert q["rs_u2u2"] == pytest.approx(A * A / 2, rel=1e-14)
+    assert q["rs_u1u1"] == 0.0 and q["rs_u3u3"] == 0.0
+    assert q["rs_flat2"] == pytest.approx(1.5, rel=1e-13)   # <cos^4> / <cos^2>^2
+    assert abs(q["rs_skew2"]) < 1e-13
+    assert q["ucfl"] == pytest.approx(A, rel=1e-14)
+    assert q["omegamax1"] == pytest.approx(A, rel=1e-14)     # sin = 1 on the grid (n % 4 == 0)
+    # the reference leaves rs_rmsvort1 unnormalised: sqrt(sum w1^2) = sqrt(N^3 A^2 / 2)
+    assert q["rs_rmsvort1"] == pytest.approx(np.sqrt(n**3 * A * A / 2), rel=1e-13)
+    assert q["rs_flatvort1"] == pytest.approx(1.5 / n**6, rel=1e-12)
+    o.set_state(*_single_mode(o, 2, 0, A))       # u3 = A cos x: w2 = A sin x, normalised
+    q = o.rs_stats()
+    assert q["rs_rmsvort2"] == pytest.approx(A / np.sqrt(2), rel=1e-14)
+    assert q["rs_flatvort2"] == pytest.approx(1.5, rel=1e-13)
+
+
+def test_rs_stats_parseval_iso_and_sheared(oracle_mod):
+    o = oracle_mod.Oracle(16, nu=0.01)
+    u = _random_state(o, 21)
+    o.set_state(*u)
+    q = o.rs_stats()
+    _, fs = o.stats()
+    for name in ("u1u1", "u2u2", "u3u3", "u1u2", "u1u3", "u2u3"):
+        assert q["rs_" + name] == pytest.approx(fs[name], rel=1e-12, abs=1e-15)
+    # Gaussian-like random field: flatness near 3
+    for c in (1, 2, 3):
+        assert 2.3 < q[f"rs_flat{c}"] < 3.7
+    # sheared frame: laboratory velocity (binv(1,3) = 0.7) in both statistics
+    s = oracle_mod.Oracle(16, f_type=2, rk3_timestep=2, deform_s=1.0, nu=0.01)
+    s.set_state(*u)
+    b, bi = s.get_frame()
+    b[6], bi[6] = -0.7, 0.7
+    s.set_frame(b, bi)
+    q = s.rs_stats()
+    _, fs = s.stats()
+    for name in ("u1u1", "u3u3", "u1u3"):
+        assert q["rs_" + name] == pytest.approx(fs[name], rel=1e-12)
+    assert q["rs_u1u1"] != pytest.approx(o.rs_stats()["rs_u1u1"], rel=1e-3)
