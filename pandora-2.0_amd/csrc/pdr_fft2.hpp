@@ -148,8 +148,8 @@ __device__ __forceinline__ void fft_ip(cx<T>* __restrict__ buf, const cx<T>* __r
   }
 }
 
-// Global store of one complex element; NT selects a non-temporal (streaming)
-// store, which does not keep the line in the caches.
+// Global store / load of one complex element; NT selects the non-temporal
+// (streaming) form, which does not keep the line in the caches.
 template <bool NT, typename T>
 __device__ __forceinline__ void put(cx<T>* p, cx<T> v) {
   if constexpr (NT) {
@@ -158,6 +158,16 @@ __device__ __forceinline__ void put(cx<T>* p, cx<T> v) {
     __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
   } else {
     *p = v;
+  }
+}
+template <bool NT, typename T>
+__device__ __forceinline__ cx<T> get(const cx<T>* p) {
+  if constexpr (NT) {
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    const v2 w = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return mk<T>(w.x, w.y);
+  } else {
+    return *p;
   }
 }
 template <bool NT, typename T>
