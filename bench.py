@@ -44,6 +44,7 @@ def kernel_bytes(g, name, csz, rsz, ntr):
         "yfwd": (3 * m2 + 3 * g.m1) * csz,
         "zfwd": (3 * g.m1 + 3 * g.mw) * csz,
         "formf": 15 * g.mw * csz,
+        "zfwdformf": (3 * g.m1 + 12 * g.mw) * csz,
         "tracers": 96 * ntr + 3 * n3 * rsz,
     }.get(name)
 
@@ -106,7 +107,7 @@ def cpu_baseline(n, ntr):
 # ---------------------------------------------------------------------------
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "rk3_step_512_shear_fp32_traffic.json")
 KERNEL_CLASS = {"zinv": "zinv", "yinv": "yinv", "xpass": "xpass", "yfwd": "yfwd", "zfwd": "zfwd",
-                "formf": "formf", "tracers": "tracers", "copy": "copy"}
+                "formf": "formf", "zfwd_formf": "zfwdformf", "tracers": "tracers", "copy": "copy"}
 
 
 def _kernel_key(name):

@@ -382,6 +382,12 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     c->fpb_yf = 3;
     while (c->fpb_yf > 1 && tiles * zc * (3 / c->fpb_yf) < 1024) c->fpb_yf = 1;
   }
+  {
+    // fused z-forward + FormF holds 3 NL values per mode in registers; above
+    // 512 points per line the tile has too many modes per thread for that
+    const char* env = std::getenv("PDR_FUSE_ZFORMF");
+    c->fuse_zf = !(env && env[0] == '0') && nz <= 512;
+  }
   fft_plan(&c->pz, nz);
   fft_plan(&c->py, ny);
   fft_plan(&c->px, nx);
@@ -607,9 +613,13 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   rk_coeffs(stage, &beta, &gamma);
   const double gdt = gamma * (*dt);
   if (tracers) HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt));
-  HIPCHK(launch_zfwd(c));
   if (c->forcing) HIPCHK(launch_force(c, tstep, *dt));  // f_ForceCalc (src/pf_rk3.f90:2063)
-  HIPCHK(launch_formf(c, cf, beta, gdt));
+  if (c->spec && c->fuse_zf) {
+    HIPCHK(launch_zfwd_formf(c, cf, beta, gdt));
+  } else {
+    HIPCHK(launch_zfwd(c));
+    HIPCHK(launch_formf(c, cf, beta, gdt));
+  }
   return PDR_OK;
 }
 
@@ -779,7 +789,8 @@ int pdr_profile_get(pdr_ctx* c, int32_t n, char* names, int32_t namelen, int64_t
     c->ev_pool.push_back(e.second.second);
   }
   c->ev_pending.clear();
-  static const char* kNames[K_NCLASS] = {"zinv", "yinv", "xpass", "yfwd", "zfwd", "formf", "tracers", "remesh", "misc"};
+  static const char* kNames[K_NCLASS] = {"zinv",    "yinv",    "xpass",  "yfwd", "zfwd", "formf",
+                                         "zfwdformf", "tracers", "remesh", "misc"};
   std::string all;
   for (int k = 0; k < K_NCLASS && k < n; ++k) {
     if (counts) counts[k] = c->prof_cnt[k];

@@ -1,0 +1,103 @@
+// pdr_formf.hpp -- FormF, low-storage RK3 update and continuity fix of one
+// wave mode, shared by the stand-alone FormF kernel and the fused z-forward +
+// FormF kernel.  Generic over the isotropic and BF_ROT shear forms through
+// the host-built operator tensors of Coef:
+//   f_RK3FormFIsotropic / f_RK3FormFBFRot (src/pf_rk3.f90:2323-2673) with
+//   f_RK3BFRot{Nonlinear,Viscous,Pressure,Source} (2864-3209), the isotropic
+//   forcing (2421-2467), f_RK3RungeKutta (3457-3498) and
+//   f_RK3ContinuityFixIsotropic (3504-3531).
+#pragma once
+#include "pdr_internal.hpp"
+
+namespace pdr {
+
+__device__ __forceinline__ double wvn_m(int k, int kmax) {
+  return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1));
+}
+
+// Mode (i, j, k) at element e of the wave arrays; a, b, c are its three
+// components of the nonlinear term NL (already normalised and truncated).
+template <typename T>
+__device__ __forceinline__ void formf_mode(const Geo& g, const Coef& cf, int i, int j, int k, long long e, cx<T> a,
+                                           cx<T> b, cx<T> c, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
+                                           cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1,
+                                           cx<T>* __restrict__ r2, const double2* __restrict__ frc, int fnk,
+                                           double fkf, double beta, double gdt) {
+  const double kv[3] = {wvn_m(i, g.ki), wvn_m(j, g.kj), wvn_m(k, g.kk)};
+  const double kmag2 = (kv[0] * kv[0]) + (kv[1] * kv[1]) + (kv[2] * kv[2]);
+  if (!(kmag2 > 0.0)) return;  // the k = 0 mode is never updated (src/pf_rk3.f90:2385)
+  cx<T> u[3] = {u0[e], u1[e], u2[e]};
+  cx<T> r[3] = {r0[e], r1[e], r2[e]};
+  cx<T> n[3];
+  if (cf.use_nl) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double* Bm = &cf.Bm[3 * q];
+      n[q] = add(add(scl(a, T(Bm[0])), scl(b, T(Bm[1]))), scl(c, T(Bm[2])));
+    }
+  } else {
+    n[0] = n[1] = n[2] = mk<T>(T(0), T(0));
+  }
+  // k . M k, p1 = M k
+  double p1[3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l) p1[l] = cf.M[3 * l] * kv[0] + cf.M[3 * l + 1] * kv[1] + cf.M[3 * l + 2] * kv[2];
+  const double p4 = kv[0] * p1[0] + kv[1] * p1[1] + kv[2] * p1[2];
+  // p2 = k . nlf ; p3 = 2 sum_c (sum_m k_m P[m][c]) u_c
+  cx<T> p23 = add(add(scl(n[0], T(kv[0])), scl(n[1], T(kv[1]))), scl(n[2], T(kv[2])));
+  if (cf.has_hom) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double q = 2.0 * (kv[0] * cf.P[c] + kv[1] * cf.P[3 + c] + kv[2] * cf.P[6 + c]);
+      p23 = add(p23, scl(u[c], T(q)));
+    }
+  }
+  const T visc = cf.use_visc ? T(-cf.nu * p4) : T(0);
+  cx<T> tot[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T pc = (p4 > 0.0) ? T(p1[c] / p4) : T(0);
+    cx<T> t = sub(scl(u[c], visc), n[c]);
+    t = add(t, scl(p23, pc));
+    if (cf.has_hom) {
+      const double* H = &cf.H[3 * c];
+      const cx<T> h = add(add(scl(u[0], T(2.0 * H[0])), scl(u[1], T(2.0 * H[1]))), scl(u[2], T(2.0 * H[2])));
+      t = sub(t, h);
+    }
+    tot[c] = t;
+  }
+  // forcing on 0 < |k| <= FORCE_KF, made solenoidal (src/pf_rk3.f90:2421-2467);
+  // frc is [i][j+NK][k+NK][c] (isotropic frame only, fnk = 0 otherwise)
+  if (fnk > 0) {
+    const int jj = int(kv[1]), kq = int(kv[2]);
+    if (i <= fnk && jj >= -fnk && jj <= fnk && kq >= -fnk && kq <= fnk && sqrt(kmag2) <= fkf) {
+      const int w = 2 * fnk + 1;
+      const double2* f = frc + ((long long)(i * w + jj + fnk) * w + kq + fnk) * 3;
+      const double kbr = (kv[0] * f[0].x + kv[1] * f[1].x) + kv[2] * f[2].x;
+      const double kbi = (kv[0] * f[0].y + kv[1] * f[1].y) + kv[2] * f[2].y;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double q = kv[c] / kmag2;
+        tot[c] = add(tot[c], mk<T>(T(f[c].x - q * kbr), T(f[c].y - q * kbi)));
+      }
+    }
+  }
+  const T tb = T(beta), tg = T(gdt);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    r[c] = add(scl(r[c], tb), tot[c]);
+    u[c] = add(u[c], scl(r[c], tg));
+  }
+  // continuity fix u -= k (k.u) / |k|^2 (src/pf_rk3.f90:3504-3531)
+  const cx<T> kd = add(add(scl(u[0], T(kv[0])), scl(u[1], T(kv[1]))), scl(u[2], T(kv[2])));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) u[c] = sub(u[c], scl(kd, T(kv[c] / kmag2)));
+  u0[e] = u[0];
+  u1[e] = u[1];
+  u2[e] = u[2];
+  r0[e] = r[0];
+  r1[e] = r[1];
+  r2[e] = r[2];
+}
+
+}  // namespace pdr

@@ -67,6 +67,7 @@ enum KernelClass {
   K_YFWD,
   K_ZFWD,
   K_FORMF,
+  K_ZFF,  // fused z-forward + FormF
   K_TRACER,
   K_REMESH,
   K_MISC,
@@ -119,6 +120,7 @@ struct pdr_ctx {
   double ucfl = 0.0;
   // launch shapes
   bool spec = false;  // length-specialised kernels (pdr_kernels2.hip)
+  bool fuse_zf = true;  // fused z-forward + FormF (specialised path; PDR_FUSE_ZFORMF=0 disables)
   int tj_z = 1, ti_y = 1;
   size_t lds_max = 65536;
   hipStream_t stream = nullptr;
@@ -141,6 +143,7 @@ hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl, int 
 hipError_t launch_yfwd(pdr_ctx* c, int z0, int nzc);
 hipError_t launch_zfwd(pdr_ctx* c);
 hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
+hipError_t launch_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
 hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
 hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt);
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
@@ -164,6 +167,7 @@ int xpass_tiles(const pdr_ctx* c);  // x-pass workgroups per z plane (specialise
 Coef make_coef(const pdr_ctx* c);
 hipError_t launch2_yfwd(pdr_ctx* c, int z0, int nzc);
 hipError_t launch2_zfwd(pdr_ctx* c);
+hipError_t launch2_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
 
 // records the text returned by pdr_last_error() and returns code
 int set_error(int code, const std::string& msg);

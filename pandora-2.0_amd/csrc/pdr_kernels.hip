@@ -24,6 +24,7 @@
 #include "pdr_internal.hpp"
 #include "pdr_rng.hpp"
 #include "pdr_rsstats.hpp"
+#include "pdr_formf.hpp"
 
 namespace pdr {
 
@@ -337,82 +338,8 @@ __global__ void __launch_bounds__(kThreads)
   const long long ij = e / g.nkk;
   const int j = int(ij % g.nkj);
   const int i = int(ij / g.nkj);
-  const double kv[3] = {wvn(i, g.ki), wvn(j, g.kj), wvn(k, g.kk)};
-  const double kmag2 = (kv[0] * kv[0]) + (kv[1] * kv[1]) + (kv[2] * kv[2]);
-  if (!(kmag2 > 0.0)) return;  // the k = 0 mode is never updated (src/pf_rk3.f90:2385)
-  cx<T> u[3] = {u0[e], u1[e], u2[e]};
-  cx<T> r[3] = {r0[e], r1[e], r2[e]};
-  cx<T> n[3];
-  if (cf.use_nl) {
-    const cx<T> a = nl[e], b = nl[g.mw + e], c = nl[2 * g.mw + e];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double* Bm = &cf.Bm[3 * q];
-      n[q] = add(add(scl(a, T(Bm[0])), scl(b, T(Bm[1]))), scl(c, T(Bm[2])));
-    }
-  } else {
-    n[0] = n[1] = n[2] = zero_c<T>();
-  }
-  // k . M k, p1 = M k
-  double p1[3];
-#pragma unroll
-  for (int l = 0; l < 3; ++l) p1[l] = cf.M[3 * l] * kv[0] + cf.M[3 * l + 1] * kv[1] + cf.M[3 * l + 2] * kv[2];
-  const double p4 = kv[0] * p1[0] + kv[1] * p1[1] + kv[2] * p1[2];
-  // p2 = k . nlf ; p3 = 2 sum_c (sum_m k_m P[m][c]) u_c
-  cx<T> p23 = add(add(scl(n[0], T(kv[0])), scl(n[1], T(kv[1]))), scl(n[2], T(kv[2])));
-  if (cf.has_hom) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double q = 2.0 * (kv[0] * cf.P[c] + kv[1] * cf.P[3 + c] + kv[2] * cf.P[6 + c]);
-      p23 = add(p23, scl(u[c], T(q)));
-    }
-  }
-  const T visc = cf.use_visc ? T(-cf.nu * p4) : T(0);
-  cx<T> tot[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T pc = (p4 > 0.0) ? T(p1[c] / p4) : T(0);
-    cx<T> t = sub(scl(u[c], visc), n[c]);
-    t = add(t, scl(p23, pc));
-    if (cf.has_hom) {
-      const double* H = &cf.H[3 * c];
-      const cx<T> h = add(add(scl(u[0], T(2.0 * H[0])), scl(u[1], T(2.0 * H[1]))), scl(u[2], T(2.0 * H[2])));
-      t = sub(t, h);
-    }
-    tot[c] = t;
-  }
-  // forcing on 0 < |k| <= FORCE_KF, made solenoidal (src/pf_rk3.f90:2421-2467);
-  // frc is [i][j+NK][k+NK][c] (isotropic frame only, fnk = 0 otherwise)
-  if (fnk > 0) {
-    const int jj = int(kv[1]), kq = int(kv[2]);
-    if (i <= fnk && jj >= -fnk && jj <= fnk && kq >= -fnk && kq <= fnk && sqrt(kmag2) <= fkf) {
-      const int w = 2 * fnk + 1;
-      const double2* f = frc + ((long long)(i * w + jj + fnk) * w + kq + fnk) * 3;
-      const double kbr = (kv[0] * f[0].x + kv[1] * f[1].x) + kv[2] * f[2].x;
-      const double kbi = (kv[0] * f[0].y + kv[1] * f[1].y) + kv[2] * f[2].y;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const double q = kv[c] / kmag2;
-        tot[c] = add(tot[c], mk<T>(T(f[c].x - q * kbr), T(f[c].y - q * kbi)));
-      }
-    }
-  }
-  const T tb = T(beta), tg = T(gdt);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    r[c] = add(scl(r[c], tb), tot[c]);
-    u[c] = add(u[c], scl(r[c], tg));
-  }
-  // continuity fix u -= k (k.u) / |k|^2 (src/pf_rk3.f90:3504-3531)
-  const cx<T> kd = add(add(scl(u[0], T(kv[0])), scl(u[1], T(kv[1]))), scl(u[2], T(kv[2])));
-#pragma unroll
-  for (int c = 0; c < 3; ++c) u[c] = sub(u[c], scl(kd, T(kv[c] / kmag2)));
-  u0[e] = u[0];
-  u1[e] = u[1];
-  u2[e] = u[2];
-  r0[e] = r[0];
-  r1[e] = r[1];
-  r2[e] = r[2];
+  formf_mode<T>(g, cf, i, j, k, e, nl[e], nl[g.mw + e], nl[2 * g.mw + e], u0, u1, u2, r0, r1, r2, frc, fnk, fkf,
+                beta, gdt);
 }
 
 // ---------------------------------------------------------------------------
@@ -839,6 +766,13 @@ hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt) {
   }
   prof_end(c, K_MISC);
   return hipGetLastError();
+}
+
+hipError_t launch_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
+  prof_begin(c, K_ZFF);
+  hipError_t e = launch2_zfwd_formf(c, cf, beta, gdt);
+  prof_end(c, K_ZFF);
+  return e;
 }
 
 hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {

@@ -7,6 +7,7 @@
 #include "pdr_fft2.hpp"
 #include "pdr_internal.hpp"
 #include "pdr_rsstats.hpp"
+#include "pdr_formf.hpp"
 
 namespace pdr {
 
@@ -550,6 +551,82 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
   }
 }
 
+// ---------------------------------------------------------------------------
+// pass 5 + FormF: forward z FFT of the three NL fields of an (i, j-tile), then
+// FormF / RK3 / continuity fix of the tile's modes straight from registers:
+// NL never goes through HBM (f_RK3SolveNavierStokes, src/pf_rk3.f90:2067-2130,
+// does the same per i slice).  The modes of a tile, (i, j0..j0+L-1, all k), are
+// contiguous in the wave arrays.
+// ---------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
+    k2_zfwd_formf(Geo g, Coef cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
+                  cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
+                  const double2* __restrict__ frc, int fnk, double fkf, double beta, double gdt,
+                  const cx<T>* __restrict__ twg, int ntile, T norm) {
+  constexpr int L = tile_lines_fwd<T>(N);
+  constexpr int LD = lds_ld<T>(N);
+  constexpr int K = N / 3;
+  constexpr int NKK = 2 * K + 1;
+  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
+  constexpr int QF = (L * NKK + kBlock2 - 1) / kBlock2;  // modes per thread
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* buf = tw + N;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = int(bid % unsigned(ntile));
+  const int i = int(bid / unsigned(ntile));
+  const int j0 = tile * L;
+  const int nkj = g.nkj;
+  const int nl_lines = (nkj - j0) < L ? (nkj - j0) : L;
+  load_tw<T, N>(tw, twg);
+  cx<T> pre[QE];
+  cx<T> nlv[3][QF];
+  auto load = [&](int f) {
+    const cx<T>* src = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
+      const int z = idx / L;
+      const int line = idx - z * L;
+      pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && j0 + line < nkj) ? src[z * g.ldj + line] : czero<T>();
+    }
+  };
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    load(f);
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
+      if (L * N % kBlock2 == 0 || idx < L * N) {
+        const int z = idx / L;
+        buf[(idx - z * L) * LD + lds_off<T>(z)] = pre[q];
+      }
+    }
+    __syncthreads();
+    fft_ip<T, N, L, LD, -1>(buf, tw);
+#pragma unroll
+    for (int q = 0; q < QF; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
+      const int line = idx / NKK;
+      const int k = idx - line * NKK;
+      const int zz = (k <= K) ? k : N - NKK + k;
+      nlv[f][q] = (idx < nl_lines * NKK) ? scl(buf[line * LD + lds_off<T>(zz)], norm) : czero<T>();
+    }
+    __syncthreads();
+  }
+  const long long e0 = ((long long)i * nkj + j0) * NKK;
+#pragma unroll
+  for (int q = 0; q < QF; ++q) {
+    const int idx = int(threadIdx.x) + kBlock2 * q;
+    if (idx < nl_lines * NKK) {
+      const int line = idx / NKK;
+      formf_mode<T>(g, cf, i, j0 + line, idx - line * NKK, e0 + idx, nlv[0][q], nlv[1][q], nlv[2][q], u0, u1, u2,
+                    r0, r1, r2, frc, fnk, fkf, beta, gdt);
+    }
+  }
+}
+
 template <typename T, int N>
 size_t lds_lines() {
   return size_t(lds_lines_bytes<T>(N));
@@ -639,6 +716,22 @@ hipError_t zfwd_n(pdr_ctx* c) {
   const double norm = 1.0 / (double(g.nx) * double(g.ny) * double(g.nz));
   k2_zfwd<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
       g, (const cx<T>*)c->w1, (cx<T>*)c->nl, (const cx<T>*)c->tab.twz, nt, T(norm));
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t zfwd_formf_n(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
+  const Geo& g = c->g;
+  constexpr int L = tile_lines_fwd<T>(N);
+  const int nt = (g.nkj + L - 1) / L;
+  const size_t lds = lds_fwd<T, N>();
+  hipError_t e = big_lds((const void*)k2_zfwd_formf<T, N>, lds);
+  if (e != hipSuccess) return e;
+  const double norm = 1.0 / (double(g.nx) * double(g.ny) * double(g.nz));
+  k2_zfwd_formf<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
+      g, cf, (const cx<T>*)c->w1, (cx<T>*)c->u[0], (cx<T>*)c->u[1], (cx<T>*)c->u[2], (cx<T>*)c->rk[0],
+      (cx<T>*)c->rk[1], (cx<T>*)c->rk[2], (const double2*)c->force, c->forcing ? c->force_nk : 0, c->p.force_kf,
+      beta, gdt, (const cx<T>*)c->tab.twz, nt, T(norm));
   return hipGetLastError();
 }
 
@@ -753,6 +846,17 @@ hipError_t launch2_yfwd(pdr_ctx* c, int z0, int nzc) {
 hipError_t launch2_zfwd(pdr_ctx* c) {
   switch (c->g.nz) {
 #define PDR_CASE(n) PDR_DISPATCH(n, (zfwd_n<float, n>(c)), (zfwd_n<double, n>(c)))
+    PDR_SIZES(PDR_CASE)
+#undef PDR_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch2_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
+  switch (c->g.nz) {
+#define PDR_CASE(n) \
+  PDR_DISPATCH(n, (zfwd_formf_n<float, n>(c, cf, beta, gdt)), (zfwd_formf_n<double, n>(c, cf, beta, gdt)))
     PDR_SIZES(PDR_CASE)
 #undef PDR_CASE
     default:

@@ -390,10 +390,10 @@ class Pandora:
         _check(self.L.pdr_profile_enable(self.h, 1 if enable else 0))
 
     def profile_get(self):
-        n = 9
-        names = C.create_string_buffer(256)
+        n = 16
+        names = C.create_string_buffer(512)
         cnt = np.zeros(n, dtype=np.int64)
         ms = np.zeros(n)
-        _check(self.L.pdr_profile_get(self.h, n, names, 256, cnt, ms))
+        _check(self.L.pdr_profile_get(self.h, n, names, 512, cnt, ms))
         keys = names.value.decode().split(",")
         return {k: (int(cnt[i]), float(ms[i])) for i, k in enumerate(keys)}
