@@ -16,7 +16,7 @@
 //              (f_RK3TransformkRs2Fs, src/pf_rk3.f90:2242-2316; pf_fftw.f90:841-879)
 //   k_formf    per mode: FormF + low-storage RK3 update + continuity fix
 //              (src/pf_rk3.f90:2323-2673, 3457-3531)
-//   k_tracers  per tracer: Neville interpolation + RK3 + periodic wrap
+//   k_tracers_col  per tracer (one lane per molecule column): Neville interpolation + RK3 + wrap
 //              (src/pp_interp.f90:313-560, src/pp_rk3.f90:57-303, src/pp_mpi.f90:1011-1057)
 //   k_remesh   per (i, j) line: integer k-index gather of the Rogallo remesh
 //              (src/pf_rk3.f90:4056-4178)
@@ -399,116 +399,119 @@ __global__ void __launch_bounds__(kThreads) k_zero_d2(double2* __restrict__ p, i
 }
 
 // ---------------------------------------------------------------------------
-// Tracers: molecule search + Neville (y, z, x) + RK3 + wrap.  Four lanes per
-// tracer: lane c < 3 gathers and interpolates velocity component c, lane 3
-// only takes part in the exchange; every lane of the quad then applies the
-// same RK3 update and lane c stores component c.  The molecule is walked one
-// z plane at a time (P x P points of one component, Neville along y for each
-// x column), which keeps the registers per lane low and the loads of a plane
-// independent.  Field ur is [z][y][x][3]; positions are FP64 (3, np).
+// Tracers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ long long pmod(long long a, long long n) {
   long long r = a % n;
   return r < 0 ? r + n : r;
 }
 
-// One component of the interpolated velocity at x (f = ur + c, point stride 3).
-// The operation sequence per (x, z) column, per x column and for the final x
-// line is that of p_InterpNeville (src/pp_interp.f90:474-560).
+// ---------------------------------------------------------------------------
+// Tracers, one lane per x column of the molecule: a group of G lanes (G = P
+// rounded up to a power of two) serves one tracer; lane a < P gathers the
+// P x P points (y, z) of column x_a, all three components per point (one
+// 3-element load each), and applies the y then z Neville passes to them; the
+// group exchanges the P column values with __shfl and every lane runs the x
+// pass, the RK3 update and the wrap; lane c < 3 stores component c.  The
+// operation sequence per column and component is that of the reference
+// (y for each (x, z), then z for each x, then x).
+// ---------------------------------------------------------------------------
 template <typename T, int P>
-__device__ double interp_c(const Geo& g, const TracerFrame& tf, const T* __restrict__ f, const double* xp) {
+__global__ void __launch_bounds__(kThreads)
+    k_tracers_col(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
+                  double* __restrict__ uprk, long long np, double beta, double gdt) {
+  constexpr int G = P <= 2 ? 2 : (P <= 4 ? 4 : 8);
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long q = gt / G;
+  const int a = int(gt % G);
+  const bool live = q < np;
+  const long long qq = live ? q : np - 1;  // np >= 1: dead lanes read a valid tracer
+  double x[3] = {xpa[3 * qq], xpa[3 * qq + 1], xpa[3 * qq + 2]};
   const int nhalo = P / 2;
-  // g_DomainInitHalo (src/pg_domain.f90:360-384) and p_InterpFindMolecule (src/pp_interp.f90:406-467)
+  // g_DomainInitHalo (src/pg_domain.f90:360-384), p_InterpFindMolecule (src/pp_interp.f90:406-467)
   double lxh = double(nhalo) * tf.dx, lyh = double(nhalo) * tf.dy, lzh = double(nhalo) * tf.dz;
   lxh = lxh - (0.5 * tf.dx);
   lyh = lyh - (0.5 * tf.dy);
   lzh = lzh - (0.5 * tf.dz);
-  const double xd1 = xp[0] + lxh + tf.dx + (0.5 * tf.dx);
-  const double xd2 = xp[1] + lyh + tf.dy + (0.5 * tf.dy);
-  const double xd3 = xp[2] + lzh + tf.dz + (0.5 * tf.dz);
+  const double xd1 = x[0] + lxh + tf.dx + (0.5 * tf.dx);
+  const double xd2 = x[1] + lyh + tf.dy + (0.5 * tf.dy);
+  const double xd3 = x[2] + lzh + tf.dz + (0.5 * tf.dz);
   const int i0 = int(xd1 / tf.dx) - nhalo - nhalo;
   const int j0 = int(xd2 / tf.dy) - nhalo - nhalo;
   const int k0 = int(xd3 / tf.dz) - nhalo - nhalo;
-  double ix[P], iy[P], iz[P];
+  double iy[P], iz[P];
 #pragma unroll
   for (int m = 0; m < P; ++m) {
-    ix[m] = double(i0 + m) * tf.dx;
     iy[m] = double(j0 + m) * tf.dy;
     iz[m] = double(k0 + m) * tf.dz;
   }
-  int gx3[P];
+  double vc[3] = {0.0, 0.0, 0.0};  // this lane's column after the y and z passes
+  if (a < P) {
+    const long long gx3 = 3 * pmod(i0 + a, g.nx);
+    double vz[3][P];  // [component][z]
 #pragma unroll
-  for (int m = 0; m < P; ++m) gx3[m] = 3 * int(pmod(i0 + m, g.nx));
-  double vz[P][P];  // [x][z] after the y pass
+    for (int qz = 0; qz < P; ++qz) {
+      const long long zoff = pmod(k0 + qz, g.nz) * g.ny;
+      T w[P][3];  // [y][component]
 #pragma unroll
-  for (int q = 0; q < P; ++q) {
-    const long long zoff = pmod(k0 + q, g.nz) * g.ny;
-    T w[P][P];  // [x][y]
+      for (int b = 0; b < P; ++b) {
+        const T* pt = ur + (zoff + pmod(j0 + b, g.ny)) * g.nx * 3 + gx3;
+        w[b][0] = pt[0];
+        w[b][1] = pt[1];
+        w[b][2] = pt[2];
+      }
 #pragma unroll
-    for (int b = 0; b < P; ++b) {
-      const T* row = f + (zoff + pmod(j0 + b, g.ny)) * g.nx * 3;
+      for (int c = 0; c < 3; ++c) {
+        double v[P];
 #pragma unroll
-      for (int a = 0; a < P; ++a) w[a][b] = row[gx3[a]];
+        for (int b = 0; b < P; ++b) v[b] = double(w[b][c]);
+#pragma unroll
+        for (int col = 1; col <= P - 1; ++col)
+#pragma unroll
+          for (int nv = 1; nv <= P - col; ++nv) {
+            const double d1 = x[1] - iy[nv + col - 1];
+            const double d2 = iy[nv - 1] - x[1];
+            const double d3 = iy[nv - 1] - iy[nv + col - 1];
+            v[nv - 1] = ((d1 * v[nv - 1]) + (d2 * v[nv])) / d3;
+          }
+        vz[c][qz] = v[0];
+      }
     }
 #pragma unroll
-    for (int a = 0; a < P; ++a) {
-      double v[P];
-#pragma unroll
-      for (int b = 0; b < P; ++b) v[b] = double(w[a][b]);
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int col = 1; col <= P - 1; ++col)
 #pragma unroll
         for (int nv = 1; nv <= P - col; ++nv) {
-          const double d1 = xp[1] - iy[nv + col - 1];
-          const double d2 = iy[nv - 1] - xp[1];
-          const double d3 = iy[nv - 1] - iy[nv + col - 1];
-          v[nv - 1] = ((d1 * v[nv - 1]) + (d2 * v[nv])) / d3;
+          const double d1 = x[2] - iz[nv + col - 1];
+          const double d2 = iz[nv - 1] - x[2];
+          const double d3 = iz[nv - 1] - iz[nv + col - 1];
+          vz[c][nv - 1] = ((d1 * vz[c][nv - 1]) + (d2 * vz[c][nv])) / d3;
         }
-      vz[a][q] = v[0];
+      vc[c] = vz[c][0];
     }
   }
-#pragma unroll
-  for (int col = 1; col <= P - 1; ++col)
-#pragma unroll
-    for (int nv = 1; nv <= P - col; ++nv) {
-      const double d1 = xp[2] - iz[nv + col - 1];
-      const double d2 = iz[nv - 1] - xp[2];
-      const double d3 = iz[nv - 1] - iz[nv + col - 1];
-#pragma unroll
-      for (int a = 0; a < P; ++a) vz[a][nv - 1] = ((d1 * vz[a][nv - 1]) + (d2 * vz[a][nv])) / d3;
-    }
-  double vx[P];
-#pragma unroll
-  for (int a = 0; a < P; ++a) vx[a] = vz[a][0];
-#pragma unroll
-  for (int col = 1; col <= P - 1; ++col)
-#pragma unroll
-    for (int nv = 1; nv <= P - col; ++nv) {
-      const double d1 = xp[0] - ix[nv + col - 1];
-      const double d2 = ix[nv - 1] - xp[0];
-      const double d3 = ix[nv - 1] - ix[nv + col - 1];
-      vx[nv - 1] = ((d1 * vx[nv - 1]) + (d2 * vx[nv])) / d3;
-    }
-  return vx[0];
-}
-
-template <typename T, int P>
-__global__ void __launch_bounds__(kThreads)
-    k_tracers(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
-              double* __restrict__ uprk, long long np, double beta, double gdt) {
-  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long q = gt >> 2;
-  const int c = int(gt & 3);
-  const bool live = q < np;
-  const long long qq = live ? q : np - 1;  // np >= 1: dead lanes read a valid tracer
-  double x[3] = {xpa[3 * qq], xpa[3 * qq + 1], xpa[3 * qq + 2]};
-  const double mine = (c < 3) ? interp_c<T, P>(g, tf, ur + c, x) : 0.0;
-  double r[3] = {uprk[3 * qq], uprk[3 * qq + 1], uprk[3 * qq + 2]};
-  const int lane0 = int(threadIdx.x & 63u) & ~3;  // first lane of the quad within the wave
+  // x pass over the P columns of the group
+  const int lane0 = int(threadIdx.x & 63u) - a;  // first lane of the group within the wave
   double uf[3];
-  uf[0] = __shfl(mine, lane0 + 0);
-  uf[1] = __shfl(mine, lane0 + 1);
-  uf[2] = __shfl(mine, lane0 + 2);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    double vx[P];
+#pragma unroll
+    for (int m = 0; m < P; ++m) vx[m] = __shfl(vc[c], lane0 + m);
+#pragma unroll
+    for (int col = 1; col <= P - 1; ++col)
+#pragma unroll
+      for (int nv = 1; nv <= P - col; ++nv) {
+        const double xa = double(i0 + nv + col - 1) * tf.dx, xb = double(i0 + nv - 1) * tf.dx;
+        const double d1 = x[0] - xa;
+        const double d2 = xb - x[0];
+        const double d3 = xb - xa;
+        vx[nv - 1] = ((d1 * vx[nv - 1]) + (d2 * vx[nv])) / d3;
+      }
+    uf[c] = vx[0];
+  }
+  double r[3] = {uprk[3 * qq], uprk[3 * qq + 1], uprk[3 * qq + 2]};
   if (!tf.bf) {
     // p_RK3SolveParticle tracer branch (src/pp_rk3.f90:103-134)
 #pragma unroll
@@ -549,9 +552,9 @@ __global__ void __launch_bounds__(kThreads)
   if (x[1] >= x2max) x[1] = x[1] - tf.ly;
   if (x[2] < x3min) x[2] = x[2] + tf.lz;
   if (x[2] > x3max) x[2] = x[2] - tf.lz;
-  if (live && c < 3) {
-    xpa[3 * q + c] = (c == 0) ? x[0] : ((c == 1) ? x[1] : x[2]);
-    uprk[3 * q + c] = (c == 0) ? r[0] : ((c == 1) ? r[1] : r[2]);
+  if (live && a < 3) {
+    xpa[3 * q + a] = (a == 0) ? x[0] : ((a == 1) ? x[1] : x[2]);
+    uprk[3 * q + a] = (a == 0) ? r[0] : ((a == 1) ? r[1] : r[2]);
   }
 }
 
@@ -784,21 +787,16 @@ hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
 
 template <typename T>
 static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt) {
-  const long long nb = (4 * c->np + kThreads - 1) / kThreads;  // four lanes per tracer
   const T* ur = (const T*)c->ur;
+  auto go = [&](auto kern, int lanes) {
+    const long long nb = (lanes * c->np + kThreads - 1) / kThreads;
+    kern<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
+  };
   switch (tf.order) {
-    case 2:
-      k_tracers<T, 2><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
-      break;
-    case 6:
-      k_tracers<T, 6><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
-      break;
-    case 8:
-      k_tracers<T, 8><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
-      break;
-    default:
-      k_tracers<T, 4><<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
-      break;
+    case 2: go(k_tracers_col<T, 2>, 2); break;
+    case 6: go(k_tracers_col<T, 6>, 8); break;
+    case 8: go(k_tracers_col<T, 8>, 8); break;
+    default: go(k_tracers_col<T, 4>, 4); break;
   }
   return hipGetLastError();
 }
