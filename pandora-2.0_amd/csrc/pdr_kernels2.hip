@@ -220,6 +220,106 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
 }
 
 // ---------------------------------------------------------------------------
+// pass 1, register form: the three velocity fields of a tile are loaded once
+// and kept in registers; the vorticity fields are formed from them, so the
+// velocity is read from HBM exactly once.  Narrower tiles (PDR_ZINV_LINES)
+// keep the registers per thread within two waves per SIMD.
+// ---------------------------------------------------------------------------
+#ifndef PDR_ZINV_LINES_F32
+#define PDR_ZINV_LINES_F32 8
+#endif
+#ifndef PDR_ZINV_LINES_F64
+#define PDR_ZINV_LINES_F64 4
+#endif
+#ifndef PDR_ZINV_REG
+#define PDR_ZINV_REG 1
+#endif
+template <typename T>
+constexpr int zinv_lines(int n) {
+  return n <= 512 ? (sizeof(T) == 4 ? PDR_ZINV_LINES_F32 : PDR_ZINV_LINES_F64) : tile_lines<T>(n);
+}
+template <typename T>
+constexpr int lds_zinv_bytes(int n) {
+  return (n + zinv_lines<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
+}
+#define PDR_ZINV_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_zinv_bytes<T>(N)))))
+
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
+    k2_zinv_r(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
+              const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
+  constexpr int L = zinv_lines<T>(N);
+  constexpr int LD = lds_ld<T>(N);
+  constexpr int K = N / 3;
+  constexpr int NKK = 2 * K + 1;
+  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* buf = tw + N;
+  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
+  const int tile = int(bid % unsigned(ntile));
+  const int i = int(bid / unsigned(ntile));
+  const int j0 = tile * L;
+  const int nkj = g.nkj;
+  load_tw<T, N>(tw, twg);
+  const long long base = ((long long)i * nkj + j0) * NKK;
+  const cx<T>* uu[3] = {u0 + base, u1 + base, u2 + base};
+  cx<T> ua[QE], ub[QE], uc[QE];  // u1, u2, u3 of this thread's elements
+#pragma unroll
+  for (int q = 0; q < QE; ++q) {
+    const int idx = int(threadIdx.x) + kBlock2 * q;
+    const int line = idx / N;
+    const int z = idx - line * N;
+    const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : -1);
+    const bool ok = (L * N % kBlock2 == 0 || idx < L * N) && j0 + line < nkj && k >= 0;
+    const int e = line * NKK + k;
+    ua[q] = ok ? uu[0][e] : czero<T>();
+    ub[q] = ok ? uu[1][e] : czero<T>();
+    uc[q] = ok ? uu[2][e] : czero<T>();
+  }
+  const double wi = double(i);
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {  // unrolled: every field selection is static
+#pragma unroll
+    for (int q = 0; q < QE; ++q) {
+      const int idx = int(threadIdx.x) + kBlock2 * q;
+      if (L * N % kBlock2 == 0 || idx < L * N) {
+        const int line = idx / N;
+        const int z = idx - line * N;
+        const cx<T> un[3] = {ua[q], ub[q], uc[q]};
+        cx<T> v;
+        if (f < 3) {
+          v = un[f];
+        } else {
+          // w_c = i sum_n (G[c][n] . k) u_n, the same operations as k2_zinv
+          const int c = f - 3;
+          const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : 0);
+          const double kv[3] = {wi, wvn2(j0 + line, g.kj), wvn2(k, K)};
+          cx<T> acc = czero<T>();
+#pragma unroll
+          for (int n = 0; n < 3; ++n) {
+            const double* G = &cf.G[(c * 3 + n) * 3];
+            if ((G[0] != 0.0) || (G[1] != 0.0) || (G[2] != 0.0))
+              acc = add(acc, scl(un[n], T(G[0] * kv[0] + G[1] * kv[1] + G[2] * kv[2])));
+          }
+          v = mk<T>(-acc.y, acc.x);
+        }
+        buf[line * LD + lds_off<T>(z)] = v;
+      }
+    }
+    __syncthreads();
+    fft_ip<T, N, L, LD, +1>(buf, tw);
+    cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
+    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+      const int z = idx / L;
+      const int line = idx - z * L;
+      if (j0 + line < nkj) put<kNtZinv>(&dst[z * g.ldj + line], buf[line * LD + lds_off<T>(z)]);
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // pass 2: inverse y FFT, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
@@ -648,15 +748,27 @@ hipError_t big_lds(const void* fn, size_t bytes) {
 template <typename T, int N>
 hipError_t zinv_n(pdr_ctx* c, const Coef& cf) {
   const Geo& g = c->g;
-  constexpr int L = tile_lines<T>(N);
-  const int nt = (g.nkj + L - 1) / L;
-  const size_t lds = lds_lines<T, N>();
-  hipError_t e = big_lds((const void*)k2_zinv<T, N>, lds);
-  if (e != hipSuccess) return e;
-  k2_zinv<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
-      g, cf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1], (const cx<T>*)c->u[2], (cx<T>*)c->w1,
-      (const cx<T>*)c->tab.twz, nt);
-  return hipGetLastError();
+  if constexpr (PDR_ZINV_REG && N <= 512) {  // longer lines hold too many values per thread
+    constexpr int L = zinv_lines<T>(N);
+    const int nt = (g.nkj + L - 1) / L;
+    const size_t lds = size_t(lds_zinv_bytes<T>(N));
+    hipError_t e = big_lds((const void*)k2_zinv_r<T, N>, lds);
+    if (e != hipSuccess) return e;
+    k2_zinv_r<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
+        g, cf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1], (const cx<T>*)c->u[2], (cx<T>*)c->w1,
+        (const cx<T>*)c->tab.twz, nt);
+    return hipGetLastError();
+  } else {
+    constexpr int L = tile_lines<T>(N);
+    const int nt = (g.nkj + L - 1) / L;
+    const size_t lds = lds_lines<T, N>();
+    hipError_t e = big_lds((const void*)k2_zinv<T, N>, lds);
+    if (e != hipSuccess) return e;
+    k2_zinv<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
+        g, cf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1], (const cx<T>*)c->u[2], (cx<T>*)c->w1,
+        (const cx<T>*)c->tab.twz, nt);
+    return hipGetLastError();
+  }
 }
 
 template <typename T, int N>
@@ -721,6 +833,9 @@ hipError_t zfwd_n(pdr_ctx* c) {
 
 template <typename T, int N>
 hipError_t zfwd_formf_n(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
+  if constexpr (N > 512) {  // not used above 512 points per line (see pdr_create)
+    return hipErrorNotSupported;
+  } else {
   const Geo& g = c->g;
   constexpr int L = tile_lines_fwd<T>(N);
   const int nt = (g.nkj + L - 1) / L;
@@ -733,6 +848,7 @@ hipError_t zfwd_formf_n(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
       (cx<T>*)c->rk[1], (cx<T>*)c->rk[2], (const double2*)c->force, c->forcing ? c->force_nk : 0, c->p.force_kf,
       beta, gdt, (const cx<T>*)c->tab.twz, nt, T(norm));
   return hipGetLastError();
+  }
 }
 
 }  // namespace
