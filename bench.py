@@ -106,8 +106,9 @@ def cpu_baseline(n, ntr):
 # this script), written to profiles/; the traffic JSON feeds roofline.traffic.
 # ---------------------------------------------------------------------------
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "rk3_step_512_shear_fp32_traffic.json")
-KERNEL_CLASS = {"zinv": "zinv", "yinv": "yinv", "xpass": "xpass", "yfwd": "yfwd", "zfwd": "zfwd",
-                "formf": "formf", "zfwd_formf": "zfwdformf", "tracers": "tracers", "copy": "copy"}
+KERNEL_CLASS = {"zinv": "zinv", "zinv_r": "zinv", "yinv": "yinv", "xpass": "xpass", "yfwd": "yfwd",
+                "zfwd": "zfwd", "formf": "formf", "zfwd_formf": "zfwdformf", "tracers": "tracers",
+                "tracers_col": "tracers", "copy": "copy"}
 
 
 def _kernel_key(name):
@@ -120,17 +121,23 @@ def _kernel_key(name):
     return cls, base
 
 
-def rocpd_summary(trace_db, fetch_db, write_db, out_json, copy_bytes=1 << 30):
-    """Markdown tables of a kernel trace and of per-dispatch FETCH_SIZE /
+def rocpd_summary(trace_db, fetch_db, write_db, out_json, sq_dbs=(), copy_bytes=1 << 30):
+    """Markdown tables of a kernel trace, of per-dispatch FETCH_SIZE /
     WRITE_SIZE (KiB), calibrated by the copy kernel k_copy (copy_bytes read and
-    written per launch, bench.py's bandwidth probe)."""
+    written per launch, bench.py's bandwidth probe), and of the SQ_ counters of
+    the sq_dbs runs (per-dispatch sums, averaged over dispatches)."""
     import collections
     import sqlite3
+
+    import re
 
     rows = sqlite3.connect(trace_db).execute("select name, duration from kernels").fetchall()
     agg = collections.OrderedDict()
     for name, dur in rows:
         cls, base = _kernel_key(name)
+        m = re.search(r"k2?_[a-z0-9_]+<(float|double)", name)
+        if m:
+            base += "<f32>" if m.group(1) == "float" else "<f64>"
         a = agg.setdefault(base, [0, 0.0, 1e30, 0.0])
         a[0] += 1
         a[1] += dur
@@ -152,6 +159,10 @@ def rocpd_summary(trace_db, fetch_db, write_db, out_json, copy_bytes=1 << 30):
                 vals[_kernel_key(kn)[1]][did] += float(v)
         return {k: sum(d.values()) / len(d) for k, d in vals.items()}
 
+    def counters(db):
+        con = sqlite3.connect(db)
+        return sorted({r[0] for r in con.execute("select distinct counter_name from counters_collection")})
+
     fetch = per_dispatch(fetch_db, "FETCH_SIZE")
     write = per_dispatch(write_db, "WRITE_SIZE")
     ff = copy_bytes / (fetch["k_copy"] * 1024.0)
@@ -170,6 +181,19 @@ def rocpd_summary(trace_db, fetch_db, write_db, out_json, copy_bytes=1 << 30):
         if cls and cls != "copy":
             kern[cls] = {"kernel": base, "fetch_kib": fk, "write_kib": wk, "read_bytes": rb,
                          "write_bytes": wb, "traffic_bytes": rb + wb}
+    sq = collections.defaultdict(dict)
+    for db in sq_dbs:
+        for cn in counters(db):
+            for base, v in per_dispatch(db, cn).items():
+                sq[base][cn] = v
+    if sq:
+        names = sorted({c for d in sq.values() for c in d})
+        print()
+        print("| kernel | " + " | ".join(names) + " |")
+        print("|---|" + "---|" * len(names))
+        for base in sorted(sq):
+            print(f"| {base} | " + " | ".join(f"{sq[base][c]:.3g}" if c in sq[base] else "-" for c in names)
+                  + " |")
     if out_json:
         with open(out_json, "w") as f:
             json.dump({"workload": {"n": 512, "tracers": 1 << 20, "precision": "fp32-mixed"},
@@ -199,8 +223,9 @@ def main():
         sp.add_argument("--fetch", required=True)
         sp.add_argument("--write", required=True)
         sp.add_argument("--json", default=None)
+        sp.add_argument("--sq", nargs="*", default=[])
         a = sp.parse_args()
-        rocpd_summary(a.trace, a.fetch, a.write, a.json)
+        rocpd_summary(a.trace, a.fetch, a.write, a.json, a.sq)
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,10 +251,10 @@ def main():
     if not args.no_fp64:
         k64 = max(3, args.steps // 3)
         print(f"[bench] FP64 {n}^3: 1 warmup, {k64} timed", file=sys.stderr, flush=True)
-        dt64, _, ok64 = run_gpu(pa, n, pa.PDR_FP64, k64, 1, ntr, False)
+        dt64, prof64, ok64 = run_gpu(pa, n, pa.PDR_FP64, k64, 1, ntr, True)
         sps64 = k64 / dt64
         fp64 = {"steps_per_s": sps64, "grid_point_steps_per_s": sps64 * n**3, "ms_per_step": 1e3 / sps64,
-                "finite": bool(ok64)}
+                "finite": bool(ok64), "prof": prof64}
 
     class G:
         pass
@@ -244,12 +269,19 @@ def main():
         if cnt and ms > best and kernel_bytes(g, k, 8, 4, ntr):
             dom, best = k, ms
     roof = None
-    kernels = {}
-    for k, (cnt, ms) in prof.items():
-        if cnt:
-            b = kernel_bytes(g, k, 8, 4, ntr)
-            kernels[k] = {"launches": cnt, "avg_ms": ms / cnt,
+
+    def per_kernel(pr, csz, rsz):
+        out = {}
+        for k, (cnt, ms) in pr.items():
+            if cnt:
+                b = kernel_bytes(g, k, csz, rsz, ntr)
+                out[k] = {"launches": cnt, "avg_ms": ms / cnt,
                           "GBps": (b / (ms / cnt * 1e-3) / 1e9) if b else None}
+        return out
+
+    kernels = per_kernel(prof, 8, 4)
+    if fp64:
+        fp64["kernels"] = per_kernel(fp64.pop("prof"), 16, 8)
     if dom:
         cnt, ms = prof[dom]
         avg = ms / cnt * 1e-3

@@ -48,7 +48,7 @@ module pdr_rk3_amd
   public :: f_RK3Stage1, f_RK3Stage2, f_RK3Stage3, f_RK3Remesh
   public :: pdr_PutVelocity, pdr_GetVelocity, pdr_PutParticles, pdr_GetParticles
   public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_Spectrum, pdr_ErrorMessage
-  public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing, pdr_Stats
+  public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing, pdr_Stats, pdr_RsStats
 
   interface
     integer(c_int) function c_pdr_create(out, p) bind(C, name="pdr_create")
@@ -150,6 +150,11 @@ module pdr_rk3_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: c
       real(c_double), intent(out) :: q(33), spec(*)
+    end function
+    integer(c_int) function c_pdr_rs_stats(c, q) bind(C, name="pdr_rs_stats")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: q(25)
     end function
     type(c_ptr) function c_pdr_last_error() bind(C, name="pdr_last_error")
       import :: c_ptr
@@ -329,6 +334,15 @@ contains
     integer, intent(inout)           :: ierr
     ierr = int(c_pdr_stats(ctx, q, spec))
   end subroutine pdr_Stats
+
+  !> f_FluidstatsRs2D + f_FluidstatsRsCollect on the device: q(1:25) in the
+  !! order of pdr_rs_stats_t (rs_u1u1..rs_u2u3, rs_skew1..3, rs_flat1..3,
+  !! rs_rmsvort1..3, rs_skewvort1..3, rs_flatvort1..3, omegamax1..3, ucfl)
+  subroutine pdr_RsStats ( q, ierr )
+    real(kind=c_double), intent(out) :: q(25)
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_rs_stats(ctx, q))
+  end subroutine pdr_RsStats
 
   !> Text of the last library error
   function pdr_ErrorMessage () result ( msg )
