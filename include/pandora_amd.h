@@ -180,6 +180,39 @@ int pdr_check_finite(pdr_ctx* ctx);
 int pdr_particles_upload(pdr_ctx* ctx, const double* xp, int64_t np);
 /* Read positions xp(1:3,np) and, if up_rk3 != NULL, particle_up_rk3(1:3,np). */
 int pdr_particles_download(pdr_ctx* ctx, double* xp, double* up_rk3);
+/* particle_up(1:3,np) (the laboratory-frame fluid velocity at the tracer from
+ * the last stage: uf, or binv uf with shear, src/pp_rk3.f90:124, 268) and
+ * particle_xp0(1:3,np) (P_KEEP_INITIAL_POSITION); either pointer may be NULL.
+ * The reference keeps both in p_arrays (src/pp_arrays.f90:25-42) and its
+ * restart; these move them in and out.  particle_up starts at zero on upload. */
+int pdr_particles_download_aux(pdr_ctx* ctx, double* up, double* xp0);
+int pdr_particles_upload_aux(pdr_ctx* ctx, const double* up, const double* xp0);
+
+/* Tracer statistics: p_StatsPrimary, p_StatsDispersion and
+ * p_StatsRelativeVelocity (src/pp_stats.f90:119-390), called by f_RK3Stage1
+ * (src/pf_rk3.f90:62-99).  enable = P_TRACK_PART statistics; keep_initial =
+ * P_KEEP_INITIAL_POSITION (particle_xp0 = particle_xp now, at upload and at the
+ * release step; it follows the periodic wraps and the remesh of its particle,
+ * src/pp_control.f90:1540-1588, src/pp_mpi.f90:1095-1129); ts_release =
+ * P_TS_RELEASE: particles move from tstep = ts_release on (particle_up_rk3 and
+ * particle_xp0 reset at that step, src/pp_control.f90:378-510) and the
+ * statistics are taken in stage 1 of every tstep > ts_release, on the device
+ * without a host synchronisation.  Resets the saved displacement used for dfp. */
+int pdr_particle_stats_config(pdr_ctx* ctx, int32_t enable, int32_t keep_initial, int32_t ts_release);
+
+/* The last record, in the order of p_StatsWrite (src/pp_stats.f90:379-390).
+ * Tracers have dp = 0, so the particle kinetic energy, particle Reynolds number
+ * and Stokes numbers are 0; without gravity vs (upgrav) is 0.  vt is the mean
+ * particle_up (p_RK3AbsoluteVelocity), xrel the mean squared laboratory-frame
+ * displacement binv (x - x0) and dfp = (xrel - xrel_previous) / (2 dt). */
+typedef struct pdr_pstats_t {
+  double rs_u1u1, rs_u2u2, rs_u3u3, rs_u1u2, rs_u1u3, rs_u2u3, tke, pken;
+  double vrel1, vrel2, vrel3, vrel_abs;
+  double xrel1, xrel2, xrel3, dfp1, dfp2, dfp3, x1x2, x1x3, x2x3;
+  double vt1, vt2, vt3, vs1, vs2, vs3, rep, stk, ste;
+} pdr_pstats_t;
+/* *tstep = the record's tstep (-1 if none yet). */
+int pdr_particle_stats(pdr_ctx* ctx, int32_t* tstep, pdr_pstats_t* out);
 
 /* ---- initial conditions, restart, statistics ----------------------------- */
 

@@ -73,6 +73,10 @@ def lib():
         L.orc_get_tracer_field.argtypes = [vp, dp, dp, dp]
         L.orc_interpolate.argtypes = [vp, dp, dp, dp, dp, C.c_int64, dp]
         L.orc_correct_position.argtypes = [vp, dp, C.c_int64]
+        L.orc_get_particles_aux.argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.orc_set_particles_aux.argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.orc_particle_stats_config.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+        L.orc_particle_stats.argtypes = [vp, dp]
         L.orc_fft.argtypes = [C.c_int, C.c_int, dp, dp]
         L.orc_spectrum.argtypes = [vp, dp, dp, dp, dp]
         L.orc_kkmax.restype = C.c_int
@@ -113,6 +117,10 @@ RS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "r
             "rs_skew3", "rs_flat1", "rs_flat2", "rs_flat3", "rs_rmsvort1", "rs_rmsvort2", "rs_rmsvort3",
             "rs_skewvort1", "rs_skewvort2", "rs_skewvort3", "rs_flatvort1", "rs_flatvort2",
             "rs_flatvort3", "omegamax1", "omegamax2", "omegamax3", "ucfl")
+# p_StatsWrite record (src/pp_stats.f90:379-390)
+PSTATS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "tke", "pken",
+                "vrel1", "vrel2", "vrel3", "vrel_abs", "xrel1", "xrel2", "xrel3", "dfp1", "dfp2", "dfp3",
+                "x1x2", "x1x3", "x2x3", "vt1", "vt2", "vt3", "vs1", "vs2", "vs3", "rep", "stk", "ste")
 
 
 def stats_finish(kkmax: int, nu: float, raw: np.ndarray, uu: np.ndarray):
@@ -156,6 +164,7 @@ class Oracle:
         self.nx, self.ny, self.nz = nx, ny, nz
         self.shape = (self.nki, self.nkj, self.nkk)
         self.np = 0
+        self._keep_x0 = False
 
     def close(self):
         if self.h:
@@ -303,6 +312,29 @@ class Oracle:
         ur = np.zeros((self.np, 3))
         lib().orc_get_particles(self.h, xp.reshape(-1), ur.reshape(-1))
         return xp, ur
+
+    def get_particles_aux(self):
+        """(particle_up, particle_xp0 or None)"""
+        up = np.zeros((self.np, 3))
+        x0 = np.zeros((self.np, 3))
+        lib().orc_get_particles_aux(self.h, up.ctypes.data, x0.ctypes.data)
+        return up, (x0 if self._keep_x0 else None)
+
+    def set_particles_aux(self, up=None, xp0=None):
+        a = None if up is None else np.ascontiguousarray(up, np.float64).reshape(-1, 3)
+        b = None if xp0 is None else np.ascontiguousarray(xp0, np.float64).reshape(-1, 3)
+        lib().orc_set_particles_aux(self.h, None if a is None else a.ctypes.data,
+                                    None if b is None else b.ctypes.data)
+
+    def particle_stats_config(self, enable=True, keep_initial=False, ts_release=0):
+        self._keep_x0 = bool(keep_initial)
+        lib().orc_particle_stats_config(self.h, int(enable), int(keep_initial), int(ts_release))
+
+    def particle_stats(self):
+        """(tstep, {name: value}) of the last tracer-statistics record (tstep -1: none)."""
+        r = np.zeros(30)
+        t = lib().orc_particle_stats(self.h, r)
+        return int(t), dict(zip(PSTATS_NAMES, r.tolist()))
 
     def tracer_field(self):
         out = [np.zeros((self.nz, self.ny, self.nx)) for _ in range(3)]

@@ -135,8 +135,15 @@ struct orc {
   double* ur[3];        /* real-space velocity for tracers */
   long np;
   double* xp;
-  double* up;
+  double* up;           /* particle_up: lab-frame fluid velocity of the last stage */
   double* uprk;
+  double* xp0;          /* particle_xp0 (P_KEEP_INITIAL_POSITION) or NULL */
+  int pst_on;           /* tracer statistics at stage 1 (P_TRACK_PART) */
+  int keep_x0;          /* P_KEEP_INITIAL_POSITION */
+  int ts_release;       /* P_TS_RELEASE */
+  double xrel_prev[3];  /* p_stats' saved xrel (zero at start) */
+  double pst[30];       /* last p_StatsWrite record */
+  int pst_tstep;        /* its tstep, -1 before the first */
   fftp fz, fy, fx;
   int nthreads;
   int forcing;          /* F_TYPE 1 and FORCE_STOP > 0 */
@@ -159,6 +166,7 @@ static int nthr(const orc* o) {
 orc* orc_create(const orc_params* p) {
   orc* o = (orc*)calloc(1, sizeof(orc));
   o->p = *p;
+  o->pst_tstep = -1;
   o->nx = p->nodes_x; o->ny = p->nodes_y; o->nz = p->nodes_z;
   /* src/pg_domain.f90:220-228 */
   o->ki = o->nx / 3; o->kj = o->ny / 3; o->kk = o->nz / 3;
@@ -324,7 +332,7 @@ void orc_destroy(orc* o) {
   if (!o) return;
   for (int c = 0; c < 3; c++) { free(o->u[c]); free(o->rk[c]); free(o->ur[c]); }
   for (int f = 0; f < 6; f++) free(o->w[f]);
-  free(o->xp); free(o->up); free(o->uprk);
+  free(o->xp); free(o->up); free(o->uprk); free(o->xp0);
   fftp_free(&o->fz); fftp_free(&o->fy); fftp_free(&o->fx);
   free(o);
 }
@@ -1000,7 +1008,17 @@ int orc_remesh(orc* o, int tstep) {
   const int a = tstep - (b110 * o->p.f_remesh_t) - 1;
   const int b = 2 * b110 * o->p.f_remesh_t;
   if ((a % b) != 0) return 0; /* Fortran mod truncates like C % */
-  /* particles first: x1 += x3, wrap x1 */
+  /* particles first: p_ControlRemeshInitial (x0_1 += x0_3, shifted as x1 + x3
+   * wraps, src/pp_mpi.f90:1095-1129), then x1 += x3, wrap x1 */
+  for (long q = 0; q < o->np && o->xp0; q++) {
+    const double* x = &o->xp[3 * q];
+    double* x0 = &o->xp0[3 * q];
+    const double x1 = x[0] + x[2];
+    const double x1min = -0.5 * o->dx, x1max = x1min + o->lx;
+    x0[0] = x0[0] + x0[2];
+    if (x1 < x1min) x0[0] = x0[0] + o->lx;
+    if (x1 >= x1max) x0[0] = x0[0] - o->lx;
+  }
   for (long q = 0; q < o->np; q++) {
     double* x = &o->xp[3 * q];
     x[0] = x[0] + x[2];
@@ -1041,12 +1059,46 @@ int orc_remesh(orc* o, int tstep) {
 /* Tracers                                                                    */
 /* ------------------------------------------------------------------------- */
 void orc_set_particles(orc* o, const double* xp, int64_t np) {
-  free(o->xp); free(o->up); free(o->uprk);
+  free(o->xp); free(o->up); free(o->uprk); free(o->xp0);
   o->np = np;
   o->xp = (double*)malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
   o->up = (double*)calloc(3 * (size_t)(np > 0 ? np : 1), sizeof(double));
   o->uprk = (double*)calloc(3 * (size_t)(np > 0 ? np : 1), sizeof(double));
   memcpy(o->xp, xp, sizeof(double) * 3 * (size_t)np);
+  o->xp0 = NULL;
+  if (o->keep_x0) {  /* particle_xp0 = particle_xp at placement, src/pp_control.f90:405-407 */
+    o->xp0 = (double*)malloc(sizeof(double) * 3 * (size_t)(np > 0 ? np : 1));
+    memcpy(o->xp0, xp, sizeof(double) * 3 * (size_t)np);
+  }
+}
+
+void orc_get_particles_aux(const orc* o, double* up, double* xp0) {
+  if (up) memcpy(up, o->up, sizeof(double) * 3 * (size_t)o->np);
+  if (xp0 && o->xp0) memcpy(xp0, o->xp0, sizeof(double) * 3 * (size_t)o->np);
+}
+
+void orc_set_particles_aux(orc* o, const double* up, const double* xp0) {
+  if (up) memcpy(o->up, up, sizeof(double) * 3 * (size_t)o->np);
+  if (xp0 && o->xp0) memcpy(o->xp0, xp0, sizeof(double) * 3 * (size_t)o->np);
+}
+
+void orc_particle_stats_config(orc* o, int enable, int keep_initial, int ts_release) {
+  o->pst_on = enable;
+  o->ts_release = ts_release;
+  if (keep_initial && !o->xp0 && o->np > 0) {
+    o->xp0 = (double*)malloc(sizeof(double) * 3 * (size_t)o->np);
+    memcpy(o->xp0, o->xp, sizeof(double) * 3 * (size_t)o->np);
+  }
+  if (!keep_initial) { free(o->xp0); o->xp0 = NULL; }
+  o->keep_x0 = keep_initial;
+  for (int c = 0; c < 3; c++) o->xrel_prev[c] = 0.0;
+  for (int a = 0; a < 30; a++) o->pst[a] = 0.0;
+  o->pst_tstep = -1;
+}
+
+int orc_particle_stats(const orc* o, double* rec) {
+  if (rec) memcpy(rec, o->pst, sizeof(o->pst));
+  return o->pst_tstep;
 }
 
 void orc_get_particles(const orc* o, double* xp, double* uprk) {
@@ -1185,8 +1237,83 @@ static void tracers_stage(orc* o, int stage, double dt) {
           if (T9(o->bval, m, n)) xm[m - 1] = xm[m - 1] + T9(o->bmat, m, n) * xl[n - 1];
       for (int c = 0; c < 3; c++) x[c] = xm[c];
     }
+    const double xu[3] = {x[0], x[1], x[2]};
     wrap_one(o, x);
+    /* p_ControlUpdateInitial (src/pp_control.f90:1574-1585): xp0 + xp_corrected - xp,
+     * evaluated left to right on every call */
+    if (o->xp0)
+      for (int c = 0; c < 3; c++) o->xp0[3 * q + c] = (o->xp0[3 * q + c] + x[c]) - xu[c];
   }
+}
+
+/* p_StatsPrimary + p_StatsDispersion (src/pp_stats.f90:119-233, 303-370) for
+ * tracers (dp = 0: particle kinetic energy, response time, Stokes numbers are
+ * zero), with the mean velocity of p_RK3AbsoluteVelocity[BF] (src/pp_rk3.f90:
+ * 602-710) that p_StatsRelativeVelocity reports as vt.  Sequential sums in
+ * particle order, as on one rank. */
+static void pstats_primary(orc* o, double dt) {
+  double s[6] = {0, 0, 0, 0, 0, 0}, va[3] = {0, 0, 0};
+  double lx[3] = {0, 0, 0}, lx2[3] = {0, 0, 0}, l12 = 0, l13 = 0, l23 = 0;
+  for (long q = 0; q < o->np; q++) {
+    const double* u = &o->up[3 * q];
+    s[0] = s[0] + u[0] * u[0];
+    s[1] = s[1] + u[1] * u[1];
+    s[2] = s[2] + u[2] * u[2];
+    s[3] = s[3] + u[0] * u[1];
+    s[4] = s[4] + u[0] * u[2];
+    s[5] = s[5] + u[1] * u[2];
+    for (int c = 0; c < 3; c++) va[c] = va[c] + u[c];
+    if (o->xp0) {
+      double xm[3], xl[3] = {0, 0, 0};
+      for (int c = 0; c < 3; c++) xm[c] = o->xp[3 * q + c] - o->xp0[3 * q + c];
+      if (o->p.f_type == 1) {
+        for (int c = 0; c < 3; c++) xl[c] = xm[c];
+      } else {
+        for (int m = 1; m <= 3; m++)
+          for (int n = 1; n <= 3; n++)
+            if (T9(o->bval, m, n)) xl[m - 1] = xl[m - 1] + T9(o->binv, m, n) * xm[n - 1];
+      }
+      for (int c = 0; c < 3; c++) { lx[c] = lx[c] + xl[c]; lx2[c] = lx2[c] + (xl[c] * xl[c]); }
+      l12 = l12 + (xl[0] * xl[1]);
+      l13 = l13 + (xl[0] * xl[2]);
+      l23 = l23 + (xl[1] * xl[2]);
+    }
+  }
+  const double npg = (double)o->np;
+  double* r = o->pst;
+  for (int a = 0; a < 30; a++) r[a] = 0.0;
+  for (int a = 0; a < 6; a++) r[a] = s[a] / npg;
+  r[6] = 0.5 * (r[0] + r[1] + r[2]);
+  r[7] = 0.0;                                         /* pken: dp = 0 */
+  for (int c = 0; c < 3; c++) r[21 + c] = va[c] / npg; /* vt = vabs */
+  if (o->xp0) {
+    double xrel[3];
+    for (int c = 0; c < 3; c++) xrel[c] = lx2[c] / npg;
+    r[18] = l12 / npg;
+    r[19] = l13 / npg;
+    r[20] = l23 / npg;
+    for (int c = 0; c < 3; c++) {
+      r[12 + c] = xrel[c];
+      r[15 + c] = 0.5 * (xrel[c] - o->xrel_prev[c]) / dt;
+      o->xrel_prev[c] = xrel[c];
+    }
+  }
+}
+
+/* p_RK3RelativeVelocity for tracers (uf = 0, src/pp_rk3.f90:407-464) summed over
+ * the stage-1 velocities, and p_StatsRelativeVelocity (src/pp_stats.f90:240-296). */
+static void pstats_relvel(orc* o, int tstep) {
+  double ab = 0, v[3] = {0, 0, 0};
+  for (long q = 0; q < o->np; q++) {
+    const double* u = &o->up[3 * q];
+    const double a = (u[0] * u[0]) + (u[1] * u[1]) + (u[2] * u[2]);
+    ab = ab + a;
+    for (int c = 0; c < 3; c++) v[c] = v[c] + u[c] * u[c];
+  }
+  const double npg = (double)o->np;
+  for (int c = 0; c < 3; c++) o->pst[8 + c] = v[c] / npg;
+  o->pst[11] = sqrt(ab / npg);
+  o->pst_tstep = tstep;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1195,9 +1322,23 @@ static void tracers_stage(orc* o, int stage, double dt) {
 void orc_stage(orc* o, int stage, int tstep, double time, double* dt) {
   (void)time;
   if (stage == 1) orc_remesh(o, tstep);
+  /* f_RK3Stage1: p_StatsPrimary before the stage, p_StatsRelativeVelocity after
+   * it (src/pf_rk3.f90:62-99), when tstep > P_TS_RELEASE */
+  const int pst = stage == 1 && o->pst_on && o->np > 0 && tstep > o->ts_release;
+  if (pst) pstats_primary(o, *dt);
   orc_prepare_realspace(o);
   orc_realspace(o, stage, tstep, dt);
-  if (o->np > 0) tracers_stage(o, stage, *dt);
+  /* particles move from tstep = P_TS_RELEASE on (src/pf_rk3.f90:1146); at the
+   * release, p_ControlInitPartVelocity[BF] (src/pp_control.f90:378-510) resets
+   * particle_up_rk3 and particle_xp0 (particle_up is reassigned by the solve) */
+  if (o->np > 0 && tstep >= o->ts_release) {
+    if (stage == 1 && tstep == o->ts_release) {
+      memset(o->uprk, 0, sizeof(double) * 3 * (size_t)o->np);
+      if (o->xp0) memcpy(o->xp0, o->xp, sizeof(double) * 3 * (size_t)o->np);
+    }
+    tracers_stage(o, stage, *dt);
+  }
+  if (pst) pstats_relvel(o, tstep);
   orc_force_calc(o, tstep, *dt);  /* src/pf_rk3.f90:2063 */
   solve_ns(o, *dt, stage);
 }

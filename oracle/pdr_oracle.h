@@ -96,6 +96,16 @@ void   orc_get_tracer_field(const orc* o, double* u1r, double* u2r, double* u3r)
 void   orc_interpolate(const orc* o, const double* f1, const double* f2, const double* f3,
                        const double* xp, int64_t np, double* uf);
 void   orc_correct_position(const orc* o, double* xp, int64_t np);
+/* particle_up (lab-frame fluid velocity of the last stage) and particle_xp0 */
+void   orc_get_particles_aux(const orc* o, double* up, double* xp0);
+void   orc_set_particles_aux(orc* o, const double* up, const double* xp0);
+/* Tracer statistics (src/pp_stats.f90:119-390): computed in stage 1 when
+ * enable and tstep > ts_release; keep_initial = P_KEEP_INITIAL_POSITION (xp0 =
+ * xp now and at the release step).  Particles move from tstep = ts_release on.
+ * orc_particle_stats copies the last record (30 values in p_StatsWrite order)
+ * and returns its tstep (-1 if none yet). */
+void   orc_particle_stats_config(orc* o, int enable, int keep_initial, int ts_release);
+int    orc_particle_stats(const orc* o, double* rec);
 
 /* Plain complex FFT used by the oracle (out = sum_n in[n] exp(sign*2*pi*i*n*k/n)). */
 void   orc_fft(int n, int sign, const double* in, double* out);

@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("PDR_FC", "/opt/rocm/bin/amdflang")
 ARCH = "gfx950"
 
-SOURCES = ["pdr_api.hip", "pdr_kernels.hip", "pdr_kernels2.hip", "pdr_host.hip", "pdr_stats.hip"]
+SOURCES = ["pdr_api.hip", "pdr_kernels.hip", "pdr_kernels2.hip", "pdr_host.hip", "pdr_stats.hip", "pdr_pstats.hip"]
 
 
 def _run(cmd, cwd=None):
@@ -39,7 +39,7 @@ def build_hip(verbose: bool = True) -> str:
     for src in SOURCES:
         obj = os.path.join(LIB, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
-               # no packed-FP32 instructions (wrong results seen on this hardware pool)
+               # no packed-FP32 instructions: some of their forms have given wrong results on gfx950
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
                "-I", os.path.join(ROOT, "include"), "-I", CSRC,

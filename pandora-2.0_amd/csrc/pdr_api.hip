@@ -245,6 +245,9 @@ int pdr_destroy(pdr_ctx* c) {
   if (c->ur) (void)hipFree(c->ur);
   if (c->xp) (void)hipFree(c->xp);
   if (c->uprk) (void)hipFree(c->uprk);
+  if (c->up) (void)hipFree(c->up);
+  if (c->xp0) (void)hipFree(c->xp0);
+  if (c->pst_buf) (void)hipFree(c->pst_buf);
   if (c->umax) (void)hipFree(c->umax);
   if (c->red) (void)hipFree(c->red);
   if (c->force) (void)hipFree(c->force);
@@ -569,8 +572,13 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
     int rc = maybe_remesh(c, tstep, nullptr);
     if (rc != PDR_OK) return rc;
   }
+  // f_RK3Stage1: p_StatsPrimary after the remesh, p_StatsRelativeVelocity after
+  // the particle advance (src/pf_rk3.f90:62-99), for tstep > P_TS_RELEASE
+  const bool pstats = stage == 1 && c->pst_on && c->np > 0 && tstep > c->ts_release;
+  if (pstats) HIPCHK(launch_pstats_primary(c, *dt));
   const Coef cf = make_coef(c);
-  const bool tracers = c->np > 0;
+  // particles move from tstep = P_TS_RELEASE on (src/pf_rk3.f90:1146)
+  const bool tracers = c->np > 0 && tstep >= c->ts_release;
   const bool need_cfl =
       stage == 1 && (c->p.rk3_timestep == PDR_RK3_TIMESTEP_CFL ||
                      (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE && c->p.f_type == PDR_F_ISOTROPIC));
@@ -612,7 +620,16 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   double beta, gamma;
   rk_coeffs(stage, &beta, &gamma);
   const double gdt = gamma * (*dt);
-  if (tracers) HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt));
+  if (tracers) {
+    if (stage == 1 && tstep == c->ts_release) {
+      // p_ControlInitPartVelocity[BF] at the release (src/pp_control.f90:378-510):
+      // particle_up_rk3 = 0, particle_xp0 = particle_xp (particle_up is set by the advance)
+      HIPCHK(hipMemsetAsync(c->uprk, 0, 24 * size_t(c->np), c->stream));
+      if (c->xp0) HIPCHK(hipMemcpyAsync(c->xp0, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt));
+  }
+  if (pstats) HIPCHK(launch_pstats_rel(c, tstep));
   if (c->forcing) HIPCHK(launch_force(c, tstep, *dt));  // f_ForceCalc (src/pf_rk3.f90:2063)
   if (c->spec && c->fuse_zf) {
     HIPCHK(launch_zfwd_formf(c, cf, beta, gdt));
@@ -697,18 +714,77 @@ int pdr_check_finite(pdr_ctx* c) {
 int pdr_particles_upload(pdr_ctx* c, const double* xp, int64_t np) {
   if (!c || np < 0 || (np > 0 && !xp)) return fail(PDR_ERR_ARG, "pdr_particles_upload: bad argument");
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (c->xp) { (void)hipFree(c->xp); c->dev_bytes -= 24 * size_t(c->np); c->xp = nullptr; }
-  if (c->uprk) { (void)hipFree(c->uprk); c->dev_bytes -= 24 * size_t(c->np); c->uprk = nullptr; }
+  double** arrays[4] = {&c->xp, &c->uprk, &c->up, &c->xp0};
+  for (double** a : arrays)
+    if (*a) {
+      (void)hipFree(*a);
+      c->dev_bytes -= 24 * size_t(c->np);
+      *a = nullptr;
+    }
   c->np = 0;
   if (np == 0) return PDR_OK;
   int rc = dalloc(c, (void**)&c->xp, 24 * size_t(np));
   if (rc == PDR_OK) rc = dalloc(c, (void**)&c->uprk, 24 * size_t(np));
+  if (rc == PDR_OK) rc = dalloc(c, (void**)&c->up, 24 * size_t(np));
+  if (rc == PDR_OK && c->keep_x0) rc = dalloc(c, (void**)&c->xp0, 24 * size_t(np));
   if (rc == PDR_OK && !c->ur) rc = dalloc(c, &c->ur, 3 * size_t(c->g.nr) * c->rsz);
   if (rc != PDR_OK) return rc;
   HIPCHK(hipMemcpy(c->xp, xp, 24 * size_t(np), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(c->uprk, 0, 24 * size_t(np)));
+  HIPCHK(hipMemset(c->up, 0, 24 * size_t(np)));
+  // particle_xp0 = particle_xp at placement (src/pp_control.f90:405-407)
+  if (c->xp0) HIPCHK(hipMemcpy(c->xp0, xp, 24 * size_t(np), hipMemcpyHostToDevice));
   c->np = np;
   return PDR_OK;
+}
+
+int pdr_particles_download_aux(pdr_ctx* c, double* up, double* xp0) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  if (c->np == 0) return fail(PDR_ERR_STATE, "no particles set");
+  if (xp0 && !c->xp0) return fail(PDR_ERR_STATE, "initial positions are not kept (pdr_particle_stats_config)");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (up) HIPCHK(hipMemcpy(up, c->up, 24 * size_t(c->np), hipMemcpyDeviceToHost));
+  if (xp0) HIPCHK(hipMemcpy(xp0, c->xp0, 24 * size_t(c->np), hipMemcpyDeviceToHost));
+  return PDR_OK;
+}
+
+int pdr_particles_upload_aux(pdr_ctx* c, const double* up, const double* xp0) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  if (c->np == 0) return fail(PDR_ERR_STATE, "no particles set");
+  if (xp0 && !c->xp0) return fail(PDR_ERR_STATE, "initial positions are not kept (pdr_particle_stats_config)");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (up) HIPCHK(hipMemcpy(c->up, up, 24 * size_t(c->np), hipMemcpyHostToDevice));
+  if (xp0) HIPCHK(hipMemcpy(c->xp0, xp0, 24 * size_t(c->np), hipMemcpyHostToDevice));
+  return PDR_OK;
+}
+
+int pdr_particle_stats_config(pdr_ctx* c, int32_t enable, int32_t keep_initial, int32_t ts_release) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (keep_initial && !c->xp0 && c->np > 0) {
+    int rc = dalloc(c, (void**)&c->xp0, 24 * size_t(c->np));
+    if (rc != PDR_OK) return rc;
+    HIPCHK(hipMemcpy(c->xp0, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToDevice));
+  }
+  if (!keep_initial && c->xp0) {
+    (void)hipFree(c->xp0);
+    c->dev_bytes -= 24 * size_t(c->np);
+    c->xp0 = nullptr;
+  }
+  c->keep_x0 = keep_initial ? 1 : 0;
+  c->ts_release = ts_release;
+  c->pst_on = enable ? 1 : 0;
+  if (enable) {
+    int rc = pstats_alloc(c);
+    if (rc != PDR_OK) return rc;
+  }
+  return pstats_reset(c);
+}
+
+int pdr_particle_stats(pdr_ctx* c, int32_t* tstep, pdr_pstats_t* out) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  static_assert(sizeof(pdr_pstats_t) == 30 * sizeof(double), "pdr_pstats_t is 30 doubles");
+  return pstats_read(c, tstep, reinterpret_cast<double*>(out));
 }
 
 int pdr_particles_download(pdr_ctx* c, double* xp, double* uprk) {

@@ -100,7 +100,15 @@ struct pdr_ctx {
   void* ur = nullptr;  // real-space velocity for tracers, [z][y][x][3]
   double* xp = nullptr;
   double* uprk = nullptr;
+  double* up = nullptr;   // particle_up (3, np): laboratory-frame fluid velocity of the last stage
+  double* xp0 = nullptr;  // particle_xp0 (3, np) with P_KEEP_INITIAL_POSITION, else null
   int64_t np = 0;
+  // tracer statistics (p_StatsPrimary / p_StatsRelativeVelocity)
+  int pst_on = 0, keep_x0 = 0, ts_release = 0;
+  double xrel_prev[3] = {0.0, 0.0, 0.0};
+  double pst[30] = {};
+  int pst_tstep = -1;
+  double* pst_buf = nullptr;  // device partial sums
   unsigned long long* umax = nullptr;  // CFL reduction (bits of a non-negative double)
   double* red = nullptr;               // spectrum reduction scratch
   void* stat_buf = nullptr;            // statistics partials (allocated on first use)
@@ -148,6 +156,12 @@ hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
 hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt);
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);
+// tracer statistics (pdr_pstats.hip)
+int pstats_alloc(pdr_ctx* c);
+int pstats_reset(pdr_ctx* c);
+hipError_t launch_pstats_primary(pdr_ctx* c, double dt);
+hipError_t launch_pstats_rel(pdr_ctx* c, int tstep);
+int pstats_read(pdr_ctx* c, int32_t* tstep, double* rec);
 hipError_t launch_finite(pdr_ctx* c, int* flag);
 hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s);
 

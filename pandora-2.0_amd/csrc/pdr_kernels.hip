@@ -419,7 +419,8 @@ __device__ __forceinline__ long long pmod(long long a, long long n) {
 template <typename T, int P>
 __global__ void __launch_bounds__(kThreads)
     k_tracers_col(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
-                  double* __restrict__ uprk, long long np, double beta, double gdt) {
+                  double* __restrict__ uprk, double* __restrict__ upa, double* __restrict__ xp0,
+                  long long np, double beta, double gdt) {
   constexpr int G = P <= 2 ? 2 : (P <= 4 ? 4 : 8);
   const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long q = gt / G;
@@ -512,10 +513,12 @@ __global__ void __launch_bounds__(kThreads)
     uf[c] = vx[0];
   }
   double r[3] = {uprk[3 * qq], uprk[3 * qq + 1], uprk[3 * qq + 2]};
+  double up[3];  // particle_up: the fluid velocity, laboratory frame
   if (!tf.bf) {
     // p_RK3SolveParticle tracer branch (src/pp_rk3.f90:103-134)
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
+      up[m] = uf[m];
       r[m] = (beta * r[m]) + uf[m];
       x[m] = x[m] + (gdt * r[m]);
     }
@@ -531,6 +534,7 @@ __global__ void __launch_bounds__(kThreads)
       }
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
+      up[m] = ul[m];
       r[m] = (beta * r[m]) + ul[m];
       xl[m] = xl[m] + (gdt * r[m]);
     }
@@ -546,6 +550,7 @@ __global__ void __launch_bounds__(kThreads)
   const double x1min = -0.5 * tf.dx, x1max = x1min + tf.lx;
   const double x2min = -0.5 * tf.dy, x2max = x2min + tf.ly;
   const double x3min = -0.5 * tf.dz, x3max = x3min + tf.lz;
+  const double xu[3] = {x[0], x[1], x[2]};
   if (x[0] < x1min) x[0] = x[0] + tf.lx;
   if (x[0] >= x1max) x[0] = x[0] - tf.lx;
   if (x[1] < x2min) x[1] = x[1] + tf.ly;
@@ -553,8 +558,16 @@ __global__ void __launch_bounds__(kThreads)
   if (x[2] < x3min) x[2] = x[2] + tf.lz;
   if (x[2] > x3max) x[2] = x[2] - tf.lz;
   if (live && a < 3) {
-    xpa[3 * q + a] = (a == 0) ? x[0] : ((a == 1) ? x[1] : x[2]);
+    const double xa = (a == 0) ? x[0] : ((a == 1) ? x[1] : x[2]);
+    xpa[3 * q + a] = xa;
     uprk[3 * q + a] = (a == 0) ? r[0] : ((a == 1) ? r[1] : r[2]);
+    upa[3 * q + a] = (a == 0) ? up[0] : ((a == 1) ? up[1] : up[2]);
+    if (xp0) {
+      // p_ControlUpdateInitial (src/pp_control.f90:1574-1585): xp0 + x_wrapped - x,
+      // left to right, on every stage
+      const double xb = (a == 0) ? xu[0] : ((a == 1) ? xu[1] : xu[2]);
+      xp0[3 * q + a] = (xp0[3 * q + a] + xa) - xb;
+    }
   }
 }
 
@@ -593,11 +606,20 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // p_MPIRemeshPosition (src/pp_mpi.f90:1062-1092)
-__global__ void k_remesh_particles(double* __restrict__ xp, long long np, double lx, double dx) {
+__global__ void k_remesh_particles(double* __restrict__ xp, double* __restrict__ xp0, long long np, double lx,
+                                   double dx) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
-  double x1 = xp[3 * q] + xp[3 * q + 2];
+  const double xs = xp[3 * q] + xp[3 * q + 2];
   const double x1min = -0.5 * dx, x1max = x1min + lx;
+  if (xp0) {
+    // p_MPIRemeshInitial (src/pp_mpi.f90:1095-1129): x0_1 += x0_3, shifted as x1 + x3 wraps
+    double x01 = xp0[3 * q] + xp0[3 * q + 2];
+    if (xs < x1min) x01 = x01 + lx;
+    if (xs >= x1max) x01 = x01 - lx;
+    xp0[3 * q] = x01;
+  }
+  double x1 = xs;
   if (x1 < x1min) x1 = x1 + lx;
   if (x1 >= x1max) x1 = x1 - lx;
   xp[3 * q] = x1;
@@ -790,7 +812,8 @@ static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, doub
   const T* ur = (const T*)c->ur;
   auto go = [&](auto kern, int lanes) {
     const long long nb = (lanes * c->np + kThreads - 1) / kThreads;
-    kern<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->np, beta, gdt);
+    kern<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->up, c->xp0, c->np,
+                                                         beta, gdt);
   };
   switch (tf.order) {
     case 2: go(k_tracers_col<T, 2>, 2); break;
@@ -830,7 +853,7 @@ hipError_t launch_remesh(pdr_ctx* c, int bfrot) {
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx) {
   if (c->np <= 0) return hipSuccess;
   const long long nb = (c->np + kThreads - 1) / kThreads;
-  k_remesh_particles<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->xp, c->np, lx, dx);
+  k_remesh_particles<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->xp, c->xp0, c->np, lx, dx);
   return hipGetLastError();
 }
 

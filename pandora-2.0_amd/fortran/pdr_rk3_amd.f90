@@ -49,6 +49,7 @@ module pdr_rk3_amd
   public :: pdr_PutVelocity, pdr_GetVelocity, pdr_PutParticles, pdr_GetParticles
   public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_Spectrum, pdr_ErrorMessage
   public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing, pdr_Stats, pdr_RsStats
+  public :: pdr_ParticleStatsConfig, pdr_ParticleStats, pdr_GetParticlesAux
 
   interface
     integer(c_int) function c_pdr_create(out, p) bind(C, name="pdr_create")
@@ -155,6 +156,22 @@ module pdr_rk3_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: c
       real(c_double), intent(out) :: q(25)
+    end function
+    integer(c_int) function c_pdr_particle_stats_config(c, enable, keep, release) &
+        bind(C, name="pdr_particle_stats_config")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: enable, keep, release
+    end function
+    integer(c_int) function c_pdr_particle_stats(c, tstep, q) bind(C, name="pdr_particle_stats")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), intent(out) :: tstep
+      real(c_double), intent(out) :: q(30)
+    end function
+    integer(c_int) function c_pdr_particles_download_aux(c, up, xp0) bind(C, name="pdr_particles_download_aux")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, up, xp0
     end function
     type(c_ptr) function c_pdr_last_error() bind(C, name="pdr_last_error")
       import :: c_ptr
@@ -343,6 +360,35 @@ contains
     integer, intent(inout)           :: ierr
     ierr = int(c_pdr_rs_stats(ctx, q))
   end subroutine pdr_RsStats
+
+  !> Tracer statistics of f_RK3Stage1 (p_StatsPrimary, p_StatsDispersion,
+  !! p_StatsRelativeVelocity): track = P_TRACK_PART statistics on/off,
+  !! keep = P_KEEP_INITIAL_POSITION, release = P_TS_RELEASE
+  subroutine pdr_ParticleStatsConfig ( track, keep, release, ierr )
+    integer, intent(in)    :: track, keep, release
+    integer, intent(inout) :: ierr
+    ierr = int(c_pdr_particle_stats_config(ctx, int(track, c_int32_t), int(keep, c_int32_t), &
+                                           int(release, c_int32_t)))
+  end subroutine pdr_ParticleStatsConfig
+
+  !> The last p_StatsWrite record: q(1:30) = u1u1, u2u2, u3u3, u1u2, u1u3, u2u3,
+  !! tke, ptke, vrel1..3, vrelabs, xrel1..3, dfp1..3, x1x2, x1x3, x2x3, vt(3),
+  !! vs(3), Rep, Stk, Ste; tstep = its time step (-1 if none yet)
+  subroutine pdr_ParticleStats ( tstep, q, ierr )
+    integer, intent(out)             :: tstep
+    real(kind=c_double), intent(out) :: q(30)
+    integer, intent(inout)           :: ierr
+    integer(c_int32_t)               :: t
+    ierr = int(c_pdr_particle_stats(ctx, t, q))
+    tstep = int(t)
+  end subroutine pdr_ParticleStats
+
+  !> particle_up(1:3,np) and particle_xp0(1:3,np) from the device
+  subroutine pdr_GetParticlesAux ( up, xp0, ierr )
+    real(kind=c_double), intent(out), target :: up(:,:), xp0(:,:)
+    integer, intent(inout)                   :: ierr
+    ierr = int(c_pdr_particles_download_aux(ctx, c_loc(up), c_loc(xp0)))
+  end subroutine pdr_GetParticlesAux
 
   !> Text of the last library error
   function pdr_ErrorMessage () result ( msg )

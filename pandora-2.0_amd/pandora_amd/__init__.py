@@ -53,6 +53,10 @@ RS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "r
             "rs_skew3", "rs_flat1", "rs_flat2", "rs_flat3", "rs_rmsvort1", "rs_rmsvort2", "rs_rmsvort3",
             "rs_skewvort1", "rs_skewvort2", "rs_skewvort3", "rs_flatvort1", "rs_flatvort2",
             "rs_flatvort3", "omegamax1", "omegamax2", "omegamax3", "ucfl")
+# fields of pdr_pstats_t (the p_StatsWrite record), in order
+PSTATS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "tke", "pken",
+                "vrel1", "vrel2", "vrel3", "vrel_abs", "xrel1", "xrel2", "xrel3", "dfp1", "dfp2", "dfp3",
+                "x1x2", "x1x3", "x2x3", "vt1", "vt2", "vt3", "vs1", "vs2", "vs3", "rep", "stk", "ste")
 
 
 class Params(C.Structure):
@@ -108,6 +112,10 @@ def load_library(path: str = LIB_PATH):
         "pdr_check_finite": (C.c_int, [vp]),
         "pdr_particles_upload": (C.c_int, [vp, dp, i64]),
         "pdr_particles_download": (C.c_int, [vp, vp, vp]),
+        "pdr_particles_download_aux": (C.c_int, [vp, vp, vp]),
+        "pdr_particles_upload_aux": (C.c_int, [vp, vp, vp]),
+        "pdr_particle_stats_config": (C.c_int, [vp, i32, i32, i32]),
+        "pdr_particle_stats": (C.c_int, [vp, C.POINTER(i32), dp]),
         "pdr_init_field": (C.c_int, [vp, C.c_uint64, d, d]),
         "pdr_init_particles": (C.c_int, [vp, C.c_uint64, i64]),
         "pdr_init_field_host": (C.c_int, [i32, i32, i32, C.c_uint64, d, d, dp, dp, dp]),
@@ -209,6 +217,7 @@ class Pandora:
         self.nx, self.ny, self.nz = nx, ny, nz
         self.shape = (self.nki, self.nkj, self.nkk)
         self.precision = precision
+        self._keep_x0 = False
 
     def close(self):
         if getattr(self, "h", None):
@@ -347,6 +356,35 @@ class Pandora:
         ur = np.zeros((n, 3))
         _check(self.L.pdr_particles_download(self.h, xp.ctypes.data, ur.ctypes.data))
         return xp, ur
+
+    def get_particles_aux(self):
+        """(particle_up, particle_xp0 or None)"""
+        n = int(self.sizes()[7])
+        up = np.zeros((n, 3))
+        _check(self.L.pdr_particles_download_aux(self.h, up.ctypes.data, None))
+        if not self._keep_x0:
+            return up, None
+        x0 = np.zeros((n, 3))
+        _check(self.L.pdr_particles_download_aux(self.h, None, x0.ctypes.data))
+        return up, x0
+
+    def set_particles_aux(self, up=None, xp0=None):
+        a = None if up is None else np.ascontiguousarray(up, np.float64).reshape(-1, 3)
+        b = None if xp0 is None else np.ascontiguousarray(xp0, np.float64).reshape(-1, 3)
+        _check(self.L.pdr_particles_upload_aux(self.h, None if a is None else a.ctypes.data,
+                                               None if b is None else b.ctypes.data))
+
+    def particle_stats_config(self, enable=True, keep_initial=False, ts_release=0):
+        """Tracer statistics in stage 1 (p_StatsPrimary/Dispersion/RelativeVelocity)."""
+        _check(self.L.pdr_particle_stats_config(self.h, int(enable), int(keep_initial), int(ts_release)))
+        self._keep_x0 = bool(keep_initial)
+
+    def particle_stats(self):
+        """(tstep, {name: value}) of the last tracer-statistics record (tstep -1: none)."""
+        t = C.c_int32(-1)
+        r = np.zeros(len(PSTATS_NAMES))
+        _check(self.L.pdr_particle_stats(self.h, C.byref(t), r))
+        return int(t.value), dict(zip(PSTATS_NAMES, r.tolist()))
 
     # ---- initial field / restart / statistics -------------------------------
     def init_field(self, seed: int, u0: float, kp: float):
