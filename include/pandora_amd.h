@@ -276,6 +276,26 @@ typedef struct pdr_stats_t {
  * spec_ken1..3, spec_dis, spec_disskew and dens_mod, bin kk at index kk-1. */
 int pdr_stats(pdr_ctx* ctx, pdr_stats_t* out, double* spec);
 
+/* Streamwise 1-D spectra of f_FluidstatsFs2D + f_FluidstatsFsCollect
+ * (src/pf_fluidstats.f90:1154-1283) for the current state: spec_1i[i],
+ * spec_2i[i], i = 0..NKI-1, the sums over the (y, z) points of 2 |u_lab,c|^2 of
+ * the y- and z-transformed velocity divided by NODES_X (times binv(1,1)^3 with
+ * shear), as the reference forms them in stage 1 from the 1r1w arrays
+ * (src/pf_rk3.f90:1504-1590).  Computed from the wave state in the statistics
+ * pass (Parseval: 2 NY NZ / NX^2 sum_{j,k} |u_lab(i,j,k)|^2).  h (may be NULL)
+ * receives the f_FluidstatsHomogeneous record (src/pf_fluidstats.f90:
+ * 2568-2936) the reference writes for shear runs; its ln_int_21, ln_int_12 and
+ * ln_int_13 are never assigned, so lstar = ln_int_11 / 0 (infinite) there and
+ * here.  Either spectrum pointer may be NULL. */
+typedef struct pdr_homog_t {
+  double sstar, kstar, lstar;                   /* S* = S tn_edy, K*, L*          */
+  double ln_int1, ln_int2, ln_int3;
+  double ln_int_11, ln_int_12, ln_int_13;       /* pi E11(0) / (2 u1u1), 0, 0     */
+  double fn_ken1, fn_ken2, fn_ken3;
+  double fn_int1, fn_int2, fn_int3;
+} pdr_homog_t;
+int pdr_spectra_1d(pdr_ctx* ctx, double* spec_1i, double* spec_2i, pdr_homog_t* h);
+
 /* Real-space statistics of the current velocity: f_FluidstatsRs2D and
  * f_FluidstatsRsCollect (src/pf_fluidstats.f90:827-951, 962-1146), which the
  * reference computes on the real-space planes of stage 1 (src/pf_rk3.f90:

@@ -87,6 +87,8 @@ def lib():
         L.orc_rs_raw.argtypes = [vp, dp]
         L.orc_rs_finish.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_rs_stats.argtypes = [vp, dp]
+        L.orc_spectra_1d.argtypes = [vp, dp, dp]
+        L.orc_homogeneous.argtypes = [vp, dp]
         L.orc_force_nk.restype = C.c_int
         L.orc_force_nk.argtypes = [vp]
         L.orc_set_forcing.argtypes = [vp, dp, dp, dp]
@@ -117,6 +119,9 @@ RS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "r
             "rs_skew3", "rs_flat1", "rs_flat2", "rs_flat3", "rs_rmsvort1", "rs_rmsvort2", "rs_rmsvort3",
             "rs_skewvort1", "rs_skewvort2", "rs_skewvort3", "rs_flatvort1", "rs_flatvort2",
             "rs_flatvort3", "omegamax1", "omegamax2", "omegamax3", "ucfl")
+# f_FluidstatsHomogeneous record (src/pf_fluidstats.f90:2910-2932)
+HOMOG_NAMES = ("sstar", "kstar", "lstar", "ln_int1", "ln_int2", "ln_int3", "ln_int_11", "ln_int_12",
+               "ln_int_13", "fn_ken1", "fn_ken2", "fn_ken3", "fn_int1", "fn_int2", "fn_int3")
 # p_StatsWrite record (src/pp_stats.f90:379-390)
 PSTATS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "tke", "pken",
                 "vrel1", "vrel2", "vrel3", "vrel_abs", "xrel1", "xrel2", "xrel3", "dfp1", "dfp2", "dfp3",
@@ -206,6 +211,18 @@ class Oracle:
         out = np.zeros(25)
         lib().orc_rs_stats(self.h, out)
         return dict(zip(RS_NAMES, out))
+
+    def spectra_1d(self):
+        """(spec_1i, spec_2i) over i = 0..K (f_FluidstatsFs2D/FsCollect)."""
+        a = np.zeros(self.nki)
+        b = np.zeros(self.nki)
+        lib().orc_spectra_1d(self.h, a, b)
+        return a, b
+
+    def homogeneous(self):
+        out = np.zeros(len(HOMOG_NAMES))
+        lib().orc_homogeneous(self.h, out)
+        return dict(zip(HOMOG_NAMES, out.tolist()))
 
     # ---- forcing -----------------------------------------------------------
     @property

@@ -745,6 +745,84 @@ void orc_rs_stats(orc* o, double* out) {
   orc_rs_finish(o->nx, o->ny, o->nz, raw, out);
 }
 
+/* f_FluidstatsFs2D + f_FluidstatsFsCollect (src/pf_fluidstats.f90:1154-1283) on
+ * the 1r1w arrays that f_RK3TransformjiFs2RsGetSpectrum fills (src/pf_rk3.f90:
+ * 1504-1590): per z slice, the y-transformed velocity over (i, y) divided by
+ * NODES_X, taken to the laboratory frame with shear; spec_1i(i) += 2 |u1|^2
+ * (times binv(1,1)^3 with shear), y inner, z outer.  Overwrites the 1r2w arrays. */
+void orc_spectra_1d(orc* o, double* s1, double* s2) {
+  orc_prepare_realspace(o);
+  const int ny = o->ny, nz = o->nz, nki = o->nki, nkj = o->nkj, KJ = o->kj;
+  const int shear = (o->p.f_type != 1);
+  const double b11 = T9(o->binv, 1, 1);
+  const double b3 = (b11 * b11) * b11;
+  const double nxd = (double)o->nx;
+  double* in = (double*)malloc(sizeof(double) * 2 * (size_t)ny);
+  double* out = (double*)malloc(sizeof(double) * 2 * (size_t)ny);
+  double* pc = (double*)malloc(sizeof(double) * 3 * 2 * (size_t)ny * nki); /* [c][y][i] */
+  for (int i = 0; i < nki; i++) { s1[i] = 0.0; s2[i] = 0.0; }
+  for (int z = 0; z < nz; z++) {
+    for (int c = 0; c < 3; c++)
+      for (int i = 0; i < nki; i++) {
+        for (int y = 0; y < 2 * ny; y++) in[y] = 0.0;
+        const double* src = &o->w[c][(((long)i * nz + z) * nkj) * 2];
+        for (int j = 0; j <= KJ; j++) { in[2 * j] = src[2 * j]; in[2 * j + 1] = src[2 * j + 1]; }
+        for (int j = KJ + 1; j < nkj; j++) {
+          int y = ny - nkj + j;
+          in[2 * y] = src[2 * j]; in[2 * y + 1] = src[2 * j + 1];
+        }
+        fft_exec(&o->fy, in, out, +1);
+        for (int y = 0; y < ny; y++) {
+          double* d = &pc[2 * (((long)c * ny + y) * nki + i)];
+          d[0] = out[2 * y] / nxd;
+          d[1] = out[2 * y + 1] / nxd;
+        }
+      }
+    for (int y = 0; y < ny; y++)
+      for (int i = 0; i < nki; i++) {
+        const double* a = &pc[2 * (((long)0 * ny + y) * nki + i)];
+        const double* b = &pc[2 * (((long)1 * ny + y) * nki + i)];
+        const double* c3 = &pc[2 * (((long)2 * ny + y) * nki + i)];
+        double ur[3], ui[3];
+        if (shear) {
+          lab_velocity(o, a[0], b[0], c3[0], ur);
+          lab_velocity(o, a[1], b[1], c3[1], ui);
+        } else {
+          ur[0] = a[0]; ur[1] = b[0]; ui[0] = a[1]; ui[1] = b[1];
+        }
+        double p11 = ur[0] * ur[0] + ui[0] * ui[0];
+        double p22 = ur[1] * ur[1] + ui[1] * ui[1];
+        if (shear) { p11 = p11 * b3; p22 = p22 * b3; }
+        s1[i] = s1[i] + (2.0 * p11);
+        s2[i] = s2[i] + (2.0 * p22);
+      }
+  }
+  free(in); free(out); free(pc);
+}
+
+/* f_FluidstatsHomogeneous (src/pf_fluidstats.f90:2568-2936) for the current
+ * state: out[15] = S*, K*, L*, ln_int1..3, ln_int_11..13, fn_ken1..3,
+ * fn_int1..3.  ln_int_21, ln_int_12 and ln_int_13 are never assigned there
+ * (zero), so L* = ln_int_11 / 0. */
+void orc_homogeneous(orc* o, double* out) {
+  double* spec = (double*)malloc(sizeof(double) * 7 * (size_t)o->kkmax);
+  double q[33];
+  orc_stats(o, spec, q);
+  double* s1 = (double*)malloc(sizeof(double) * 2 * (size_t)o->nki);
+  orc_spectra_1d(o, s1, s1 + o->nki);
+  const double onepi = (double)3.14159265358979323846f;
+  const double zero = 0.0;
+  const double l11 = onepi * s1[0] / (2.0 * q[1]);
+  out[0] = q[28] * o->p.f_deform_s;
+  out[1] = 2.0 * q[1] / (q[2] + q[3]);
+  out[2] = l11 / (2.0 * zero);
+  out[3] = q[25]; out[4] = q[26]; out[5] = q[27];
+  out[6] = l11; out[7] = 0.0; out[8] = 0.0;
+  out[9] = q[11]; out[10] = q[12]; out[11] = q[13];
+  out[12] = q[17]; out[13] = q[18]; out[14] = q[19];
+  free(spec); free(s1);
+}
+
 /* ------------------------------------------------------------------------- */
 /* z forward transform of u4..u6_1r2w for one i slice, x 1/(NX NY NZ),       */
 /* truncated to the stored k modes.  src/pf_rk3.f90:2148-2316,              */

@@ -32,7 +32,12 @@ struct StatsFrame {
   double bi[9];  // masked binv, [m][n]
   double nu;
   int kkmax;
+  int nki;
 };
+
+// Width of the per-workgroup histogram: 13 shell arrays of kkmax, then the
+// streamwise bins (hi and lo of sum |u1_lab|^2, then of |u2_lab|^2) over i.
+__host__ __device__ inline int stats_hist(int kkmax, int nki) { return 13 * kkmax + 4 * nki; }
 
 __device__ __forceinline__ double wvn_s(int k, int kmax) {
   return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1));
@@ -47,8 +52,9 @@ __device__ __forceinline__ void two_sum_add(double* hi, double* lo, double v) {
 }
 
 // LDS: hist[13][kkmax] = hi sums of 6 quantities, their lo parts, mode counts;
-// then 12 doubles x kStatThreads for the Reynolds-stress tree reduction.
-// Partial of workgroup b: part[b][13 * kkmax + 12].
+// the streamwise bins s1 (hi, lo), s2 (hi, lo) of nki each; then 12 doubles x
+// kStatThreads for the Reynolds-stress tree reduction.
+// Partial of workgroup b: part[b][stats_hist + 12].
 template <typename T>
 __global__ void __launch_bounds__(kStatThreads)
     k_stats(Geo g, StatsFrame sf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
@@ -59,7 +65,8 @@ __global__ void __launch_bounds__(kStatThreads)
 #pragma clang fp contract(off)
   extern __shared__ double sh[];
   const int kkmax = sf.kkmax;
-  const int nh = 13 * kkmax;
+  const int nh = stats_hist(kkmax, sf.nki);
+  double* s1 = sh + 13 * kkmax;
   double* red = sh + nh;
   for (int q = threadIdx.x; q < nh; q += kStatThreads) sh[q] = 0.0;
   __syncthreads();
@@ -90,6 +97,9 @@ __global__ void __launch_bounds__(kStatThreads)
     ph[3] = ul[0][0] * ul[1][0] + ul[0][1] * ul[1][1];
     ph[4] = ul[0][0] * ul[2][0] + ul[0][1] * ul[2][1];
     ph[5] = ul[1][0] * ul[2][0] + ul[1][1] * ul[2][1];
+    // streamwise spectra: sum over (j, k) of |u_lab|^2 per i (all stored modes)
+    two_sum_add(&s1[i], &s1[sf.nki + i], ph[0]);
+    two_sum_add(&s1[2 * sf.nki + i], &s1[3 * sf.nki + i], ph[1]);
     if (i != 0) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) ph[q] = ph[q] * 2.0;
@@ -153,7 +163,7 @@ __global__ void __launch_bounds__(kStatThreads)
 
 template <typename T>
 hipError_t stats_t(pdr_ctx* c, const StatsFrame& sf, int nblk, double* part, double* out) {
-  const int width = 13 * sf.kkmax + 12;
+  const int width = stats_hist(sf.kkmax, sf.nki) + 12;
   const size_t lds = size_t(width - 12 + 12 * kStatThreads) * sizeof(double);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_stats<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -173,9 +183,10 @@ int kkmax_of(const Geo& g) {
 }
 
 // raw[7 * kkmax] (ken, ken1, ken2, ken3, dis, disskew, count) and uu[6]
-int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6]) {
+int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6], std::vector<double>* s12 = nullptr) {
   const int kkmax = kkmax_of(c->g);
-  const size_t lds = size_t(13 * kkmax + 12 * kStatThreads) * sizeof(double);
+  const int nki = c->g.nki;
+  const size_t lds = size_t(stats_hist(kkmax, nki) + 12 * kStatThreads) * sizeof(double);
   if (lds > c->lds_max) return set_error(PDR_ERR_UNSUPPORTED, "statistics: spectrum too long for the LDS");
   StatsFrame sf;
   for (int m = 1; m <= 3; ++m)
@@ -185,9 +196,10 @@ int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6]) {
     }
   sf.nu = c->p.f_nu;
   sf.kkmax = kkmax;
+  sf.nki = nki;
   const long long want = (c->g.mw + 4LL * kStatThreads - 1) / (4LL * kStatThreads);
   const int nblk = int(want < kStatBlocks ? (want < 1 ? 1 : want) : kStatBlocks);
-  const int width = 13 * kkmax + 12;
+  const int width = stats_hist(kkmax, nki) + 12;
   const size_t need = (size_t(nblk) + 1) * size_t(width) * sizeof(double);
   if (c->stat_bytes < need) {
     if (c->stat_buf) {
@@ -217,7 +229,16 @@ int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6]) {
     for (int q = 0; q < kkmax; ++q)
       raw[size_t(a * kkmax + q)] = h[size_t(a * kkmax + q)] + h[size_t((6 + a) * kkmax + q)];
   for (int q = 0; q < kkmax; ++q) raw[size_t(6 * kkmax + q)] = h[size_t(12 * kkmax + q)];
-  for (int a = 0; a < 6; ++a) uu[a] = h[size_t(13 * kkmax + a)] + h[size_t(13 * kkmax + 6 + a)];
+  const size_t nh = size_t(stats_hist(kkmax, nki));
+  for (int a = 0; a < 6; ++a) uu[a] = h[nh + a] + h[nh + 6 + a];
+  if (s12) {  // [s1 over i, s2 over i]
+    s12->assign(size_t(2) * nki, 0.0);
+    const size_t o = size_t(13) * kkmax;
+    for (int i = 0; i < nki; ++i) {
+      (*s12)[size_t(i)] = h[o + i] + h[o + nki + i];
+      (*s12)[size_t(nki + i)] = h[o + 2 * nki + i] + h[o + 3 * nki + i];
+    }
+  }
   return PDR_OK;
 }
 
@@ -259,12 +280,14 @@ const double kOnePiRef = double(3.14159265358979323846f);
 
 extern "C" {
 
-int pdr_stats(pdr_ctx* ctx, pdr_stats_t* q, double* spec) {
-  if (!ctx || !q) return pdr::set_error(PDR_ERR_ARG, "pdr_stats: null argument");
-  std::vector<double> raw;
-  double uu[6];
-  int rc = pdr::stats_raw(ctx, raw, uu);
-  if (rc != PDR_OK) return rc;
+}  // extern "C"
+
+namespace pdr {
+namespace {
+
+// Mode-density normalisation and secondary quantities of f_FluidstatsFs
+void stats_finish(pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6], pdr_stats_t* q,
+                  double* spec) {
   const int kkmax = pdr::kkmax_of(ctx->g);
   const double nu = ctx->p.f_nu;
   std::vector<double> sp(size_t(7) * kkmax, 0.0);
@@ -325,6 +348,70 @@ int pdr_stats(pdr_ctx* ctx, pdr_stats_t* q, double* spec) {
   q->disskew = fn[5] * std::pow(15.0 / fn[4], 3.0 / 2.0) * std::sqrt(nu) / 35.0;
   q->c_epsilon = fn[4] * li[0] / (fint * fint * fint);
   if (spec) std::copy(sp.begin(), sp.end(), spec);
+}
+
+}  // namespace
+}  // namespace pdr
+
+extern "C" {
+
+int pdr_stats(pdr_ctx* ctx, pdr_stats_t* q, double* spec) {
+  if (!ctx || !q) return pdr::set_error(PDR_ERR_ARG, "pdr_stats: null argument");
+  std::vector<double> raw;
+  double uu[6];
+  int rc = pdr::stats_raw(ctx, raw, uu);
+  if (rc != PDR_OK) return rc;
+  pdr::stats_finish(ctx, raw, uu, q, spec);
+  return PDR_OK;
+}
+
+// f_FluidstatsFs2D + f_FluidstatsFsCollect (src/pf_fluidstats.f90:1154-1283).
+// The reference sums 2 |u_lab(i, y, z) / NX|^2 over the (y, z) points of the
+// y- and z-transformed velocity (src/pf_rk3.f90:1504-1590); by Parseval for
+// those unnormalised transforms this equals 2 NY NZ / NX^2 sum_{j,k} |u_lab(i,j,k)|^2,
+// which the statistics pass accumulates per i from the wave state.
+int pdr_spectra_1d(pdr_ctx* ctx, double* spec_1i, double* spec_2i, pdr_homog_t* h) {
+  if (!ctx) return pdr::set_error(PDR_ERR_ARG, "pdr_spectra_1d: null context");
+  std::vector<double> raw, s12;
+  double uu[6];
+  int rc = pdr::stats_raw(ctx, raw, uu, &s12);
+  if (rc != PDR_OK) return rc;
+  const pdr::Geo& g = ctx->g;
+  const bool shear = ctx->p.f_type != PDR_F_ISOTROPIC;
+  const double b11 = T9(ctx->binv, 1, 1);
+  const double f = 2.0 * (double(g.ny) * double(g.nz)) / (double(g.nx) * double(g.nx)) *
+                   (shear ? (b11 * b11) * b11 : 1.0);
+  std::vector<double> s1(size_t(g.nki)), s2(size_t(g.nki));
+  for (int i = 0; i < g.nki; ++i) {
+    s1[size_t(i)] = f * s12[size_t(i)];
+    s2[size_t(i)] = f * s12[size_t(g.nki + i)];
+  }
+  if (spec_1i) std::copy(s1.begin(), s1.end(), spec_1i);
+  if (spec_2i) std::copy(s2.begin(), s2.end(), spec_2i);
+  if (h) {
+    // f_FluidstatsHomogeneous (src/pf_fluidstats.f90:2568-2936); ln_int_21,
+    // ln_int_12 and ln_int_13 are never assigned there, so they are zero and
+    // L* = ln_int_11 / 0
+    pdr_stats_t q;
+    pdr::stats_finish(ctx, raw, uu, &q, nullptr);
+    const double zero = 0.0;
+    const double l11 = pdr::kOnePiRef * s1[0] / (2.0 * q.u1u1);
+    h->sstar = q.tn_edy * ctx->p.f_deform_s;
+    h->kstar = 2.0 * q.u1u1 / (q.u2u2 + q.u3u3);
+    h->lstar = l11 / (2.0 * zero);
+    h->ln_int1 = q.ln_int1;
+    h->ln_int2 = q.ln_int2;
+    h->ln_int3 = q.ln_int3;
+    h->ln_int_11 = l11;
+    h->ln_int_12 = 0.0;
+    h->ln_int_13 = 0.0;
+    h->fn_ken1 = q.fn_ken1;
+    h->fn_ken2 = q.fn_ken2;
+    h->fn_ken3 = q.fn_ken3;
+    h->fn_int1 = q.fn_int1;
+    h->fn_int2 = q.fn_int2;
+    h->fn_int3 = q.fn_int3;
+  }
   return PDR_OK;
 }
 

@@ -53,6 +53,9 @@ RS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "r
             "rs_skew3", "rs_flat1", "rs_flat2", "rs_flat3", "rs_rmsvort1", "rs_rmsvort2", "rs_rmsvort3",
             "rs_skewvort1", "rs_skewvort2", "rs_skewvort3", "rs_flatvort1", "rs_flatvort2",
             "rs_flatvort3", "omegamax1", "omegamax2", "omegamax3", "ucfl")
+# fields of pdr_homog_t (the f_FluidstatsHomogeneous record), in order
+HOMOG_NAMES = ("sstar", "kstar", "lstar", "ln_int1", "ln_int2", "ln_int3", "ln_int_11", "ln_int_12",
+               "ln_int_13", "fn_ken1", "fn_ken2", "fn_ken3", "fn_int1", "fn_int2", "fn_int3")
 # fields of pdr_pstats_t (the p_StatsWrite record), in order
 PSTATS_NAMES = ("rs_u1u1", "rs_u2u2", "rs_u3u3", "rs_u1u2", "rs_u1u3", "rs_u2u3", "tke", "pken",
                 "vrel1", "vrel2", "vrel3", "vrel_abs", "xrel1", "xrel2", "xrel3", "dfp1", "dfp2", "dfp3",
@@ -97,6 +100,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_download_rk3": (C.c_int, [vp, dp, dp, dp]),
         "pdr_stats": (C.c_int, [vp, dp, dp]),
         "pdr_rs_stats": (C.c_int, [vp, dp]),
+        "pdr_spectra_1d": (C.c_int, [vp, dp, dp, dp]),
         "pdr_set_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_get_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_set_frame": (C.c_int, [vp, dp, dp]),
@@ -403,6 +407,14 @@ class Pandora:
         spec = np.zeros(7 * self.kkmax)
         _check(self.L.pdr_stats(self.h, q, spec))
         return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
+
+    def spectra_1d(self):
+        """(spec_1i, spec_2i, homogeneous-record dict) of the current state on the device."""
+        a = np.zeros(self.nki)
+        b = np.zeros(self.nki)
+        h = np.zeros(len(HOMOG_NAMES))
+        _check(self.L.pdr_spectra_1d(self.h, a, b, h))
+        return a, b, dict(zip(HOMOG_NAMES, h.tolist()))
 
     def rs_stats(self):
         """f_FluidstatsRs2D/RsCollect of the current state on the device (dict of RS_NAMES)."""

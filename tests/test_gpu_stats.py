@@ -151,3 +151,28 @@ def test_cfl_dt_sheared_frame(oracle_mod):
         assert g.dt == pytest.approx(o.dt, rel=1e-13)
     assert tg == pytest.approx(to, rel=1e-13)
     assert g.rs_stats()["ucfl"] == pytest.approx(o.rs_stats()["ucfl"], rel=1e-12)
+
+
+@pytest.mark.parametrize("shear", [False, True])
+def test_spectra_1d_and_homogeneous(oracle_mod, shear):
+    note(f"streamwise 1-D spectra and homogeneous record, N=48x32x40, shear={shear}")
+    kw = dict(f_type=2, rk3_timestep=2, deform_s=2.0, nu=0.005) if shear else dict(nu=0.01)
+    g = pa.Pandora(nx=48, ny=32, nz=40, **kw)
+    o = oracle_mod.Oracle(nx=48, ny=32, nz=40, **kw)
+    rng = np.random.default_rng(17)
+    u = [rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape) for _ in range(3)]
+    g.set_state(*u)
+    o.set_state(*u)
+    if shear:
+        b, bi = o.get_frame()
+        b[6], bi[6] = -0.3, 0.3
+        o.set_frame(b, bi)
+        g.set_frame(b, bi)
+    s1, s2, h = g.spectra_1d()
+    o1, o2 = o.spectra_1d()
+    np.testing.assert_allclose(s1, o1, rtol=1e-12)
+    np.testing.assert_allclose(s2, o2, rtol=1e-12)
+    ho = o.homogeneous()
+    for k, v in ho.items():
+        assert h[k] == pytest.approx(v, rel=1e-11), k
+    g.close()

@@ -217,3 +217,50 @@ def test_rs_stats_parseval_iso_and_sheared(oracle_mod):
     for name in ("u1u1", "u3u3", "u1u3"):
         assert q["rs_" + name] == pytest.approx(fs[name], rel=1e-12)
     assert q["rs_u1u1"] != pytest.approx(o.rs_stats()["rs_u1u1"], rel=1e-3)
+
+
+# ---- streamwise 1-D spectra (f_FluidstatsFs2D / FsCollect) --------------------
+
+@pytest.mark.parametrize("shear", [False, True])
+def test_spectra_1d_parseval(oracle_mod, shear):
+    """The oracle transforms the velocity in z and y (zero padding, FFT, /NX) and
+    sums 2 |u_lab|^2 over the (y, z) plane for each i.  By Parseval for the
+    unnormalised y and z transforms this is 2 NY NZ / NX^2 sum_{j,k} |u_lab|^2,
+    computed here from the wave state directly (a non-cubic box pins where each
+    length enters; the sheared frame pins the laboratory-frame velocity)."""
+    nx, ny, nz = 18, 12, 20
+    kw = dict(f_type=2, rk3_timestep=2, deform_s=1.0) if shear else {}
+    o = oracle_mod.Oracle(nx=nx, ny=ny, nz=nz, nu=0.01, **kw)
+    rng = np.random.default_rng(31)
+    u = [rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape) for _ in range(3)]
+    o.set_state(*u)
+    b13 = 0.0
+    if shear:
+        b, bi = o.get_frame()
+        b[6], bi[6] = -0.45, 0.45
+        o.set_frame(b, bi)
+        b13 = 0.45
+    s1, s2 = o.spectra_1d()
+    ul1 = u[0] + b13 * u[2]
+    e1 = 2.0 * ny * nz / nx**2 * np.sum(np.abs(ul1) ** 2, axis=(1, 2))
+    e2 = 2.0 * ny * nz / nx**2 * np.sum(np.abs(u[1]) ** 2, axis=(1, 2))
+    np.testing.assert_allclose(s1, e1, rtol=1e-13)
+    np.testing.assert_allclose(s2, e2, rtol=1e-13)
+
+
+def test_homogeneous_record(oracle_mod):
+    """f_FluidstatsHomogeneous: S* = S tau_edy, K* = 2 u1u1 / (u2u2 + u3u3),
+    l11 = pi E11(0) / (2 u1u1); ln_int_21, ln_int_12, ln_int_13 are never set
+    in the reference, so L* = l11 / 0 and l22i = l33i = 0."""
+    o = oracle_mod.Oracle(16, f_type=2, rk3_timestep=2, deform_s=2.5, nu=0.01)
+    o.set_state(*_random_state(o, 12))
+    _, q = o.stats()
+    s1, _ = o.spectra_1d()
+    h = o.homogeneous()
+    assert h["sstar"] == pytest.approx(2.5 * q["tn_edy"], rel=1e-15)
+    assert h["kstar"] == pytest.approx(2 * q["u1u1"] / (q["u2u2"] + q["u3u3"]), rel=1e-15)
+    assert h["ln_int_11"] == pytest.approx(ONEPI * s1[0] / (2 * q["u1u1"]), rel=1e-15)
+    assert h["lstar"] == float("inf") and h["ln_int_12"] == 0.0 and h["ln_int_13"] == 0.0
+    for c in (1, 2, 3):
+        assert h[f"fn_ken{c}"] == q[f"fn_ken{c}"] and h[f"fn_int{c}"] == q[f"fn_int{c}"]
+        assert h[f"ln_int{c}"] == q[f"ln_int{c}"]
