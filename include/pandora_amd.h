@@ -186,6 +186,13 @@ int pdr_particles_download(pdr_ctx* ctx, double* xp, double* up_rk3);
  * The reference keeps both in p_arrays (src/pp_arrays.f90:25-42) and its
  * restart; these move them in and out.  particle_up starts at zero on upload. */
 int pdr_particles_download_aux(pdr_ctx* ctx, double* up, double* xp0);
+/* Re-sort the device slots of the tracers by the (z, y) grid row of their
+ * position every `every` steps (stage 1; 0 = never; default 10, and once at
+ * upload), so that the interpolation gathers of neighbouring lanes share cache
+ * lines.  No reference counterpart: the reference's particle order is its MPI
+ * ownership order.  Results do not depend on it; every transfer to and from the
+ * host uses the caller's numbering. */
+int pdr_set_particle_sort(pdr_ctx* ctx, int32_t every);
 int pdr_particles_upload_aux(pdr_ctx* ctx, const double* up, const double* xp0);
 
 /* Tracer statistics: p_StatsPrimary, p_StatsDispersion and

@@ -247,6 +247,8 @@ int pdr_destroy(pdr_ctx* c) {
   if (c->uprk) (void)hipFree(c->uprk);
   if (c->up) (void)hipFree(c->up);
   if (c->xp0) (void)hipFree(c->xp0);
+  if (c->pid) (void)hipFree(c->pid);
+  particles_free_sort(c);
   if (c->pst_buf) (void)hipFree(c->pst_buf);
   if (c->umax) (void)hipFree(c->umax);
   if (c->red) (void)hipFree(c->red);
@@ -574,6 +576,10 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   }
   // f_RK3Stage1: p_StatsPrimary after the remesh, p_StatsRelativeVelocity after
   // the particle advance (src/pf_rk3.f90:62-99), for tstep > P_TS_RELEASE
+  if (stage == 1 && c->np > 1 && c->sort_every > 0 && ++c->since_sort >= c->sort_every) {
+    int rc = particles_sort(c);
+    if (rc != PDR_OK) return rc;
+  }
   const bool pstats = stage == 1 && c->pst_on && c->np > 0 && tstep > c->ts_release;
   if (pstats) HIPCHK(launch_pstats_primary(c, *dt));
   const Coef cf = make_coef(c);
@@ -714,6 +720,7 @@ int pdr_check_finite(pdr_ctx* c) {
 int pdr_particles_upload(pdr_ctx* c, const double* xp, int64_t np) {
   if (!c || np < 0 || (np > 0 && !xp)) return fail(PDR_ERR_ARG, "pdr_particles_upload: bad argument");
   HIPCHK(hipStreamSynchronize(c->stream));
+  particles_free_sort(c);
   double** arrays[4] = {&c->xp, &c->uprk, &c->up, &c->xp0};
   for (double** a : arrays)
     if (*a) {
@@ -721,20 +728,43 @@ int pdr_particles_upload(pdr_ctx* c, const double* xp, int64_t np) {
       c->dev_bytes -= 24 * size_t(c->np);
       *a = nullptr;
     }
+  if (c->pid) {
+    (void)hipFree(c->pid);
+    c->dev_bytes -= 4 * size_t(c->np);
+    c->pid = nullptr;
+  }
   c->np = 0;
   if (np == 0) return PDR_OK;
+  if (np > int64_t(0xFFFFFFFF)) return fail(PDR_ERR_ARG, "pdr_particles_upload: at most 2^32 - 1 particles");
   int rc = dalloc(c, (void**)&c->xp, 24 * size_t(np));
   if (rc == PDR_OK) rc = dalloc(c, (void**)&c->uprk, 24 * size_t(np));
   if (rc == PDR_OK) rc = dalloc(c, (void**)&c->up, 24 * size_t(np));
   if (rc == PDR_OK && c->keep_x0) rc = dalloc(c, (void**)&c->xp0, 24 * size_t(np));
+  if (rc == PDR_OK) rc = dalloc(c, (void**)&c->pid, 4 * size_t(np));
   if (rc == PDR_OK && !c->ur) rc = dalloc(c, &c->ur, 3 * size_t(c->g.nr) * c->rsz);
   if (rc != PDR_OK) return rc;
+  std::vector<uint32_t> ident(static_cast<size_t>(np));
+  for (size_t s = 0; s < ident.size(); ++s) ident[s] = uint32_t(s);
+  HIPCHK(hipMemcpy(c->pid, ident.data(), 4 * size_t(np), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->xp, xp, 24 * size_t(np), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(c->uprk, 0, 24 * size_t(np)));
   HIPCHK(hipMemset(c->up, 0, 24 * size_t(np)));
   // particle_xp0 = particle_xp at placement (src/pp_control.f90:405-407)
   if (c->xp0) HIPCHK(hipMemcpy(c->xp0, xp, 24 * size_t(np), hipMemcpyHostToDevice));
   c->np = np;
+  c->since_sort = 0;
+  if (c->sort_every > 0) {
+    rc = particles_sort(c);
+    if (rc != PDR_OK) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return PDR_OK;
+}
+
+int pdr_set_particle_sort(pdr_ctx* c, int32_t every) {
+  if (!c || every < 0) return fail(PDR_ERR_ARG, "pdr_set_particle_sort: bad argument");
+  c->sort_every = every;
+  c->since_sort = 0;
   return PDR_OK;
 }
 
@@ -743,9 +773,9 @@ int pdr_particles_download_aux(pdr_ctx* c, double* up, double* xp0) {
   if (c->np == 0) return fail(PDR_ERR_STATE, "no particles set");
   if (xp0 && !c->xp0) return fail(PDR_ERR_STATE, "initial positions are not kept (pdr_particle_stats_config)");
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (up) HIPCHK(hipMemcpy(up, c->up, 24 * size_t(c->np), hipMemcpyDeviceToHost));
-  if (xp0) HIPCHK(hipMemcpy(xp0, c->xp0, 24 * size_t(c->np), hipMemcpyDeviceToHost));
-  return PDR_OK;
+  int rc = up ? particles_to_host(c, c->up, up) : PDR_OK;
+  if (rc == PDR_OK && xp0) rc = particles_to_host(c, c->xp0, xp0);
+  return rc;
 }
 
 int pdr_particles_upload_aux(pdr_ctx* c, const double* up, const double* xp0) {
@@ -753,9 +783,9 @@ int pdr_particles_upload_aux(pdr_ctx* c, const double* up, const double* xp0) {
   if (c->np == 0) return fail(PDR_ERR_STATE, "no particles set");
   if (xp0 && !c->xp0) return fail(PDR_ERR_STATE, "initial positions are not kept (pdr_particle_stats_config)");
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (up) HIPCHK(hipMemcpy(c->up, up, 24 * size_t(c->np), hipMemcpyHostToDevice));
-  if (xp0) HIPCHK(hipMemcpy(c->xp0, xp0, 24 * size_t(c->np), hipMemcpyHostToDevice));
-  return PDR_OK;
+  int rc = up ? particles_from_host(c, up, c->up) : PDR_OK;
+  if (rc == PDR_OK && xp0) rc = particles_from_host(c, xp0, c->xp0);
+  return rc;
 }
 
 int pdr_particle_stats_config(pdr_ctx* c, int32_t enable, int32_t keep_initial, int32_t ts_release) {
@@ -791,9 +821,9 @@ int pdr_particles_download(pdr_ctx* c, double* xp, double* uprk) {
   if (!c) return fail(PDR_ERR_ARG, "null context");
   if (c->np == 0) return fail(PDR_ERR_STATE, "no particles set");
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (xp) HIPCHK(hipMemcpy(xp, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToHost));
-  if (uprk) HIPCHK(hipMemcpy(uprk, c->uprk, 24 * size_t(c->np), hipMemcpyDeviceToHost));
-  return PDR_OK;
+  int rc = xp ? particles_to_host(c, c->xp, xp) : PDR_OK;
+  if (rc == PDR_OK && uprk) rc = particles_to_host(c, c->uprk, uprk);
+  return rc;
 }
 
 int pdr_debug_prepare(pdr_ctx* c, int32_t field, double* out) {

@@ -109,6 +109,14 @@ struct pdr_ctx {
   double pst[30] = {};
   int pst_tstep = -1;
   double* pst_buf = nullptr;  // device partial sums
+  // tracer order: slot s holds the tracer uploaded as number pid[s]; the slots
+  // are re-sorted by (z, y) grid row every sort_every steps (0: never)
+  uint32_t* pid = nullptr;
+  int sort_every = 10;
+  int since_sort = 0;
+  void* sort_buf = nullptr;   // keys, indices and the radix-sort workspace
+  size_t sort_bytes = 0;
+  double* spare = nullptr;    // 3 * np doubles for the reordering
   unsigned long long* umax = nullptr;  // CFL reduction (bits of a non-negative double)
   double* red = nullptr;               // spectrum reduction scratch
   void* stat_buf = nullptr;            // statistics partials (allocated on first use)
@@ -156,6 +164,11 @@ hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
 hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt);
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);
+// tracer order (pdr_sort.hip)
+int particles_sort(pdr_ctx* c);
+int particles_to_host(pdr_ctx* c, const double* dev, double* host);
+int particles_from_host(pdr_ctx* c, const double* host, double* dev);
+void particles_free_sort(pdr_ctx* c);
 // tracer statistics (pdr_pstats.hip)
 int pstats_alloc(pdr_ctx* c);
 int pstats_reset(pdr_ctx* c);

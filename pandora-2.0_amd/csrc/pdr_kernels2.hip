@@ -80,11 +80,12 @@ constexpr int lds_x_bytes(int n) {
 #ifndef PDR_NT_WRITES
 #define PDR_NT_WRITES 31
 #endif
-// Streaming loads of single-use inputs, bit mask: 2 yinv, 4 xpass, 8 yfwd, 16 zfwd
-// (the velocity read by zinv is read again for the vorticity and stays cached).
+// Streaming loads of single-use inputs, bit mask: 1 zinv (register form),
+// 2 yinv, 4 xpass, 8 yfwd, 16 zfwd.
 #ifndef PDR_NT_READS
 #define PDR_NT_READS 0
 #endif
+constexpr bool kNrZinv = (PDR_NT_READS & 1) != 0;
 constexpr bool kNrYinv = (PDR_NT_READS & 2) != 0;
 constexpr bool kNrXpass = (PDR_NT_READS & 4) != 0;
 constexpr bool kNrYfwd = (PDR_NT_READS & 8) != 0;
@@ -273,9 +274,9 @@ __global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
     const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : -1);
     const bool ok = (L * N % kBlock2 == 0 || idx < L * N) && j0 + line < nkj && k >= 0;
     const int e = line * NKK + k;
-    ua[q] = ok ? uu[0][e] : czero<T>();
-    ub[q] = ok ? uu[1][e] : czero<T>();
-    uc[q] = ok ? uu[2][e] : czero<T>();
+    ua[q] = ok ? get<kNrZinv>(&uu[0][e]) : czero<T>();
+    ub[q] = ok ? get<kNrZinv>(&uu[1][e]) : czero<T>();
+    uc[q] = ok ? get<kNrZinv>(&uu[2][e]) : czero<T>();
   }
   const double wi = double(i);
 #pragma unroll

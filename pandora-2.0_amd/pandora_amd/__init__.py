@@ -119,6 +119,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_particles_download_aux": (C.c_int, [vp, vp, vp]),
         "pdr_particles_upload_aux": (C.c_int, [vp, vp, vp]),
         "pdr_particle_stats_config": (C.c_int, [vp, i32, i32, i32]),
+        "pdr_set_particle_sort": (C.c_int, [vp, i32]),
         "pdr_particle_stats": (C.c_int, [vp, C.POINTER(i32), dp]),
         "pdr_init_field": (C.c_int, [vp, C.c_uint64, d, d]),
         "pdr_init_particles": (C.c_int, [vp, C.c_uint64, i64]),
@@ -377,6 +378,10 @@ class Pandora:
         b = None if xp0 is None else np.ascontiguousarray(xp0, np.float64).reshape(-1, 3)
         _check(self.L.pdr_particles_upload_aux(self.h, None if a is None else a.ctypes.data,
                                                None if b is None else b.ctypes.data))
+
+    def set_particle_sort(self, every: int):
+        """Re-sort the tracer slots by grid row every `every` steps (0: never)."""
+        _check(self.L.pdr_set_particle_sort(self.h, int(every)))
 
     def particle_stats_config(self, enable=True, keep_initial=False, ts_release=0):
         """Tracer statistics in stage 1 (p_StatsPrimary/Dispersion/RelativeVelocity)."""
