@@ -71,6 +71,60 @@ def run_gpu(pa, n, precision, steps, warmup, ntr, profile):
     return (t1 - t0), prof, ok
 
 
+ENS_SIZES = (32, 48, 64, 96)
+
+
+def ensemble_bench(pa, per_size=16, steps=50, warmup=5):
+    """The batching form: 16 decaying isotropic members each of 32^3, 48^3, 64^3
+    and 96^3 (k0 = 4, nu = 0.01, dt = 0.005, FP32-mixed, seeds 0..63), advanced by
+    one graph launch per RK3 step; timed beside stepping the same members one by
+    one with pdr_run."""
+    sizes = [n for n in ENS_SIZES for _ in range(per_size)]
+    members = []
+    for k, n in enumerate(sizes):
+        g = pa.Pandora(n, rk3_timestep=pa.RK3_TIMESTEP_FIXED, rk3_dt=0.005, nu=0.01, precision=pa.PDR_FP32_MIXED)
+        g.init_field(k, 2.0 / 3.0, 4.0)
+        members.append(g)
+    e = pa.Ensemble(members)
+    split = e.autotune(3)
+    t = e.run(1, warmup, [0.0] * len(sizes))
+    t0 = time.perf_counter()
+    t = e.run(1 + warmup, steps, t)
+    t1 = time.perf_counter()
+    ens_sps = steps / (t1 - t0)
+    seq_steps = 5
+    for g in members:
+        g.synchronize()
+    s0 = time.perf_counter()
+    for g in members:
+        g.run(1, seq_steps, 0.0)
+    for g in members:
+        g.synchronize()
+    s1 = time.perf_counter()
+    seq_sps = seq_steps / (s1 - s0)
+    pts = sum(n**3 for n in sizes)
+
+    class G:
+        pass
+
+    b = 0
+    for n in sizes:
+        g = G()
+        g.nx = g.ny = g.nz = n
+        g.nki, g.nkj, g.nkk = n // 3 + 1, 2 * (n // 3) + 1, 2 * (n // 3) + 1
+        g.mw = g.nki * g.nkj * g.nkk
+        g.m1 = g.nki * g.nkj * n
+        b += model_bytes_per_step(g, 8, 4, 0)
+    e.close()
+    for g in members:
+        g.close()
+    return {"members": f"{per_size} each of " + ", ".join(f"{n}^3" for n in ENS_SIZES) + " (FP32-mixed)",
+            "ensemble_steps_per_s": ens_sps, "grid_point_steps_per_s": ens_sps * pts,
+            "one_by_one_steps_per_s": seq_sps, "batching_speedup": ens_sps / seq_sps,
+            "autotuned_split": list(split), "model_bytes_per_step": b,
+            "model_frac_of_peak": b * ens_sps / HBM_PEAK, "steps": steps}
+
+
 def cpu_baseline(n, ntr):
     """One RK3 stage (a third of a step) of the same workload on the FP64 oracle,
     OpenMP over the host cores; scaled to grid-point-steps/s."""
@@ -235,6 +289,7 @@ def main():
     ap.add_argument("--tracers", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this path runs on one GPU (the whole box stays resident on one "
@@ -292,6 +347,10 @@ def main():
                 "traffic_source": (os.path.relpath(TRAFFIC_JSON, ROOT) + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                    "copy-calibrated)") if traffic else None,
                 "bytes_per_launch": b, "avg_launch_ms": avg * 1e3}
+    ens = None
+    if not args.no_ensemble:
+        print("[bench] batched ensemble 16 x {32,48,64,96}^3", file=sys.stderr, flush=True)
+        ens = ensemble_bench(pa)
     bstep = model_bytes_per_step(g, 8, 4, ntr)
     copy_gbps = pa.copy_bandwidth(1 << 30, 20)
     cpu = None
@@ -323,6 +382,7 @@ def main():
         "measured_copy_GBps": copy_gbps,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ensemble": ens,
         "finite": bool(ok32),
     }
     print(json.dumps(out), flush=True)

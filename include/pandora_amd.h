@@ -328,6 +328,32 @@ int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out);
  * tot[6] = {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}. */
 int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6]);
 
+/* ---- batched ensembles ---------------------------------------------------- */
+
+/* Many independent realisations of uneven sizes advanced together: each RK3
+ * pass is one launch over the tiles of every member (per-member work
+ * descriptors) and the 15 launches of a step are captured once into a HIP graph
+ * that pdr_ensemble_run replays.  No reference counterpart (the reference runs
+ * one box per job); each member runs the device code pdr_step runs for it, with
+ * the same arguments (compiled into other kernels: equal to rounding).
+ * Members are contexts from pdr_create, not owned by the ensemble, all of one
+ * precision, cubic with 32..256 points (a specialised length), isotropic with
+ * RK3_TIMESTEP = FIXED, unforced and without tracers; others give
+ * PDR_ERR_UNSUPPORTED.  Their buffers must not be reallocated while the
+ * ensemble exists (state uploads and statistics are fine). */
+typedef struct pdr_ensemble pdr_ensemble;
+int pdr_ensemble_create(pdr_ensemble** out, pdr_ctx* const* members, int32_t count);
+/* nsteps g_RK3Calc steps of every member; simtime[k] of member k advances by
+ * its RK3_DT per step (tstep0 is accepted for symmetry with pdr_run). */
+int pdr_ensemble_run(pdr_ensemble* e, int32_t tstep0, int32_t nsteps, double* simtime);
+/* Fields per workgroup of the inverse / forward y passes (1, 2, 3, 6 / 1, 3;
+ * 0 = each member's own choice); rebuilds the graph. */
+int pdr_ensemble_set_split(pdr_ensemble* e, int32_t fpb_y, int32_t fpb_yf);
+/* Times `steps` steps for every split, keeps the fastest and restores the
+ * members' state and RK registers; returns the choice in *fpb_y, *fpb_yf. */
+int pdr_ensemble_autotune(pdr_ensemble* e, int32_t steps, int32_t* fpb_y, int32_t* fpb_yf);
+int pdr_ensemble_destroy(pdr_ensemble* e);
+
 /* ---- diagnostics ---------------------------------------------------------- */
 
 /* Copy 1r2w intermediate field f (0..5 = u1..u6_1r2w after the z pass of

@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FC = os.environ.get("PDR_FC", "/opt/rocm/bin/amdflang")
 ARCH = "gfx950"
 
-SOURCES = ["pdr_api.hip", "pdr_kernels.hip", "pdr_kernels2.hip", "pdr_host.hip", "pdr_stats.hip", "pdr_pstats.hip", "pdr_sort.hip"]
+SOURCES = ["pdr_api.hip", "pdr_kernels.hip", "pdr_kernels2.hip", "pdr_host.hip", "pdr_stats.hip", "pdr_pstats.hip", "pdr_sort.hip", "pdr_ens.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -51,6 +51,28 @@ struct Coef {
   int pad;
 };
 
+// One member of a batched ensemble (pdr_ens.hip): everything a batched pass
+// kernel needs to run that member's tiles, read from device memory by the
+// workgroups whose block index falls in [start[p], start[p] + blocks[p]).
+struct EnsMember {
+  Geo g;
+  Coef cf;
+  int n;                  // line length (cubic member)
+  int nt[5];              // tiles per line group of pass p (zinv, yinv, xpass, yfwd, zfwd+FormF)
+  int fpb_y, fpb_yf;      // fields per workgroup of the y passes
+  unsigned start[5];      // first block of the member in pass p
+  double dt;              // RK3_DT
+  double norm;            // 1 / (NX NY NZ)
+  const void* u[3];
+  void* rk[3];
+  void* w1;
+  void* w2;
+  unsigned long long* umax;
+  const void* twz;
+  const void* twy;
+  const void* twx;
+};
+
 // Tracer frame transforms (p_RK3Mov2Lab / p_RK3Lab2Mov, src/pp_rk3.f90:989-1043).
 struct TracerFrame {
   double binv[9];  // masked, [m][n]
@@ -164,6 +186,13 @@ hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
 hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt);
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);
+// batched ensemble passes (pdr_kernels2.hip)
+bool ens_size_supported(int n);
+int ens_tiles(int prec, int n, const Geo& g, int fpb_y, int fpb_yf, int nt[5], unsigned blocks[5]);
+size_t ens_lds(int prec, int n, int pass);
+hipError_t ens_prepare(int prec, size_t lds_max);
+hipError_t launch_ens_pass(int prec, int pass, const EnsMember* mem, int nm, unsigned nblk, size_t lds,
+                           double beta, double gamma, hipStream_t s);
 // tracer order (pdr_sort.hip)
 int particles_sort(pdr_ctx* c);
 int particles_to_host(pdr_ctx* c, const double* dev, double* host);

@@ -50,9 +50,12 @@ template <typename T>
 constexpr int tile_lines_fwd(int n) {
   return (sizeof(T) == 4 && n <= 512) ? PDR_LINES_FWD_F32 : tile_lines<T>(n);
 }
+// Short lines take more rows per workgroup, so that a workgroup still moves
+// about a thousand points (2 rows of 512; 16 rows of 64).
 template <typename T>
 constexpr int xrows(int n) {
-  return sizeof(T) == 4 ? (n <= 512 ? PDR_XROWS_F32 : 2) : (n <= 512 ? PDR_XROWS_F64 : 2);
+  return n <= 64 ? 16 : (n <= 128 ? 8 : (n <= 256 ? 4 : (sizeof(T) == 4 ? (n <= 512 ? PDR_XROWS_F32 : 2)
+                                                                        : (n <= 512 ? PDR_XROWS_F64 : 2))));
 }
 
 // LDS bytes of a workgroup, and the waves per SIMD that the 160 KB of LDS of a
@@ -246,8 +249,7 @@ constexpr int lds_zinv_bytes(int n) {
 #define PDR_ZINV_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_zinv_bytes<T>(N)))))
 
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
-    k2_zinv_r(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
+__device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const Coef& cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
               const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
   constexpr int L = zinv_lines<T>(N);
   constexpr int LD = lds_ld<T>(N);
@@ -257,7 +259,6 @@ __global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
-  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
   const int i = int(bid / unsigned(ntile));
   const int j0 = tile * L;
@@ -320,12 +321,18 @@ __global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
   }
 }
 
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
+    k2_zinv_r(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
+              const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
+  k2_zinv_r_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, cf, u0, u1, u2, w1, twg, ntile);
+}
+
 // ---------------------------------------------------------------------------
 // pass 2: inverse y FFT, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
-    k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
+__device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
   constexpr int L = tile_lines<T>(N);
   constexpr int LD = lds_ld<T>(N);
@@ -335,7 +342,6 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
-  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
   const int zg = int(bid / unsigned(ntile));
   const int zl = zg % nzc;        // plane within the chunk
@@ -381,12 +387,18 @@ __global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
   }
 }
 
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
+    k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
+            int ntile, int z0, int nzc, int fpb) {
+  k2_yinv_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, w1, w2, twg, ntile, z0, nzc, fpb);
+}
+
 // ---------------------------------------------------------------------------
 // pass 3: RB rows (z, y0..y0+RB-1) per block: c2r, product, r2c
 // ---------------------------------------------------------------------------
 template <typename T, int N, bool RS>
-__global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
-    k2_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
+__device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const Coef& cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
              unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile, int z0,
              double* __restrict__ rsp) {
   constexpr int RB = xrows<T>(N);
@@ -398,7 +410,6 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
   __shared__ T red[kBlock2];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
-  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
   const int zl = int(bid / unsigned(ntile));  // plane within the chunk
   const int z = z0 + zl;
@@ -535,12 +546,19 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
   }
 }
 
+template <typename T, int N, bool RS>
+__global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
+    k2_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
+             unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile, int z0,
+             double* __restrict__ rsp) {
+  k2_xpass_body<T, N, RS>(xcd_remap(blockIdx.x, gridDim.x), g, cf, w2, ur, want_tr, want_cfl, umax, twg, ntile, z0, rsp);
+}
+
 // ---------------------------------------------------------------------------
 // pass 4: forward y FFT + truncation, (z, i-tile) per block
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
-    k2_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
+__device__ __forceinline__ void k2_yfwd_body(unsigned bid, const Geo& g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
   constexpr int L = tile_lines_fwd<T>(N);
   constexpr int LD = lds_ld<T>(N);
@@ -550,7 +568,6 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
-  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
   const int zg = int(bid / unsigned(ntile));
   const int zl = zg % nzc;
@@ -594,6 +611,13 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
     }
     __syncthreads();
   }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
+    k2_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
+            int ntile, int z0, int nzc, int fpb) {
+  k2_yfwd_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, w2, w1, twg, ntile, z0, nzc, fpb);
 }
 
 // ---------------------------------------------------------------------------
@@ -660,8 +684,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
 // contiguous in the wave arrays.
 // ---------------------------------------------------------------------------
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
-    k2_zfwd_formf(Geo g, Coef cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
+__device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, const Coef& cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
                   cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
                   const double2* __restrict__ frc, int fnk, double fkf, double beta, double gdt,
                   const cx<T>* __restrict__ twg, int ntile, T norm) {
@@ -674,7 +697,6 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
-  const unsigned bid = xcd_remap(blockIdx.x, gridDim.x);
   const int tile = int(bid % unsigned(ntile));
   const int i = int(bid / unsigned(ntile));
   const int j0 = tile * L;
@@ -726,6 +748,15 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
                     r0, r1, r2, frc, fnk, fkf, beta, gdt);
     }
   }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
+    k2_zfwd_formf(Geo g, Coef cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
+                  cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
+                  const double2* __restrict__ frc, int fnk, double fkf, double beta, double gdt,
+                  const cx<T>* __restrict__ twg, int ntile, T norm) {
+  k2_zfwd_formf_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, cf, w1, u0, u1, u2, r0, r1, r2, frc, fnk, fkf, beta, gdt, twg, ntile, norm);
 }
 
 template <typename T, int N>
@@ -852,6 +883,224 @@ hipError_t zfwd_formf_n(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched ensemble passes: one launch runs a pass for every member.  Block b
+// (after the XCD remap, so that a member's tiles share one XCD's L2) belongs to
+// the last member whose start is <= b; the member's descriptor supplies the
+// geometry, coefficients, buffers and tile counts, and a switch on its line
+// length selects the compile-time body that the single-box kernel also uses,
+// so each member computes what its own launches would (up to the rounding of
+// multiply-adds that the compiler contracts differently in another kernel).
+// ---------------------------------------------------------------------------
+#define PDR_ENS_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+
+// member of block b of a pass: binary search of the members' first blocks
+__device__ __forceinline__ int ens_member(const EnsMember* __restrict__ mem, int nm, int pass, unsigned b) {
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mem[mid].start[pass] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock2) k_ens_zinv(const EnsMember* __restrict__ mem, int nm) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const EnsMember& e = mem[ens_member(mem, nm, 0, b)];
+  const unsigned lb = b - e.start[0];
+  switch (e.n) {
+#define PDR_EC(n)                                                                                                \
+  case n:                                                                                                        \
+    k2_zinv_r_body<T, n>(lb, e.g, e.cf, (const cx<T>*)e.u[0], (const cx<T>*)e.u[1], (const cx<T>*)e.u[2],        \
+                         (cx<T>*)e.w1, (const cx<T>*)e.twz, e.nt[0]);                                            \
+    break;
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    default:
+      break;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock2) k_ens_yinv(const EnsMember* __restrict__ mem, int nm) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const EnsMember& e = mem[ens_member(mem, nm, 1, b)];
+  const unsigned lb = b - e.start[1];
+  switch (e.n) {
+#define PDR_EC(n)                                                                                                \
+  case n:                                                                                                        \
+    k2_yinv_body<T, n>(lb, e.g, (const cx<T>*)e.w1, (cx<T>*)e.w2, (const cx<T>*)e.twy, e.nt[1], 0, e.g.nz,      \
+                       e.fpb_y);                                                                                 \
+    break;
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    default:
+      break;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock2) k_ens_xpass(const EnsMember* __restrict__ mem, int nm) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const EnsMember& e = mem[ens_member(mem, nm, 2, b)];
+  const unsigned lb = b - e.start[2];
+  switch (e.n) {
+#define PDR_EC(n)                                                                                                \
+  case n:                                                                                                        \
+    k2_xpass_body<T, n, false>(lb, e.g, e.cf, (cx<T>*)e.w2, (T*)nullptr, 0, 0, e.umax, (const cx<T>*)e.twx,      \
+                               e.nt[2], 0, nullptr);                                                             \
+    break;
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    default:
+      break;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock2) k_ens_yfwd(const EnsMember* __restrict__ mem, int nm) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const EnsMember& e = mem[ens_member(mem, nm, 3, b)];
+  const unsigned lb = b - e.start[3];
+  switch (e.n) {
+#define PDR_EC(n)                                                                                                \
+  case n:                                                                                                        \
+    k2_yfwd_body<T, n>(lb, e.g, (const cx<T>*)e.w2, (cx<T>*)e.w1, (const cx<T>*)e.twy, e.nt[3], 0, e.g.nz,      \
+                       e.fpb_yf);                                                                                \
+    break;
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    default:
+      break;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock2) k_ens_zfwd_formf(const EnsMember* __restrict__ mem, int nm, double beta,
+                                                          double gamma) {
+  const unsigned b = xcd_remap(blockIdx.x, gridDim.x);
+  const EnsMember& e = mem[ens_member(mem, nm, 4, b)];
+  const unsigned lb = b - e.start[4];
+  const double gdt = gamma * e.dt;
+  switch (e.n) {
+#define PDR_EC(n)                                                                                                \
+  case n:                                                                                                        \
+    k2_zfwd_formf_body<T, n>(lb, e.g, e.cf, (const cx<T>*)e.w1, (cx<T>*)e.u[0], (cx<T>*)e.u[1], (cx<T>*)e.u[2], \
+                             (cx<T>*)e.rk[0], (cx<T>*)e.rk[1], (cx<T>*)e.rk[2], nullptr, 0, 0.0, beta, gdt,     \
+                             (const cx<T>*)e.twz, e.nt[4], T(e.norm));                                           \
+    break;
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    default:
+      break;
+  }
+}
+
+template <typename T, int N>
+int ens_tiles_n(const Geo& g, int fpb_y, int fpb_yf, int nt[5], unsigned blocks[5]) {
+  nt[0] = (g.nkj + zinv_lines<T>(N) - 1) / zinv_lines<T>(N);
+  nt[1] = (g.nki + tile_lines<T>(N) - 1) / tile_lines<T>(N);
+  nt[2] = (g.ny + xrows<T>(N) - 1) / xrows<T>(N);
+  nt[3] = (g.nki + tile_lines_fwd<T>(N) - 1) / tile_lines_fwd<T>(N);
+  nt[4] = (g.nkj + tile_lines_fwd<T>(N) - 1) / tile_lines_fwd<T>(N);
+  blocks[0] = unsigned(nt[0] * g.nki);
+  blocks[1] = unsigned(nt[1] * g.nz * (6 / fpb_y));
+  blocks[2] = unsigned(nt[2] * g.nz);
+  blocks[3] = unsigned(nt[3] * g.nz * (3 / fpb_yf));
+  blocks[4] = unsigned(nt[4] * g.nki);
+  return 0;
+}
+
+template <typename T, int N>
+size_t ens_lds_n(int pass) {
+  switch (pass) {
+    case 0: return size_t(lds_zinv_bytes<T>(N));
+    case 1: return lds_lines<T, N>();
+    case 2: return lds_x<T, N>();
+    default: return lds_fwd<T, N>();
+  }
+}
+
+}  // namespace
+
+bool ens_size_supported(int n) {
+  switch (n) {
+#define PDR_EC(n) case n:
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    return true;
+    default:
+      return false;
+  }
+}
+
+int ens_tiles(int prec, int n, const Geo& g, int fpb_y, int fpb_yf, int nt[5], unsigned blocks[5]) {
+  switch (n) {
+#define PDR_EC(n)                                                                                     \
+  case n:                                                                                             \
+    return prec ? ens_tiles_n<float, n>(g, fpb_y, fpb_yf, nt, blocks)                                 \
+                : ens_tiles_n<double, n>(g, fpb_y, fpb_yf, nt, blocks);
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    default:
+      return -1;
+  }
+}
+
+size_t ens_lds(int prec, int n, int pass) {
+  switch (n) {
+#define PDR_EC(n) \
+  case n:         \
+    return prec ? ens_lds_n<float, n>(pass) : ens_lds_n<double, n>(pass);
+    PDR_ENS_SIZES(PDR_EC)
+#undef PDR_EC
+    default:
+      return 0;
+  }
+}
+
+template <typename T>
+static hipError_t ens_prepare_t(size_t lds_max) {
+  const void* fns[5] = {(const void*)k_ens_zinv<T>, (const void*)k_ens_yinv<T>, (const void*)k_ens_xpass<T>,
+                        (const void*)k_ens_yfwd<T>, (const void*)k_ens_zfwd_formf<T>};
+  for (const void* f : fns) {
+    hipError_t e = big_lds(f, lds_max);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t ens_prepare(int prec, size_t lds_max) {
+  return prec ? ens_prepare_t<float>(lds_max) : ens_prepare_t<double>(lds_max);
+}
+
+template <typename T>
+static void ens_launch_t(int pass, const EnsMember* mem, int nm, unsigned nblk, size_t lds, double beta,
+                         double gamma, hipStream_t s) {
+  const dim3 grid(nblk);
+  switch (pass) {
+    case 0: k_ens_zinv<T><<<grid, kBlock2, lds, s>>>(mem, nm); break;
+    case 1: k_ens_yinv<T><<<grid, kBlock2, lds, s>>>(mem, nm); break;
+    case 2: k_ens_xpass<T><<<grid, kBlock2, lds, s>>>(mem, nm); break;
+    case 3: k_ens_yfwd<T><<<grid, kBlock2, lds, s>>>(mem, nm); break;
+    default: k_ens_zfwd_formf<T><<<grid, kBlock2, lds, s>>>(mem, nm, beta, gamma); break;
+  }
+}
+
+hipError_t launch_ens_pass(int prec, int pass, const EnsMember* mem, int nm, unsigned nblk, size_t lds,
+                           double beta, double gamma, hipStream_t s) {
+  if (prec)
+    ens_launch_t<float>(pass, mem, nm, nblk, lds, beta, gamma, s);
+  else
+    ens_launch_t<double>(pass, mem, nm, nblk, lds, beta, gamma, s);
+  return hipGetLastError();
+}
+
+namespace {
 }  // namespace
 
 #define PDR_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)

@@ -101,6 +101,11 @@ def load_library(path: str = LIB_PATH):
         "pdr_stats": (C.c_int, [vp, dp, dp]),
         "pdr_rs_stats": (C.c_int, [vp, dp]),
         "pdr_spectra_1d": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_ensemble_create": (C.c_int, [C.POINTER(vp), C.POINTER(vp), i32]),
+        "pdr_ensemble_run": (C.c_int, [vp, i32, i32, dp]),
+        "pdr_ensemble_set_split": (C.c_int, [vp, i32, i32]),
+        "pdr_ensemble_autotune": (C.c_int, [vp, i32, C.POINTER(i32), C.POINTER(i32)]),
+        "pdr_ensemble_destroy": (C.c_int, [vp]),
         "pdr_set_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_get_forcing": (C.c_int, [vp, i32, dp, dp, dp]),
         "pdr_set_frame": (C.c_int, [vp, dp, dp]),
@@ -452,3 +457,40 @@ class Pandora:
         _check(self.L.pdr_profile_get(self.h, n, names, 512, cnt, ms))
         keys = names.value.decode().split(",")
         return {k: (int(cnt[i]), float(ms[i])) for i, k in enumerate(keys)}
+
+
+class Ensemble:
+    """Batched ensemble of member contexts (pdr_ensemble_*): one graph launch per
+    RK3 step advances every member."""
+
+    def __init__(self, members):
+        self.L = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        _check(self.L.pdr_ensemble_create(C.byref(h), arr, len(self.members)))
+        self.h = h
+
+    def run(self, tstep0: int, nsteps: int, simtime):
+        t = np.ascontiguousarray(simtime, dtype=np.float64).copy()
+        _check(self.L.pdr_ensemble_run(self.h, tstep0, nsteps, t))
+        return t
+
+    def set_split(self, fpb_y: int, fpb_yf: int):
+        _check(self.L.pdr_ensemble_set_split(self.h, fpb_y, fpb_yf))
+
+    def autotune(self, steps: int = 5):
+        a, b = C.c_int32(0), C.c_int32(0)
+        _check(self.L.pdr_ensemble_autotune(self.h, steps, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pdr_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
