@@ -50,6 +50,23 @@ template <typename T>
 constexpr int tile_lines_fwd(int n) {
   return (sizeof(T) == 4 && n <= 512) ? PDR_LINES_FWD_F32 : tile_lines<T>(n);
 }
+// y passes of short lines: a workgroup takes PP whole z planes (all NKI
+// i-lines of each) instead of a tile of i-lines of one plane, so that it
+// moves about three thousand points and no tile is left part-empty
+// (PP = 0: tiles of tile_lines i-lines within one plane).
+template <typename T>
+constexpr int y_planes(int n) {
+  return n <= 32 ? 8 : (n <= 48 ? 4 : (n <= 64 ? 2 : (n <= 96 ? 1 : 0)));
+}
+template <typename T>
+constexpr int y_lines(int n) {
+  return y_planes<T>(n) ? y_planes<T>(n) * (n / 3 + 1) : tile_lines<T>(n);
+}
+template <typename T>
+constexpr int y_lines_fwd(int n) {
+  return y_planes<T>(n) ? y_planes<T>(n) * (n / 3 + 1) : tile_lines_fwd<T>(n);
+}
+
 // Short lines take more rows per workgroup, so that a workgroup still moves
 // about a thousand points (2 rows of 512; 16 rows of 64).
 template <typename T>
@@ -71,6 +88,14 @@ constexpr int lds_lines_bytes(int n) {
 template <typename T>
 constexpr int lds_fwd_bytes(int n) {
   return (n + tile_lines_fwd<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
+}
+template <typename T>
+constexpr int lds_y_bytes(int n) {
+  return (n + y_lines<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
+}
+template <typename T>
+constexpr int lds_yf_bytes(int n) {
+  return (n + y_lines_fwd<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
 }
 template <typename T>
 constexpr int lds_x_bytes(int n) {
@@ -105,6 +130,8 @@ constexpr bool kNtZfwd = (PDR_NT_WRITES & 16) != 0;
 #define PDR_LINE_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_lines_bytes<T>(N)))))
 #define PDR_FWD_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_fwd_bytes<T>(N)))))
 #define PDR_X_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_x_bytes<T>(N)))))
+#define PDR_Y_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_y_bytes<T>(N)))))
+#define PDR_YF_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_yf_bytes<T>(N)))))
 
 // Line kernels issue all global loads of a field (QE elements per thread)
 // before the LDS stores of that field, so that one HBM latency is exposed per
@@ -334,7 +361,8 @@ __global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
 template <typename T, int N>
 __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
-  constexpr int L = tile_lines<T>(N);
+  constexpr int PP = y_planes<T>(N);
+  constexpr int L = y_lines<T>(N);
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKJ = 2 * K + 1;
@@ -344,10 +372,10 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   cx<T>* buf = tw + N;
   const int tile = int(bid % unsigned(ntile));
   const int zg = int(bid / unsigned(ntile));
-  const int zl = zg % nzc;        // plane within the chunk
-  const int fg = zg / nzc;        // field group
+  const int zl = PP ? 0 : zg % nzc;   // plane within the chunk (PP: tile = group of PP planes)
+  const int fg = PP ? zg : zg / nzc;  // field group
   const int z = z0 + zl;
-  const int i0 = tile * L;
+  const int i0 = PP ? 0 : tile * L;
   const int nki = g.nki;
   const int lstride = g.nz * g.ldj;
   load_tw<T, N>(tw, twg);
@@ -361,7 +389,16 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
       const int y = idx - line * N;
       const int j = (y <= K) ? y : ((y >= N - K) ? y - (N - NKJ) : -1);
       cx<T> v = czero<T>();
-      if ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki && j >= 0) v = get<kNrYinv>(&src[line * lstride + j]);
+      if constexpr (PP > 0) {
+        const int p = line / (K + 1);
+        const int i = line - p * (K + 1);
+        const int zz = tile * PP + p;
+        if ((L * N % kBlock2 == 0 || idx < L * N) && zz < nzc && j >= 0)
+          v = get<kNrYinv>(&src[((long long)i * g.nz + zz) * g.ldj + j]);
+      } else {
+        if ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki && j >= 0)
+          v = get<kNrYinv>(&src[line * lstride + j]);
+      }
       pre[q] = v;
     }
   };
@@ -381,14 +418,21 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
     for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
       const int y = idx / L;
       const int line = idx - y * L;
-      if (i0 + line < nki) put<kNtYinv>(&dst[y * g.ldi + line], buf[line * LD + lds_off<T>(y)]);
+      if constexpr (PP > 0) {
+        const int p = line / (K + 1);
+        const int zz = tile * PP + p;
+        if (zz < nzc)
+          put<kNtYinv>(&dst[((long long)zz * N + y) * g.ldi + (line - p * (K + 1))], buf[line * LD + lds_off<T>(y)]);
+      } else {
+        if (i0 + line < nki) put<kNtYinv>(&dst[y * g.ldi + line], buf[line * LD + lds_off<T>(y)]);
+      }
     }
     __syncthreads();
   }
 }
 
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_LINE_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_Y_WAVES
     k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
   k2_yinv_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, w1, w2, twg, ntile, z0, nzc, fpb);
@@ -560,7 +604,8 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
 template <typename T, int N>
 __device__ __forceinline__ void k2_yfwd_body(unsigned bid, const Geo& g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
-  constexpr int L = tile_lines_fwd<T>(N);
+  constexpr int PP = y_planes<T>(N);
+  constexpr int L = y_lines_fwd<T>(N);
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKJ = 2 * K + 1;
@@ -570,10 +615,10 @@ __device__ __forceinline__ void k2_yfwd_body(unsigned bid, const Geo& g, const c
   cx<T>* buf = tw + N;
   const int tile = int(bid % unsigned(ntile));
   const int zg = int(bid / unsigned(ntile));
-  const int zl = zg % nzc;
-  const int fg = zg / nzc;
+  const int zl = PP ? 0 : zg % nzc;   // PP: tile = group of PP planes
+  const int fg = PP ? zg : zg / nzc;
   const int z = z0 + zl;
-  const int i0 = tile * L;
+  const int i0 = PP ? 0 : tile * L;
   const int nki = g.nki;
   const int lstride = g.nz * g.ldj;
   load_tw<T, N>(tw, twg);
@@ -585,7 +630,16 @@ __device__ __forceinline__ void k2_yfwd_body(unsigned bid, const Geo& g, const c
       const int idx = int(threadIdx.x) + kBlock2 * q;
       const int y = idx / L;
       const int line = idx - y * L;
-      pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki) ? get<kNrYfwd>(&src[y * g.ldi + line]) : czero<T>();
+      if constexpr (PP > 0) {
+        const int p = line / (K + 1);
+        const int zz = tile * PP + p;
+        pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && zz < nzc)
+                     ? get<kNrYfwd>(&src[((long long)zz * N + y) * g.ldi + (line - p * (K + 1))])
+                     : czero<T>();
+      } else {
+        pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki) ? get<kNrYfwd>(&src[y * g.ldi + line])
+                                                                          : czero<T>();
+      }
     }
   };
   for (int f = fg * fpb; f < (fg + 1) * fpb; ++f) {
@@ -604,9 +658,14 @@ __device__ __forceinline__ void k2_yfwd_body(unsigned bid, const Geo& g, const c
     for (int idx = threadIdx.x; idx < L * NKJ; idx += kBlock2) {
       const int line = idx / NKJ;
       const int j = idx - line * NKJ;
-      if (i0 + line < nki) {
-        const int yy = (j <= K) ? j : N - NKJ + j;
-        put<kNtYfwd>(&dst[line * lstride + j], buf[line * LD + lds_off<T>(yy)]);
+      const int yy = (j <= K) ? j : N - NKJ + j;
+      if constexpr (PP > 0) {
+        const int p = line / (K + 1);
+        const int zz = tile * PP + p;
+        if (zz < nzc)
+          put<kNtYfwd>(&dst[((long long)(line - p * (K + 1)) * g.nz + zz) * g.ldj + j], buf[line * LD + lds_off<T>(yy)]);
+      } else {
+        if (i0 + line < nki) put<kNtYfwd>(&dst[line * lstride + j], buf[line * LD + lds_off<T>(yy)]);
       }
     }
     __syncthreads();
@@ -614,7 +673,7 @@ __device__ __forceinline__ void k2_yfwd_body(unsigned bid, const Geo& g, const c
 }
 
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
+__global__ void __launch_bounds__(kBlock2) PDR_YF_WAVES
     k2_yfwd(Geo g, const cx<T>* __restrict__ w2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
   k2_yfwd_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, w2, w1, twg, ntile, z0, nzc, fpb);
@@ -768,6 +827,10 @@ size_t lds_fwd() {
   return size_t(lds_fwd_bytes<T>(N));
 }
 template <typename T, int N>
+size_t lds_y() {
+  return size_t(lds_y_bytes<T>(N) > lds_yf_bytes<T>(N) ? lds_y_bytes<T>(N) : lds_yf_bytes<T>(N));
+}
+template <typename T, int N>
 size_t lds_x() {
   return size_t(lds_x_bytes<T>(N) - kBlock2 * int(sizeof(T)));  // dynamic part
 }
@@ -806,12 +869,13 @@ hipError_t zinv_n(pdr_ctx* c, const Coef& cf) {
 template <typename T, int N>
 hipError_t yinv_n(pdr_ctx* c, int z0, int nzc) {
   const Geo& g = c->g;
-  constexpr int L = tile_lines<T>(N);
-  const int nt = (g.nki + L - 1) / L;
-  const size_t lds = lds_lines<T, N>();
+  constexpr int PP = y_planes<T>(N);
+  constexpr int L = y_lines<T>(N);
+  const int nt = PP ? (nzc + PP - 1) / PP : (g.nki + L - 1) / L;
+  const size_t lds = size_t(lds_y_bytes<T>(N));
   hipError_t e = big_lds((const void*)k2_yinv<T, N>, lds);
   if (e != hipSuccess) return e;
-  k2_yinv<T, N><<<dim3(unsigned(nt * nzc * (6 / c->fpb_y))), kBlock2, lds, c->stream>>>(
+  k2_yinv<T, N><<<dim3(unsigned(nt * (PP ? 1 : nzc) * (6 / c->fpb_y))), kBlock2, lds, c->stream>>>(
       g, (const cx<T>*)c->w1, (cx<T>*)c->w2, (const cx<T>*)c->tab.twy, nt, z0, nzc, c->fpb_y);
   return hipGetLastError();
 }
@@ -839,12 +903,13 @@ hipError_t xpass_n(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, int z0, int nz
 template <typename T, int N>
 hipError_t yfwd_n(pdr_ctx* c, int z0, int nzc) {
   const Geo& g = c->g;
-  constexpr int L = tile_lines_fwd<T>(N);
-  const int nt = (g.nki + L - 1) / L;
-  const size_t lds = lds_fwd<T, N>();
+  constexpr int PP = y_planes<T>(N);
+  constexpr int L = y_lines_fwd<T>(N);
+  const int nt = PP ? (nzc + PP - 1) / PP : (g.nki + L - 1) / L;
+  const size_t lds = size_t(lds_yf_bytes<T>(N));
   hipError_t e = big_lds((const void*)k2_yfwd<T, N>, lds);
   if (e != hipSuccess) return e;
-  k2_yfwd<T, N><<<dim3(unsigned(nt * nzc * (3 / c->fpb_yf))), kBlock2, lds, c->stream>>>(
+  k2_yfwd<T, N><<<dim3(unsigned(nt * (PP ? 1 : nzc) * (3 / c->fpb_yf))), kBlock2, lds, c->stream>>>(
       g, (const cx<T>*)c->w2, (cx<T>*)c->w1, (const cx<T>*)c->tab.twy, nt, z0, nzc, c->fpb_yf);
   return hipGetLastError();
 }
@@ -1003,14 +1068,15 @@ __global__ void __launch_bounds__(kBlock2) k_ens_zfwd_formf(const EnsMember* __r
 template <typename T, int N>
 int ens_tiles_n(const Geo& g, int fpb_y, int fpb_yf, int nt[5], unsigned blocks[5]) {
   nt[0] = (g.nkj + zinv_lines<T>(N) - 1) / zinv_lines<T>(N);
-  nt[1] = (g.nki + tile_lines<T>(N) - 1) / tile_lines<T>(N);
+  constexpr int PP = y_planes<T>(N);
+  nt[1] = PP ? (g.nz + PP - 1) / PP : (g.nki + y_lines<T>(N) - 1) / y_lines<T>(N);
   nt[2] = (g.ny + xrows<T>(N) - 1) / xrows<T>(N);
-  nt[3] = (g.nki + tile_lines_fwd<T>(N) - 1) / tile_lines_fwd<T>(N);
+  nt[3] = PP ? (g.nz + PP - 1) / PP : (g.nki + y_lines_fwd<T>(N) - 1) / y_lines_fwd<T>(N);
   nt[4] = (g.nkj + tile_lines_fwd<T>(N) - 1) / tile_lines_fwd<T>(N);
   blocks[0] = unsigned(nt[0] * g.nki);
-  blocks[1] = unsigned(nt[1] * g.nz * (6 / fpb_y));
+  blocks[1] = unsigned(nt[1] * (PP ? 1 : g.nz) * (6 / fpb_y));
   blocks[2] = unsigned(nt[2] * g.nz);
-  blocks[3] = unsigned(nt[3] * g.nz * (3 / fpb_yf));
+  blocks[3] = unsigned(nt[3] * (PP ? 1 : g.nz) * (3 / fpb_yf));
   blocks[4] = unsigned(nt[4] * g.nki);
   return 0;
 }
@@ -1019,8 +1085,9 @@ template <typename T, int N>
 size_t ens_lds_n(int pass) {
   switch (pass) {
     case 0: return size_t(lds_zinv_bytes<T>(N));
-    case 1: return lds_lines<T, N>();
+    case 1: return size_t(lds_y_bytes<T>(N));
     case 2: return lds_x<T, N>();
+    case 3: return size_t(lds_yf_bytes<T>(N));
     default: return lds_fwd<T, N>();
   }
 }
@@ -1152,12 +1219,14 @@ size_t spec_lds_need(int prec, int nx, int ny, int nz) {
     PDR_LDS(float, ny, lds_lines)
     PDR_LDS(float, nz, lds_fwd)
     PDR_LDS(float, ny, lds_fwd)
+    PDR_LDS(float, ny, lds_y)
     PDR_LDS(float, nx, lds_x)
   } else {
     PDR_LDS(double, nz, lds_lines)
     PDR_LDS(double, ny, lds_lines)
     PDR_LDS(double, nz, lds_fwd)
     PDR_LDS(double, ny, lds_fwd)
+    PDR_LDS(double, ny, lds_y)
     PDR_LDS(double, nx, lds_x)
   }
 #undef PDR_LDS
