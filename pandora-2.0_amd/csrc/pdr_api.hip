@@ -357,6 +357,7 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     const char* env = std::getenv("PDR_GENERIC_KERNELS");
     const bool generic = env && env[0] && env[0] != '0';
     c->spec = !generic && spec_supported(nx) && spec_supported(ny) && spec_supported(nz) &&
+              spec_shape_ok(nx, ny, nz) &&
               spec_lds_need(c->prec, nx, ny, nz) <= c->lds_max;
   }
   // The x-pencil buffer may hold a chunk of zc planes instead of the whole box

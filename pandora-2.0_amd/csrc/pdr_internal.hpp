@@ -214,6 +214,7 @@ void prof_end(pdr_ctx* c, int cls);
 void fft_plan(FftPlan* p, int n);
 
 // length-specialised kernels (pdr_kernels2.hip)
+bool spec_shape_ok(int nx, int ny, int nz);
 bool spec_supported(int n);
 size_t spec_lds_need(int prec, int nx, int ny, int nz);
 hipError_t launch2_zinv(pdr_ctx* c, const Coef& cf);

@@ -1184,6 +1184,14 @@ int xpass_tiles(const pdr_ctx* c) {
   }
 }
 
+// The whole-plane tiles of the y passes of short lines take the i-lines per
+// plane from the compile-time length of the pass: NKI = NY/3 + 1 must then be
+// the box's NX/3 + 1.
+bool spec_shape_ok(int nx, int ny, int nz) {
+  (void)nz;
+  return y_planes<float>(ny) == 0 || nx / 3 == ny / 3;
+}
+
 bool spec_supported(int n) {
   switch (n) {
 #define PDR_CASE(n) case n:
