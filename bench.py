@@ -217,6 +217,8 @@ def rocpd_summary(trace_db, fetch_db, write_db, out_json, sq_dbs=(), copy_bytes=
         con = sqlite3.connect(db)
         return sorted({r[0] for r in con.execute("select distinct counter_name from counters_collection")})
 
+    if not (fetch_db and write_db):  # kernel trace only
+        return
     fetch = per_dispatch(fetch_db, "FETCH_SIZE")
     write = per_dispatch(write_db, "WRITE_SIZE")
     ff = copy_bytes / (fetch["k_copy"] * 1024.0)
@@ -274,8 +276,8 @@ def main():
         sp = argparse.ArgumentParser()
         sp.add_argument("--rocpd-summary", action="store_true")
         sp.add_argument("--trace", required=True)
-        sp.add_argument("--fetch", required=True)
-        sp.add_argument("--write", required=True)
+        sp.add_argument("--fetch", default=None)
+        sp.add_argument("--write", default=None)
         sp.add_argument("--json", default=None)
         sp.add_argument("--sq", nargs="*", default=[])
         a = sp.parse_args()
@@ -290,6 +292,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the batched-ensemble measurement and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this path runs on one GPU (the whole box stays resident on one "
@@ -297,6 +301,9 @@ def main():
         sys.exit(2)
     import pandora_amd as pa
 
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(pa)}), flush=True)
+        return
     n, ntr = args.n, args.tracers
     print(f"[bench] FP32-mixed {n}^3 shear + {ntr} tracers: {args.warmup} warmup, {args.steps} timed",
           file=sys.stderr, flush=True)
