@@ -71,6 +71,91 @@ def run_gpu(pa, n, precision, steps, warmup, ntr, profile):
     return (t1 - t0), prof, ok
 
 
+# The large single-box cases, run on request (--case): forced isotropic
+# turbulence at 1024^3 and homogeneous shear at 1536^3, each FP32-mixed with its
+# tracers, then the FP64 GPU form of the same case for the error of E(k),
+# epsilon and the tracer positions (the two forms do not fit on the GPU at once,
+# so they run one after the other from the same seeded field).
+CASES = {
+    "forced1024": dict(n=1024, tracers=16_777_216, seed=3, tseed=103, kp=3.0,
+                       kw=dict(f_type=1, rk3_timestep=2, rk3_dt=0.0005, nu=0.0008, force_tf=0.2,
+                               force_kf=2.5, force_sigma=0.5**0.5, force_stop=100000, force_seed=3),
+                       label="1024^3 forced isotropic (OU |k|<=2.5, T_F=0.2, sigma^2 T_F=0.1), nu=0.0008, "
+                             "dt=0.0005, 16,777,216 tracers"),
+    # its FP64 form runs without tracers: with the 87 GB FP64 tracer field it would not fit in
+    # the 288 GB of one GPU (tracers do not act on the flow, so E(k) still compares one flow)
+    "shear1536": dict(n=1536, tracers=67_108_864, fp64_tracers=0, seed=4, tseed=104, kp=6.0,
+                      kw=dict(f_type=2, rk3_timestep=1, deform_s=1.0, remesh_t=2500, nu=0.0003),
+                      label="1536^3 homogeneous shear, S=1, nu=0.0003, dt=0.0002 (F_REMESH_T=2500), "
+                            "67,108,864 tracers"),
+}
+
+
+def run_case_prec(pa, name, tag, steps, warmup, out_path):
+    """One precision of a large case, in its own process (the two forms of the
+    1536^3 case do not fit on the GPU together, and a fresh process starts with
+    all of the device memory free): timings, E(k), epsilon and the tracer
+    positions go to out_path (.npz)."""
+    import numpy as np
+    c = CASES[name]
+    n = c["n"]
+    prec = pa.PDR_FP32_MIXED if tag == "fp32" else pa.PDR_FP64
+    print(f"[bench] {name} {tag}: init", file=sys.stderr, flush=True)
+    g = pa.Pandora(n, precision=prec, **c["kw"])
+    g.init_field(c["seed"], 2.0 / 3.0, c["kp"])
+    ntr = c["tracers"] if tag == "fp32" else c.get("fp64_tracers", c["tracers"])
+    if ntr:
+        g.init_particles(c["tseed"], ntr)
+    t = g.run(1, warmup, 0.0) if warmup else 0.0
+    g.synchronize()
+    g.profile(True)
+    print(f"[bench] {name} {tag}: {steps} timed steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    t = g.run(1 + warmup, steps, t)
+    g.synchronize()
+    dt = time.perf_counter() - t0
+    prof = g.profile_get()
+    spec, q = g.stats()
+    xp = g.get_particles()[0] if ntr else np.zeros((0, 3))
+    summary = {"steps_per_s": steps / dt, "grid_point_steps_per_s": steps / dt * n**3, "tracers": ntr,
+               "kernels": {k: {"launches": v[0], "avg_ms": v[1] / v[0]} for k, v in prof.items() if v[0]},
+               "finite": bool(g.check_finite()), "device_gib": int(g.sizes()[6]) / 2**30}
+    g.close()
+    np.savez(out_path, ek=spec["ken"], eps=np.array([q["fn_dis"]]), xp=xp, summary=json.dumps(summary))
+
+
+def run_case(name, steps, warmup):
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    c = CASES[name]
+    n = c["n"]
+    out = {"case": name, "workload": c["label"], "steps": steps, "warmup": warmup}
+    data = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        for tag in ("fp32", "fp64"):
+            path = os.path.join(tmp, f"{tag}.npz")
+            subprocess.run([sys.executable, os.path.abspath(__file__), "--case", name, "--case-prec", tag,
+                            "--case-out", path, "--steps", str(steps), "--warmup", str(warmup)], check=True)
+            with np.load(path) as z:
+                data[tag] = {"ek": z["ek"], "eps": float(z["eps"][0]), "xp": z["xp"],
+                             "summary": json.loads(str(z["summary"]))}
+            out[tag] = data[tag]["summary"]
+    e32, e64 = data["fp32"]["ek"], data["fp64"]["ek"]
+    big = e64 > 1e-6 * e64.max()
+    out["fp32_over_fp64"] = out["fp32"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    out["ek_max_rel_err"] = float(np.max(np.abs(e32[big] - e64[big]) / e64[big]))
+    out["eps_rel_err"] = abs(data["fp32"]["eps"] - data["fp64"]["eps"]) / abs(data["fp64"]["eps"])
+    out["tracer_rms_err_dx"] = None
+    if len(data["fp32"]["xp"]) and len(data["fp32"]["xp"]) == len(data["fp64"]["xp"]):
+        d = np.abs(data["fp32"]["xp"] - data["fp64"]["xp"])
+        L = 2.0 * float(np.float32(np.pi))
+        d = np.minimum(d, L - d) / (L / n)
+        out["tracer_rms_err_dx"] = float(np.sqrt(np.mean(np.sum(d * d, axis=1))))
+    return out
+
+
 ENS_SIZES = (32, 48, 64, 96)
 
 
@@ -294,15 +379,25 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched-ensemble measurement and print its JSON object")
+    ap.add_argument("--case", choices=sorted(CASES), default=None,
+                    help="run one large single-box case (FP32-mixed and FP64) and print its JSON object")
+    ap.add_argument("--case-prec", choices=("fp32", "fp64"), default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--case-out", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this path runs on one GPU (the whole box stays resident on one "
               "MI355X); multi-GPU runs are not supported", file=sys.stderr)
         sys.exit(2)
+    if args.case and not args.case_prec:  # the parent only starts the two children
+        print(json.dumps(run_case(args.case, args.steps, args.warmup)), flush=True)
+        return
     import pandora_amd as pa
 
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(pa)}), flush=True)
+        return
+    if args.case and args.case_prec:
+        run_case_prec(pa, args.case, args.case_prec, args.steps, args.warmup, args.case_out)
         return
     n, ntr = args.n, args.tracers
     print(f"[bench] FP32-mixed {n}^3 shear + {ntr} tracers: {args.warmup} warmup, {args.steps} timed",

@@ -220,8 +220,14 @@ static int dalloc(pdr_ctx* c, void** p, size_t bytes) {
   if (e != hipSuccess) {
     (void)hipGetLastError();  // clear it: later launch checks read hipGetLastError()
     *p = nullptr;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    (void)hipGetLastError();
     return fail(PDR_ERR_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes failed: " +
-                                   hipGetErrorString(e));
+                                   hipGetErrorString(e) + " (context holds " + std::to_string(c->dev_bytes) +
+                                   " bytes; device free " + std::to_string(free_b) + " of " +
+                                   std::to_string(total_b) + ", x-pencil chunk " + std::to_string(c->g.zc) +
+                                   " planes)");
   }
   c->dev_bytes += bytes;
   return PDR_OK;
@@ -370,12 +376,20 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     double mb = env && env[0] ? std::atof(env) : 0.0;
     const double plane = 6.0 * double(ny) * double(g.ldi) * double(c->csz);
     if (!(env && env[0])) {
+      // free device memory; if the query fails, the device's total memory
       size_t free_b = 0, total_b = 0;
-      if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
-        const double need = double(g.mw) * 6.0 * double(c->csz) + 6.0 * double(g.m1) * double(c->csz) +
-                            plane * double(nz) + 3.0 * double(g.nr) * double(c->rsz);
-        if (need > 0.9 * double(free_b)) mb = kZChunkMB;
+      if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        int dev = 0;
+        hipDeviceProp_t pr;
+        free_b = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess)
+                     ? size_t(pr.totalGlobalMem)
+                     : 0;
+        (void)hipGetLastError();
       }
+      const double need = double(g.mw) * 6.0 * double(c->csz) + 6.0 * double(g.m1) * double(c->csz) +
+                          plane * double(nz) + 3.0 * double(g.nr) * double(c->rsz);
+      if (free_b == 0 || need > 0.9 * double(free_b)) mb = kZChunkMB;
     }
     int zc = mb > 0.0 ? int(mb * 1048576.0 / plane) : nz;
     zc = std::max(1, std::min(zc, nz));
