@@ -59,7 +59,7 @@ void fft_plan(FftPlan* p, int n) {
   while (m % 5 == 0 && p->npass < kMaxPasses) { p->radix[p->npass++] = 5; m /= 5; }
 }
 
-void prof_begin(pdr_ctx* c, int cls) {
+void prof_begin(pdr_ctx* c, int cls, hipStream_t s) {
   if (!c->prof) return;
   hipEvent_t a, b;
   if (c->ev_pool.size() >= 2) {
@@ -71,15 +71,15 @@ void prof_begin(pdr_ctx* c, int cls) {
     if (hipEventCreate(&a) != hipSuccess) return;
     if (hipEventCreate(&b) != hipSuccess) return;
   }
-  (void)hipEventRecord(a, c->stream);
+  (void)hipEventRecord(a, s ? s : c->stream);
   c->ev_pending.push_back({cls, {a, b}});
 }
 
-void prof_end(pdr_ctx* c, int cls) {
+void prof_end(pdr_ctx* c, int cls, hipStream_t s) {
   if (!c->prof || c->ev_pending.empty()) return;
   auto& last = c->ev_pending.back();
   if (last.first != cls) return;
-  (void)hipEventRecord(last.second.second, c->stream);
+  (void)hipEventRecord(last.second.second, s ? s : c->stream);
 }
 
 }  // namespace pdr
@@ -268,6 +268,10 @@ int pdr_destroy(pdr_ctx* c) {
     (void)hipEventDestroy(e.second.second);
   }
   for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->tstream) (void)hipStreamSynchronize(c->tstream);
+  if (c->ev_ur) (void)hipEventDestroy(c->ev_ur);
+  if (c->ev_tr) (void)hipEventDestroy(c->ev_tr);
+  if (c->tstream) (void)hipStreamDestroy(c->tstream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PDR_OK;
@@ -414,8 +418,11 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
 
   int rc = PDR_OK;
   he = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&c->tstream, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_ur, hipEventDisableTiming);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&c->ev_tr, hipEventDisableTiming);
   if (he != hipSuccess) {
-    delete c;
+    pdr_destroy(c);
     return fail(PDR_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
   }
   const size_t wbytes = size_t(g.mw) * c->csz;
@@ -642,15 +649,23 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   rk_coeffs(stage, &beta, &gamma);
   const double gdt = gamma * (*dt);
   if (tracers) {
+    // the advance reads the tracer field the x pass wrote and nothing the forward
+    // passes touch: it runs on the tracer stream beside them
+    HIPCHK(hipEventRecord(c->ev_ur, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->tstream, c->ev_ur, 0));
     if (stage == 1 && tstep == c->ts_release) {
       // p_ControlInitPartVelocity[BF] at the release (src/pp_control.f90:378-510):
       // particle_up_rk3 = 0, particle_xp0 = particle_xp (particle_up is set by the advance)
-      HIPCHK(hipMemsetAsync(c->uprk, 0, 24 * size_t(c->np), c->stream));
-      if (c->xp0) HIPCHK(hipMemcpyAsync(c->xp0, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->uprk, 0, 24 * size_t(c->np), c->tstream));
+      if (c->xp0)
+        HIPCHK(hipMemcpyAsync(c->xp0, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToDevice, c->tstream));
     }
-    HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt));
+    HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt, c->tstream));
+    if (pstats) HIPCHK(launch_pstats_rel(c, tstep, c->tstream));
+    HIPCHK(hipEventRecord(c->ev_tr, c->tstream));
+  } else if (pstats) {
+    HIPCHK(launch_pstats_rel(c, tstep, c->stream));
   }
-  if (pstats) HIPCHK(launch_pstats_rel(c, tstep));
   if (c->forcing) HIPCHK(launch_force(c, tstep, *dt));  // f_ForceCalc (src/pf_rk3.f90:2063)
   if (c->spec && c->fuse_zf) {
     HIPCHK(launch_zfwd_formf(c, cf, beta, gdt));
@@ -658,6 +673,7 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
     HIPCHK(launch_zfwd(c));
     HIPCHK(launch_formf(c, cf, beta, gdt));
   }
+  if (tracers) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_tr, 0));
   return PDR_OK;
 }
 

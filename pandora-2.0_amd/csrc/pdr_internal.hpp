@@ -162,6 +162,12 @@ struct pdr_ctx {
   int tj_z = 1, ti_y = 1;
   size_t lds_max = 65536;
   hipStream_t stream = nullptr;
+  // the tracer advance runs on its own stream beside the forward passes of the
+  // stage: ev_ur orders it after the x pass that writes the tracer field,
+  // ev_tr orders the next stage after it
+  hipStream_t tstream = nullptr;
+  hipEvent_t ev_ur = nullptr;
+  hipEvent_t ev_tr = nullptr;
   size_t dev_bytes = 0;
   // profiling
   bool prof = false;
@@ -183,7 +189,7 @@ hipError_t launch_zfwd(pdr_ctx* c);
 hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
 hipError_t launch_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
 hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
-hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt);
+hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s);
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);
 // batched ensemble passes (pdr_kernels2.hip)
@@ -202,14 +208,14 @@ void particles_free_sort(pdr_ctx* c);
 int pstats_alloc(pdr_ctx* c);
 int pstats_reset(pdr_ctx* c);
 hipError_t launch_pstats_primary(pdr_ctx* c, double dt);
-hipError_t launch_pstats_rel(pdr_ctx* c, int tstep);
+hipError_t launch_pstats_rel(pdr_ctx* c, int tstep, hipStream_t s);
 int pstats_read(pdr_ctx* c, int32_t* tstep, double* rec);
 hipError_t launch_finite(pdr_ctx* c, int* flag);
 hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s);
 
 // profiling hooks (pdr_api.hip)
-void prof_begin(pdr_ctx* c, int cls);
-void prof_end(pdr_ctx* c, int cls);
+void prof_begin(pdr_ctx* c, int cls, hipStream_t s = nullptr);
+void prof_end(pdr_ctx* c, int cls, hipStream_t s = nullptr);
 
 void fft_plan(FftPlan* p, int n);
 

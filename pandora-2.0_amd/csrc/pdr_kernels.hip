@@ -808,11 +808,11 @@ hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
 }
 
 template <typename T>
-static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt) {
+static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s) {
   const T* ur = (const T*)c->ur;
   auto go = [&](auto kern, int lanes) {
     const long long nb = (lanes * c->np + kThreads - 1) / kThreads;
-    kern<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->g, tf, ur, c->xp, c->uprk, c->up, c->xp0, c->np,
+    kern<<<dim3((unsigned)nb), kThreads, 0, s>>>(c->g, tf, ur, c->xp, c->uprk, c->up, c->xp0, c->np,
                                                          beta, gdt);
   };
   switch (tf.order) {
@@ -824,11 +824,11 @@ static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, doub
   return hipGetLastError();
 }
 
-hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt) {
+hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s) {
   if (c->np <= 0) return hipSuccess;
-  prof_begin(c, K_TRACER);
-  hipError_t e = c->prec ? tracers_t<float>(c, tf, beta, gdt) : tracers_t<double>(c, tf, beta, gdt);
-  prof_end(c, K_TRACER);
+  prof_begin(c, K_TRACER, s);
+  hipError_t e = c->prec ? tracers_t<float>(c, tf, beta, gdt, s) : tracers_t<double>(c, tf, beta, gdt, s);
+  prof_end(c, K_TRACER, s);
   return e;
 }
 

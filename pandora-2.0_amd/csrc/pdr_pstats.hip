@@ -213,16 +213,16 @@ hipError_t launch_pstats_primary(pdr_ctx* c, double dt) {
   return hipGetLastError();
 }
 
-hipError_t launch_pstats_rel(pdr_ctx* c, int tstep) {
+hipError_t launch_pstats_rel(pdr_ctx* c, int tstep, hipStream_t s) {
   const PsLayout L = ps_layout();
   const int nb = ps_blocks(c->np);
-  k_pstats_rel<<<dim3(unsigned(nb)), kPsThreads, 0, c->stream>>>(c->up, c->np, c->pst_buf + L.rel_part);
+  k_pstats_rel<<<dim3(unsigned(nb)), kPsThreads, 0, s>>>(c->up, c->np, c->pst_buf + L.rel_part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_ps_reduce<<<1, 64, 0, c->stream>>>(c->pst_buf + L.rel_part, nb, kRelW, c->pst_buf + L.rel_sum);
+  k_ps_reduce<<<1, 64, 0, s>>>(c->pst_buf + L.rel_part, nb, kRelW, c->pst_buf + L.rel_sum);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_ps_finish_rel<<<1, 1, 0, c->stream>>>(c->pst_buf, L.rel_sum, L.rec, L.tstep, double(c->np), double(tstep));
+  k_ps_finish_rel<<<1, 1, 0, s>>>(c->pst_buf, L.rel_sum, L.rec, L.tstep, double(c->np), double(tstep));
   return hipGetLastError();
 }
 
