@@ -51,7 +51,8 @@ constexpr int plan_npass(int n) {
 }
 
 // LDS layout of a tile of lines: line l, element o at l * LD + o + (o >> PS),
-// i.e. one pad element per 256 bytes inside a line (PS = log2(256 / sizeof(cx<T>)))
+// i.e. one pad element per 16 elements inside a line (PS = 4; 128 bytes at FP32,
+// 256 at FP64, measured faster at 512 points than one per 8, 32 or 64 at FP32)
 // and a line stride LD = N + (N >> PS) + 1.  Without the pad the stride-R writes
 // of the first Stockham passes (NS = 1, 8) fall on a few banks (8-way conflicts
 // for radix 8); with it every access pattern of these kernels stays within ~1.5x
@@ -59,7 +60,10 @@ constexpr int plan_npass(int n) {
 // keeps the line-fastest (transposing) accesses spread as well.
 template <typename T>
 constexpr int lds_pad_shift() {
-  return sizeof(T) == 4 ? 5 : 4;
+#ifndef PDR_LDS_PAD_F32
+#define PDR_LDS_PAD_F32 4
+#endif
+  return sizeof(T) == 4 ? PDR_LDS_PAD_F32 : 4;
 }
 template <typename T>
 constexpr int lds_ld(int n) {
