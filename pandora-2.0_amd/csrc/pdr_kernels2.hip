@@ -24,7 +24,13 @@ __device__ __forceinline__ cx<T> czero() {
 
 // Tile shapes.  Lines per tile of the y/z kernels and rows per block of the x
 // kernel; the build may override the FP32 / FP64 values for lengths <= 512
-// (PDR_LINES_F32, PDR_LINES_F64, PDR_XROWS_F32, PDR_XROWS_F64) to tune.
+// (PDR_LINES_F32, PDR_LINES_F64, PDR_XROWS_F32, PDR_XROWS_F64) and the FP32
+// heights of longer lines (PDR_LINES_MID_F32 and PDR_LINES_MID_FWD_F32 for
+// 512 < n <= 1024, PDR_LINES_LONG_F32 and PDR_LINES_LONG_FWD_F32 above) to tune.
+// Longer lines keep at least 8 lines in the inverse kernels, whose global
+// accesses are runs of one element of each line (64-byte runs at 8 FP32 lines;
+// 1.45 against 1.26 steps/s at 1536^3 with 4), while the forward kernels of
+// lines above 1024 run faster with the two workgroups per CU that 4 lines allow.
 #ifndef PDR_LINES_F32
 #define PDR_LINES_F32 16
 #endif
@@ -37,18 +43,31 @@ __device__ __forceinline__ cx<T> czero() {
 #ifndef PDR_XROWS_F64
 #define PDR_XROWS_F64 2
 #endif
+#ifndef PDR_LINES_MID_F32
+#define PDR_LINES_MID_F32 8
+#endif
+#ifndef PDR_LINES_LONG_F32
+#define PDR_LINES_LONG_F32 8
+#endif
 template <typename T>
 constexpr int tile_lines(int n) {
-  return sizeof(T) == 4 ? (n <= 512 ? PDR_LINES_F32 : (n <= 1024 ? 8 : 4))
+  return sizeof(T) == 4 ? (n <= 512 ? PDR_LINES_F32 : (n <= 1024 ? PDR_LINES_MID_F32 : PDR_LINES_LONG_F32))
                         : (n <= 512 ? PDR_LINES_F64 : (n <= 1024 ? 4 : 2));
 }
 // forward y/z kernels (3 fields per tile) may use their own FP32 tile height
 #ifndef PDR_LINES_FWD_F32
 #define PDR_LINES_FWD_F32 8
 #endif
+#ifndef PDR_LINES_MID_FWD_F32
+#define PDR_LINES_MID_FWD_F32 8
+#endif
+#ifndef PDR_LINES_LONG_FWD_F32
+#define PDR_LINES_LONG_FWD_F32 4
+#endif
 template <typename T>
 constexpr int tile_lines_fwd(int n) {
-  return (sizeof(T) == 4 && n <= 512) ? PDR_LINES_FWD_F32 : tile_lines<T>(n);
+  return sizeof(T) != 4 ? tile_lines<T>(n)
+                        : (n <= 512 ? PDR_LINES_FWD_F32 : (n <= 1024 ? PDR_LINES_MID_FWD_F32 : PDR_LINES_LONG_FWD_F32));
 }
 // y passes of short lines: a workgroup takes PP whole z planes (all NKI
 // i-lines of each) instead of a tile of i-lines of one plane, so that it
