@@ -164,6 +164,8 @@ static int nthr(const orc* o) {
 }
 
 orc* orc_create(const orc_params* p) {
+  /* NS_DEFORMED: 1 BF_ROT and 3 R_ROT are restated; R_CONV (2) is not */
+  if (p->f_type != 1 && p->ns_deformed != 1 && p->ns_deformed != 3) return NULL;
   orc* o = (orc*)calloc(1, sizeof(orc));
   o->p = *p;
   o->pst_tstep = -1;
@@ -383,6 +385,22 @@ static inline void deriv(const double* u, double k, double* d) {
   d[0] = -k * u[1];
 }
 
+/* Rogallo forms (NS_DEFORMED = 2 R_CONV, 3 R_ROT) of a sheared run; only R_ROT
+ * is restated here, the create call rejects R_CONV. */
+static inline int rogallo(const orc* o) { return o->p.f_type != 1 && o->p.ns_deformed != 1; }
+
+/* f_RK3DerivativeRRot (src/pf_rk3.f90:3749-3780): derivative along the sheared
+ * direction j, sum over m of bmat(m,j) k_m; the existence test there has an empty
+ * body, so every m contributes (absent entries of bmat are zero). */
+static inline void deriv_rrot(const orc* o, const double* uk, const double* k, int j, double* d) {
+  d[0] = 0.0;
+  d[1] = 0.0;
+  for (int m = 1; m <= 3; m++) {
+    d[1] = d[1] + (T9(o->bmat, m, j) * k[m - 1] * uk[0]);
+    d[0] = d[0] - (T9(o->bmat, m, j) * k[m - 1] * uk[1]);
+  }
+}
+
 /* f_RK3VorticityBFRot, src/pf_rk3.f90:3786-3843.  om is (0:1,1:3) row major [c][2]. */
 static void vort_bfrot(const orc* o, const double* un, const double* k, int n, double om[3][2]) {
   for (int c = 0; c < 3; c++) om[c][0] = om[c][1] = 0.0;
@@ -437,6 +455,20 @@ static void mode_vorticity(const orc* o, const double* u1, const double* u2, con
     om[0][0] = om[0][0] - dvdz[0]; om[0][1] = om[0][1] - dvdz[1];
     om[2][0] = om[2][0] + dvdx[0]; om[2][1] = om[2][1] + dvdx[1];
     deriv(u3, ki_, dwdx); deriv(u3, kj_, dwdy);
+    om[0][0] = om[0][0] + dwdy[0]; om[0][1] = om[0][1] + dwdy[1];
+    om[1][0] = om[1][0] - dwdx[0]; om[1][1] = om[1][1] - dwdx[1];
+  } else if (rogallo(o)) {
+    /* NS_R_ROT branches of f_RK3PrepareVelocity (src/pf_rk3.f90:416-421, 533-537, 645-649) */
+    const double kv[3] = {ki_, kj_, kk_};
+    double dudy[2], dudz[2], dvdx[2], dvdz[2], dwdx[2], dwdy[2];
+    deriv_rrot(o, u1, kv, 2, dudy); deriv_rrot(o, u1, kv, 3, dudz);
+    om[0][0] = 0.0; om[0][1] = 0.0;
+    om[1][0] = dudz[0]; om[1][1] = dudz[1];
+    om[2][0] = -dudy[0]; om[2][1] = -dudy[1];
+    deriv_rrot(o, u2, kv, 1, dvdx); deriv_rrot(o, u2, kv, 3, dvdz);
+    om[0][0] = om[0][0] - dvdz[0]; om[0][1] = om[0][1] - dvdz[1];
+    om[2][0] = om[2][0] + dvdx[0]; om[2][1] = om[2][1] + dvdx[1];
+    deriv_rrot(o, u3, kv, 1, dwdx); deriv_rrot(o, u3, kv, 2, dwdy);
     om[0][0] = om[0][0] + dwdy[0]; om[0][1] = om[0][1] + dwdy[1];
     om[1][0] = om[1][0] - dwdx[0]; om[1][1] = om[1][1] - dwdx[1];
   } else {
@@ -619,7 +651,7 @@ void orc_realspace(orc* o, int stage, int tstep, double* dt) {
         double uu[3] = {r[q], r[plane + q], r[2 * plane + q]};
         double ww[3] = {r[3 * plane + q], r[4 * plane + q], r[5 * plane + q]};
         double nn[3];
-        if (o->p.f_type == 1) nl_iso(uu, ww, nn); else nl_bfrot(o, uu, ww, nn);
+        if (o->p.f_type == 1 || rogallo(o)) nl_iso(uu, ww, nn); else nl_bfrot(o, uu, ww, nn);  /* 1080-1093 */
         r[3 * plane + q] = nn[0]; r[4 * plane + q] = nn[1]; r[5 * plane + q] = nn[2];
       }
       if (want_tracers) {
@@ -962,9 +994,89 @@ static void bf_source(const orc* o, const double u[3][2], double h[3][2]) {
               }
 }
 
-/* f_RK3RungeKutta (src/pf_rk3.f90:3457-3498) + f_RK3ContinuityFixIsotropic (3504-3531) */
-static inline void rk_update(double* rk[3], double* u[3], long e, const double tot[3][2], double beta,
-                             double gdt, const double* kv, double kmag2) {
+/* ------------------------------------------------------------------------- */
+/* FormF terms for the R_ROT shear form: src/pf_rk3.f90:2945-3249            */
+/* ------------------------------------------------------------------------- */
+/* f_RK3RRotViscous (2945-2979): -nu bmat(l,j) bmat(n,j) k_l k_n u, term by term */
+static void rr_viscous(const orc* o, const double u[3][2], const double* k, double v[3][2]) {
+  const double* B = o->bmat;
+  const int* bv = o->bval;
+  const double nu = o->p.f_nu;
+  for (int c = 0; c < 3; c++) v[c][0] = v[c][1] = 0.0;
+  for (int l = 1; l <= 3; l++)
+    for (int j = 1; j <= 3; j++)
+      if (T9(bv, l, j))
+        for (int n = 1; n <= 3; n++)
+          if (T9(bv, n, j))
+            for (int c = 0; c < 3; c++)
+              for (int r = 0; r < 2; r++)
+                v[c][r] = v[c][r] - nu * T9(B, l, j) * T9(B, n, j) * k[l - 1] * k[n - 1] * u[c][r];
+}
+
+/* f_RK3RRotPressure (3078-3156): (p1_c / p4) (p2 + p3) with p1 = bmat^T k,
+ * p2 = k . bmat nl, p3 = k . bmat amat u (no factor two), p4 = |bmat^T k|^2 */
+static void rr_pressure(const orc* o, const double u[3][2], const double* k, const double nl[3][2],
+                        double pf[3][2]) {
+  const double* B = o->bmat;
+  const double* A = o->amat;
+  const int* bv = o->bval;
+  const int* av = o->aval;
+  double p1[3] = {0, 0, 0}, p2[2] = {0, 0}, p3[2] = {0, 0}, p4 = 0.0;
+  for (int m = 1; m <= 3; m++)
+    for (int r = 1; r <= 3; r++) {
+      if (!T9(bv, m, r)) continue;
+      for (int q = 0; q < 2; q++) p2[q] = p2[q] + (k[m - 1] * T9(B, m, r) * nl[r - 1][q]);
+      for (int kk = 1; kk <= 3; kk++)
+        if (T9(av, r, kk))
+          for (int q = 0; q < 2; q++) {
+            const double t = k[m - 1] * T9(B, m, r) * T9(A, r, kk) * u[kk - 1][q];
+            p3[q] = p3[q] + (o->p.rrot_p3_two ? 2.0 * t : t);
+          }
+      for (int kk = 1; kk <= 3; kk++)
+        if (T9(bv, kk, r)) p4 = p4 + (k[m - 1] * T9(B, m, r) * k[kk - 1] * T9(B, kk, r));
+    }
+  for (int i = 1; i <= 3; i++)
+    for (int p = 1; p <= 3; p++)
+      if (T9(bv, p, i)) p1[i - 1] = p1[i - 1] + k[p - 1] * T9(B, p, i);
+  for (int c = 0; c < 3; c++)
+    for (int q = 0; q < 2; q++) pf[c][q] = (p4 > 0.0) ? (p1[c] / p4) * (p2[q] + p3[q]) : 0.0;
+}
+
+/* f_RK3RRotSource (3218-3249): amat u over the existing entries */
+static void rr_source(const orc* o, const double u[3][2], double h[3][2]) {
+  for (int i = 1; i <= 3; i++)
+    for (int q = 0; q < 2; q++) {
+      h[i - 1][q] = 0.0;
+      for (int j = 1; j <= 3; j++)
+        if (T9(o->aval, i, j)) h[i - 1][q] = h[i - 1][q] + (T9(o->amat, i, j) * u[j - 1][q]);
+    }
+}
+
+/* f_RK3ContinuityFixRRot (src/pf_rk3.f90:3620-3693): map to the moving frame
+ * with bmat, remove k (k . u) / |k|^2, map back with binv */
+static void continuity_rrot(const orc* o, double* u[3], long e, const double* kv, double kmag2) {
+  double um[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int m = 1; m <= 3; m++)
+    for (int n = 1; n <= 3; n++)
+      if (T9(o->bval, m, n))
+        for (int q = 0; q < 2; q++) um[m - 1][q] = um[m - 1][q] + (T9(o->bmat, m, n) * u[n - 1][e + q]);
+  if (!(kmag2 > 0.0)) return;
+  for (int q = 0; q < 2; q++) {
+    const double kd = kv[0] * um[0][q] + kv[1] * um[1][q] + kv[2] * um[2][q];
+    for (int c = 0; c < 3; c++) um[c][q] = um[c][q] - (kv[c] * kd / kmag2);
+  }
+  for (int m = 1; m <= 3; m++)
+    for (int q = 0; q < 2; q++) {
+      double a = 0.0;
+      for (int n = 1; n <= 3; n++)
+        if (T9(o->bval, m, n)) a = a + (T9(o->binv, m, n) * um[n - 1][q]);
+      u[m - 1][e + q] = a;
+    }
+}
+
+/* f_RK3RungeKutta (src/pf_rk3.f90:3457-3498) */
+static inline void rk_only(double* rk[3], double* u[3], long e, const double tot[3][2], double beta,
+                           double gdt) {
   for (int c = 0; c < 3; c++) {
     rk[c][e] = (beta * rk[c][e]) + tot[c][0];
     rk[c][e + 1] = (beta * rk[c][e + 1]) + tot[c][1];
@@ -973,6 +1085,12 @@ static inline void rk_update(double* rk[3], double* u[3], long e, const double t
     u[c][e] = u[c][e] + (gdt * rk[c][e]);
     u[c][e + 1] = u[c][e + 1] + (gdt * rk[c][e + 1]);
   }
+}
+
+/* f_RK3RungeKutta + f_RK3ContinuityFixIsotropic (3504-3531) */
+static inline void rk_update(double* rk[3], double* u[3], long e, const double tot[3][2], double beta,
+                             double gdt, const double* kv, double kmag2) {
+  rk_only(rk, u, e, tot, beta, gdt);
   double kd0 = kv[0] * u[0][e] + kv[1] * u[1][e] + kv[2] * u[2][e];
   double kd1 = kv[0] * u[0][e + 1] + kv[1] * u[1][e + 1] + kv[2] * u[2][e + 1];
   for (int c = 0; c < 3; c++) {
@@ -1042,6 +1160,24 @@ static void solve_ns(orc* o, double dt, int stage) {
               tot[c][0] = (((visc0 - nn[c][0]) + pr0) + fo[c][0]) + 0.0;
               tot[c][1] = (((visc1 - nn[c][1]) + pr1) + fo[c][1]) + 0.0;
             }
+          } else if (rogallo(o)) {
+            /* f_RK3FormFRRot (src/pf_rk3.f90:2680-2856); the nonlinear term enters
+             * only for F_SHEAR_DNS, unmapped */
+            double nlf[3][2], pf[3][2], hom[3][2], visc[3][2];
+            for (int c = 0; c < 3; c++)
+              for (int r = 0; r < 2; r++) nlf[c][r] = (o->p.f_type == 2) ? nn[c][r] : 0.0;
+            rr_pressure(o, uu, kv, nlf, pf);
+            rr_source(o, uu, hom);
+            if (o->p.f_type == 2 || o->p.f_type == 3) rr_viscous(o, uu, kv, visc);
+            for (int c = 0; c < 3; c++)
+              for (int r = 0; r < 2; r++) {
+                if (o->p.f_type == 2) tot[c][r] = (((visc[c][r] - nlf[c][r]) + pf[c][r]) + 0.0) - hom[c][r];
+                else if (o->p.f_type == 3) tot[c][r] = ((visc[c][r] + pf[c][r]) + 0.0) - hom[c][r];
+                else tot[c][r] = pf[c][r] - hom[c][r];
+              }
+            rk_only(o->rk, o->u, e, tot, beta, gdt);
+            continuity_rrot(o, o->u, e, kv, kmag2);
+            continue;
           } else {
             double nlf[3][2], pf[3][2], hom[3][2], visc[3][2];
             if (o->p.f_type == 2) bf_nonlinear(o, nn, nlf);
@@ -1287,7 +1423,7 @@ static void tracers_stage(orc* o, int stage, double dt) {
   if (stage == 2) { beta = -FIVE_OVER_NINE; gamma = FIFTEEN_OVER_SIXTEEN; }
   if (stage == 3) { beta = -ONEFIVETHREE_OVER_ONETWOEIGHT; gamma = EIGHT_OVER_FIFTEEN; }
   const double* f[3] = {o->ur[0], o->ur[1], o->ur[2]};
-  const int bf = (o->p.f_type != 1);
+  const int bf = (o->p.f_type != 1) && !rogallo(o);  /* R forms: p_RK3SolveParticleR (307-403) */
 #pragma omp parallel for num_threads(nthr(o))
   for (long q = 0; q < o->np; q++) {
     double* x = &o->xp[3 * q];
@@ -1340,7 +1476,17 @@ static void pstats_primary(orc* o, double dt) {
     s[3] = s[3] + u[0] * u[1];
     s[4] = s[4] + u[0] * u[2];
     s[5] = s[5] + u[1] * u[2];
-    for (int c = 0; c < 3; c++) va[c] = va[c] + u[c];
+    if (rogallo(o)) {
+      /* p_RK3AbsoluteVelocityR (src/pp_rk3.f90:720-773): binv particle_up */
+      for (int m = 1; m <= 3; m++) {
+        double a = 0.0;
+        for (int n = 1; n <= 3; n++)
+          if (T9(o->bval, m, n)) a = a + T9(o->binv, m, n) * u[n - 1];
+        va[m - 1] = va[m - 1] + a;
+      }
+    } else {
+      for (int c = 0; c < 3; c++) va[c] = va[c] + u[c];
+    }
     if (o->xp0) {
       double xm[3], xl[3] = {0, 0, 0};
       for (int c = 0; c < 3; c++) xm[c] = o->xp[3 * q + c] - o->xp0[3 * q + c];
@@ -1474,6 +1620,10 @@ static void lab_frame(const orc* o, const double* kv, const double um[3][2], dou
         ul[m - 1][0] = ul[m - 1][0] + T9(o->binv, m, n) * um[n - 1][0];
         ul[m - 1][1] = ul[m - 1][1] + T9(o->binv, m, n) * um[n - 1][1];
       }
+  /* f_FluidstatsFsPrimaryQuantsRRot (src/pf_fluidstats.f90:1801-1970): the
+   * Rogallo forms hold laboratory components, ulab = umov */
+  if (rogallo(o))
+    for (int n = 0; n < 3; n++) { ul[n][0] = um[n][0]; ul[n][1] = um[n][1]; }
 }
 
 /* bin kk in 1..kkmax with kk - 1/2 <= |k| < kk + 1/2, or 0 (src/pf_fluidstats.f90:1530-1545) */

@@ -29,7 +29,7 @@ extern "C" {
 typedef struct orc_params {
   int32_t nodes_x, nodes_y, nodes_z;
   int32_t f_type;          /* 1 iso, 2 shear DNS, 3 shear RDT viscous, 4 shear RDT inviscid */
-  int32_t ns_deformed;     /* 1 BF_ROT (only form restated here for shear) */
+  int32_t ns_deformed;     /* 1 BF_ROT, 3 R_ROT (the shear forms restated here) */
   int32_t rk3_timestep;    /* 1 DECIDE, 2 FIXED, 3 CFL */
   double  rk3_dt;
   double  rk3_cfl;
@@ -41,7 +41,8 @@ typedef struct orc_params {
   int32_t p_interp_order;  /* even orders (4 is the reference default) */
   int32_t nthreads;        /* 0 = OpenMP default */
   int32_t force_stop;      /* FORCE_STOP: OU forcing (F_TYPE 1) while tstep < FORCE_STOP */
-  int32_t pad0;
+  int32_t rrot_p3_two;     /* test switch, 0 = the reference: 1 doubles the amat term p3 of the
+                              R_ROT pressure (the factor two left commented out there) */
   double  force_kf;        /* FORCE_KF */
   double  force_sigma;     /* FORCE_SIGMA */
   int64_t force_seed;      /* seed of the counter-based forcing draws */

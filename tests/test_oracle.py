@@ -281,6 +281,132 @@ def test_shear_energy_budget(oracle_mod):
     assert abs(lhs - rhs) < 2e-4 * (abs(P0) + abs(e0))
 
 
+def test_rrot_form_with_zero_shear_reduces_to_isotropic(oracle_mod):
+    """NS_R_ROT with S = 0: bmat = binv = I, so the sheared derivatives, the
+    R_ROT pressure, viscous and continuity terms reduce to the isotropic ones."""
+    import pandora_amd as pa
+    n = 20
+    u = pa.init_field_host(n, 3, 2.0 / 3.0, 4.0)
+    a = oracle_mod.Oracle(n, f_type=1, rk3_timestep=2, rk3_dt=0.004, nu=0.01)
+    b = oracle_mod.Oracle(n, f_type=2, ns_deformed=3, rk3_timestep=2, rk3_dt=0.004, nu=0.01, deform_s=0.0)
+    a.set_state(*u)
+    b.set_state(*u)
+    ta = tb = 0.0
+    for s in range(1, 4):
+        ta = a.step(s, ta)
+        tb = b.step(s, tb)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert rel(y, x) < 1e-14
+
+
+def _rrot_budget_gap(oracle_mod, p3_two, remesh_t=100):
+    """|dK/dt - (-S <u1 u3> - eps)| / (|P| + |eps|) of an R_ROT run, whose state
+    is the laboratory velocity on the sheared grid (lab wavenumbers bmat^T k)."""
+    import pandora_amd as pa
+    n, S, nu = 32, 5.0, 0.02
+    o = oracle_mod.Oracle(n, f_type=2, ns_deformed=3, rk3_timestep=1, remesh_t=remesh_t, deform_s=S, nu=nu,
+                          rrot_p3_two=p3_two)
+    o.set_state(*pa.init_field_host(n, 5, 2.0 / 3.0, 4.0))
+
+    def budget():
+        b, _ = o.get_frame()
+        B = b.reshape(3, 3, order="F")
+        u = o.get_state()
+        KI, KJ, KK = kgrid(o)
+        w = _energy_weights(o)
+        kl = [B[0, j] * KI + B[1, j] * KJ + B[2, j] * KK for j in range(3)]
+        kl2 = kl[0] ** 2 + kl[1] ** 2 + kl[2] ** 2
+        K = 0.5 * sum((w * np.abs(x) ** 2).sum() for x in u)
+        P = -S * (w * (u[0] * np.conj(u[2])).real).sum()
+        eps = nu * sum((w * kl2 * np.abs(x) ** 2).sum() for x in u)
+        return K, P, eps
+
+    t = 0.0
+    nst = 60 * remesh_t // 100  # St = 0.3
+    for s in range(1, nst + 1):
+        t = o.step(s, t)
+    K0, P0, e0 = budget()
+    t1 = o.step(nst + 1, t)
+    K1, P1, e1 = budget()
+    lhs = (K1 - K0) / (t1 - t)
+    rhs = 0.5 * ((P0 - e0) + (P1 - e1))
+    assert abs(P0) > 0.05 * abs(e0)
+    return abs(lhs - rhs) / (abs(P0) + abs(e0))
+
+
+def test_rrot_shear_energy_budget(oracle_mod):
+    """With the amat term of the R_ROT pressure doubled (see the next test), the
+    pressure keeps bmat^T k . u = 0 as the frame shears and the R_ROT step obeys
+    dK/dt = -S <u1 u3> - eps up to a first-order error in dt (the gap halves with
+    dt), which pins its source, viscous and nonlinear terms.  With the
+    reference's factor one, half of that divergence is left to the continuity
+    fix, which removes it along binv k (not bmat^T k) and so changes the energy:
+    the budget is off by a fixed fraction at every dt."""
+    g1, g2 = _rrot_budget_gap(oracle_mod, 1, 100), _rrot_budget_gap(oracle_mod, 1, 200)
+    assert g1 < 5e-3 and g2 < 0.6 * g1
+    assert _rrot_budget_gap(oracle_mod, 0, 100) > 0.1
+
+
+def _rrot_vs_bfrot(oracle_mod, remesh_t, p3_two):
+    import pandora_amd as pa
+    S, nu, n = 5.0, 0.02, 24
+    u = pa.init_field_host(n, 5, 2.0 / 3.0, 4.0)
+    bf = oracle_mod.Oracle(n, f_type=2, ns_deformed=1, rk3_timestep=1, remesh_t=remesh_t, deform_s=S, nu=nu)
+    rr = oracle_mod.Oracle(n, f_type=2, ns_deformed=3, rk3_timestep=1, remesh_t=remesh_t, deform_s=S, nu=nu,
+                           rrot_p3_two=p3_two)
+    bf.set_state(*u)
+    rr.set_state(*u)
+    t1 = t2 = 0.0
+    for s in range(1, remesh_t // 5 + 1):  # to t = 0.02 = St 0.1
+        t1 = bf.step(s, t1)
+        t2 = rr.step(s, t2)
+    assert t1 == t2
+    _, bi = bf.get_frame()
+    BI = bi.reshape(3, 3, order="F")
+    U = bf.get_state()
+    ul = [BI[m, 0] * U[0] + BI[m, 1] * U[1] + BI[m, 2] * U[2] for m in range(3)]
+    return max(rel(rr.get_state()[c], ul[c]) for c in range(3))
+
+
+def test_rrot_follows_bfrot_once_its_pressure_source_is_doubled(oracle_mod):
+    """BF_ROT (pinned above) carries binv-mapped velocities, R_ROT laboratory ones:
+    the same shear flow.  With the factor two of the amat term of the pressure
+    (which src/pf_rk3.f90:3118-3121 leaves commented out) restored, the R_ROT
+    laboratory velocity converges to binv u of BF_ROT as dt -> 0, which checks
+    the sheared derivatives, nonlinear form, viscous, source, pressure and
+    continuity terms of R_ROT; with the reference's factor one the two stay apart
+    by the same amount at every dt (a different flow, kept as in the reference)."""
+    d1, d4 = _rrot_vs_bfrot(oracle_mod, 400, 1), _rrot_vs_bfrot(oracle_mod, 1600, 1)
+    assert d4 < 0.35 * d1 and d4 < 1e-4
+    r1, r4 = _rrot_vs_bfrot(oracle_mod, 400, 0), _rrot_vs_bfrot(oracle_mod, 1600, 0)
+    assert r4 > 0.8 * r1 and r4 > 1e-4
+
+
+def test_rrot_remesh_moves_modes_without_adding_u3(oracle_mod):
+    """R forms copy the moved modes as they are (src/pf_rk3.f90:4155-4162)."""
+    n, R = 24, 2
+    o = oracle_mod.Oracle(n, f_type=2, ns_deformed=3, rk3_timestep=1, remesh_t=R, deform_s=1.0)
+    u = random_state(o, 9, solenoidal=False)
+    o.set_state(*u)
+    assert o.remesh(R + 1)
+    v = o.get_state()
+    K = o.nkk // 2
+    for i in range(o.nki):
+        for k in range(o.nkk):
+            s = k if k <= K else k - o.nkk
+            src = s + i
+            if src > K:
+                continue
+            ko = src % o.nkk
+            for c in range(3):
+                assert np.array_equal(v[c][i, :, k], u[c][i, :, ko])
+
+
+def test_rconv_form_is_not_restated(oracle_mod):
+    with pytest.raises(ValueError):
+        oracle_mod.Oracle(16, f_type=2, ns_deformed=2, rk3_timestep=1, remesh_t=5, deform_s=1.0)
+
+
 def test_remesh_map_enumeration(oracle_mod):
     """knew = k_signed - i, dropped below -K (src/pf_rk3.f90:4100-4125)."""
     for K in [4, 10, 21, 32]:

@@ -69,6 +69,41 @@ def test_uniform_flow_record(oracle_mod):
     np.testing.assert_allclose(xp - xp0, np.broadcast_to(20 * dt * U, xp.shape), atol=1e-12)
 
 
+def test_rrot_uniform_flow_record(oracle_mod):
+    """R_ROT run (laboratory velocity on the sheared grid): tracers take the
+    isotropic advance (p_RK3SolveParticleR, src/pp_rk3.f90:307-403), so each moves
+    by U dt per step, while vt is binv particle_up (p_RK3AbsoluteVelocityR,
+    720-773) and the dispersion binv (x - x0), with binv(1,3) = S t at the start
+    of the step (FIXED dt: no remesh)."""
+    n, dt, S = 16, 0.05, 0.8
+    U = np.array([0.7, -0.4, 0.25])
+    o = oracle_mod.Oracle(n, f_type=2, ns_deformed=3, rk3_timestep=2, rk3_dt=dt, nu=0.01, deform_s=S)
+    _uniform(o, U)
+    rng = np.random.default_rng(6)
+    dx = TWOPI / n
+    o.set_particles(rng.uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(300, 3)))
+    o.particle_stats_config(True, keep_initial=True, ts_release=0)
+    t = 0.0
+    for s in range(1, 13):
+        t = o.step(s, t)
+        ts, r = o.particle_stats()
+        assert ts == s
+        if s == 1:
+            continue
+        b13 = S * (s - 1) * dt
+        assert r["vt1"] == pytest.approx(U[0] + b13 * U[2], rel=1e-12)
+        assert r["vt2"] == pytest.approx(U[1], rel=1e-12)
+        assert r["vt3"] == pytest.approx(U[2], rel=1e-12)
+        assert r["rs_u1u3"] == pytest.approx(U[0] * U[2], rel=1e-12)
+        d = (s - 1) * dt * U
+        xl = np.array([d[0] + b13 * d[2], d[1], d[2]])
+        for c in range(3):
+            assert r[f"xrel{c + 1}"] == pytest.approx(xl[c] ** 2, rel=1e-10)
+        assert r["x1x3"] == pytest.approx(xl[0] * xl[2], rel=1e-10)
+    u = o.get_state()
+    assert u[0][0, 0, 0] == U[0] and np.abs(u[0]).sum() == abs(U[0])
+
+
 def test_release_step(oracle_mod):
     """Particles stay put before P_TS_RELEASE, xp0 is taken at the release, and
     statistics start one step later (tstep > P_TS_RELEASE)."""

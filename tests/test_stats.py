@@ -159,6 +159,29 @@ def test_sheared_frame_bins_by_lab_wavenumber(oracle_mod):
     assert q["u1u1"] == pytest.approx(np.sum(w * np.abs(ul1) ** 2), rel=1e-13)
 
 
+def test_rrot_frame_takes_the_state_as_laboratory_velocity(oracle_mod):
+    """f_FluidstatsFsPrimaryQuantsRRot (src/pf_fluidstats.f90:1801-1970): R_ROT
+    holds laboratory velocities, so its statistics of u equal the BF_ROT ones of
+    bmat u (whose laboratory velocity binv bmat u is u again); same bins, same
+    mode density."""
+    r = oracle_mod.Oracle(16, f_type=2, ns_deformed=3, rk3_timestep=2, deform_s=1.0, nu=0.01)
+    f = oracle_mod.Oracle(16, f_type=2, ns_deformed=1, rk3_timestep=2, deform_s=1.0, nu=0.01)
+    u = _random_state(r, 9)
+    b, bi = r.get_frame()
+    b[6], bi[6] = -0.7, 0.7
+    for o in (r, f):
+        o.set_frame(b, bi)
+    r.set_state(*u)
+    f.set_state(u[0] - 0.7 * u[2], u[1], u[2])
+    (sr, qr), (sf, qf) = r.stats(), f.stats()
+    for k in sr:
+        np.testing.assert_allclose(sr[k], sf[k], rtol=1e-12, atol=1e-14 * np.abs(sf[k]).max())
+    for k in qr:
+        assert qr[k] == pytest.approx(qf[k], rel=1e-12, abs=1e-15), k
+    w = np.where(np.arange(r.nki)[:, None, None] > 0, 2.0, 1.0)
+    assert qr["u1u1"] == pytest.approx(np.sum(w * np.abs(u[0]) ** 2), rel=1e-13)
+
+
 # ---- real-space statistics (f_FluidstatsRs2D / RsCollect) --------------------
 
 def _single_mode(o, comp, axis, amp):
