@@ -125,6 +125,35 @@ Coef pdr::make_coef(const pdr_ctx* c) {
   const int* bv = c->bval;
   const int* av = c->aval;
   auto G = [&](int cc, int n, int m) -> double& { return f.G[((cc - 1) * 3 + (n - 1)) * 3 + (m - 1)]; };
+  if (rogallo_form(c)) {
+    // NS_R_ROT: omega = curl' u with d'_j = i sum_m bmat(m,j) k_m
+    // (f_RK3DerivativeRRot, src/pf_rk3.f90:3749-3780; accumulation 416-421,
+    // 533-537, 645-649); the nonlinear term is the isotropic omega x u of the
+    // unmapped velocity (1088-1089), FormF is f_RK3FormFRRot (2680-2856)
+    for (int m = 1; m <= 3; ++m) {
+      G(1, 3, m) += T9(B, m, 2);
+      G(1, 2, m) -= T9(B, m, 3);
+      G(2, 1, m) += T9(B, m, 3);
+      G(2, 3, m) -= T9(B, m, 1);
+      G(3, 2, m) += T9(B, m, 1);
+      G(3, 1, m) -= T9(B, m, 2);
+    }
+    for (int r = 1; r <= 3; ++r)
+      for (int m = 1; m <= 3; ++m) {
+        f.V[3 * (r - 1) + (m - 1)] = T9(bv, r, m) ? T9(BI, r, m) : 0.0;
+        f.Bm[3 * (r - 1) + (m - 1)] = T9(bv, r, m) ? T9(B, r, m) : 0.0;
+        f.M[3 * (r - 1) + (m - 1)] = T9(bv, m, r) ? T9(B, m, r) : 0.0;  // p1 = bmat^T k
+        // amat / 2: P enters as 2 sum_m p1_m P[m][c] u_c, H as 2 H u
+        f.P[3 * (r - 1) + (m - 1)] = T9(av, r, m) ? 0.5 * T9(A, r, m) : 0.0;
+        f.H[3 * (r - 1) + (m - 1)] = T9(av, r, m) ? 0.5 * T9(A, r, m) : 0.0;
+      }
+    f.nu = c->p.f_nu;
+    f.use_nl = (c->p.f_type == PDR_F_SHEAR_DNS) ? 1 : 0;
+    f.use_visc = (c->p.f_type != PDR_F_SHEAR_RDT_INVISCID) ? 1 : 0;
+    f.has_hom = 1;
+    f.form = 1;
+    return f;
+  }
   for (int n = 1; n <= 3; ++n)
     for (int m = 1; m <= 3; ++m) {
       if (T9(bv, m, 1)) {
@@ -180,7 +209,9 @@ static TracerFrame make_tracer_frame(const pdr_ctx* c) {
       t.binv[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->binv, m, n) : 0.0;
       t.bmat[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->bmat, m, n) : 0.0;
     }
-  t.bf = (c->p.f_type != PDR_F_ISOTROPIC) ? 1 : 0;
+  // BF_ROT tracers move in the moving frame (p_RK3SolveParticleBF); R_ROT ones
+  // take the isotropic update (p_RK3SolveParticleR, src/pp_rk3.f90:307-403)
+  t.bf = (c->p.f_type != PDR_F_ISOTROPIC && !rogallo_form(c)) ? 1 : 0;
   t.order = c->p.p_interp_order;
   t.lx = kTwoPiRef;
   t.ly = kTwoPiRef;
@@ -295,8 +326,9 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     return fail(PDR_ERR_UNSUPPORTED, "grid sizes must be even, >= 8 and of the form 2^a 3^b 5^c");
   if (p->f_type < 1 || p->f_type > 4)
     return fail(PDR_ERR_UNSUPPORTED, "F_TYPE must be 1 (isotropic) or 2..4 (shear DNS/RDT)");
-  if (p->f_type != PDR_F_ISOTROPIC && p->ns_deformed != PDR_NS_BF_ROT)
-    return fail(PDR_ERR_UNSUPPORTED, "shear flows support NS_DEFORMED = 1 (BF_ROT) only");
+  if (p->f_type != PDR_F_ISOTROPIC && p->ns_deformed != PDR_NS_BF_ROT && p->ns_deformed != PDR_NS_R_ROT)
+    return fail(PDR_ERR_UNSUPPORTED, "shear flows support NS_DEFORMED = 1 (BF_ROT) and 3 (R_ROT); "
+                                     "2 (R_CONV) is not built");
   if (p->dealiasing != 1) return fail(PDR_ERR_UNSUPPORTED, "DEALIASING must be 1 (2/3 rule by storage)");
   if (p->rk3_timestep < 1 || p->rk3_timestep > 3) return fail(PDR_ERR_ARG, "RK3_TIMESTEP must be 1, 2 or 3");
   if (p->rk3_cfl >= 1.0) return fail(PDR_ERR_ARG, "STOP: Courant number is greater than 1.0 (g_RK3Check)");

@@ -28,8 +28,13 @@ __device__ __forceinline__ void formf_mode(const Geo& g, const Coef& cf, int i, 
   if (!(kmag2 > 0.0)) return;  // the k = 0 mode is never updated (src/pf_rk3.f90:2385)
   cx<T> u[3] = {u0[e], u1[e], u2[e]};
   cx<T> r[3] = {r0[e], r1[e], r2[e]};
+  const bool rg = cf.form != 0;  // NS_R_ROT (f_RK3FormFRRot, src/pf_rk3.f90:2680-2856)
   cx<T> n[3];
-  if (cf.use_nl) {
+  if (cf.use_nl && rg) {
+    n[0] = a;
+    n[1] = b;
+    n[2] = c;
+  } else if (cf.use_nl) {
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       const double* Bm = &cf.Bm[3 * q];
@@ -38,17 +43,19 @@ __device__ __forceinline__ void formf_mode(const Geo& g, const Coef& cf, int i, 
   } else {
     n[0] = n[1] = n[2] = mk<T>(T(0), T(0));
   }
-  // k . M k, p1 = M k
+  // p1 = M k; p4 = k . M k (BF_ROT, M = bmat bmat^T) or |bmat^T k|^2 (R_ROT, M = bmat^T)
   double p1[3];
 #pragma unroll
   for (int l = 0; l < 3; ++l) p1[l] = cf.M[3 * l] * kv[0] + cf.M[3 * l + 1] * kv[1] + cf.M[3 * l + 2] * kv[2];
-  const double p4 = kv[0] * p1[0] + kv[1] * p1[1] + kv[2] * p1[2];
-  // p2 = k . nlf ; p3 = 2 sum_c (sum_m k_m P[m][c]) u_c
-  cx<T> p23 = add(add(scl(n[0], T(kv[0])), scl(n[1], T(kv[1]))), scl(n[2], T(kv[2])));
+  const double p4 = rg ? (p1[0] * p1[0] + p1[1] * p1[1]) + p1[2] * p1[2] : kv[0] * p1[0] + kv[1] * p1[1] + kv[2] * p1[2];
+  // p2 = kp . nlf ; p3 = 2 sum_c (sum_m kp_m P[m][c]) u_c, along kp = k (BF_ROT)
+  // or bmat^T k (R_ROT: f_RK3RRotPressure, 3078-3156)
+  const double kp[3] = {rg ? p1[0] : kv[0], rg ? p1[1] : kv[1], rg ? p1[2] : kv[2]};
+  cx<T> p23 = add(add(scl(n[0], T(kp[0])), scl(n[1], T(kp[1]))), scl(n[2], T(kp[2])));
   if (cf.has_hom) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const double q = 2.0 * (kv[0] * cf.P[c] + kv[1] * cf.P[3 + c] + kv[2] * cf.P[6 + c]);
+      const double q = 2.0 * (kp[0] * cf.P[c] + kp[1] * cf.P[3 + c] + kp[2] * cf.P[6 + c]);
       p23 = add(p23, scl(u[c], T(q)));
     }
   }
@@ -88,10 +95,25 @@ __device__ __forceinline__ void formf_mode(const Geo& g, const Coef& cf, int i, 
     r[c] = add(scl(r[c], tb), tot[c]);
     u[c] = add(u[c], scl(r[c], tg));
   }
-  // continuity fix u -= k (k.u) / |k|^2 (src/pf_rk3.f90:3504-3531)
-  const cx<T> kd = add(add(scl(u[0], T(kv[0])), scl(u[1], T(kv[1]))), scl(u[2], T(kv[2])));
+  if (rg) {
+    // f_RK3ContinuityFixRRot (src/pf_rk3.f90:3620-3693): um = bmat u,
+    // um -= k (k.um) / |k|^2, u = binv um
+    cx<T> um[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) u[c] = sub(u[c], scl(kd, T(kv[c] / kmag2)));
+    for (int m = 0; m < 3; ++m)
+      um[m] = add(add(scl(u[0], T(cf.Bm[3 * m])), scl(u[1], T(cf.Bm[3 * m + 1]))), scl(u[2], T(cf.Bm[3 * m + 2])));
+    const cx<T> kd = add(add(scl(um[0], T(kv[0])), scl(um[1], T(kv[1]))), scl(um[2], T(kv[2])));
+#pragma unroll
+    for (int m = 0; m < 3; ++m) um[m] = sub(um[m], scl(kd, T(kv[m] / kmag2)));
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      u[m] = add(add(scl(um[0], T(cf.V[3 * m])), scl(um[1], T(cf.V[3 * m + 1]))), scl(um[2], T(cf.V[3 * m + 2])));
+  } else {
+    // continuity fix u -= k (k.u) / |k|^2 (src/pf_rk3.f90:3504-3531)
+    const cx<T> kd = add(add(scl(u[0], T(kv[0])), scl(u[1], T(kv[1]))), scl(u[2], T(kv[2])));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = sub(u[c], scl(kd, T(kv[c] / kmag2)));
+  }
   u0[e] = u[0];
   u1[e] = u[1];
   u2[e] = u[2];

@@ -48,7 +48,9 @@ struct Coef {
   int use_nl;    // F_TYPE 1, 2
   int use_visc;  // F_TYPE 1, 2, 3
   int has_hom;   // shear
-  int pad;
+  int form;      // 0: isotropic / BF_ROT tensors; 1: NS_R_ROT (nonlinear term of the
+                 // unmapped velocity, pressure along M k = bmat^T k, continuity
+                 // fix through Bm and V)
 };
 
 // One member of a batched ensemble (pdr_ens.hip): everything a batched pass
@@ -228,6 +230,10 @@ hipError_t launch2_yinv(pdr_ctx* c, int z0, int nzc);
 hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl, int z0, int nzc, double* rsp);
 int xpass_tiles(const pdr_ctx* c);  // x-pass workgroups per z plane (specialised path)
 Coef make_coef(const pdr_ctx* c);
+// NS_R_ROT shear run (state = laboratory velocity on the sheared grid)
+inline bool rogallo_form(const pdr_ctx* c) {
+  return c->p.f_type != PDR_F_ISOTROPIC && c->p.ns_deformed == PDR_NS_R_ROT;
+}
 hipError_t launch2_yfwd(pdr_ctx* c, int z0, int nzc);
 hipError_t launch2_zfwd(pdr_ctx* c);
 hipError_t launch2_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);

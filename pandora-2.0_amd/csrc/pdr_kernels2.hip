@@ -544,9 +544,11 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
       const T v1 = V[0] * u1 + V[1] * u2 + V[2] * u3;
       const T v2 = V[3] * u1 + V[4] * u2 + V[5] * u3;
       const T v3 = V[6] * u1 + V[7] * u2 + V[8] * u3;
-      nl[q][0] = o2 * v3 - o3 * v2;
-      nl[q][1] = o3 * v1 - o1 * v3;
-      nl[q][2] = o1 * v2 - o2 * v1;
+      // omega x (binv u) (BF_ROT), or omega x u (isotropic with binv = I, R_ROT)
+      const T w1 = cf.form ? u1 : v1, w2 = cf.form ? u2 : v2, w3 = cf.form ? u3 : v3;
+      nl[q][0] = o2 * w3 - o3 * w2;
+      nl[q][1] = o3 * w1 - o1 * w3;
+      nl[q][2] = o1 * w2 - o2 * w1;
       if (want_tr && y0 + r < ny) {
         T* d = urow + idx * 3;
         put1<kNtXpass>(d, u1);

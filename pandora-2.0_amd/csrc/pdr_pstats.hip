@@ -1,7 +1,7 @@
 // pdr_pstats.hip -- tracer statistics on the device: p_StatsPrimary,
 // p_StatsDispersion and p_StatsRelativeVelocity (src/pp_stats.f90:119-390) for
-// tracers (dp = 0), with the mean particle velocity of p_RK3AbsoluteVelocity[BF]
-// (src/pp_rk3.f90:602-710) that the record reports as vt.
+// tracers (dp = 0), with the mean particle velocity of p_RK3AbsoluteVelocity[BF|R]
+// (src/pp_rk3.f90:602-773) that the record reports as vt.
 //
 // Stage 1 of a step (src/pf_rk3.f90:62-99) runs two reductions over the
 // particles: before the particle advance, the second moments and the mean of
@@ -46,6 +46,7 @@ PsLayout ps_layout() {
 
 struct PsFrame {
   double bi[9];  // binv over the existing entries, [m][n]; identity for isotropic runs
+  int vt_binv;   // R_ROT: vt is the mean of binv particle_up (p_RK3AbsoluteVelocityR)
 };
 
 template <int W>
@@ -76,9 +77,18 @@ __global__ void __launch_bounds__(kPsThreads)
     s[3] += u1 * u2;
     s[4] += u1 * u3;
     s[5] += u2 * u3;
-    s[6] += u1;
-    s[7] += u2;
-    s[8] += u3;
+    if (f.vt_binv) {
+      const double u[3] = {u1, u2, u3};
+      for (int m = 0; m < 3; ++m) {
+        double a = 0.0;
+        for (int n = 0; n < 3; ++n) a = a + f.bi[3 * m + n] * u[n];
+        s[6 + m] += a;
+      }
+    } else {
+      s[6] += u1;
+      s[7] += u2;
+      s[8] += u3;
+    }
     if (xp0) {
       double xm[3], xl[3];
       for (int c = 0; c < 3; ++c) xm[c] = xp[3 * p + c] - xp0[3 * p + c];
@@ -200,6 +210,7 @@ hipError_t launch_pstats_primary(pdr_ctx* c, double dt) {
   for (int m = 1; m <= 3; ++m)
     for (int n = 1; n <= 3; ++n)
       f.bi[3 * (m - 1) + (n - 1)] = iso ? (m == n ? 1.0 : 0.0) : (T9(c->bval, m, n) ? T9(c->binv, m, n) : 0.0);
+  f.vt_binv = rogallo_form(c) ? 1 : 0;
   const int nb = ps_blocks(c->np);
   k_pstats_primary<<<dim3(unsigned(nb)), kPsThreads, 0, c->stream>>>(c->up, c->xp, c->xp0, c->np, f,
                                                                      c->pst_buf + L.prim_part);

@@ -33,6 +33,8 @@ struct StatsFrame {
   double nu;
   int kkmax;
   int nki;
+  int lab_state;  // NS_R_ROT: the state holds laboratory velocities
+                  // (f_FluidstatsFsPrimaryQuantsRRot, src/pf_fluidstats.f90:1801-1970)
 };
 
 // Width of the per-workgroup histogram: 13 shell arrays of kkmax, then the
@@ -90,16 +92,18 @@ __global__ void __launch_bounds__(kStatThreads)
         ul[m][1] = ul[m][1] + sf.bi[3 * m + n] * um[n][1];
       }
     const double km = sqrt((kl[0] * kl[0]) + (kl[1] * kl[1]) + (kl[2] * kl[2]));
+    // streamwise spectra: sum over (j, k) of |binv u|^2 per i (all stored modes;
+    // f_FluidstatsFs2D maps with binv in every sheared form)
+    two_sum_add(&s1[i], &s1[sf.nki + i], ul[0][0] * ul[0][0] + ul[0][1] * ul[0][1]);
+    two_sum_add(&s1[2 * sf.nki + i], &s1[3 * sf.nki + i], ul[1][0] * ul[1][0] + ul[1][1] * ul[1][1]);
+    const double(*v)[2] = sf.lab_state ? um : ul;
     double ph[6];
-    ph[0] = ul[0][0] * ul[0][0] + ul[0][1] * ul[0][1];
-    ph[1] = ul[1][0] * ul[1][0] + ul[1][1] * ul[1][1];
-    ph[2] = ul[2][0] * ul[2][0] + ul[2][1] * ul[2][1];
-    ph[3] = ul[0][0] * ul[1][0] + ul[0][1] * ul[1][1];
-    ph[4] = ul[0][0] * ul[2][0] + ul[0][1] * ul[2][1];
-    ph[5] = ul[1][0] * ul[2][0] + ul[1][1] * ul[2][1];
-    // streamwise spectra: sum over (j, k) of |u_lab|^2 per i (all stored modes)
-    two_sum_add(&s1[i], &s1[sf.nki + i], ph[0]);
-    two_sum_add(&s1[2 * sf.nki + i], &s1[3 * sf.nki + i], ph[1]);
+    ph[0] = v[0][0] * v[0][0] + v[0][1] * v[0][1];
+    ph[1] = v[1][0] * v[1][0] + v[1][1] * v[1][1];
+    ph[2] = v[2][0] * v[2][0] + v[2][1] * v[2][1];
+    ph[3] = v[0][0] * v[1][0] + v[0][1] * v[1][1];
+    ph[4] = v[0][0] * v[2][0] + v[0][1] * v[2][1];
+    ph[5] = v[1][0] * v[2][0] + v[1][1] * v[2][1];
     if (i != 0) {
 #pragma unroll
       for (int q = 0; q < 6; ++q) ph[q] = ph[q] * 2.0;
@@ -197,6 +201,7 @@ int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6], std::vector<do
   sf.nu = c->p.f_nu;
   sf.kkmax = kkmax;
   sf.nki = nki;
+  sf.lab_state = rogallo_form(c) ? 1 : 0;
   const long long want = (c->g.mw + 4LL * kStatThreads - 1) / (4LL * kStatThreads);
   const int nblk = int(want < kStatBlocks ? (want < 1 ? 1 : want) : kStatBlocks);
   const int width = stats_hist(kkmax, nki) + 12;
