@@ -49,7 +49,7 @@ extern "C" {
 #define PDR_F_SHEAR_RDT_VISCOUS   3
 #define PDR_F_SHEAR_RDT_INVISCID  4
 #define PDR_NS_BF_ROT             1
-#define PDR_NS_R_CONV             2  /* not built: rejected with PDR_ERR_UNSUPPORTED */
+#define PDR_NS_R_CONV             2
 #define PDR_NS_R_ROT              3
 #define PDR_RK3_TIMESTEP_DECIDE   1
 #define PDR_RK3_TIMESTEP_FIXED    2
@@ -63,7 +63,7 @@ typedef struct pdr_ctx pdr_ctx;
 typedef struct pdr_params {
   int32_t nodes_x, nodes_y, nodes_z; /* NODES_X/Y/Z: 2^a * {1,3,5} (radix 2/3/5 FFTs)  */
   int32_t f_type;                    /* F_TYPE: 1 isotropic, 2 shear DNS, 3/4 shear RDT */
-  int32_t ns_deformed;               /* NS_DEFORMED: 1 = NS_BF_ROT, 3 = NS_R_ROT        */
+  int32_t ns_deformed;               /* NS_DEFORMED: 1 BF_ROT, 2 R_CONV, 3 R_ROT        */
   int32_t dealiasing;                /* DEALIASING: 1 = cubic 2/3 rule by storage       */
   int32_t rk3_timestep;              /* RK3_TIMESTEP: 1 DECIDE, 2 FIXED, 3 CFL          */
   int32_t f_remesh_t;                /* F_REMESH_T (shear, DECIDE)                      */

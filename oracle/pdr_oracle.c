@@ -131,7 +131,8 @@ struct orc {
   double dx, dy, dz, lx, ly, lz;
   double* u[3];
   double* rk[3];
-  double* w[6];         /* u1..u6_1r2w */
+  double* w[12];        /* u1..u6_1r2w (u1..u12 for R_CONV) */
+  int nf;               /* inverse-transformed fields: 6, or 12 for R_CONV */
   double* ur[3];        /* real-space velocity for tracers */
   long np;
   double* xp;
@@ -164,8 +165,8 @@ static int nthr(const orc* o) {
 }
 
 orc* orc_create(const orc_params* p) {
-  /* NS_DEFORMED: 1 BF_ROT and 3 R_ROT are restated; R_CONV (2) is not */
-  if (p->f_type != 1 && p->ns_deformed != 1 && p->ns_deformed != 3) return NULL;
+  /* NS_DEFORMED: 1 BF_ROT, 2 R_CONV, 3 R_ROT */
+  if (p->f_type != 1 && (p->ns_deformed < 1 || p->ns_deformed > 3)) return NULL;
   orc* o = (orc*)calloc(1, sizeof(orc));
   o->p = *p;
   o->pst_tstep = -1;
@@ -199,7 +200,9 @@ orc* orc_create(const orc_params* p) {
     o->rk[c] = (double*)calloc((size_t)(2 * o->mw), sizeof(double));
     o->ur[c] = NULL;
   }
-  for (int f = 0; f < 6; f++) o->w[f] = (double*)calloc((size_t)(2 * o->m1), sizeof(double));
+  /* R_CONV transforms u and its nine raw-wavenumber gradients (src/pf_arrays.f90:788-964) */
+  o->nf = (p->f_type != 1 && p->ns_deformed == 2) ? 12 : 6;
+  for (int f = 0; f < o->nf; f++) o->w[f] = (double*)calloc((size_t)(2 * o->m1), sizeof(double));
   fftp_init(&o->fz, o->nz);
   fftp_init(&o->fy, o->ny);
   fftp_init(&o->fx, o->nx);
@@ -333,7 +336,7 @@ void orc_destroy(orc* o) {
   free(o->force);
   if (!o) return;
   for (int c = 0; c < 3; c++) { free(o->u[c]); free(o->rk[c]); free(o->ur[c]); }
-  for (int f = 0; f < 6; f++) free(o->w[f]);
+  for (int f = 0; f < o->nf; f++) free(o->w[f]);
   free(o->xp); free(o->up); free(o->uprk); free(o->xp0);
   fftp_free(&o->fz); fftp_free(&o->fy); fftp_free(&o->fx);
   free(o);
@@ -385,8 +388,7 @@ static inline void deriv(const double* u, double k, double* d) {
   d[0] = -k * u[1];
 }
 
-/* Rogallo forms (NS_DEFORMED = 2 R_CONV, 3 R_ROT) of a sheared run; only R_ROT
- * is restated here, the create call rejects R_CONV. */
+/* Rogallo forms (NS_DEFORMED = 2 R_CONV, 3 R_ROT) of a sheared run */
 static inline int rogallo(const orc* o) { return o->p.f_type != 1 && o->p.ns_deformed != 1; }
 
 /* f_RK3DerivativeRRot (src/pf_rk3.f90:3749-3780): derivative along the sheared
@@ -493,7 +495,8 @@ void orc_prepare_realspace(orc* o) {
   {
     double* in = (double*)malloc(sizeof(double) * 2 * nz);
     double* out = (double*)malloc(sizeof(double) * 2 * nz);
-    double* om = (double*)malloc(sizeof(double) * 2 * (size_t)nkk * 3);
+    double* om = (double*)malloc(sizeof(double) * 2 * (size_t)nkk * 9);
+    const int conv = (o->nf == 12);
 #pragma omp for schedule(dynamic)
     for (int i = 0; i < o->nki; i++) {
       const double wi = wvn(i, o->ki);
@@ -502,12 +505,25 @@ void orc_prepare_realspace(orc* o) {
         const long base = ((long)i * nkj + j) * nkk;
         for (int k = 0; k < nkk; k++) {
           const double wk = wvn(k, K);
+          if (conv) {
+            /* NS_R_CONV branches of f_RK3PrepareVelocity (src/pf_rk3.f90:422-427, 538-543,
+             * 650-656): u(4 + 3d + c) = d u_c / d x_d with the raw wavenumbers */
+            const double kd[3] = {wi, wj, wk};
+            for (int c = 0; c < 3; c++)
+              for (int d = 0; d < 3; d++) {
+                double g[2];
+                deriv(&o->u[c][2 * (base + k)], kd[d], g);
+                om[2 * ((3 * d + c) * nkk + k)] = g[0];
+                om[2 * ((3 * d + c) * nkk + k) + 1] = g[1];
+              }
+            continue;
+          }
           double t[3][2];
           mode_vorticity(o, &o->u[0][2 * (base + k)], &o->u[1][2 * (base + k)],
                          &o->u[2][2 * (base + k)], wi, wj, wk, t);
           for (int c = 0; c < 3; c++) { om[2 * (c * nkk + k)] = t[c][0]; om[2 * (c * nkk + k) + 1] = t[c][1]; }
         }
-        for (int f = 0; f < 6; f++) {
+        for (int f = 0; f < o->nf; f++) {
           const double* src = (f < 3) ? &o->u[f][2 * base] : &om[2 * (f - 3) * nkk];
           /* zero-padded buffer: K+1 positive, NZ-2K-1 zeros, K negative */
           for (int z = 0; z < 2 * nz; z++) in[z] = 0.0;
@@ -560,6 +576,18 @@ static inline void nl_bfrot(const orc* o, const double* u, const double* om, dou
   n[0] = n1; n[1] = n2; n[2] = n3;
 }
 
+/* f_RK3ComputeNonlinearRConv, src/pf_rk3.f90:1717-1774: NL_i = sum_j sum_k
+ * bmat(k,j) du_i/dx_k u_j over the existing entries; g[3k + i] = du_i/dx_k */
+static void nl_rconv(const orc* o, const double* u, const double* g, double* n) {
+  for (int i = 1; i <= 3; i++) {
+    double a = 0.0;
+    for (int j = 1; j <= 3; j++)
+      for (int k = 1; k <= 3; k++)
+        if (T9(o->bval, k, j)) a = a + (T9(o->bmat, k, j) * g[3 * (k - 1) + (i - 1)] * u[j - 1]);
+    n[i - 1] = a;
+  }
+}
+
 /* ------------------------------------------------------------------------- */
 /* f_RK3RealSpace: per z-plane y/x inverse transforms, nonlinear term,       */
 /* forward x/y transforms with truncation.                                   */
@@ -572,7 +600,7 @@ static inline void nl_bfrot(const orc* o, const double* u, const double* om, dou
 static void plane_fields(const orc* o, int z, double* in, double* out, double* pc, double* r) {
   const int nx = o->nx, ny = o->ny, nz = o->nz, nki = o->nki, nkj = o->nkj, KJ = o->kj;
   const long plane = (long)nx * ny;
-  for (int f = 0; f < 6; f++) {
+  for (int f = 0; f < o->nf; f++) {
     /* y transforms for each i (src/pf_rk3.f90:1437-1498, pf_fftw.f90:673-759) */
     for (int i = 0; i < nki; i++) {
       for (int y = 0; y < 2 * ny; y++) in[y] = 0.0;
@@ -631,7 +659,7 @@ void orc_realspace(orc* o, int stage, int tstep, double* dt) {
     double* in = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
     double* out = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
     double* pc = (double*)malloc(sizeof(double) * 2 * (size_t)ny * nki);   /* ji plane */
-    double* r = (double*)malloc(sizeof(double) * 6 * (size_t)plane);       /* u1..u6_2r */
+    double* r = (double*)malloc(sizeof(double) * (size_t)o->nf * (size_t)plane);  /* u1..u6(12)_2r */
 #pragma omp for schedule(dynamic)
     for (int z = 0; z < nz; z++) {
       plane_fields(o, z, in, out, pc, r);
@@ -647,6 +675,16 @@ void orc_realspace(orc* o, int stage, int tstep, double* dt) {
         }
       }
       /* nonlinear term */
+      if (o->nf == 12) {
+        for (long q = 0; q < plane; q++) {
+          double uu[3] = {r[q], r[plane + q], r[2 * plane + q]};
+          double g[9];
+          for (int a = 0; a < 9; a++) g[a] = r[(3 + a) * plane + q];
+          double nn[3];
+          nl_rconv(o, uu, g, nn);
+          r[3 * plane + q] = nn[0]; r[4 * plane + q] = nn[1]; r[5 * plane + q] = nn[2];
+        }
+      } else
       for (long q = 0; q < plane; q++) {
         double uu[3] = {r[q], r[plane + q], r[2 * plane + q]};
         double ww[3] = {r[3 * plane + q], r[4 * plane + q], r[5 * plane + q]};
@@ -720,7 +758,7 @@ void orc_rs_raw(orc* o, double* raw) {
   double* in = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
   double* out = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
   double* pc = (double*)malloc(sizeof(double) * 2 * (size_t)ny * o->nki);
-  double* r = (double*)malloc(sizeof(double) * 6 * (size_t)plane);
+  double* r = (double*)malloc(sizeof(double) * (size_t)o->nf * (size_t)plane);
   for (int z = 0; z < nz; z++) {
     plane_fields(o, z, in, out, pc, r);
     for (long q = 0; q < plane; q++) {

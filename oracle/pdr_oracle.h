@@ -29,7 +29,7 @@ extern "C" {
 typedef struct orc_params {
   int32_t nodes_x, nodes_y, nodes_z;
   int32_t f_type;          /* 1 iso, 2 shear DNS, 3 shear RDT viscous, 4 shear RDT inviscid */
-  int32_t ns_deformed;     /* 1 BF_ROT, 3 R_ROT (the shear forms restated here) */
+  int32_t ns_deformed;     /* 1 BF_ROT, 2 R_CONV, 3 R_ROT */
   int32_t rk3_timestep;    /* 1 DECIDE, 2 FIXED, 3 CFL */
   double  rk3_dt;
   double  rk3_cfl;

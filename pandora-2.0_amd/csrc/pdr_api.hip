@@ -279,6 +279,8 @@ int pdr_destroy(pdr_ctx* c) {
   }
   if (c->w1) (void)hipFree(c->w1);
   if (c->w2) (void)hipFree(c->w2);
+  for (void* q : {c->w1b, c->w1c, c->w2b, c->w2c})
+    if (q) (void)hipFree(q);
   if (c->ur) (void)hipFree(c->ur);
   if (c->xp) (void)hipFree(c->xp);
   if (c->uprk) (void)hipFree(c->uprk);
@@ -326,9 +328,8 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     return fail(PDR_ERR_UNSUPPORTED, "grid sizes must be even, >= 8 and of the form 2^a 3^b 5^c");
   if (p->f_type < 1 || p->f_type > 4)
     return fail(PDR_ERR_UNSUPPORTED, "F_TYPE must be 1 (isotropic) or 2..4 (shear DNS/RDT)");
-  if (p->f_type != PDR_F_ISOTROPIC && p->ns_deformed != PDR_NS_BF_ROT && p->ns_deformed != PDR_NS_R_ROT)
-    return fail(PDR_ERR_UNSUPPORTED, "shear flows support NS_DEFORMED = 1 (BF_ROT) and 3 (R_ROT); "
-                                     "2 (R_CONV) is not built");
+  if (p->f_type != PDR_F_ISOTROPIC && (p->ns_deformed < PDR_NS_BF_ROT || p->ns_deformed > PDR_NS_R_ROT))
+    return fail(PDR_ERR_UNSUPPORTED, "NS_DEFORMED must be 1 (BF_ROT), 2 (R_CONV) or 3 (R_ROT)");
   if (p->dealiasing != 1) return fail(PDR_ERR_UNSUPPORTED, "DEALIASING must be 1 (2/3 rule by storage)");
   if (p->rk3_timestep < 1 || p->rk3_timestep > 3) return fail(PDR_ERR_ARG, "RK3_TIMESTEP must be 1, 2 or 3");
   if (p->rk3_cfl >= 1.0) return fail(PDR_ERR_ARG, "STOP: Courant number is greater than 1.0 (g_RK3Check)");
@@ -391,14 +392,15 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   c->tj_z = pick(nz, g.nkj);
   c->ti_y = pick(ny, g.nki);
   if (6 * size_t(nx + 1) * c->csz > c->lds_max || 2 * size_t(nz + 1) * c->csz > c->lds_max ||
-      2 * size_t(ny + 1) * c->csz > c->lds_max) {
+      2 * size_t(ny + 1) * c->csz > c->lds_max || (conv_form(c) && conv_xpass_lds(nx, c->csz) > c->lds_max)) {
     delete c;
     return fail(PDR_ERR_UNSUPPORTED, "grid too large for the LDS of this device");
   }
   {
     const char* env = std::getenv("PDR_GENERIC_KERNELS");
     const bool generic = env && env[0] && env[0] != '0';
-    c->spec = !generic && spec_supported(nx) && spec_supported(ny) && spec_supported(nz) &&
+    // NS_R_CONV runs on the generic kernels (its x pass takes twelve rows)
+    c->spec = !generic && !conv_form(c) && spec_supported(nx) && spec_supported(ny) && spec_supported(nz) &&
               spec_shape_ok(nx, ny, nz) &&
               spec_lds_need(c->prec, nx, ny, nz) <= c->lds_max;
   }
@@ -465,6 +467,12 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   if (rc == PDR_OK) rc = dalloc(c, &c->w1, 6 * size_t(g.m1) * c->csz);
   if (rc == PDR_OK) rc = dalloc(c, &c->w2, 6 * size_t(g.m2) * c->csz);
   if (rc == PDR_OK) c->nl = static_cast<char*>(c->w1) + 3 * size_t(g.m1) * c->csz;
+  if (rc == PDR_OK && conv_form(c)) {
+    for (void** q : {&c->w1b, &c->w1c})
+      if (rc == PDR_OK) rc = dalloc(c, q, 6 * size_t(g.m1) * c->csz);
+    for (void** q : {&c->w2b, &c->w2c})
+      if (rc == PDR_OK) rc = dalloc(c, q, 6 * size_t(g.m2) * c->csz);
+  }
   if (rc == PDR_OK) rc = dalloc(c, (void**)&c->umax, 256);
   const size_t fbytes = force_bytes(c);
   if (rc == PDR_OK && fbytes) {
@@ -642,13 +650,19 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   const bool need_cfl =
       stage == 1 && (c->p.rk3_timestep == PDR_RK3_TIMESTEP_CFL ||
                      (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE && c->p.f_type == PDR_F_ISOTROPIC));
-  HIPCHK(launch_zinv(c, cf));
-  if (need_cfl) HIPCHK(hipMemsetAsync(c->umax, 0, sizeof(unsigned long long), c->stream));
-  for (int z0 = 0; z0 < c->g.nz; z0 += c->g.zc) {
-    const int nzc = std::min(c->g.zc, c->g.nz - z0);
-    HIPCHK(launch_yinv(c, z0, nzc));
-    HIPCHK(launch_xpass(c, cf, tracers, need_cfl, z0, nzc, nullptr));
-    HIPCHK(launch_yfwd(c, z0, nzc));
+  if (conv_form(c)) {
+    if (need_cfl) HIPCHK(hipMemsetAsync(c->umax, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(launch_conv_front(c, cf, tracers, need_cfl, nullptr));
+    HIPCHK(launch_yfwd(c, 0, c->g.nz));
+  } else {
+    HIPCHK(launch_zinv(c, cf));
+    if (need_cfl) HIPCHK(hipMemsetAsync(c->umax, 0, sizeof(unsigned long long), c->stream));
+    for (int z0 = 0; z0 < c->g.nz; z0 += c->g.zc) {
+      const int nzc = std::min(c->g.zc, c->g.nz - z0);
+      HIPCHK(launch_yinv(c, z0, nzc));
+      HIPCHK(launch_xpass(c, cf, tracers, need_cfl, z0, nzc, nullptr));
+      HIPCHK(launch_yfwd(c, z0, nzc));
+    }
   }
   if (stage == 1) {
     // f_RK3Tstep (src/pf_rk3.f90:3852-3948)
@@ -891,7 +905,8 @@ int pdr_particles_download(pdr_ctx* c, double* xp, double* uprk) {
 
 int pdr_debug_prepare(pdr_ctx* c, int32_t field, double* out) {
   if (!c || !out || field < 0 || field > 5) return fail(PDR_ERR_ARG, "pdr_debug_prepare: bad argument");
-  const Coef cf = make_coef(c);
+  // NS_R_CONV: fields 3..5 are du/dx (the reference's u4..u6)
+  const Coef cf = conv_form(c) ? conv_coef(make_coef(c), 0) : make_coef(c);
   HIPCHK(launch_zinv(c, cf));
   const Geo& g = c->g;
   const size_t off = size_t(field) * size_t(g.m1) * c->csz;

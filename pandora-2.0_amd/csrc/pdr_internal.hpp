@@ -118,6 +118,7 @@ struct pdr_ctx {
   // work buffers
   void* w1 = nullptr;  // 6 fields x m1 (1r2w layout)
   void* w2 = nullptr;  // 6 fields x m2 (x-pencil layout, zc planes of z)
+  void *w1b = nullptr, *w1c = nullptr, *w2b = nullptr, *w2c = nullptr;  // NS_R_CONV: du/dy, du/dz sweeps
   void* nl = nullptr;  // NL [c][i][j][k] written by pass 5 (aliases w1 fields 3..5)
   int fpb_y = 6;       // fields per workgroup of the chunked inverse / forward y passes
   int fpb_yf = 3;
@@ -230,10 +231,18 @@ hipError_t launch2_yinv(pdr_ctx* c, int z0, int nzc);
 hipError_t launch2_xpass(pdr_ctx* c, const Coef& cf, bool tracers, bool cfl, int z0, int nzc, double* rsp);
 int xpass_tiles(const pdr_ctx* c);  // x-pass workgroups per z plane (specialised path)
 Coef make_coef(const pdr_ctx* c);
-// NS_R_ROT shear run (state = laboratory velocity on the sheared grid)
+// Rogallo shear run, NS_R_ROT or NS_R_CONV (state = laboratory velocity on the
+// sheared grid; the two share FormF, remesh, tracer and statistics forms)
 inline bool rogallo_form(const pdr_ctx* c) {
-  return c->p.f_type != PDR_F_ISOTROPIC && c->p.ns_deformed == PDR_NS_R_ROT;
+  return c->p.f_type != PDR_F_ISOTROPIC && (c->p.ns_deformed == PDR_NS_R_ROT || c->p.ns_deformed == PDR_NS_R_CONV);
 }
+inline bool conv_form(const pdr_ctx* c) {
+  return c->p.f_type != PDR_F_ISOTROPIC && c->p.ns_deformed == PDR_NS_R_CONV;
+}
+// LDS of the convective x pass: two buffers of 6 complex lines of NX + 1
+inline size_t conv_xpass_lds(int nx, size_t csz) { return 12 * size_t(nx + 1) * csz; }
+Coef conv_coef(const Coef& cf, int d);  // G selecting field 3 + c = d u_c / d x_d
+hipError_t launch_conv_front(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, double* rsp);
 hipError_t launch2_yfwd(pdr_ctx* c, int z0, int nzc);
 hipError_t launch2_zfwd(pdr_ctx* c);
 hipError_t launch2_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);

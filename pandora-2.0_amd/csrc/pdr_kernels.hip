@@ -250,6 +250,130 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // ---------------------------------------------------------------------------
+// pass 3, NS_R_CONV: x c2r of u and its nine gradients (12 real rows in 6
+// complex lines: wa holds u and du/dx, fields 3..5 of wb and wc hold du/dy and
+// du/dz), the convective product NL_i = sum_j sum_k bmat(k,j) du_i/dx_k u_j
+// (f_RK3ComputeNonlinearRConv, src/pf_rk3.f90:1717-1774), x r2c of NL into
+// fields 0..2 of wa.  The tracer field, the CFL maximum and the real-space
+// moments take u, binv u and fields 3..5 (du/dx), as the reference's
+// f_FluidstatsRs2D reads u4..u6 (src/pf_fluidstats.f90:846-917).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_xpass_conv(Geo g, Coef cf, cx<T>* __restrict__ wa, const cx<T>* __restrict__ wb,
+                 const cx<T>* __restrict__ wc, T* __restrict__ ur, int want_tr, int want_cfl,
+                 unsigned long long* __restrict__ umax, FftPlan px, const cx<T>* __restrict__ tw,
+                 double* __restrict__ rsp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ T red[kThreads];
+  const int nx = g.nx, KI = g.ki, nki = g.nki;
+  const int ld = nx + 1;
+  cx<T>* A = reinterpret_cast<cx<T>*>(smem);
+  cx<T>* B = A + 6 * ld;
+  const int y = blockIdx.x;
+  const int z = blockIdx.y;
+  const long long row = ((long long)z * g.ny + y) * g.ldi;
+  auto field = [&](int q) -> const cx<T>* {  // real row q of the 12
+    return (q < 6 ? wa + (long long)q * g.m2 : (q < 9 ? wb + (long long)(q - 3) * g.m2 : wc + (long long)(q - 6) * g.m2)) +
+           row;
+  };
+  for (int idx = threadIdx.x; idx < 6 * nx; idx += blockDim.x) {
+    const int p = idx / nx;
+    const int m = idx - p * nx;
+    cx<T> X1 = zero_c<T>(), X2 = zero_c<T>();
+    const cx<T>* fa = field(2 * p);
+    const cx<T>* fb = field(2 * p + 1);
+    if (m <= KI) {
+      X1 = fa[m];
+      X2 = fb[m];
+      if (m == 0) {  // c2r uses only the real part of the DC term
+        X1.y = T(0);
+        X2.y = T(0);
+      }
+    } else if (m >= nx - KI) {
+      X1 = conj_c(fa[nx - m]);
+      X2 = conj_c(fb[nx - m]);
+    }
+    A[p * ld + m] = mk<T>(X1.x - X2.y, X1.y + X2.x);
+  }
+  __syncthreads();
+  cx<T>* R = fft_lines<T, +1>(A, B, px, 6, ld, tw);
+  cx<T>* O = (R == A) ? B : A;
+  T lmax = T(0);
+  RsAcc acc;
+  if (rsp) rs_init(acc);
+  const T V[9] = {T(cf.V[0]), T(cf.V[1]), T(cf.V[2]), T(cf.V[3]), T(cf.V[4]),
+                  T(cf.V[5]), T(cf.V[6]), T(cf.V[7]), T(cf.V[8])};
+  const T Bk[9] = {T(cf.Bm[0]), T(cf.Bm[1]), T(cf.Bm[2]), T(cf.Bm[3]), T(cf.Bm[4]),
+                   T(cf.Bm[5]), T(cf.Bm[6]), T(cf.Bm[7]), T(cf.Bm[8])};  // [k][j]
+  const long long rr = ((long long)z * g.ny + y) * nx;
+  for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+    T f[12];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+      const cx<T> v = R[p * ld + x];
+      f[2 * p] = v.x;
+      f[2 * p + 1] = v.y;
+    }
+    const T u[3] = {f[0], f[1], f[2]};
+    T nl[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      T a = T(0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a = a + (Bk[3 * k + j] * f[3 + 3 * k + i] * u[j]);
+      nl[i] = a;
+    }
+    const T v1 = V[0] * u[0] + V[1] * u[1] + V[2] * u[2];
+    const T v2 = V[3] * u[0] + V[4] * u[1] + V[5] * u[2];
+    const T v3 = V[6] * u[0] + V[7] * u[1] + V[8] * u[2];
+    if (want_tr) {
+      T* d = ur + (rr + x) * 3;
+      d[0] = u[0];
+      d[1] = u[1];
+      d[2] = u[2];
+    }
+    if (want_cfl) {
+      const T sm = fabs(v1) + fabs(v2) + fabs(v3);
+      lmax = sm > lmax ? sm : lmax;
+    }
+    if (rsp) rs_add(acc, double(v1), double(v2), double(v3), double(f[3]), double(f[4]), double(f[5]));
+    O[x] = mk<T>(nl[0], nl[1]);
+    O[ld + x] = mk<T>(nl[2], T(0));
+  }
+  __syncthreads();
+  cx<T>* Q = fft_lines<T, -1>(O, R, px, 2, ld, tw);
+  cx<T>* d0 = wa + row;
+  cx<T>* d1 = wa + g.m2 + row;
+  cx<T>* d2 = wa + 2 * g.m2 + row;
+  for (int m = threadIdx.x; m < nki; m += blockDim.x) {
+    const int mr = (m == 0) ? 0 : nx - m;
+    const cx<T> Z = Q[m], Zc = conj_c(Q[mr]);
+    const cx<T> s = add(Z, Zc), df = sub(Z, Zc);
+    d0[m] = mk<T>(T(0.5) * s.x, T(0.5) * s.y);
+    d1[m] = mk<T>(T(0.5) * df.y, T(-0.5) * df.x);
+    const cx<T> W = Q[ld + m], Wc = conj_c(Q[ld + mr]);
+    const cx<T> s3 = add(W, Wc);
+    d2[m] = mk<T>(T(0.5) * s3.x, T(0.5) * s3.y);
+  }
+  if (want_cfl) {
+    red[threadIdx.x] = lmax;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + s] ? red[threadIdx.x] : red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(umax, (unsigned long long)__double_as_longlong(double(red[0])));
+  }
+  if (rsp) {
+    __shared__ double rs_lds[kRsWidth * (kThreads / 64)];
+    rs_block_out(acc, rs_lds, rsp + ((long long)z * g.ny + y) * kRsWidth);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // pass 4: forward y FFT + truncation, x-pencils -> 1r2w (NL fields 0..2)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -661,7 +785,7 @@ static hipError_t set_lds(const void* fn, size_t bytes) {
 }
 
 template <typename T>
-static hipError_t zinv_t(pdr_ctx* c, const Coef& cf, int fsel) {
+static hipError_t zinv_t(pdr_ctx* c, const Coef& cf, int fsel, void* w1) {
   const Geo& g = c->g;
   const int tj = c->tj_z;
   const size_t lds = 2 * size_t(tj) * (g.nz + 1) * sizeof(cx<T>);
@@ -669,27 +793,29 @@ static hipError_t zinv_t(pdr_ctx* c, const Coef& cf, int fsel) {
   if (e != hipSuccess) return e;
   dim3 grid((g.nkj + tj - 1) / tj, g.nki);
   k_zinv<T><<<grid, kThreads, lds, c->stream>>>(g, cf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1],
-                                                (const cx<T>*)c->u[2], (cx<T>*)c->w1, c->pz,
+                                                (const cx<T>*)c->u[2], (cx<T>*)w1, c->pz,
                                                 (const cx<T>*)c->tab.twz, tj, fsel);
   return hipGetLastError();
 }
 
 hipError_t launch_zinv(pdr_ctx* c, const Coef& cf) {
   prof_begin(c, K_ZINV);
-  hipError_t e = c->spec ? launch2_zinv(c, cf) : c->prec ? zinv_t<float>(c, cf, -1) : zinv_t<double>(c, cf, -1);
+  hipError_t e = c->spec   ? launch2_zinv(c, cf)
+                 : c->prec ? zinv_t<float>(c, cf, -1, c->w1)
+                           : zinv_t<double>(c, cf, -1, c->w1);
   prof_end(c, K_ZINV);
   return e;
 }
 
 template <typename T>
-static hipError_t yinv_t(pdr_ctx* c) {
+static hipError_t yinv_t(pdr_ctx* c, const void* w1, void* w2) {
   const Geo& g = c->g;
   const int ti = c->ti_y;
   const size_t lds = 2 * size_t(ti) * (g.ny + 1) * sizeof(cx<T>);
   hipError_t e = set_lds<T>((const void*)k_yinv<T>, lds);
   if (e != hipSuccess) return e;
   dim3 grid((g.nki + ti - 1) / ti, g.nz);
-  k_yinv<T><<<grid, kThreads, lds, c->stream>>>(g, (const cx<T>*)c->w1, (cx<T>*)c->w2, c->py,
+  k_yinv<T><<<grid, kThreads, lds, c->stream>>>(g, (const cx<T>*)w1, (cx<T>*)w2, c->py,
                                                 (const cx<T>*)c->tab.twy, ti);
   return hipGetLastError();
 }
@@ -698,7 +824,9 @@ static hipError_t yinv_t(pdr_ctx* c) {
 // for them: zc = NZ); the specialised ones take a chunk of z planes.
 hipError_t launch_yinv(pdr_ctx* c, int z0, int nzc) {
   prof_begin(c, K_YINV);
-  hipError_t e = c->spec ? launch2_yinv(c, z0, nzc) : c->prec ? yinv_t<float>(c) : yinv_t<double>(c);
+  hipError_t e = c->spec   ? launch2_yinv(c, z0, nzc)
+                 : c->prec ? yinv_t<float>(c, c->w1, c->w2)
+                           : yinv_t<double>(c, c->w1, c->w2);
   prof_end(c, K_YINV);
   return e;
 }
@@ -713,6 +841,51 @@ static hipError_t xpass_t(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, double*
   k_xpass<T><<<grid, kThreads, lds, c->stream>>>(g, cf, (cx<T>*)c->w2, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0,
                                                  c->umax, c->px, (const cx<T>*)c->tab.twx, rsp);
   return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t xpass_conv_t(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, double* rsp) {
+  const Geo& g = c->g;
+  const size_t lds = conv_xpass_lds(g.nx, c->csz);
+  hipError_t e = set_lds<T>((const void*)k_xpass_conv<T>, lds);
+  if (e != hipSuccess) return e;
+  dim3 grid(g.ny, g.nz);
+  k_xpass_conv<T><<<grid, kThreads, lds, c->stream>>>(g, cf, (cx<T>*)c->w2, (const cx<T>*)c->w2b,
+                                                      (const cx<T>*)c->w2c, (T*)c->ur, tr ? 1 : 0, cfl ? 1 : 0,
+                                                      c->umax, c->px, (const cx<T>*)c->tab.twx, rsp);
+  return hipGetLastError();
+}
+
+Coef conv_coef(const Coef& cf, int d) {
+  Coef r = cf;
+  for (int q = 0; q < 27; ++q) r.G[q] = 0.0;
+  for (int cc = 0; cc < 3; ++cc) r.G[(cc * 3 + cc) * 3 + d] = 1.0;  // field 3 + c = i k_d u_c
+  return r;
+}
+
+// NS_R_CONV front of a stage (generic kernels, whole box): three inverse z/y
+// sweeps give u with du/dx, du/dy and du/dz (fields 3..5 of w, wb, wc), then
+// the convective x pass leaves NL in fields 0..2 of the x-pencil buffer.
+hipError_t launch_conv_front(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, double* rsp) {
+  void* w1s[3] = {c->w1, c->w1b, c->w1c};
+  void* w2s[3] = {c->w2, c->w2b, c->w2c};
+  hipError_t e = hipSuccess;
+  prof_begin(c, K_ZINV);
+  for (int d = 0; d < 3 && e == hipSuccess; ++d) {
+    const Coef cd = conv_coef(cf, d);
+    e = c->prec ? zinv_t<float>(c, cd, -1, w1s[d]) : zinv_t<double>(c, cd, -1, w1s[d]);
+  }
+  prof_end(c, K_ZINV);
+  if (e != hipSuccess) return e;
+  prof_begin(c, K_YINV);
+  for (int d = 0; d < 3 && e == hipSuccess; ++d)
+    e = c->prec ? yinv_t<float>(c, w1s[d], w2s[d]) : yinv_t<double>(c, w1s[d], w2s[d]);
+  prof_end(c, K_YINV);
+  if (e != hipSuccess) return e;
+  prof_begin(c, K_XPASS);
+  e = c->prec ? xpass_conv_t<float>(c, cf, tr, cfl, rsp) : xpass_conv_t<double>(c, cf, tr, cfl, rsp);
+  prof_end(c, K_XPASS);
+  return e;
 }
 
 hipError_t launch_xpass(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, int z0, int nzc, double* rsp) {

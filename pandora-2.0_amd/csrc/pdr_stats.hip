@@ -446,11 +446,16 @@ int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out) {
   // the real-space planes of the current state: pass 1, then the y and x
   // passes with the moments accumulated in the x pass (its outputs are scratch)
   const pdr::Coef cf = pdr::make_coef(c);
-  hipError_t e = pdr::launch_zinv(c, cf);
-  for (int z0 = 0; e == hipSuccess && z0 < g.nz; z0 += g.zc) {
-    const int nzc = std::min(g.zc, g.nz - z0);
-    e = pdr::launch_yinv(c, z0, nzc);
-    if (e == hipSuccess) e = pdr::launch_xpass(c, cf, false, false, z0, nzc, part);
+  hipError_t e = hipSuccess;
+  if (pdr::conv_form(c)) {
+    e = pdr::launch_conv_front(c, cf, false, false, part);
+  } else {
+    e = pdr::launch_zinv(c, cf);
+    for (int z0 = 0; e == hipSuccess && z0 < g.nz; z0 += g.zc) {
+      const int nzc = std::min(g.zc, g.nz - z0);
+      e = pdr::launch_yinv(c, z0, nzc);
+      if (e == hipSuccess) e = pdr::launch_xpass(c, cf, false, false, z0, nzc, part);
+    }
   }
   if (e == hipSuccess) {
     pdr::k_rs_reduce<<<dim3(unsigned(pdr::kRsWidth)), pdr::kStatThreads, 0, c->stream>>>(part, nblk, dres);

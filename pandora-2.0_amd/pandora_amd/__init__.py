@@ -27,7 +27,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "pandora_amd.h"
 PDR_FP64 = 0
 PDR_FP32_MIXED = 1
 F_ISOTROPIC, F_SHEAR_DNS, F_SHEAR_RDT_VISCOUS, F_SHEAR_RDT_INVISCID = 1, 2, 3, 4
-NS_BF_ROT, NS_R_CONV, NS_R_ROT = 1, 2, 3  # NS_R_CONV is not built (PdrError)
+NS_BF_ROT, NS_R_CONV, NS_R_ROT = 1, 2, 3
 RK3_TIMESTEP_DECIDE, RK3_TIMESTEP_FIXED, RK3_TIMESTEP_CFL = 1, 2, 3
 ERRORS = {0: "PDR_OK", -1: "PDR_ERR_ARG", -2: "PDR_ERR_HIP", -3: "PDR_ERR_NOMEM",
           -4: "PDR_ERR_UNSUPPORTED", -5: "PDR_ERR_IO", -6: "PDR_ERR_STATE", -7: "PDR_ERR_NONFINITE"}

@@ -40,7 +40,7 @@ def _params(**kw):
     (dict(nodes_x=28, nodes_y=28, nodes_z=28), -4),        # 7 is not a 2/3/5 radix
     (dict(nodes_x=15), -4),                               # odd x length (real transforms)
     (dict(f_type=5), -4),                                 # isotropic deformed grid
-    (dict(f_type=2, ns_deformed=2), -4),                  # R_CONV (convective Rogallo form)
+    (dict(f_type=2, ns_deformed=4), -4),                  # no such NS_DEFORMED form
     (dict(dealiasing=2), -4),
     (dict(rk3_cfl=1.0), -1),                              # g_RK3Check abort condition
     (dict(force_stop=10, force_tf=0.0, force_kf=2.5), -1),  # OU time scale must be > 0

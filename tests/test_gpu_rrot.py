@@ -1,10 +1,11 @@
-"""GPU parity of the NS_R_ROT shear form (Rogallo rotational form: sheared-direction
+"""GPU parity of the Rogallo shear forms.  NS_R_ROT (rotational form: sheared-direction
 vorticity, the isotropic omega x u of the unmapped velocity, f_RK3FormFRRot and
 f_RK3ContinuityFixRRot, src/pf_rk3.f90:416-421, 1088-1089, 2680-2856, 2945-3249,
 3620-3693), its tracers (p_RK3SolveParticleR / p_RK3AbsoluteVelocityR,
 src/pp_rk3.f90:307-403, 720-773) and statistics (f_FluidstatsFsPrimaryQuantsRRot,
-src/pf_fluidstats.f90:1801-1970), through the C-ABI against the FP64 oracle on the
-same inputs.  Criteria as for BF_ROT: FP64 norm-wise <= 1e-12 after 10 steps, the
+src/pf_fluidstats.f90:1801-1970); NS_R_CONV (convective form: nine raw-wavenumber
+gradients, NL_i = sum bmat(k,j) du_i/dx_k u_j, src/pf_rk3.f90:422-427, 1717-1774,
+then the same FormF).  Through the C-ABI against the FP64 oracle on the same inputs.  Criteria as for BF_ROT: FP64 norm-wise <= 1e-12 after 10 steps, the
 frame and the remesh bit-exact, FP32-mixed E(k) within 1e-4 of FP64."""
 import sys
 
@@ -186,7 +187,94 @@ def test_rrot_tracers_and_statistics_fp64(oracle_mod):
     g.close()
 
 
-def test_rconv_form_is_rejected():
-    note("NS_R_CONV rejected")
+# ---- NS_R_CONV: nine gradients through three inverse sweeps, generic kernels ----
+
+CONV = dict(SHEAR, ns_deformed=2)
+
+
+def test_rconv_pass1_gradients_match_oracle(oracle_mod):
+    note("R_CONV pass 1 (u and du/dx + inverse z FFT) N=32")
+    n = 32
+    o, g = _pair(oracle_mod, n, **CONV)
+    u = pa.init_field_host(n, 22, 2.0 / 3.0, 4.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    o.prepare_realspace()
+    for f in range(6):
+        assert rel(g.debug_prepare(f), o.get_1r2w(f)) < 1e-13, f
+    g.close()
+
+
+@pytest.mark.parametrize("n", [32, 48])
+def test_rconv_step_parity_fp64_with_remesh(oracle_mod, n):
+    """N = 32 is alias-free (the oracle's R_CONV equals its R_ROT there); N = 48 =
+    3K aliases, where the convective form takes its own path."""
+    note(f"R_CONV shear DNS FP64 N={n}, 10 steps with remesh at 4 and 10")
+    o, g = _pair(oracle_mod, n, **CONV)
+    u = pa.init_field_host(n, 600 + n, 2.0 / 3.0, 6.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    _run(o, g, 10)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert rel(a, b) < 1e-12
+    g.close()
+
+
+def test_rconv_fp32_mixed_spectrum_against_fp64(oracle_mod):
+    note("R_CONV FP32-mixed N=64 shear, 10 steps")
+    n = 64
+    o, g = _pair(oracle_mod, n, precision=pa.PDR_FP32_MIXED, **CONV)
+    u = pa.init_field_host(n, 79, 2.0 / 3.0, 6.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    _run(o, g, 10)
+    eo, _, _, tot_o = o.spectrum()
+    eg, _, _, tot_g = g.spectrum()
+    live = eo > 1e-6 * eo.max()
+    assert np.abs(eg[live] - eo[live]).max() / eo[live].max() < 1e-4
+    assert abs(tot_g[0] - tot_o[0]) / tot_o[0] < 1e-5
+    g.close()
+
+
+def test_rconv_tracers_and_statistics_fp64(oracle_mod):
+    note("R_CONV tracers + tracer statistics FP64 N=32, 1000 tracers, 6 steps, then statistics")
+    n = 32
+    o, g = _pair(oracle_mod, n, **CONV)
+    u = pa.init_field_host(n, 9, 2.0 / 3.0, 6.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    dx = TWOPI / n
+    xp = np.random.default_rng(10).uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(1000, 3))
+    o.set_particles(xp)
+    g.set_particles(xp)
+    o.particle_stats_config(True, keep_initial=True)
+    g.particle_stats_config(True, keep_initial=True)
+    to = tg = 0.0
+    for s in range(1, 7):
+        to = o.step(s, to)
+        tg = g.rk3_calc(s, tg)
+        (t_o, ro), (t_g, rg) = o.particle_stats(), g.particle_stats()
+        assert t_g == t_o
+        if t_o >= 0:
+            for k, v in ro.items():
+                assert rg[k] == pytest.approx(v, rel=1e-9, abs=1e-13), (s, k)
+    xo, _ = o.get_particles()
+    xg, _ = g.get_particles()
+    d = np.abs(xg - xo)
+    assert np.minimum(d, TWOPI - d).max() < 1e-11
+    o.set_state(*g.get_state())
+    o.set_frame(*g.get_frame())
+    sg, qg = g.stats()
+    so, qo = o.stats()
+    for k, v in qo.items():
+        assert qg[k] == pytest.approx(v, rel=1e-12, abs=1e-300), k
+    rg_, ro_ = g.rs_stats(), o.rs_stats()
+    for k, v in ro_.items():
+        assert rg_[k] == pytest.approx(v, rel=1e-11, abs=1e-14), k
+    g.close()
+
+
+def test_unknown_form_is_rejected():
+    note("NS_DEFORMED = 4 rejected")
     with pytest.raises(pa.PdrError):
-        pa.Pandora(32, f_type=2, ns_deformed=pa.NS_R_CONV, rk3_timestep=1, deform_s=1.0, remesh_t=5)
+        pa.Pandora(32, f_type=2, ns_deformed=4, rk3_timestep=1, deform_s=1.0, remesh_t=5)

@@ -402,9 +402,36 @@ def test_rrot_remesh_moves_modes_without_adding_u3(oracle_mod):
                 assert np.array_equal(v[c][i, :, k], u[c][i, :, ko])
 
 
-def test_rconv_form_is_not_restated(oracle_mod):
-    with pytest.raises(ValueError):
-        oracle_mod.Oracle(16, f_type=2, ns_deformed=2, rk3_timestep=1, remesh_t=5, deform_s=1.0)
+def _rconv_vs_rrot(oracle_mod, n):
+    import pandora_amd as pa
+    kw = dict(f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=3, nu=0.005)
+    u = pa.init_field_host(n, 5, 2.0 / 3.0, 6.0)
+    a = oracle_mod.Oracle(n, ns_deformed=3, **kw)
+    b = oracle_mod.Oracle(n, ns_deformed=2, **kw)
+    a.set_state(*u)
+    b.set_state(*u)
+    t1 = t2 = 0.0
+    for s in range(1, 11):  # remeshes at steps 4 and 10
+        t1 = a.step(s, t1)
+        t2 = b.step(s, t2)
+    return max(rel(x, y) for x, y in zip(b.get_state(), a.get_state()))
+
+
+@pytest.mark.parametrize("n", [32, 40])
+def test_rconv_equals_rrot_when_alias_free(oracle_mod, n):
+    """R_CONV's NL_i = sum_jk bmat(k,j) du_i/dx_k u_j (src/pf_rk3.f90:1717-1774,
+    from nine raw-wavenumber gradients) is R_ROT's omega' x u plus the sheared
+    gradient of |u|^2/2.  Without aliasing (N > 3K, so products of retained
+    modes never fold back onto them) the truncated difference is exactly a
+    gradient, which the R_ROT pressure projects out: both forms give the same
+    state to rounding, through two remeshes."""
+    assert _rconv_vs_rrot(oracle_mod, n) < 1e-14
+
+
+def test_rconv_differs_from_rrot_when_products_alias(oracle_mod):
+    """N = 48 = 3K: the product modes +-2K fold onto -+K, the two forms of the
+    nonlinear term alias differently and the states part (as in the reference)."""
+    assert _rconv_vs_rrot(oracle_mod, 48) > 1e-6
 
 
 def test_remesh_map_enumeration(oracle_mod):
