@@ -21,7 +21,7 @@ module pdr_rk3_amd
   type, bind(C), public :: pdr_params_t
     integer(c_int32_t) :: nodes_x = 0, nodes_y = 0, nodes_z = 0
     integer(c_int32_t) :: f_type = 1
-    integer(c_int32_t) :: ns_deformed = 1
+    integer(c_int32_t) :: ns_deformed = 1   ! 1 BF_ROT, 2 R_CONV, 3 R_ROT
     integer(c_int32_t) :: dealiasing = 1
     integer(c_int32_t) :: rk3_timestep = 2
     integer(c_int32_t) :: f_remesh_t = 0
