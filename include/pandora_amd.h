@@ -51,6 +51,9 @@ extern "C" {
 #define PDR_NS_BF_ROT             1
 #define PDR_NS_R_CONV             2
 #define PDR_NS_R_ROT              3
+#define PDR_DEALIASING_TWOTHIRD   1
+#define PDR_DEALIASING_SPHERICAL  2
+#define PDR_DEALIASING_LES        3
 #define PDR_RK3_TIMESTEP_DECIDE   1
 #define PDR_RK3_TIMESTEP_FIXED    2
 #define PDR_RK3_TIMESTEP_CFL      3
@@ -64,7 +67,9 @@ typedef struct pdr_params {
   int32_t nodes_x, nodes_y, nodes_z; /* NODES_X/Y/Z: 2^a * {1,3,5} (radix 2/3/5 FFTs)  */
   int32_t f_type;                    /* F_TYPE: 1 isotropic, 2 shear DNS, 3/4 shear RDT */
   int32_t ns_deformed;               /* NS_DEFORMED: 1 BF_ROT, 2 R_CONV, 3 R_ROT        */
-  int32_t dealiasing;                /* DEALIASING: 1 = cubic 2/3 rule by storage       */
+  int32_t dealiasing;                /* DEALIASING: 1 cubic 2/3 rule by storage; 2 also
+                                        spherical |k_lab| <= KMAX; 3 LES |k_lab| <=
+                                        LES_CUTOFF KMAX (KMAX = min(KI, KJ, KK))         */
   int32_t rk3_timestep;              /* RK3_TIMESTEP: 1 DECIDE, 2 FIXED, 3 CFL          */
   int32_t f_remesh_t;                /* F_REMESH_T (shear, DECIDE)                      */
   int32_t b110;                      /* B110 (box stretch in x1; 1 for all configs)     */
@@ -79,6 +84,7 @@ typedef struct pdr_params {
   double  force_kf;                  /* FORCE_KF: forced modes 0 < |k| <= FORCE_KF      */
   double  force_sigma;               /* FORCE_SIGMA: OU amplitude                       */
   int64_t force_seed;                /* seed of the forcing draws (counter-based RNG)   */
+  double  les_cutoff;                /* LES_CUTOFF (DEALIASING = 3; reference default 1) */
 } pdr_params;
 
 /* ---- library / context -------------------------------------------------- */
@@ -153,6 +159,15 @@ int pdr_stage(pdr_ctx* ctx, int32_t stage, int32_t tstep, double time, double* d
  * p_ControlRemesh (src/pp_control.f90:1658-1743).  pdr_stage(1, ...) calls it
  * first; *done (may be NULL) is set to 1 if this tstep remeshed. */
 int pdr_remesh(pdr_ctx* ctx, int32_t tstep, int32_t* done);
+
+/* Spherical / LES truncation of the state: f_ArraysDealiasSpherical and
+ * f_ArraysLESCutoff (src/pf_arrays.f90:1865-1987): u1..u3 of every mode whose
+ * laboratory wavenumber |bmat^T k| exceeds KMAX (DEALIASING = 2) or
+ * LES_CUTOFF * KMAX (DEALIASING = 3) are set to zero; no-op for DEALIASING = 1.
+ * pdr_stage applies it where f_RK3PrepareRealSpace and f_RK3SolveNavierStokes
+ * do (src/pf_rk3.f90:269-277, 2101-2109), pdr_init_field and pdr_restart_read
+ * where f_InitialiseInit does (src/pf_initialise.f90:75-83). */
+int pdr_dealias(pdr_ctx* ctx);
 
 /* Shear-frame update after a stage: g_RK3UpdateB(tstep, dt, simtime)
  * (src/pg_rk3.f90:280-385).  FP64 on the host. */

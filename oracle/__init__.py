@@ -28,7 +28,8 @@ class OrcParams(C.Structure):
         ("f_deform_s", C.c_double), ("f_remesh_t", C.c_int32), ("b110", C.c_int32),
         ("force_tf", C.c_double), ("p_interp_order", C.c_int32), ("nthreads", C.c_int32),
         ("force_stop", C.c_int32), ("rrot_p3_two", C.c_int32), ("force_kf", C.c_double),
-        ("force_sigma", C.c_double), ("force_seed", C.c_int64),
+        ("force_sigma", C.c_double), ("force_seed", C.c_int64), ("dealiasing", C.c_int32),
+        ("pad1", C.c_int32), ("les_cutoff", C.c_double),
     ]
 
 
@@ -66,6 +67,8 @@ def lib():
         L.orc_get_1r2w.argtypes = [vp, C.c_int, dp]
         L.orc_remesh.restype = C.c_int
         L.orc_remesh.argtypes = [vp, C.c_int]
+        L.orc_dealias.restype = None
+        L.orc_dealias.argtypes = [vp]
         L.orc_remesh_target.restype = C.c_int
         L.orc_remesh_target.argtypes = [C.c_int, C.c_int, C.c_int]
         L.orc_set_particles.argtypes = [vp, dp, C.c_int64]
@@ -155,13 +158,14 @@ class Oracle:
     def __init__(self, n=None, *, nx=None, ny=None, nz=None, f_type=1, ns_deformed=1,
                  rk3_timestep=2, rk3_dt=0.005, rk3_cfl=0.75, nu=0.01, deform_s=0.0,
                  remesh_t=0, b110=1, force_tf=0.0, interp_order=4, nthreads=0, force_stop=0,
-                 force_kf=0.0, force_sigma=0.0, force_seed=0, rrot_p3_two=0):
+                 force_kf=0.0, force_sigma=0.0, force_seed=0, rrot_p3_two=0, dealiasing=1, les_cutoff=1.0):
         nx = nx or n
         ny = ny or n
         nz = nz or n
         self.p = OrcParams(nx, ny, nz, f_type, ns_deformed, rk3_timestep, rk3_dt, rk3_cfl,
                            nu, deform_s, remesh_t, b110, force_tf, interp_order, nthreads,
-                           force_stop, rrot_p3_two, force_kf, force_sigma, force_seed)
+                           force_stop, rrot_p3_two, force_kf, force_sigma, force_seed, dealiasing, 0,
+                           les_cutoff)
         self.h = lib().orc_create(C.byref(self.p))
         if not self.h:
             raise ValueError(f"oracle: NS_DEFORMED={ns_deformed} is not restated")
@@ -319,6 +323,9 @@ class Oracle:
 
     def remesh(self, tstep: int) -> bool:
         return bool(lib().orc_remesh(self.h, tstep))
+
+    def dealias(self):
+        lib().orc_dealias(self.h)
 
     # ---- tracers -----------------------------------------------------------
     def set_particles(self, xp):

@@ -1578,6 +1578,31 @@ static void pstats_relvel(orc* o, int tstep) {
   o->pst_tstep = tstep;
 }
 
+/* f_ArraysDealiasSpherical / f_ArraysLESCutoff (src/pf_arrays.f90:1865-1987):
+ * zero u1..u3 where |k_lab| > cutoff, k_lab(n) = sum_m bmat(m,n) k(m) over the
+ * existing entries, cutoff = KMAX = min(KI, KJ, KK) (src/pg_domain.f90:231) or
+ * LES_CUTOFF * KMAX */
+void orc_dealias(orc* o) {
+  if (o->p.dealiasing != 2 && o->p.dealiasing != 3) return;
+  const int km_i = o->ki < o->kj ? (o->ki < o->kk ? o->ki : o->kk) : (o->kj < o->kk ? o->kj : o->kk);
+  const double kmax = (double)km_i;
+  const double cutoff = (o->p.dealiasing == 3) ? o->p.les_cutoff * kmax : kmax;
+  for (int i = 0; i < o->nki; i++)
+    for (int j = 0; j < o->nkj; j++)
+      for (int k = 0; k < o->nkk; k++) {
+        const double kv[3] = {wvn(i, o->ki), wvn(j, o->kj), wvn(k, o->kk)};
+        double kl[3] = {0.0, 0.0, 0.0};
+        for (int m = 1; m <= 3; m++)
+          for (int n = 1; n <= 3; n++)
+            if (T9(o->bval, m, n)) kl[n - 1] = kl[n - 1] + T9(o->bmat, m, n) * kv[m - 1];
+        const double kmag = sqrt((kl[0] * kl[0]) + (kl[1] * kl[1]) + (kl[2] * kl[2]));
+        if (kmag > cutoff) {
+          const long e = (((long)i * o->nkj + j) * o->nkk + k) * 2;
+          for (int c = 0; c < 3; c++) o->u[c][e] = o->u[c][e + 1] = 0.0;
+        }
+      }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Stage / step drivers                                                       */
 /* ------------------------------------------------------------------------- */
@@ -1588,6 +1613,7 @@ void orc_stage(orc* o, int stage, int tstep, double time, double* dt) {
    * it (src/pf_rk3.f90:62-99), when tstep > P_TS_RELEASE */
   const int pst = stage == 1 && o->pst_on && o->np > 0 && tstep > o->ts_release;
   if (pst) pstats_primary(o, *dt);
+  orc_dealias(o);  /* entry of f_RK3PrepareRealSpace, src/pf_rk3.f90:269-277 */
   orc_prepare_realspace(o);
   orc_realspace(o, stage, tstep, dt);
   /* particles move from tstep = P_TS_RELEASE on (src/pf_rk3.f90:1146); at the
@@ -1603,6 +1629,7 @@ void orc_stage(orc* o, int stage, int tstep, double time, double* dt) {
   if (pst) pstats_relvel(o, tstep);
   orc_force_calc(o, tstep, *dt);  /* src/pf_rk3.f90:2063 */
   solve_ns(o, *dt, stage);
+  orc_dealias(o);  /* end of f_RK3SolveNavierStokes, src/pf_rk3.f90:2101-2109 */
 }
 
 /* g_RK3UpdateB, src/pg_rk3.f90:280-385 */

@@ -46,6 +46,9 @@ typedef struct orc_params {
   double  force_kf;        /* FORCE_KF */
   double  force_sigma;     /* FORCE_SIGMA */
   int64_t force_seed;      /* seed of the counter-based forcing draws */
+  int32_t dealiasing;      /* DEALIASING: 1 two-thirds (storage), 2 spherical, 3 LES */
+  int32_t pad1;
+  double  les_cutoff;      /* LES_CUTOFF */
 } orc_params;
 
 typedef struct orc orc;
@@ -87,6 +90,7 @@ double orc_get_ucfl(const orc* o);
 
 /* Remesh (src/pf_rk3.f90:3958-4178); returns 1 if a remesh happened. */
 int    orc_remesh(orc* o, int tstep);
+void   orc_dealias(orc* o);                                   /* spherical / LES truncation */
 int    orc_remesh_target(int kk_max, int i, int k);            /* new k index or -1 */
 
 /* Tracers (src/pp_control.f90:518-580, src/pp_rk3.f90:57-303, src/pp_mpi.f90:1011-1092) */

@@ -330,7 +330,10 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     return fail(PDR_ERR_UNSUPPORTED, "F_TYPE must be 1 (isotropic) or 2..4 (shear DNS/RDT)");
   if (p->f_type != PDR_F_ISOTROPIC && (p->ns_deformed < PDR_NS_BF_ROT || p->ns_deformed > PDR_NS_R_ROT))
     return fail(PDR_ERR_UNSUPPORTED, "NS_DEFORMED must be 1 (BF_ROT), 2 (R_CONV) or 3 (R_ROT)");
-  if (p->dealiasing != 1) return fail(PDR_ERR_UNSUPPORTED, "DEALIASING must be 1 (2/3 rule by storage)");
+  if (p->dealiasing < PDR_DEALIASING_TWOTHIRD || p->dealiasing > PDR_DEALIASING_LES)
+    return fail(PDR_ERR_UNSUPPORTED, "DEALIASING must be 1 (2/3 rule), 2 (spherical) or 3 (LES)");
+  if (p->dealiasing == PDR_DEALIASING_LES && !(p->les_cutoff > 0.0))
+    return fail(PDR_ERR_ARG, "DEALIASING = 3 needs LES_CUTOFF > 0");
   if (p->rk3_timestep < 1 || p->rk3_timestep > 3) return fail(PDR_ERR_ARG, "RK3_TIMESTEP must be 1, 2 or 3");
   if (p->rk3_cfl >= 1.0) return fail(PDR_ERR_ARG, "STOP: Courant number is greater than 1.0 (g_RK3Check)");
   if (p->f_type != PDR_F_ISOTROPIC && p->rk3_timestep == PDR_RK3_TIMESTEP_DECIDE &&
@@ -644,6 +647,8 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   }
   const bool pstats = stage == 1 && c->pst_on && c->np > 0 && tstep > c->ts_release;
   if (pstats) HIPCHK(launch_pstats_primary(c, *dt));
+  // spherical / LES truncation on entry to f_RK3PrepareRealSpace (src/pf_rk3.f90:269-277)
+  HIPCHK(launch_dealias(c));
   const Coef cf = make_coef(c);
   // particles move from tstep = P_TS_RELEASE on (src/pf_rk3.f90:1146)
   const bool tracers = c->np > 0 && tstep >= c->ts_release;
@@ -719,7 +724,16 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
     HIPCHK(launch_zfwd(c));
     HIPCHK(launch_formf(c, cf, beta, gdt));
   }
+  // and at the end of f_RK3SolveNavierStokes (src/pf_rk3.f90:2101-2109)
+  HIPCHK(launch_dealias(c));
   if (tracers) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_tr, 0));
+  return PDR_OK;
+}
+
+int pdr_dealias(pdr_ctx* c) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  HIPCHK(launch_dealias(c));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return PDR_OK;
 }
 

@@ -133,11 +133,11 @@ int pdr_ensemble_create(pdr_ensemble** out, pdr_ctx* const* members, int32_t cou
     const bool ok = c->prec == prec && c->spec && c->fuse_zf && c->g.zc == c->g.nz && c->g.nx == c->g.ny &&
                     c->g.ny == c->g.nz && pdr::ens_size_supported(c->g.nx) &&
                     c->p.f_type == PDR_F_ISOTROPIC && c->p.rk3_timestep == PDR_RK3_TIMESTEP_FIXED &&
-                    !c->forcing && c->np == 0;
+                    !c->forcing && c->np == 0 && c->p.dealiasing == PDR_DEALIASING_TWOTHIRD;
     if (!ok)
       return set_error(PDR_ERR_UNSUPPORTED,
                        "pdr_ensemble_create: members must share the precision and be cubic (32..256 points), "
-                       "isotropic, FIXED time step, unforced and without tracers");
+                       "isotropic, FIXED time step, 2/3 dealiasing, unforced and without tracers");
   }
   pdr_ensemble* e = new pdr_ensemble();
   e->m.assign(members, members + count);

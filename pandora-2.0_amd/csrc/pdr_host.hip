@@ -126,7 +126,9 @@ int pdr_init_field(pdr_ctx* ctx, uint64_t seed, double init_u0, double init_kp) 
   rc = pdr_upload_state(ctx, u[0].data(), u[1].data(), u[2].data());
   if (rc != PDR_OK) return rc;
   for (int c = 0; c < 3; ++c) std::fill(u[c].begin(), u[c].end(), 0.0);
-  return pdr_upload_rk3(ctx, u[0].data(), u[1].data(), u[2].data());
+  rc = pdr_upload_rk3(ctx, u[0].data(), u[1].data(), u[2].data());
+  // f_InitialiseInit truncates the new field (src/pf_initialise.f90:75-83)
+  return rc == PDR_OK ? pdr_dealias(ctx) : rc;
 }
 
 int pdr_init_particles(pdr_ctx* ctx, uint64_t seed, int64_t np) {
@@ -221,7 +223,8 @@ int pdr_restart_read(pdr_ctx* ctx, const char* path) {
   for (int c = 0; c < 3; ++c) u[c].resize(static_cast<size_t>(n));
   int rc = pdr_restart_read_host(path, n, u[0].data(), u[1].data(), u[2].data());
   if (rc != PDR_OK) return rc;
-  return pdr_upload_state(ctx, u[0].data(), u[1].data(), u[2].data());
+  rc = pdr_upload_state(ctx, u[0].data(), u[1].data(), u[2].data());
+  return rc == PDR_OK ? pdr_dealias(ctx) : rc;  // as after a restart in f_InitialiseInit
 }
 
 }  // extern "C"

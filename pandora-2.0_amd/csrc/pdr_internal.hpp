@@ -243,6 +243,7 @@ inline bool conv_form(const pdr_ctx* c) {
 inline size_t conv_xpass_lds(int nx, size_t csz) { return 12 * size_t(nx + 1) * csz; }
 Coef conv_coef(const Coef& cf, int d);  // G selecting field 3 + c = d u_c / d x_d
 hipError_t launch_conv_front(pdr_ctx* c, const Coef& cf, bool tr, bool cfl, double* rsp);
+hipError_t launch_dealias(pdr_ctx* c);  // no-op for DEALIASING = 1
 hipError_t launch2_yfwd(pdr_ctx* c, int z0, int nzc);
 hipError_t launch2_zfwd(pdr_ctx* c);
 hipError_t launch2_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);

@@ -525,6 +525,42 @@ __global__ void __launch_bounds__(kThreads) k_zero_d2(double2* __restrict__ p, i
 }
 
 // ---------------------------------------------------------------------------
+// Spherical / LES truncation (f_ArraysDealiasSpherical, f_ArraysLESCutoff,
+// src/pf_arrays.f90:1865-1987): zero u1..u3 where |bmat^T k| > cutoff.  The
+// decision needs only the indices and the frame, so the kernel reads nothing
+// and writes the masked modes only.
+// ---------------------------------------------------------------------------
+struct MaskFrame {
+  double bm[9];  // bmat over the existing entries, [m][n]
+  double cutoff;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_dealias(Geo g, MaskFrame mf, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1, cx<T>* __restrict__ u2) {
+#pragma clang fp contract(off)
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < g.mw; e += stride) {
+    const int k = int(e % g.nkk);
+    const long long ij = e / g.nkk;
+    const int j = int(ij % g.nkj);
+    const int i = int(ij / g.nkj);
+    const double kv[3] = {wvn(i, g.ki), wvn(j, g.kj), wvn(k, g.kk)};
+    double kl[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 3; ++n) kl[n] = kl[n] + mf.bm[3 * m + n] * kv[m];
+    const double km = sqrt((kl[0] * kl[0]) + (kl[1] * kl[1]) + (kl[2] * kl[2]));
+    if (km > mf.cutoff) {
+      u0[e] = zero_c<T>();
+      u1[e] = zero_c<T>();
+      u2[e] = zero_c<T>();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Tracers
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ long long pmod(long long a, long long n) {
@@ -1029,6 +1065,26 @@ hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx) {
   if (c->np <= 0) return hipSuccess;
   const long long nb = (c->np + kThreads - 1) / kThreads;
   k_remesh_particles<<<dim3((unsigned)nb), kThreads, 0, c->stream>>>(c->xp, c->xp0, c->np, lx, dx);
+  return hipGetLastError();
+}
+
+hipError_t launch_dealias(pdr_ctx* c) {
+  if (c->p.dealiasing == PDR_DEALIASING_TWOTHIRD) return hipSuccess;
+  const Geo& g = c->g;
+  MaskFrame mf;
+  for (int m = 1; m <= 3; ++m)
+    for (int n = 1; n <= 3; ++n) mf.bm[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->bmat, m, n) : 0.0;
+  // KMAX = min(KI_MAX, KJ_MAX, KK_MAX) (src/pg_domain.f90:231)
+  const double kmax = double(std::min(g.ki, std::min(g.kj, g.kk)));
+  mf.cutoff = c->p.dealiasing == PDR_DEALIASING_LES ? c->p.les_cutoff * kmax : kmax;
+  const long long want = (g.mw + kThreads - 1) / kThreads;
+  const unsigned nb = unsigned(want < 65536 ? want : 65536);
+  if (c->prec)
+    k_dealias<float><<<nb, kThreads, 0, c->stream>>>(g, mf, (cx<float>*)c->u[0], (cx<float>*)c->u[1],
+                                                     (cx<float>*)c->u[2]);
+  else
+    k_dealias<double><<<nb, kThreads, 0, c->stream>>>(g, mf, (cx<double>*)c->u[0], (cx<double>*)c->u[1],
+                                                      (cx<double>*)c->u[2]);
   return hipGetLastError();
 }
 

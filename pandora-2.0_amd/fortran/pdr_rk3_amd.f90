@@ -22,7 +22,7 @@ module pdr_rk3_amd
     integer(c_int32_t) :: nodes_x = 0, nodes_y = 0, nodes_z = 0
     integer(c_int32_t) :: f_type = 1
     integer(c_int32_t) :: ns_deformed = 1   ! 1 BF_ROT, 2 R_CONV, 3 R_ROT
-    integer(c_int32_t) :: dealiasing = 1
+    integer(c_int32_t) :: dealiasing = 1    ! 1 two-thirds, 2 spherical, 3 LES
     integer(c_int32_t) :: rk3_timestep = 2
     integer(c_int32_t) :: f_remesh_t = 0
     integer(c_int32_t) :: b110 = 1
@@ -37,6 +37,7 @@ module pdr_rk3_amd
     real(c_double)     :: force_kf = 0.0d0
     real(c_double)     :: force_sigma = 0.0d0
     integer(c_int64_t) :: force_seed = 0
+    real(c_double)     :: les_cutoff = 1.0d0
   end type pdr_params_t
 
   integer(c_int32_t), parameter, public :: PDR_FP64 = 0, PDR_FP32_MIXED = 1

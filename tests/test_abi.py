@@ -41,7 +41,8 @@ def _params(**kw):
     (dict(nodes_x=15), -4),                               # odd x length (real transforms)
     (dict(f_type=5), -4),                                 # isotropic deformed grid
     (dict(f_type=2, ns_deformed=4), -4),                  # no such NS_DEFORMED form
-    (dict(dealiasing=2), -4),
+    (dict(dealiasing=4), -4),                             # no such dealiasing form
+    (dict(dealiasing=3, les_cutoff=0.0), -1),             # LES cutoff must be > 0
     (dict(rk3_cfl=1.0), -1),                              # g_RK3Check abort condition
     (dict(force_stop=10, force_tf=0.0, force_kf=2.5), -1),  # OU time scale must be > 0
     (dict(force_stop=10, force_tf=0.2, force_kf=0.0), -1),  # no forced band

@@ -434,6 +434,69 @@ def test_rconv_differs_from_rrot_when_products_alias(oracle_mod):
     assert _rconv_vs_rrot(oracle_mod, 48) > 1e-6
 
 
+def _klab_norm(o, b):
+    B = b.reshape(3, 3, order="F")
+    KI, KJ, KK = kgrid(o)
+    kl = [B[0, n] * KI + B[1, n] * KJ + B[2, n] * KK for n in range(3)]
+    return np.sqrt(kl[0] ** 2 + kl[1] ** 2 + kl[2] ** 2)
+
+
+@pytest.mark.parametrize("dealiasing,les", [(2, 1.0), (3, 0.6)])
+@pytest.mark.parametrize("sheared", [False, True])
+def test_dealias_masks_brute_force(oracle_mod, dealiasing, les, sheared):
+    """f_ArraysDealiasSpherical / f_ArraysLESCutoff: zero exactly the modes with
+    |bmat^T k| > KMAX (or LES_CUTOFF KMAX), KMAX = min(KI, KJ, KK); keep the rest."""
+    kw = dict(f_type=2, rk3_timestep=2, deform_s=1.0) if sheared else {}
+    o = oracle_mod.Oracle(nx=24, ny=30, nz=36, dealiasing=dealiasing, les_cutoff=les, **kw)
+    u = random_state(o, 4, solenoidal=False)
+    o.set_state(*u)
+    if sheared:
+        b, bi = o.get_frame()
+        b[6], bi[6] = -0.45, 0.45
+        o.set_frame(b, bi)
+    o.dealias()
+    b, _ = o.get_frame()
+    cut = les * min(o.nki - 1, o.nkj // 2, o.nkk // 2)
+    out = _klab_norm(o, b) > cut
+    assert 0 < out.sum() < out.size
+    for a, x in zip(o.get_state(), u):
+        assert np.all(a[out] == 0) and np.array_equal(a[~out], x[~out])
+
+
+def test_two_thirds_dealiasing_leaves_the_state(oracle_mod):
+    o = oracle_mod.Oracle(16)
+    u = random_state(o, 5)
+    o.set_state(*u)
+    o.dealias()
+    for a, x in zip(o.get_state(), u):
+        assert np.array_equal(a, x)
+
+
+def test_spherical_run_stays_inside_the_sphere(oracle_mod):
+    """Masked on entry to the transforms and after the update: after each step
+    no mode outside |k| <= KMAX carries energy, the state stays solenoidal, and
+    the run differs from the cubic 2/3 one (whose corner modes keep energy)."""
+    import pandora_amd as pa
+    n = 24
+    u = pa.init_field_host(n, 8, 2.0 / 3.0, 6.0)
+    s = oracle_mod.Oracle(n, rk3_dt=0.005, nu=0.01, dealiasing=2)
+    c = oracle_mod.Oracle(n, rk3_dt=0.005, nu=0.01)
+    s.set_state(*u)
+    c.set_state(*u)
+    out = _klab_norm(s, s.get_frame()[0]) > n // 3
+    ts = tc = 0.0
+    for st in range(1, 4):
+        ts = s.step(st, ts)
+        tc = c.step(st, tc)
+        for a in s.get_state():
+            assert np.all(a[out] == 0)
+    KI, KJ, KK = kgrid(s)
+    v = s.get_state()
+    div = KI * v[0] + KJ * v[1] + KK * v[2]
+    assert np.abs(div).max() < 1e-14 * max(np.abs(x).max() for x in v)
+    assert max(rel(a, b) for a, b in zip(s.get_state(), c.get_state())) > 1e-3
+
+
 def test_remesh_map_enumeration(oracle_mod):
     """knew = k_signed - i, dropped below -K (src/pf_rk3.f90:4100-4125)."""
     for K in [4, 10, 21, 32]:
