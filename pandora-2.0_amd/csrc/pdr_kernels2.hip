@@ -116,9 +116,17 @@ template <typename T>
 constexpr int lds_yf_bytes(int n) {
   return (n + y_lines_fwd<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
 }
+// The x tile of lines longer than PDR_X_TWG_MIN points reads its twiddles from
+// the global table (through the caches) instead of staging them in LDS: that
+// frees N complex of LDS, so two 2-row FP32 tiles of 1536 points fit one CU
+// and the FP64 one fits at all.
+#ifndef PDR_X_TWG_MIN
+#define PDR_X_TWG_MIN 1024
+#endif
+constexpr bool x_tw_global(int n) { return n > PDR_X_TWG_MIN; }
 template <typename T>
 constexpr int lds_x_bytes(int n) {
-  return (n + 3 * xrows<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>)) + kBlock2 * int(sizeof(T));
+  return ((x_tw_global(n) ? 0 : n) + 3 * xrows<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>)) + kBlock2 * int(sizeof(T));
 }
 // Streaming (non-temporal) stores per kernel, bit mask: 1 zinv, 2 yinv,
 // 4 xpass, 8 yfwd, 16 zfwd.  Every pass writes data that only the next pass
@@ -471,14 +479,16 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   constexpr int IT = (RB * N + kBlock2 - 1) / kBlock2;  // product items per thread
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[kBlock2];
-  cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
-  cx<T>* buf = tw + N;
+  constexpr bool TWG = x_tw_global(N);
+  cx<T>* const lds0 = reinterpret_cast<cx<T>*>(smem);
+  const cx<T>* const tw = TWG ? twg : lds0;
+  cx<T>* const buf = TWG ? lds0 : lds0 + N;
   const int tile = int(bid % unsigned(ntile));
   const int zl = int(bid / unsigned(ntile));  // plane within the chunk
   const int z = z0 + zl;
   const int y0 = tile * RB;
   const int ny = g.ny;
-  load_tw<T, N>(tw, twg);
+  if constexpr (!TWG) load_tw<T, N>(lds0, twg);
   cx<T>* rowp = w2 + ((long long)zl * ny + y0) * g.ldi;
   // fill: the 3 * RB lines pack pairs of real inputs (p = 0: u1 + i u2,
   // 1: u3 + i w1, 2: w2 + i w3).  Each stored mode m' = 0..KI of a pair (a, b) is
