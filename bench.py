@@ -82,9 +82,7 @@ CASES = {
                                force_kf=2.5, force_sigma=0.5**0.5, force_stop=100000, force_seed=3),
                        label="1024^3 forced isotropic (OU |k|<=2.5, T_F=0.2, sigma^2 T_F=0.1), nu=0.0008, "
                              "dt=0.0005, 16,777,216 tracers"),
-    # its FP64 form runs without tracers: with the 87 GB FP64 tracer field it would not fit in
-    # the 288 GB of one GPU (tracers do not act on the flow, so E(k) still compares one flow)
-    "shear1536": dict(n=1536, tracers=67_108_864, fp64_tracers=0, seed=4, tseed=104, kp=6.0,
+    "shear1536": dict(n=1536, tracers=67_108_864, seed=4, tseed=104, kp=6.0,
                       kw=dict(f_type=2, rk3_timestep=1, deform_s=1.0, remesh_t=2500, nu=0.0003),
                       label="1536^3 homogeneous shear, S=1, nu=0.0003, dt=0.0002 (F_REMESH_T=2500), "
                             "67,108,864 tracers"),
