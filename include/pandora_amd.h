@@ -274,6 +274,20 @@ int pdr_restart_write_host(const char* path, int64_t n, const double* u1, const 
                            const double* u3, int32_t int64_header);
 int pdr_restart_read_host(const char* path, int64_t n, double* u1, double* u2, double* u3);
 
+/* Scaled restart (F_RESTART_SCALE > 1): g_RestartPetscFluid of a file written on
+ * the grid (nx, ny, nz) / scale, then f_InitialiseCopyRestart
+ * (src/pf_initialise.f90:1184-1299, src/pf_arrays.f90:1047-1312): each coarse
+ * mode keeps its signed wavenumber on the (nx, ny, nz) grid, every other mode is
+ * zero.  The reference copies the negative-j half only on ranks whose j range
+ * starts above the positive half (IFJISPOS), so its result depends on the
+ * decomposition; this copies both halves, as a decomposition with a rank
+ * boundary there does.  Host-only form: fills the fine u1..u3. */
+int pdr_restart_read_scaled_host(const char* path, int32_t nx, int32_t ny, int32_t nz, int32_t scale,
+                                 double* u1, double* u2, double* u3);
+/* Device form: the copy above into the context's state, then the truncation
+ * of pdr_restart_read. */
+int pdr_restart_read_scaled(pdr_ctx* ctx, const char* path, int32_t scale);
+
 /* Wave-space statistics of the current velocity, computed on the device:
  * f_FluidstatsFs (src/pf_fluidstats.f90:1293-1361).  Modes are taken to the
  * laboratory frame as in f_FluidstatsFsPrimaryQuantsBFRot (1586-1792; the

@@ -151,6 +151,26 @@ def remesh_target(kmax: int, i: int, k: int) -> int:
     return lib().orc_remesh_target(kmax, i, k)
 
 
+def copy_restart(coarse, nx, ny, nz, scale):
+    """f_InitialiseCopyRestart (src/pf_initialise.f90:1184-1299) for one process
+    owning every mode, with both j halves copied: the fine state is zero except
+    i <= KIc, j and k with |j| <= KJc, |k| <= KKc, taken from the coarse arrays at
+    the same signed wavenumbers (positive k at k, negative k shifted by
+    NODES_KK - nkk_c; likewise j).  coarse: three (NKIc, NKJc, NKKc) arrays."""
+    nkjf, nkkf = 2 * (ny // 3) + 1, 2 * (nz // 3) + 1
+    kic, kjc, kkc = (nx // scale) // 3, (ny // scale) // 3, (nz // scale) // 3
+    nkjc, nkkc = 2 * kjc + 1, 2 * kkc + 1
+    kshift, jshift = nkkf - nkkc, nkjf - nkjc
+    out = [np.zeros((nx // 3 + 1, nkjf, nkkf), np.complex128) for _ in range(3)]
+    for i in range(kic + 1):
+        jrows = [(j, j) for j in range(kjc + 1)] + [(j, j - jshift) for j in range(nkjf - kjc, nkjf)]
+        for j, jc in jrows:
+            for c in range(3):
+                out[c][i, j, : kkc + 1] = coarse[c][i, jc, : kkc + 1]
+                out[c][i, j, nkkf - kkc:] = coarse[c][i, jc, nkkf - kkc - kshift:]
+    return out
+
+
 class Oracle:
     """FP64 reference-step restatement.  Wave arrays are complex128 numpy
     arrays shaped (NKI, NKJ, NKK) = Fortran arr(0:1, k, j, i) in C order."""

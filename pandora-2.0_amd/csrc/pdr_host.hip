@@ -216,6 +216,53 @@ int pdr_restart_write(pdr_ctx* ctx, const char* path, int32_t int64_header) {
   return pdr_restart_write_host(path, n, u[0].data(), u[1].data(), u[2].data(), int64_header);
 }
 
+int pdr_restart_read_scaled_host(const char* path, int32_t nx, int32_t ny, int32_t nz, int32_t scale,
+                                 double* u1, double* u2, double* u3) {
+  if (!path || scale < 1 || nx < 1 || ny < 1 || nz < 1 || !u1 || !u2 || !u3)
+    return pdr::set_error(PDR_ERR_ARG, "pdr_restart_read_scaled_host: bad argument");
+  // fine (this grid) and coarse (file) compact sizes, src/pg_domain.f90:220-228 and
+  // f_ArraysVectorCreateRestart (src/pf_arrays.f90:1067-1073)
+  const int kif = nx / 3, kjf = ny / 3, kkf = nz / 3;
+  const int nkjf = 2 * kjf + 1, nkkf = 2 * kkf + 1;
+  const int kic = (nx / scale) / 3, kjc = (ny / scale) / 3, kkc = (nz / scale) / 3;
+  const int nkic = kic + 1, nkjc = 2 * kjc + 1, nkkc = 2 * kkc + 1;
+  if (kic < 1 || kjc < 1 || kkc < 1)
+    return pdr::set_error(PDR_ERR_ARG, "pdr_restart_read_scaled_host: coarse grid too small");
+  const int64_t nc = 2LL * nkic * nkjc * nkkc;
+  std::vector<double> c[3];
+  for (auto& v : c) v.resize(size_t(nc));
+  int rc = pdr_restart_read_host(path, nc, c[0].data(), c[1].data(), c[2].data());
+  if (rc != PDR_OK) return rc;
+  double* f[3] = {u1, u2, u3};
+  const size_t nf = size_t(2) * size_t(kif + 1) * nkjf * nkkf;
+  for (auto* p : f) std::fill(p, p + nf, 0.0);
+  // signed wavenumber -> compact index: s >= 0 at s, s < 0 at n + s
+  for (int i = 0; i <= kic; ++i)
+    for (int sj = -kjc; sj <= kjc; ++sj)
+      for (int sk = -kkc; sk <= kkc; ++sk) {
+        const size_t jc = size_t(sj >= 0 ? sj : nkjc + sj), kc = size_t(sk >= 0 ? sk : nkkc + sk);
+        const size_t jf = size_t(sj >= 0 ? sj : nkjf + sj), kf = size_t(sk >= 0 ? sk : nkkf + sk);
+        const size_t ec = 2 * ((size_t(i) * nkjc + jc) * nkkc + kc);
+        const size_t ef = 2 * ((size_t(i) * nkjf + jf) * nkkf + kf);
+        for (int q = 0; q < 3; ++q) {
+          f[q][ef] = c[q][ec];
+          f[q][ef + 1] = c[q][ec + 1];
+        }
+      }
+  return PDR_OK;
+}
+
+int pdr_restart_read_scaled(pdr_ctx* ctx, const char* path, int32_t scale) {
+  if (!ctx || !path) return pdr::set_error(PDR_ERR_ARG, "pdr_restart_read_scaled: null argument");
+  const pdr::Geo& g = ctx->g;
+  std::vector<double> u[3];
+  for (auto& v : u) v.resize(size_t(2 * g.mw));
+  int rc = pdr_restart_read_scaled_host(path, g.nx, g.ny, g.nz, scale, u[0].data(), u[1].data(), u[2].data());
+  if (rc != PDR_OK) return rc;
+  rc = pdr_upload_state(ctx, u[0].data(), u[1].data(), u[2].data());
+  return rc == PDR_OK ? pdr_dealias(ctx) : rc;
+}
+
 int pdr_restart_read(pdr_ctx* ctx, const char* path) {
   if (!ctx || !path) return pdr::set_error(PDR_ERR_ARG, "pdr_restart_read: null argument");
   const long long n = 2 * ctx->g.mw;

@@ -136,6 +136,8 @@ def load_library(path: str = LIB_PATH):
         "pdr_restart_read_host": (C.c_int, [C.c_char_p, i64, dp, dp, dp]),
         "pdr_restart_write": (C.c_int, [vp, C.c_char_p, i32]),
         "pdr_restart_read": (C.c_int, [vp, C.c_char_p]),
+        "pdr_restart_read_scaled_host": (C.c_int, [C.c_char_p, i32, i32, i32, i32, dp, dp, dp]),
+        "pdr_restart_read_scaled": (C.c_int, [vp, C.c_char_p, i32]),
         "pdr_spectrum": (C.c_int, [vp, dp, dp, dp, dp]),
         "pdr_debug_prepare": (C.c_int, [vp, i32, dp]),
         "pdr_profile_enable": (C.c_int, [vp, i32]),
@@ -193,6 +195,17 @@ def restart_read_host(path, shape):
     out = [np.zeros(shape, np.complex128) for _ in range(3)]
     flat = [o.view(np.float64).reshape(-1) for o in out]
     _check(L.pdr_restart_read_host(path.encode(), flat[0].size, *flat))
+    return out
+
+
+def restart_read_scaled_host(path, nx, ny, nz, scale):
+    """Scaled restart onto the (nx, ny, nz) grid (F_RESTART_SCALE = scale): u1..u3
+    as (NKI, NKJ, NKK) complex arrays of the fine grid."""
+    L = load_library()
+    shape = (nx // 3 + 1, 2 * (ny // 3) + 1, 2 * (nz // 3) + 1)
+    out = [np.zeros(shape, np.complex128) for _ in range(3)]
+    flat = [o.view(np.float64).reshape(-1) for o in out]
+    _check(L.pdr_restart_read_scaled_host(path.encode(), nx, ny, nz, scale, *flat))
     return out
 
 
@@ -416,6 +429,9 @@ class Pandora:
 
     def restart_read(self, path: str):
         _check(self.L.pdr_restart_read(self.h, path.encode()))
+
+    def restart_read_scaled(self, path: str, scale: int):
+        _check(self.L.pdr_restart_read_scaled(self.h, path.encode(), scale))
 
     def stats(self):
         """f_FluidstatsFs on the device: (spectra dict of kkmax arrays, dict of

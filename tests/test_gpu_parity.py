@@ -252,6 +252,25 @@ def test_restart_roundtrip_on_device(tmp_path):
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
+@pytest.mark.parametrize("prec", [pa.PDR_FP64, pa.PDR_FP32_MIXED])
+def test_scaled_restart_on_device(tmp_path, prec):
+    """pdr_restart_read_scaled: a 16^3 restart onto the 32^3 context equals the
+    host copy (FP64 exactly; FP32 storage rounds each value once)."""
+    note(f"scaled restart 16^3 -> 32^3 precision={prec}")
+    uc = pa.init_field_host(16, 5, 2.0 / 3.0, 3.0)
+    path = str(tmp_path / "coarse.bin")
+    pa.restart_write_host(path, *uc)
+    g = pa.Pandora(32, precision=prec)
+    g.restart_read_scaled(path, 2)
+    want = pa.restart_read_scaled_host(path, 32, 32, 32, 2)
+    for a, b in zip(g.get_state(), want):
+        if prec == pa.PDR_FP64:
+            assert np.array_equal(a, b)
+        else:
+            assert np.array_equal(a, b.astype(np.complex64).astype(np.complex128))
+    g.close()
+
+
 def test_error_paths_on_device():
     note("error paths")
     g = pa.Pandora(16)
