@@ -48,7 +48,7 @@ module pdr_rk3_amd
   public :: pdr_Configure, g_RK3Init, g_RK3Calc, g_RK3UpdateB, g_RK3Finalise
   public :: f_RK3Stage1, f_RK3Stage2, f_RK3Stage3, f_RK3Remesh
   public :: pdr_PutVelocity, pdr_GetVelocity, pdr_PutParticles, pdr_GetParticles
-  public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_Spectrum, pdr_ErrorMessage
+  public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_RestartReadScaled, pdr_Spectrum, pdr_ErrorMessage
   public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing, pdr_Stats, pdr_RsStats
   public :: pdr_ParticleStatsConfig, pdr_ParticleStats, pdr_GetParticlesAux
 
@@ -142,6 +142,12 @@ module pdr_rk3_amd
       import :: c_int, c_ptr, c_char
       type(c_ptr), value :: c
       character(kind=c_char), intent(in) :: path(*)
+    end function
+    integer(c_int) function c_pdr_restart_read_scaled(c, path, scale) bind(C, name="pdr_restart_read_scaled")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: c
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: scale
     end function
     integer(c_int) function c_pdr_spectrum(c, ek, dk, dsk, tot) bind(C, name="pdr_spectrum")
       import :: c_int, c_ptr, c_double
@@ -334,6 +340,14 @@ contains
     integer, intent(inout)       :: ierr
     ierr = int(c_pdr_restart_read(ctx, trim(path)//c_null_char))
   end subroutine pdr_RestartRead
+
+  !> g_RestartPetscFluid(READ_FILE) with F_RESTART_SCALE > 1 and f_InitialiseCopyRestart
+  subroutine pdr_RestartReadScaled ( path, scale, ierr )
+    character(len=*), intent(in) :: path
+    integer, intent(in)          :: scale
+    integer, intent(inout)       :: ierr
+    ierr = int(c_pdr_restart_read_scaled(ctx, trim(path)//c_null_char, int(scale, c_int32_t)))
+  end subroutine pdr_RestartReadScaled
 
   !> E(k), D(k) shell sums and totals (f_FluidstatsFsPrimaryQuants)
   subroutine pdr_Spectrum ( ek, dk, dsk, tot, ierr )
