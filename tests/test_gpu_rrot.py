@@ -187,6 +187,30 @@ def test_rrot_tracers_and_statistics_fp64(oracle_mod):
     g.close()
 
 
+@pytest.mark.parametrize("prec", [pa.PDR_FP64, pa.PDR_FP32_MIXED])
+def test_rrot_chunked_plane_pass_with_tracers(oracle_mod, monkeypatch, prec):
+    """R_ROT through the chunked y/x passes (PDR_ZCHUNK_MB = 0.75: 5 planes per
+    chunk at 48^3 FP64, the last partial) with tracers and a remesh."""
+    note(f"R_ROT chunked plane pass N=48 prec={prec}, tracers, 6 steps")
+    monkeypatch.setenv("PDR_ZCHUNK_MB", "0.75")
+    n = 48
+    o, g = _pair(oracle_mod, n, precision=prec, **dict(SHEAR, remesh_t=2))
+    u = pa.init_field_host(n, 78, 2.0 / 3.0, 4.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    dx = TWOPI / n
+    xp = np.random.default_rng(9).uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(500, 3))
+    o.set_particles(xp)
+    g.set_particles(xp)
+    _run(o, g, 6)
+    tol = 1e-12 if prec == pa.PDR_FP64 else 1e-5
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert rel(a, b) < tol
+    d = np.abs(g.get_particles()[0] - o.get_particles()[0])
+    assert np.minimum(d, TWOPI - d).max() < (1e-11 if prec == pa.PDR_FP64 else 1e-5)
+    g.close()
+
+
 # ---- NS_R_CONV: nine gradients through three inverse sweeps, generic kernels ----
 
 CONV = dict(SHEAR, ns_deformed=2)
