@@ -162,3 +162,24 @@ def test_tracer_sorting_does_not_change_results():
             np.testing.assert_array_equal(a, b)
         for k, v in res[0][4].items():  # sums in another order: rounding only
             assert other[4][k] == pytest.approx(v, rel=1e-12, abs=1e-15), k
+
+
+def test_empty_tracer_set_steps_and_reports_no_record():
+    """An empty upload (np = 0) after tracers: the step runs the field path only,
+    reading positions back is the 'no particles set' error, and no tracer record
+    is formed."""
+    note("empty tracer set after 100 tracers, shear N=32, 3 steps")
+    n = 32
+    g = pa.Pandora(n, f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=3, nu=0.005)
+    g.set_state(*pa.init_field_host(n, 6, 2.0 / 3.0, 6.0))
+    g.set_particles(np.full((100, 3), 1.0))
+    g.particle_stats_config(True, keep_initial=True)
+    g.set_particles(np.zeros((0, 3)))
+    t = 0.0
+    for s in range(1, 4):
+        t = g.rk3_calc(s, t)
+    assert g.check_finite()
+    with pytest.raises(pa.PdrError):
+        g.get_particles()
+    assert g.particle_stats()[0] == -1
+    g.close()

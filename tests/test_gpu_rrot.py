@@ -298,6 +298,14 @@ def test_rconv_tracers_and_statistics_fp64(oracle_mod):
     g.close()
 
 
+def test_rconv_rejects_an_x_line_beyond_the_lds():
+    """12 (NX + 1) complex elements of LDS per x line: an FP64 1536-point line
+    (295 KB) does not fit, and create says so before allocating anything."""
+    note("R_CONV FP64 NX=1536 rejected for LDS")
+    with pytest.raises(pa.PdrError):
+        pa.Pandora(nx=1536, ny=16, nz=16, **CONV)
+
+
 def test_unknown_form_is_rejected():
     note("NS_DEFORMED = 4 rejected")
     with pytest.raises(pa.PdrError):
