@@ -325,7 +325,7 @@ def test_full_size_256_shear_fp64_with_tracers():
 
 
 def test_full_size_512_shear_fp32_with_tracers():
-    """The north-star workload (512^3 shear, 1M tracers, FP32-mixed) for 2 steps."""
+    """The benchmark workload (512^3 shear, 1M tracers, FP32-mixed) for 2 steps."""
     note("512^3 shear FP32-mixed + 1M tracers, 2 steps")
     n = 512
     g = pa.Pandora(n, f_type=2, rk3_timestep=1, deform_s=1.0, remesh_t=250, nu=0.005,
