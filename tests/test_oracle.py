@@ -140,15 +140,19 @@ def _taylor_green_reverse(o):
 def test_reference_known_answer_taylor_green_nonlinear(oracle_mod, n):
     """The reference's own known answer: L1 norms of the transformed nonlinear
     term are 3/4, 9/8, 11/8 (tf_RK3CheckNonlinearError, src/ptf_rk3.f90:1157-1181)."""
+    import json
+    import os
+    ka = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ptf_rk3_known_answers.json")))
+    assert n in ka["grids"]
     o = oracle_mod.Oracle(n)
     o.set_state(*_taylor_green_reverse(o))
     o.prepare_realspace()
     o.realspace(0)
     nl = o.nl_forward()
     l1 = [np.abs(a.view(np.float64)).sum() for a in nl]
-    assert abs(l1[0] - 3 / 4) < 1e-13
-    assert abs(l1[1] - 9 / 8) < 1e-13
-    assert abs(l1[2] - 11 / 8) < 1e-13
+    for got, name in zip(l1, ("u4", "u5", "u6")):
+        num, den = ka["nonlinear_l1"][name]
+        assert abs(got - num / den) < 1e-13, name
 
 
 def test_known_answer_needs_resolution(oracle_mod):
