@@ -79,7 +79,48 @@ def test_rrot_rdt_parity_fp64(oracle_mod, f_type):
     u = pa.init_field_host(32, 40 + f_type, 2.0 / 3.0, 4.0)
     o.set_state(*u)
     g.set_state(*u)
-    _run(o, g, 8)
+    _run(o, g, 6)
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert rel(a, b) < 1e-12
+    g.close()
+
+
+def test_rrot_cfl_time_step_parity_fp64(oracle_mod):
+    """RK3_TIMESTEP = CFL in the R_ROT form: dt from the laboratory-frame ucfl of
+    stage 1 (binv u, as f_FluidstatsRs2D maps every sheared form); the frame runs
+    on simtime (no remesh clock)."""
+    note("R_ROT CFL time step FP64 N=32, 6 steps")
+    kw = dict(f_type=2, ns_deformed=3, rk3_timestep=3, rk3_cfl=0.5, deform_s=2.0, nu=0.005)
+    o, g = _pair(oracle_mod, 32, **kw)
+    u = pa.init_field_host(32, 44, 2.0 / 3.0, 5.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    to = tg = 0.0
+    for s in range(1, 7):
+        to = o.step(s, to)
+        tg = g.rk3_calc(s, tg)
+        assert abs(g.dt - o.dt) < 1e-14 * o.dt, s
+    assert abs(tg - to) < 1e-13
+    for a, b in zip(g.get_state(), o.get_state()):
+        assert rel(a, b) < 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("shape", [(64, 32, 48), (32, 32, 64)])
+def test_rrot_non_cubic_box_parity(oracle_mod, shape):
+    note(f"R_ROT non-cubic box {shape} FP64, 6 steps with a remesh")
+    nx, ny, nz = shape
+    o = oracle_mod.Oracle(nx=nx, ny=ny, nz=nz, **SHEAR)
+    g = pa.Pandora(nx=nx, ny=ny, nz=nz, **SHEAR)
+    rng = np.random.default_rng(31)
+    u = [0.05 * (rng.standard_normal(g.shape) + 1j * rng.standard_normal(g.shape)) for _ in range(3)]
+    for c in range(3):  # real field: Hermitian i = 0 plane, zero mean
+        p = u[c][0]
+        u[c][0] = 0.5 * (p + np.conj(np.roll(p[::-1, ::-1], 1, axis=(0, 1))))
+        u[c][0, 0, 0] = 0.0
+    o.set_state(*u)
+    g.set_state(*u)
+    _run(o, g, 6)
     for a, b in zip(g.get_state(), o.get_state()):
         assert rel(a, b) < 1e-12
     g.close()
