@@ -183,6 +183,44 @@ def test_tracers_parity_isotropic_fp64(oracle_mod):
     assert d.max() < 1e-11
 
 
+@pytest.mark.parametrize("f_type", [1, 2])
+def test_tracers_on_box_faces_and_crowded_cell_fp64(oracle_mod, f_type):
+    """Tracers on the lower faces (x = -dx/2), just below the upper faces, exactly on
+    x3max (p_MPICorrectPosition keeps it: the x3 upper test is '>',
+    src/pp_mpi.f90:1011-1057) and 1000 tracers in one cell: the molecule search
+    wraps through the periodic box in every direction (p_InterpFindMolecule,
+    src/pp_interp.f90:406-467)."""
+    n = 32
+    note(f"tracers on box faces + crowded cell FP64 N={n} f_type={f_type}, 6 steps")
+    kw = dict(f_type=2, rk3_timestep=1, deform_s=20.0, remesh_t=2, nu=0.005) if f_type == 2 else \
+        dict(nu=0.01, rk3_dt=0.01)
+    o, g = _pair(oracle_mod, n, **kw)
+    u = pa.init_field_host(n, 60 + f_type, 2.0 / 3.0, 6.0)
+    o.set_state(*u)
+    g.set_state(*u)
+    dx = TWOPI / n
+    rng = np.random.default_rng(20 + f_type)
+    xp = rng.uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(3000, 3))
+    for d in range(3):
+        xp[100 * d:100 * d + 50, d] = -0.5 * dx
+        xp[100 * d + 50:100 * d + 100, d] = np.nextafter(TWOPI - 0.5 * dx, 0.0)
+    xp[300:340, 2] = TWOPI - 0.5 * dx  # exactly x3max
+    xp[400:1400] = xp[400]
+    o.set_particles(xp)
+    g.set_particles(xp)
+    to = tg = 0.0
+    for s in range(1, 7):
+        to = o.step(s, to)
+        tg = g.rk3_calc(s, tg)
+    xo, ro = o.get_particles()
+    xg, rg = g.get_particles()
+    d = np.abs(xg - xo)
+    d = np.minimum(d, TWOPI - d)
+    assert d.max() < 1e-11
+    assert rel(rg, ro) < 1e-12
+    assert xg.min() >= -0.5 * dx and xg.max() <= TWOPI - 0.5 * dx
+
+
 def test_single_mode_closed_form_on_gpu():
     """Independent of the oracle: one Fourier mode decays by R(z)^steps exactly."""
     note("single-mode closed form FP64 N=24")
