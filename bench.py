@@ -49,6 +49,16 @@ def kernel_bytes(g, name, csz, rsz, ntr):
     }.get(name)
 
 
+# The y inverse, x pass and y forward run chunk by chunk over z when the box is
+# split (from 512 planes up two half-box chunks are in flight on two streams):
+# a launch then moves its chunk's share of the bytes.
+CHUNKED = ("yinv", "xpass", "yfwd")
+
+
+def chunks_per_stage(name, launches, steps):
+    return max(1.0, launches / (3.0 * steps)) if name in CHUNKED else 1.0
+
+
 def run_gpu(pa, n, precision, steps, warmup, ntr, profile):
     g = pa.Pandora(n, f_type=pa.F_SHEAR_DNS, rk3_timestep=pa.RK3_TIMESTEP_DECIDE, deform_s=1.0,
                    remesh_t=250, nu=0.005, precision=precision)
@@ -425,28 +435,36 @@ def main():
             dom, best = k, ms
     roof = None
 
-    def per_kernel(pr, csz, rsz):
+    def per_kernel(pr, csz, rsz, nsteps):
         out = {}
         for k, (cnt, ms) in pr.items():
             if cnt:
                 b = kernel_bytes(g, k, csz, rsz, ntr)
-                out[k] = {"launches": cnt, "avg_ms": ms / cnt,
+                ch = chunks_per_stage(k, cnt, nsteps)
+                if b:
+                    b /= ch
+                out[k] = {"launches": cnt, "chunks_per_stage": ch, "avg_ms": ms / cnt,
                           "GBps": (b / (ms / cnt * 1e-3) / 1e9) if b else None}
         return out
 
-    kernels = per_kernel(prof, 8, 4)
+    kernels = per_kernel(prof, 8, 4, args.steps)
     if fp64:
-        fp64["kernels"] = per_kernel(fp64.pop("prof"), 16, 8)
+        fp64["kernels"] = per_kernel(fp64.pop("prof"), 16, 8, k64)
     if dom:
         cnt, ms = prof[dom]
         avg = ms / cnt * 1e-3
-        b = kernel_bytes(g, dom, 8, 4, ntr)
+        ch = chunks_per_stage(dom, cnt, args.steps)
+        b = kernel_bytes(g, dom, 8, 4, ntr) / ch
         traffic = measured_traffic(dom, n, ntr)
+        if traffic:
+            traffic /= ch
         roof = {"bound": "hbm", "kernel": dom, "achieved": b / avg / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": b / avg / HBM_PEAK, "traffic": traffic,
                 "traffic_source": (os.path.relpath(TRAFFIC_JSON, ROOT) + " (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                                   "copy-calibrated)") if traffic else None,
-                "bytes_per_launch": b, "avg_launch_ms": avg * 1e3}
+                                   "copy-calibrated, whole-box launch"
+                                   + (f", divided by the {ch:g} chunks of a stage" if ch > 1 else "") + ")")
+                if traffic else None,
+                "bytes_per_launch": b, "chunks_per_stage": ch, "avg_launch_ms": avg * 1e3}
     ens = None
     if not args.no_ensemble:
         print("[bench] batched ensemble 16 x {32,48,64,96}^3", file=sys.stderr, flush=True)

@@ -448,6 +448,38 @@ def test_chunked_plane_pass_parity(oracle_mod, monkeypatch, prec):
 
 
 @pytest.mark.parametrize("prec", [pa.PDR_FP64, pa.PDR_FP32_MIXED])
+@pytest.mark.parametrize("chunk_mb", ["", "0.75"])
+def test_two_chunks_in_flight_match_serial(monkeypatch, prec, chunk_mb):
+    """Odd z chunks on a second stream with their own x-pencil buffer
+    (PDR_ZSTREAMS=2) give the serial passes' result bit for bit: the chunks
+    touch disjoint planes and the CFL maximum is order-free (shear with remesh,
+    CFL time step, tracers, N=48; two half-box chunks, and 5-plane chunks)."""
+    note(f"two chunks in flight vs serial N=48 prec={prec} chunk_mb={chunk_mb or 'half box'}")
+    n = 48
+    kw = dict(f_type=2, rk3_timestep=3, rk3_cfl=0.5, deform_s=20.0, remesh_t=2, nu=0.005, precision=prec)
+    u = pa.init_field_host(n, 91, 2.0 / 3.0, 4.0)
+    rng = np.random.default_rng(12)
+    dx = TWOPI / n
+    xp = rng.uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(700, 3))
+    if chunk_mb:
+        monkeypatch.setenv("PDR_ZCHUNK_MB", chunk_mb)
+    out = []
+    for zs in ("1", "2"):
+        monkeypatch.setenv("PDR_ZSTREAMS", zs)
+        g = pa.Pandora(n, **kw)
+        g.set_state(*u)
+        g.set_particles(xp)
+        t = g.run(1, 6, 0.0)
+        out.append((t, g.get_state(), g.get_particles()))
+        g.close()
+    (ta, sa, pa_), (tb, sb, pb) = out
+    assert ta == tb
+    for a, b in zip(sa, sb):
+        assert np.array_equal(a, b)
+    assert np.array_equal(pa_[0], pb[0]) and np.array_equal(pa_[1], pb[1])
+
+
+@pytest.mark.parametrize("prec", [pa.PDR_FP64, pa.PDR_FP32_MIXED])
 def test_fused_zforward_formf_matches_separate(monkeypatch, prec):
     """The fused z-forward + FormF kernel gives the separate zfwd / formf
     kernels' result (shear with tracers, N=64, 4 steps)."""
