@@ -410,8 +410,23 @@ def main():
     n, ntr = args.n, args.tracers
     print(f"[bench] FP32-mixed {n}^3 shear + {ntr} tracers: {args.warmup} warmup, {args.steps} timed",
           file=sys.stderr, flush=True)
-    dt32, prof, ok32 = run_gpu(pa, n, pa.PDR_FP32_MIXED, args.steps, args.warmup, ntr, True)
+    dt32, prof_split, ok32 = run_gpu(pa, n, pa.PDR_FP32_MIXED, args.steps, args.warmup, ntr, True)
     sps32 = args.steps / dt32
+    # Per-kernel rates and the roofline come from a short second timed run with
+    # the chunks of a stage one after another: in the run above two chunks share
+    # the GPU, so a launch's duration is not that kernel's own.
+    rsteps = max(3, args.steps // 4)
+    print(f"[bench] FP32-mixed {n}^3, chunks one after another: 1 warmup, {rsteps} timed (kernel rates)",
+          file=sys.stderr, flush=True)
+    env_zs = os.environ.get("PDR_ZSTREAMS")
+    os.environ["PDR_ZSTREAMS"] = "1"
+    try:
+        _, prof, _ = run_gpu(pa, n, pa.PDR_FP32_MIXED, rsteps, 1, ntr, True)
+    finally:
+        if env_zs is None:
+            del os.environ["PDR_ZSTREAMS"]
+        else:
+            os.environ["PDR_ZSTREAMS"] = env_zs
     fp64 = None
     if not args.no_fp64:
         k64 = max(3, args.steps // 3)
@@ -447,13 +462,14 @@ def main():
                           "GBps": (b / (ms / cnt * 1e-3) / 1e9) if b else None}
         return out
 
-    kernels = per_kernel(prof, 8, 4, args.steps)
+    kernels = per_kernel(prof, 8, 4, rsteps)
+    kernels_split = per_kernel(prof_split, 8, 4, args.steps)
     if fp64:
         fp64["kernels"] = per_kernel(fp64.pop("prof"), 16, 8, k64)
     if dom:
         cnt, ms = prof[dom]
         avg = ms / cnt * 1e-3
-        ch = chunks_per_stage(dom, cnt, args.steps)
+        ch = chunks_per_stage(dom, cnt, rsteps)
         b = kernel_bytes(g, dom, 8, 4, ntr) / ch
         traffic = measured_traffic(dom, n, ntr)
         if traffic:
@@ -464,7 +480,9 @@ def main():
                                    "copy-calibrated, whole-box launch"
                                    + (f", divided by the {ch:g} chunks of a stage" if ch > 1 else "") + ")")
                 if traffic else None,
-                "bytes_per_launch": b, "chunks_per_stage": ch, "avg_launch_ms": avg * 1e3}
+                "bytes_per_launch": b, "chunks_per_stage": ch, "avg_launch_ms": avg * 1e3,
+                "measured_in": f"{rsteps} timed steps with the chunks of a stage one after another "
+                               "(PDR_ZSTREAMS=1), after the headline run"}
     ens = None
     if not args.no_ensemble:
         print("[bench] batched ensemble 16 x {32,48,64,96}^3", file=sys.stderr, flush=True)
@@ -497,6 +515,7 @@ def main():
         "step_model": {"bytes_per_step": bstep, "achieved_GBps": bstep * sps32 / 1e9,
                        "frac_of_peak": bstep * sps32 / HBM_PEAK},
         "kernels": kernels,
+        "kernels_two_chunks_in_flight": kernels_split,
         "measured_copy_GBps": copy_gbps,
         "roofline": roof,
         "cpu_baseline": cpu,
