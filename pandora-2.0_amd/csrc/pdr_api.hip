@@ -442,11 +442,12 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     // Two chunks in flight: the even chunks' y/x/y passes on the main stream,
     // the odd ones' on a second stream with their own x-pencil buffer, so that
     // the HBM-bound y passes of one run beside the issue-bound x pass of the
-    // other (512^3 FP32: 84.8 / 85.9 against 83.8 / 84.9 steps/s, FP64 +1 %).
-    // On from 512 planes up (PDR_ZSTREAMS=2 forces it, 1 turns it off); the same
-    // buffer bytes as one whole-box buffer.
+    // other (512^3 FP32: 84.8 / 85.9 against 83.8 / 84.9 steps/s, FP64 +1 %;
+    // 1024^3 FP64 +3 %).  On from 512 planes up when the whole-box buffer fits
+    // (PDR_ZSTREAMS=2 forces it, 1 turns it off), in the same buffer bytes; not
+    // over chunks cut for memory (1536^3 FP64: 0.750 against 0.771 steps/s).
     const char* zs = std::getenv("PDR_ZSTREAMS");
-    c->zs2 = (zs && zs[0] ? zs[0] == '2' : nz >= 512) && nz > 1;
+    c->zs2 = (zs && zs[0] ? zs[0] == '2' : (nz >= 512 && zc == nz)) && nz > 1;
     if (c->zs2) zc = std::min(zc, (nz + 1) / 2);
     g.zc = zc;
     g.m2 = (long long)zc * g.ny * g.ldi;

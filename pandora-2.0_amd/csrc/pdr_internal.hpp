@@ -171,7 +171,7 @@ struct pdr_ctx {
   hipStream_t tstream = nullptr;
   hipEvent_t ev_ur = nullptr;
   hipEvent_t ev_tr = nullptr;
-  // two z chunks in flight (nz >= 512, or PDR_ZSTREAMS=2): odd chunks run their
+  // two z chunks in flight (nz >= 512 unchunked, or PDR_ZSTREAMS=2): odd chunks run their
   // y/x/y passes on zstream with their own x-pencil buffer w2z, beside the even ones
   bool zs2 = false;
   hipStream_t zstream = nullptr;
