@@ -4,7 +4,8 @@
  * The whole periodic (or Rogallo shear-deforming) box lives on one GPU.  The
  * Fortran host keeps its own drivers and calls these entry points through
  * ISO_C_BINDING (pandora-2.0_amd/fortran/pdr_rk3_amd.f90 is the shim that keeps
- * the reference's g_RK3Calc / f_RK3Stage* / p_RK3SolveParticle* signatures).
+ * the reference's g_RK3Calc / g_RK3UpdateB / f_RK3Stage1..3 / f_RK3Remesh /
+ * p_ControlRK3 / p_RK3SolveParticle[BF|R] / p_ControlFinishTimeStep signatures).
  *
  * Conventions
  *  - Every function returns PDR_OK (0) or a negative PDR_ERR_* code; the text
@@ -73,7 +74,7 @@ typedef struct pdr_params {
   int32_t rk3_timestep;              /* RK3_TIMESTEP: 1 DECIDE, 2 FIXED, 3 CFL          */
   int32_t f_remesh_t;                /* F_REMESH_T (shear, DECIDE)                      */
   int32_t b110;                      /* B110 (box stretch in x1; 1 for all configs)     */
-  int32_t p_interp_order;            /* P_INTERP_ORDER: even, 2..8 (reference default 4) */
+  int32_t p_interp_order;            /* P_INTERP_ORDER: 1..8 (reference default 4)      */
   int32_t precision;                 /* PDR_FP64 or PDR_FP32_MIXED                      */
   int32_t force_stop;                /* FORCE_STOP: OU forcing while tstep < FORCE_STOP */
   double  rk3_dt;                    /* RK3_DT  (FIXED)                                 */
@@ -149,9 +150,11 @@ int pdr_get_dt(const pdr_ctx* ctx, double* dt);
 /* One Runge-Kutta stage: f_RK3Stage1/2/3(tstep, time, dt, ierr)
  * (src/pf_rk3.f90:40, 121, 161).  Stage 1 also does the shear remesh check
  * (f_RK3Remesh, src/pf_rk3.f90:3958) and sets *dt (f_RK3Tstep, 3852-3948).
- * Tracers, if set, are advanced inside the stage as in f_RK3RealSpace
- * (src/pf_rk3.f90:1145-1176).  The state-space statistics calls of the
- * reference stage are not made (see pdr_spectrum). */
+ * Tracers, if set, are advanced inside the stage as f_RK3RealSpace does with
+ * p_ControlRK3 + p_ControlFinishTimeStep (src/pf_rk3.f90:1145-1176), unless
+ * pdr_set_tracers_in_stage(ctx, 0) leaves that to the caller.  The
+ * state-space statistics calls of the reference stage are not made (see
+ * pdr_spectrum). */
 int pdr_stage(pdr_ctx* ctx, int32_t stage, int32_t tstep, double time, double* dt);
 
 /* Remesh check and remesh of the Rogallo frame: f_RK3Remesh(tstep, ierr)
@@ -211,6 +214,44 @@ int pdr_particles_download_aux(pdr_ctx* ctx, double* up, double* xp0);
  * host uses the caller's numbering. */
 int pdr_set_particle_sort(pdr_ctx* ctx, int32_t every);
 int pdr_particles_upload_aux(pdr_ctx* ctx, const double* up, const double* xp0);
+
+/* Who advances the tracers.  on != 0 (default): pdr_stage and pdr_step do, as
+ * f_RK3RealSpace calls p_ControlRK3 and p_ControlFinishTimeStep inside the
+ * fluid stage.  on = 0: pdr_stage only writes the stage's real-space velocity
+ * for them (u1_3r..u3_3r, src/pf_rk3.f90:1108-1115) and the caller advances
+ * them with the three entry points below, after pdr_stage and before
+ * pdr_update_b (the interpolation uses the frame of the stage either way).
+ * pdr_step and pdr_run do not advance tracers in that mode.  No reference
+ * counterpart (the reference's particle driver is called from inside
+ * f_RK3RealSpace). */
+int pdr_set_tracers_in_stage(pdr_ctx* ctx, int32_t on);
+
+/* p_ControlRK3(stage, tstep, deltat, ierr) (src/pp_control.f90:518-580): at
+ * the release step (stage 1, tstep = P_TS_RELEASE) particle_up_rk3 = 0 and
+ * particle_xp0 = particle_xp (p_ControlInitPartVelocity[BF|R]); then the
+ * interpolation of the stage's velocity at every tracer, particle_up and the
+ * RK3 update with the frame's branch: p_RK3SolveParticle (isotropic),
+ * p_RK3SolveParticleBF (NS_BF_ROT) or p_RK3SolveParticleR (NS_R_ROT,
+ * NS_R_CONV).  Positions are left unwrapped until pdr_particles_finish_step.
+ * Needs pdr_set_tracers_in_stage(ctx, 0) and the tracer field of this stage and
+ * tstep (PDR_ERR_STATE otherwise). */
+int pdr_particles_control_rk3(pdr_ctx* ctx, int32_t stage, int32_t tstep, double deltat);
+
+/* p_RK3SolveParticle / p_RK3SolveParticleBF / p_RK3SolveParticleR(npin, stage,
+ * deltat, ierr) (src/pp_rk3.f90:57-165, 169-303, 307-403): the advance alone
+ * (interpolation, particle_up, particle_up_rk3 and position update; no release
+ * handling, no wrap).  form = 0, 1, 2 selects the routine and must be the
+ * run's frame branch (PDR_ERR_ARG otherwise); npin must equal the number of
+ * tracers set: one box, so one owner of every tracer. */
+int pdr_particles_solve(pdr_ctx* ctx, int32_t form, int64_t npin, int32_t stage, double deltat);
+
+/* p_ControlFinishTimeStep(tstep, ierr) (src/pp_control.f90:1440-1450): the
+ * periodic correction of every position (p_MPICorrectPosition,
+ * src/pp_mpi.f90:1011-1057; particle_xp0 moves with it, src/pp_control.f90:
+ * 1574-1585); the MPI redistribution has nothing to do on one box.  After a
+ * stage-1 advance with tracer statistics on it also takes
+ * p_StatsRelativeVelocity, as f_RK3Stage1 does next (src/pf_rk3.f90:88-97). */
+int pdr_particles_finish_step(pdr_ctx* ctx, int32_t tstep);
 
 /* Tracer statistics: p_StatsPrimary, p_StatsDispersion and
  * p_StatsRelativeVelocity (src/pp_stats.f90:119-390), called by f_RK3Stage1
@@ -354,6 +395,25 @@ typedef struct pdr_rs_stats_t {
 } pdr_rs_stats_t;
 int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out);
 
+/* Statistics of stage 1 inside the step, as f_RK3Stage1 takes them
+ * (src/pf_rk3.f90:58-60 and f_RK3RealSpace, 1073-1078, 1124-1143): with
+ * every > 0, stage 1 of each tstep with (tstep - 1) mod every = 0 runs the
+ * wave-space statistics pass on the state after the remesh check
+ * (f_FluidstatsFs((tstep-1))) and accumulates the real-space moments in its own
+ * x pass (f_FluidstatsRs2D / RsCollect on the stage's planes), so no extra
+ * transform is made; the record stays on the device until read.  every = 0
+ * turns it off.  The record buffer is allocated here, never during a step.
+ * The 1-D spectra come from the same wave-space pass (Parseval, see
+ * pdr_spectra_1d): for DEALIASING 2 and 3 they are those of the state before
+ * the stage's entry truncation, which the reference applies first. */
+int pdr_set_stage_stats(pdr_ctx* ctx, int32_t every);
+
+/* The last stage-1 record: *tstep = its statistics step (the reference's
+ * tstep - 1; -1 if none yet), then the quantities of pdr_stats, pdr_rs_stats
+ * and pdr_spectra_1d for that state.  Any output pointer may be NULL. */
+int pdr_stage_stats(pdr_ctx* ctx, int32_t* tstep, pdr_stats_t* fs, double* spec, pdr_rs_stats_t* rs,
+                    double* spec_1i, double* spec_2i, pdr_homog_t* h);
+
 /* Shell sums before mode-density normalisation from the same device pass:
  * ek, dk, dsk have kkmax entries (energy, 2 nu k^2 E, 4 nu^2 k^4 E);
  * tot[6] = {f_ken, f_dis, f_disskew, u1u1, u2u2, u3u3}. */
@@ -392,6 +452,16 @@ int pdr_ensemble_destroy(pdr_ensemble* e);
  * (0:1, 0:NKJ-1, 0:NZ-1, 0:NKI-1) FP64.  Runs pass 1 of a stage on the
  * current state; the state is not modified. */
 int pdr_debug_prepare(pdr_ctx* ctx, int32_t field, double* out);
+
+/* Which kernel forms this context runs (no reference counterpart; for tests
+ * that must know which code a box size exercises): out[0] 1 = length-
+ * specialised kernels, 0 = runtime-radix kernels; out[1] z planes per x-pencil
+ * chunk (NZ = whole box); out[2] 1 = two chunks in flight on two streams;
+ * out[3] 1 = fused z-forward + FormF kernel; with the specialised kernels
+ * out[4] x rows per x-pass workgroup, out[5] 1 = x tile twiddles read from the
+ * global table, out[6] 1 = register z inverse (velocity tile loaded once),
+ * out[7] lines per z-inverse tile (0 for out[4..7] on the runtime-radix path). */
+int pdr_kernel_forms(const pdr_ctx* ctx, int32_t out[8]);
 
 /* Measured device copy bandwidth (read + write bytes per second, GB/s) of a
  * plain 16-byte-vector copy of `bytes` bytes, timed over `iters` launches.

@@ -34,18 +34,24 @@ def _run(cmd, cwd=None):
 
 
 def build_hip(verbose: bool = True) -> str:
+    """Compile every HIP source (in parallel, one hipcc per source) and link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(LIB, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(LIB, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
-               # no packed-FP32 instructions: some of their forms have given wrong results on gfx950
-               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-               "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
-               "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-               os.path.join(CSRC, src), "-o", obj]
-        _run(cmd)
+        cmds.append([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
+                     # no packed-FP32 instructions: some of their forms have given wrong results on gfx950
+                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                     "-Wall", "-Wno-unused-parameter", "-Wno-unused-command-line-argument",
+                     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+                     os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for f in [ex.submit(_run, cmd) for cmd in cmds]:
+            f.result()
     tmp = SO + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, SO)
@@ -64,7 +70,7 @@ def build_fortran() -> bool:
           "-module-dir", LIB])
     if os.path.exists(drv):
         _run([FC, "-O2", drv, os.path.join(LIB, "pdr_rk3_amd.o"), "-I", LIB,
-              "-L", LIB, "-lpandora_amd", f"-Wl,-rpath,{LIB}", "-Wl,-rpath,$ORIGIN",
+              "-L", LIB, "-lpandora_amd", "-Wl,-rpath,$ORIGIN",
               "-o", os.path.join(LIB, "pdr_fortran_driver")])
     return True
 

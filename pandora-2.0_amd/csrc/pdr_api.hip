@@ -272,7 +272,9 @@ const char* pdr_last_error(void) { return g_err.c_str(); }
 
 int pdr_destroy(pdr_ctx* c) {
   if (!c) return PDR_OK;
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  // every stream of the context, before any buffer is freed
+  for (hipStream_t s : {c->stream, c->zstream, c->tstream})
+    if (s) (void)hipStreamSynchronize(s);
   for (int q = 0; q < 3; ++q) {
     if (c->u[q]) (void)hipFree(c->u[q]);
     if (c->rk[q]) (void)hipFree(c->rk[q]);
@@ -294,6 +296,8 @@ int pdr_destroy(pdr_ctx* c) {
   if (c->red) (void)hipFree(c->red);
   if (c->force) (void)hipFree(c->force);
   if (c->stat_buf) (void)hipFree(c->stat_buf);
+  if (c->rs_buf) (void)hipFree(c->rs_buf);
+  stage_stats_free(c);
   if (c->tab.twz) (void)hipFree(c->tab.twz);
   if (c->tab.twy) (void)hipFree(c->tab.twy);
   if (c->tab.twx) (void)hipFree(c->tab.twx);
@@ -302,10 +306,8 @@ int pdr_destroy(pdr_ctx* c) {
     (void)hipEventDestroy(e.second.second);
   }
   for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
-  if (c->tstream) (void)hipStreamSynchronize(c->tstream);
   if (c->ev_ur) (void)hipEventDestroy(c->ev_ur);
   if (c->ev_tr) (void)hipEventDestroy(c->ev_tr);
-  if (c->zstream) (void)hipStreamSynchronize(c->zstream);
   if (c->ev_za) (void)hipEventDestroy(c->ev_za);
   if (c->ev_zb) (void)hipEventDestroy(c->ev_zb);
   if (c->zstream) (void)hipStreamDestroy(c->zstream);
@@ -344,8 +346,8 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   if (p->f_type != PDR_F_ISOTROPIC && p->rk3_timestep == PDR_RK3_TIMESTEP_DECIDE &&
       (p->f_remesh_t <= 0 || p->f_deform_s == 0.0))
     return fail(PDR_ERR_ARG, "shear with RK3_TIMESTEP=DECIDE needs F_REMESH_T > 0 and F_DEFORM_S != 0");
-  if (p->p_interp_order < 2 || p->p_interp_order > 8 || (p->p_interp_order & 1))
-    return fail(PDR_ERR_UNSUPPORTED, "P_INTERP_ORDER must be even, 2..8");
+  if (p->p_interp_order < 1 || p->p_interp_order > 8)
+    return fail(PDR_ERR_UNSUPPORTED, "P_INTERP_ORDER must be 1..8");
   if (p->precision != PDR_FP64 && p->precision != PDR_FP32_MIXED)
     return fail(PDR_ERR_ARG, "precision must be PDR_FP64 or PDR_FP32_MIXED");
   if (p->b110 != 1) return fail(PDR_ERR_UNSUPPORTED, "B110 must be 1");
@@ -537,6 +539,18 @@ int pdr_sizes(const pdr_ctx* c, int64_t out[8]) {
   return PDR_OK;
 }
 
+int pdr_kernel_forms(const pdr_ctx* c, int32_t out[8]) {
+  if (!c || !out) return fail(PDR_ERR_ARG, "pdr_kernel_forms: null argument");
+  int f[5] = {0, 0, 0, 0, 0};
+  if (c->spec) (void)spec_forms(c->prec, c->g.nx, c->g.ny, c->g.nz, f);
+  out[0] = c->spec ? 1 : 0;
+  out[1] = c->g.zc;
+  out[2] = c->zs2 ? 1 : 0;
+  out[3] = (c->spec && c->fuse_zf) ? 1 : 0;
+  for (int q = 0; q < 4; ++q) out[4 + q] = f[q];
+  return PDR_OK;
+}
+
 static int put_field(pdr_ctx* c, void* dst, const double* src, long long n) {
   if (!src) return fail(PDR_ERR_ARG, "null host array");
   if (c->prec == PDR_FP64) {
@@ -650,6 +664,14 @@ static int maybe_remesh(pdr_ctx* c, int tstep, int32_t* done) {
   return PDR_OK;
 }
 
+// p_ControlInitPartVelocity[BF|R] at the release (src/pp_control.f90:378-510):
+// particle_up_rk3 = 0, particle_xp0 = particle_xp (particle_up is set by the advance)
+static hipError_t tracers_release(pdr_ctx* c, hipStream_t s) {
+  hipError_t e = hipMemsetAsync(c->uprk, 0, 24 * size_t(c->np), s);
+  if (e == hipSuccess && c->xp0) e = hipMemcpyAsync(c->xp0, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToDevice, s);
+  return e;
+}
+
 int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt) {
   (void)time;
   if (!c || !dt) return fail(PDR_ERR_ARG, "pdr_stage: null argument");
@@ -658,6 +680,11 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
     int rc = maybe_remesh(c, tstep, nullptr);
     if (rc != PDR_OK) return rc;
   }
+  // f_FluidstatsFs((tstep-1)) after the remesh check (src/pf_rk3.f90:58-60); the
+  // real-space moments then ride on this stage's x pass (1073-1078)
+  const bool sst = stage_stats_due(c, stage, tstep);
+  double* rsp = sst ? stage_stats_rs_partials(c) : nullptr;
+  if (sst) HIPCHK(stage_stats_wave(c, tstep));
   // f_RK3Stage1: p_StatsPrimary after the remesh, p_StatsRelativeVelocity after
   // the particle advance (src/pf_rk3.f90:62-99), for tstep > P_TS_RELEASE
   if (stage == 1 && c->np > 1 && c->sort_every > 0 && ++c->since_sort >= c->sort_every) {
@@ -669,14 +696,16 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   // spherical / LES truncation on entry to f_RK3PrepareRealSpace (src/pf_rk3.f90:269-277)
   HIPCHK(launch_dealias(c));
   const Coef cf = make_coef(c);
-  // particles move from tstep = P_TS_RELEASE on (src/pf_rk3.f90:1146)
-  const bool tracers = c->np > 0 && tstep >= c->ts_release;
+  // particles move from tstep = P_TS_RELEASE on (src/pf_rk3.f90:1146): the x pass
+  // writes their field, and the stage advances them unless the caller does
+  const bool field = c->np > 0 && tstep >= c->ts_release;
+  const bool tracers = field && c->tracers_in_stage;
   const bool need_cfl =
       stage == 1 && (c->p.rk3_timestep == PDR_RK3_TIMESTEP_CFL ||
                      (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE && c->p.f_type == PDR_F_ISOTROPIC));
   if (conv_form(c)) {
     if (need_cfl) HIPCHK(hipMemsetAsync(c->umax, 0, sizeof(unsigned long long), c->stream));
-    HIPCHK(launch_conv_front(c, cf, tracers, need_cfl, nullptr));
+    HIPCHK(launch_conv_front(c, cf, field, need_cfl, rsp));
     HIPCHK(launch_yfwd(c, 0, c->g.nz));
   } else {
     HIPCHK(launch_zinv(c, cf));
@@ -689,26 +718,36 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
       HIPCHK(hipStreamWaitEvent(c->zstream, c->ev_za, 0));
     }
     int chunk = 0;
-    for (int z0 = 0; z0 < c->g.nz; z0 += c->g.zc, ++chunk) {
+    hipError_t he = hipSuccess;
+    for (int z0 = 0; z0 < c->g.nz && he == hipSuccess; z0 += c->g.zc, ++chunk) {
       const int nzc = std::min(c->g.zc, c->g.nz - z0);
       const bool alt = two && (chunk & 1);
       if (alt) {
         std::swap(c->stream, c->zstream);
         std::swap(c->w2, c->w2z);
       }
-      hipError_t he = launch_yinv(c, z0, nzc);
-      if (he == hipSuccess) he = launch_xpass(c, cf, tracers, need_cfl, z0, nzc, nullptr);
+      he = launch_yinv(c, z0, nzc);
+      if (he == hipSuccess) he = launch_xpass(c, cf, field, need_cfl, z0, nzc, rsp);
       if (he == hipSuccess) he = launch_yfwd(c, z0, nzc);
       if (alt) {
         std::swap(c->stream, c->zstream);
         std::swap(c->w2, c->w2z);
       }
-      HIPCHK(he);
     }
+    // join the second stream also when a launch failed, so that nothing freed
+    // later is still in use there
     if (two) {
-      HIPCHK(hipEventRecord(c->ev_zb, c->zstream));
-      HIPCHK(hipStreamWaitEvent(c->stream, c->ev_zb, 0));
+      hipError_t je = hipEventRecord(c->ev_zb, c->zstream);
+      if (je == hipSuccess) je = hipStreamWaitEvent(c->stream, c->ev_zb, 0);
+      if (he == hipSuccess) he = je;
     }
+    HIPCHK(he);
+  }
+  if (sst) HIPCHK(stage_stats_rs_reduce(c));  // f_FluidstatsRsCollect's sums (src/pf_rk3.f90:1124-1127)
+  if (field) {
+    c->ur_tstep = tstep;
+    c->ur_stage = stage;
+    c->tf_ur = make_tracer_frame(c);
   }
   if (stage == 1) {
     // f_RK3Tstep (src/pf_rk3.f90:3852-3948)
@@ -745,19 +784,15 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
     // passes touch: it runs on the tracer stream beside them
     HIPCHK(hipEventRecord(c->ev_ur, c->stream));
     HIPCHK(hipStreamWaitEvent(c->tstream, c->ev_ur, 0));
-    if (stage == 1 && tstep == c->ts_release) {
-      // p_ControlInitPartVelocity[BF] at the release (src/pp_control.f90:378-510):
-      // particle_up_rk3 = 0, particle_xp0 = particle_xp (particle_up is set by the advance)
-      HIPCHK(hipMemsetAsync(c->uprk, 0, 24 * size_t(c->np), c->tstream));
-      if (c->xp0)
-        HIPCHK(hipMemcpyAsync(c->xp0, c->xp, 24 * size_t(c->np), hipMemcpyDeviceToDevice, c->tstream));
-    }
-    HIPCHK(launch_tracers(c, make_tracer_frame(c), beta, gdt, c->tstream));
+    if (stage == 1 && tstep == c->ts_release) HIPCHK(tracers_release(c, c->tstream));
+    HIPCHK(launch_tracers(c, c->tf_ur, beta, gdt, c->tstream));
     if (pstats) HIPCHK(launch_pstats_rel(c, tstep, c->tstream));
     HIPCHK(hipEventRecord(c->ev_tr, c->tstream));
-  } else if (pstats) {
+  } else if (pstats && !field) {
     HIPCHK(launch_pstats_rel(c, tstep, c->stream));
   }
+  // with the advance left to the caller, p_StatsRelativeVelocity follows its
+  // pdr_particles_finish_step
   if (c->forcing) HIPCHK(launch_force(c, tstep, *dt));  // f_ForceCalc (src/pf_rk3.f90:2063)
   if (c->spec && c->fuse_zf) {
     HIPCHK(launch_zfwd_formf(c, cf, beta, gdt));
@@ -768,6 +803,73 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   // and at the end of f_RK3SolveNavierStokes (src/pf_rk3.f90:2101-2109)
   HIPCHK(launch_dealias(c));
   if (tracers) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_tr, 0));
+  return PDR_OK;
+}
+
+int pdr_set_tracers_in_stage(pdr_ctx* c, int32_t on) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->tracers_in_stage = on != 0;
+  c->adv_stage = 0;
+  return PDR_OK;
+}
+
+// 0: p_RK3SolveParticle (isotropic), 1: p_RK3SolveParticleBF, 2: p_RK3SolveParticleR
+// (the branch p_ControlRK3 takes, src/pp_control.f90:556-574)
+static int tracer_form(const pdr_ctx* c) {
+  if (c->p.f_type == PDR_F_ISOTROPIC) return 0;
+  return rogallo_form(c) ? 2 : 1;
+}
+
+static int caller_advance_check(pdr_ctx* c, int32_t stage) {
+  if (stage < 1 || stage > 3) return fail(PDR_ERR_ARG, "stage must be 1, 2 or 3");
+  if (c->np <= 0) return fail(PDR_ERR_STATE, "no particles set");
+  if (c->tracers_in_stage)
+    return fail(PDR_ERR_STATE, "pdr_stage advances the tracers; call pdr_set_tracers_in_stage(ctx, 0) first");
+  if (c->ur_stage != stage)
+    return fail(PDR_ERR_STATE, "the tracer field is not that of this stage: call pdr_stage for it first");
+  return PDR_OK;
+}
+
+int pdr_particles_control_rk3(pdr_ctx* c, int32_t stage, int32_t tstep, double deltat) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  int rc = caller_advance_check(c, stage);
+  if (rc != PDR_OK) return rc;
+  if (c->ur_tstep != tstep) return fail(PDR_ERR_STATE, "the tracer field is not that of this tstep");
+  // p_ControlInitPartVelocity[BF|R] at the release step (src/pp_control.f90:540-551)
+  if (stage == 1 && tstep == c->ts_release) HIPCHK(tracers_release(c, c->stream));
+  double beta, gamma;
+  rk_coeffs(stage, &beta, &gamma);
+  HIPCHK(launch_tracers(c, c->tf_ur, beta, gamma * deltat, c->stream, false));
+  c->adv_stage = stage;
+  c->adv_tstep = tstep;
+  return PDR_OK;
+}
+
+int pdr_particles_solve(pdr_ctx* c, int32_t form, int64_t npin, int32_t stage, double deltat) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  if (form < 0 || form > 2) return fail(PDR_ERR_ARG, "form must be 0 (isotropic), 1 (BF) or 2 (R)");
+  if (form != tracer_form(c))
+    return fail(PDR_ERR_ARG, "this run's frame takes p_RK3SolveParticle" +
+                                 std::string(tracer_form(c) == 0 ? "" : (tracer_form(c) == 1 ? "BF" : "R")));
+  int rc = caller_advance_check(c, stage);
+  if (rc != PDR_OK) return rc;
+  if (npin != c->np) return fail(PDR_ERR_ARG, "npin must equal the number of tracers set (one box, one owner)");
+  double beta, gamma;
+  rk_coeffs(stage, &beta, &gamma);
+  HIPCHK(launch_tracers(c, c->tf_ur, beta, gamma * deltat, c->stream, false));
+  c->adv_stage = stage;
+  c->adv_tstep = c->ur_tstep;
+  return PDR_OK;
+}
+
+int pdr_particles_finish_step(pdr_ctx* c, int32_t tstep) {
+  if (!c) return fail(PDR_ERR_ARG, "null context");
+  if (c->np <= 0) return fail(PDR_ERR_STATE, "no particles set");
+  HIPCHK(launch_tracers_wrap(c, make_tracer_frame(c), c->stream));
+  // f_RK3Stage1: p_StatsRelativeVelocity after the stage-1 advance (src/pf_rk3.f90:88-97)
+  if (c->adv_stage == 1 && c->pst_on && tstep > c->ts_release) HIPCHK(launch_pstats_rel(c, tstep, c->stream));
+  c->adv_stage = 0;
   return PDR_OK;
 }
 

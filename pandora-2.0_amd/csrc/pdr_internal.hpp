@@ -139,6 +139,13 @@ struct pdr_ctx {
   uint32_t* pid = nullptr;
   int sort_every = 10;
   int since_sort = 0;
+  // tracers advanced inside pdr_stage (default) or by the caller through the
+  // p_ControlRK3 / p_RK3SolveParticle* entry points (the stage then only
+  // writes the tracer field)
+  bool tracers_in_stage = true;
+  int ur_tstep = -1, ur_stage = 0;  // the step and stage whose x pass wrote the tracer field
+  pdr::TracerFrame tf_ur{};           // the frame of that stage
+  int adv_stage = 0, adv_tstep = -1;  // the last caller-driven tracer advance
   void* sort_buf = nullptr;   // keys, indices and the radix-sort workspace
   size_t sort_bytes = 0;
   double* spare = nullptr;    // 3 * np doubles for the reordering
@@ -148,6 +155,13 @@ struct pdr_ctx {
   size_t stat_bytes = 0;
   void* rs_buf = nullptr;              // real-space moment partials (allocated on first use)
   size_t rs_bytes = 0;
+  // statistics of stage 1 inside the step (pdr_set_stage_stats): every sst_every
+  // steps, into a record buffer allocated by the set call
+  int sst_every = 0;
+  void* sst_buf = nullptr;
+  size_t sst_bytes = 0;
+  int sst_tstep = -1;    // the record's statistics step (tstep - 1, as the reference passes it)
+  double sst_b11 = 1.0;  // binv(1,1) of the record's frame
   double* force = nullptr;             // OU forcing state, [i][j+NK][k+NK][c] complex FP64
   int force_nk = 0;                    // NK = aint(FORCE_KF)
   bool forcing = false;                // F_TYPE 1 with FORCE_STOP > 0
@@ -198,7 +212,11 @@ hipError_t launch_zfwd(pdr_ctx* c);
 hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
 hipError_t launch_zfwd_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt);
 hipError_t launch_force(pdr_ctx* c, int32_t tstep, double dt);
-hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s);
+// the tracer advance (interpolation, RK3 update, particle_up); wrap: also the
+// periodic correction of p_ControlFinishTimeStep (moving particle_xp0 with it)
+hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s,
+                          bool wrap = true);
+hipError_t launch_tracers_wrap(pdr_ctx* c, const TracerFrame& tf, hipStream_t s);
 hipError_t launch_remesh(pdr_ctx* c, int bfrot);
 hipError_t launch_remesh_particles(pdr_ctx* c, double lx, double dx);
 // batched ensemble passes (pdr_kernels2.hip)
@@ -220,6 +238,12 @@ hipError_t launch_pstats_primary(pdr_ctx* c, double dt);
 hipError_t launch_pstats_rel(pdr_ctx* c, int tstep, hipStream_t s);
 int pstats_read(pdr_ctx* c, int32_t* tstep, double* rec);
 hipError_t launch_finite(pdr_ctx* c, int* flag);
+// statistics of stage 1 inside the step (pdr_stats.hip)
+bool stage_stats_due(const pdr_ctx* c, int stage, int tstep);
+hipError_t stage_stats_wave(pdr_ctx* c, int tstep);
+double* stage_stats_rs_partials(pdr_ctx* c);
+hipError_t stage_stats_rs_reduce(pdr_ctx* c);
+void stage_stats_free(pdr_ctx* c);
 hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s);
 
 // profiling hooks (pdr_api.hip)
@@ -230,6 +254,7 @@ void fft_plan(FftPlan* p, int n);
 
 // length-specialised kernels (pdr_kernels2.hip)
 bool spec_shape_ok(int nx, int ny, int nz);
+int spec_forms(int prec, int nx, int ny, int nz, int out[5]);
 bool spec_supported(int n);
 size_t spec_lds_need(int prec, int nx, int ny, int nz);
 hipError_t launch2_zinv(pdr_ctx* c, const Coef& cf);

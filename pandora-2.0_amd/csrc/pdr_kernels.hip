@@ -568,6 +568,38 @@ __device__ __forceinline__ long long pmod(long long a, long long n) {
   return r < 0 ? r + n : r;
 }
 
+// p_MPICorrectPosition (src/pp_mpi.f90:1011-1057): one shift per axis into
+// [-dx/2, L - dx/2); the x3 upper test is '>', so x3 = x3max stays.
+__device__ __forceinline__ void wrap_position(const TracerFrame& tf, double x[3]) {
+  const double x1min = -0.5 * tf.dx, x1max = x1min + tf.lx;
+  const double x2min = -0.5 * tf.dy, x2max = x2min + tf.ly;
+  const double x3min = -0.5 * tf.dz, x3max = x3min + tf.lz;
+  if (x[0] < x1min) x[0] = x[0] + tf.lx;
+  if (x[0] >= x1max) x[0] = x[0] - tf.lx;
+  if (x[1] < x2min) x[1] = x[1] + tf.ly;
+  if (x[1] >= x2max) x[1] = x[1] - tf.ly;
+  if (x[2] < x3min) x[2] = x[2] + tf.lz;
+  if (x[2] > x3max) x[2] = x[2] - tf.lz;
+}
+
+// p_ControlFinishTimeStep alone (src/pp_control.f90:1440-1450 ->
+// p_ControlUpdateCount, 1454-1536): the periodic correction of every position,
+// with particle_xp0 moved by the same shift (p_ControlUpdateInitial,
+// src/pp_control.f90:1574-1585).  One thread per tracer.
+__global__ void __launch_bounds__(kThreads)
+    k_tracers_wrap(TracerFrame tf, double* __restrict__ xpa, double* __restrict__ xp0, long long np) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  double x[3] = {xpa[3 * q], xpa[3 * q + 1], xpa[3 * q + 2]};
+  const double xu[3] = {x[0], x[1], x[2]};
+  wrap_position(tf, x);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    xpa[3 * q + m] = x[m];
+    if (xp0) xp0[3 * q + m] = (xp0[3 * q + m] + x[m]) - xu[m];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tracers, one lane per x column of the molecule: a group of G lanes (G = P
 // rounded up to a power of two) serves one tracer; lane a < P gathers the
@@ -582,8 +614,8 @@ template <typename T, int P>
 __global__ void __launch_bounds__(kThreads)
     k_tracers_col(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
                   double* __restrict__ uprk, double* __restrict__ upa, double* __restrict__ xp0,
-                  long long np, double beta, double gdt) {
-  constexpr int G = P <= 2 ? 2 : (P <= 4 ? 4 : 8);
+                  long long np, double beta, double gdt, int wrap) {
+  constexpr int G = P <= 1 ? 1 : (P <= 2 ? 2 : (P <= 4 ? 4 : 8));
   const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long q = gt / G;
   const int a = int(gt % G);
@@ -591,14 +623,23 @@ __global__ void __launch_bounds__(kThreads)
   const long long qq = live ? q : np - 1;  // np >= 1: dead lanes read a valid tracer
   double x[3] = {xpa[3 * qq], xpa[3 * qq + 1], xpa[3 * qq + 2]};
   const int nhalo = P / 2;
-  // g_DomainInitHalo (src/pg_domain.f90:360-384), p_InterpFindMolecule (src/pp_interp.f90:406-467)
+  // g_DomainInitHalo (src/pg_domain.f90:360-384), p_InterpFindMolecule (src/pp_interp.f90:406-467):
+  // even orders shift the halo length by half a cell and the search by one and a half,
+  // odd orders only the search by half a cell (the molecule is centred on the nearest node)
   double lxh = double(nhalo) * tf.dx, lyh = double(nhalo) * tf.dy, lzh = double(nhalo) * tf.dz;
-  lxh = lxh - (0.5 * tf.dx);
-  lyh = lyh - (0.5 * tf.dy);
-  lzh = lzh - (0.5 * tf.dz);
-  const double xd1 = x[0] + lxh + tf.dx + (0.5 * tf.dx);
-  const double xd2 = x[1] + lyh + tf.dy + (0.5 * tf.dy);
-  const double xd3 = x[2] + lzh + tf.dz + (0.5 * tf.dz);
+  double xd1, xd2, xd3;
+  if constexpr (P % 2 == 0) {
+    lxh = lxh - (0.5 * tf.dx);
+    lyh = lyh - (0.5 * tf.dy);
+    lzh = lzh - (0.5 * tf.dz);
+    xd1 = x[0] + lxh + tf.dx + (0.5 * tf.dx);
+    xd2 = x[1] + lyh + tf.dy + (0.5 * tf.dy);
+    xd3 = x[2] + lzh + tf.dz + (0.5 * tf.dz);
+  } else {
+    xd1 = x[0] + lxh + (0.5 * tf.dx);
+    xd2 = x[1] + lyh + (0.5 * tf.dy);
+    xd3 = x[2] + lzh + (0.5 * tf.dz);
+  }
   const int i0 = int(xd1 / tf.dx) - nhalo - nhalo;
   const int j0 = int(xd2 / tf.dy) - nhalo - nhalo;
   const int k0 = int(xd3 / tf.dz) - nhalo - nhalo;
@@ -708,23 +749,14 @@ __global__ void __launch_bounds__(kThreads)
     x[1] = xm[1];
     x[2] = xm[2];
   }
-  // p_MPICorrectPosition (src/pp_mpi.f90:1011-1057): one shift, x3 upper test is '>'
-  const double x1min = -0.5 * tf.dx, x1max = x1min + tf.lx;
-  const double x2min = -0.5 * tf.dy, x2max = x2min + tf.ly;
-  const double x3min = -0.5 * tf.dz, x3max = x3min + tf.lz;
   const double xu[3] = {x[0], x[1], x[2]};
-  if (x[0] < x1min) x[0] = x[0] + tf.lx;
-  if (x[0] >= x1max) x[0] = x[0] - tf.lx;
-  if (x[1] < x2min) x[1] = x[1] + tf.ly;
-  if (x[1] >= x2max) x[1] = x[1] - tf.ly;
-  if (x[2] < x3min) x[2] = x[2] + tf.lz;
-  if (x[2] > x3max) x[2] = x[2] - tf.lz;
+  if (wrap) wrap_position(tf, x);  // p_ControlFinishTimeStep, fused into the advance
   if (live && a < 3) {
     const double xa = (a == 0) ? x[0] : ((a == 1) ? x[1] : x[2]);
     xpa[3 * q + a] = xa;
     uprk[3 * q + a] = (a == 0) ? r[0] : ((a == 1) ? r[1] : r[2]);
     upa[3 * q + a] = (a == 0) ? up[0] : ((a == 1) ? up[1] : up[2]);
-    if (xp0) {
+    if (xp0 && wrap) {
       // p_ControlUpdateInitial (src/pp_control.f90:1574-1585): xp0 + x_wrapped - x,
       // left to right, on every stage
       const double xb = (a == 0) ? xu[0] : ((a == 1) ? xu[1] : xu[2]);
@@ -1019,28 +1051,42 @@ hipError_t launch_formf(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
 }
 
 template <typename T>
-static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s) {
+static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s,
+                            int wrap) {
   const T* ur = (const T*)c->ur;
   auto go = [&](auto kern, int lanes) {
     const long long nb = (lanes * c->np + kThreads - 1) / kThreads;
     kern<<<dim3((unsigned)nb), kThreads, 0, s>>>(c->g, tf, ur, c->xp, c->uprk, c->up, c->xp0, c->np,
-                                                         beta, gdt);
+                                                         beta, gdt, wrap);
   };
   switch (tf.order) {
+    case 1: go(k_tracers_col<T, 1>, 1); break;
     case 2: go(k_tracers_col<T, 2>, 2); break;
+    case 3: go(k_tracers_col<T, 3>, 4); break;
+    case 5: go(k_tracers_col<T, 5>, 8); break;
     case 6: go(k_tracers_col<T, 6>, 8); break;
+    case 7: go(k_tracers_col<T, 7>, 8); break;
     case 8: go(k_tracers_col<T, 8>, 8); break;
     default: go(k_tracers_col<T, 4>, 4); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s) {
+hipError_t launch_tracers(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s,
+                          bool wrap) {
   if (c->np <= 0) return hipSuccess;
   prof_begin(c, K_TRACER, s);
-  hipError_t e = c->prec ? tracers_t<float>(c, tf, beta, gdt, s) : tracers_t<double>(c, tf, beta, gdt, s);
+  hipError_t e = c->prec ? tracers_t<float>(c, tf, beta, gdt, s, wrap ? 1 : 0)
+                         : tracers_t<double>(c, tf, beta, gdt, s, wrap ? 1 : 0);
   prof_end(c, K_TRACER, s);
   return e;
+}
+
+hipError_t launch_tracers_wrap(pdr_ctx* c, const TracerFrame& tf, hipStream_t s) {
+  if (c->np <= 0) return hipSuccess;
+  const long long nb = (c->np + kThreads - 1) / kThreads;
+  k_tracers_wrap<<<dim3((unsigned)nb), kThreads, 0, s>>>(tf, c->xp, c->xp0, c->np);
+  return hipGetLastError();
 }
 
 template <typename T>

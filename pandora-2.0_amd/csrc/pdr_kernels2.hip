@@ -4,6 +4,8 @@
 // each pass replaces.  Differences: compile-time transform length and tile
 // height, in-place LDS transforms, twiddles in LDS, 128-byte tile chunks on
 // 16-element-padded strides, XCD-aware tile order, several x rows per block.
+#include <type_traits>
+
 #include "pdr_fft2.hpp"
 #include "pdr_internal.hpp"
 #include "pdr_rsstats.hpp"
@@ -1213,6 +1215,55 @@ int xpass_tiles(const pdr_ctx* c) {
     default:
       return c->g.ny;
   }
+}
+
+// Compile-time forms the specialised kernels take for this box: out[0] x rows
+// per x-pass workgroup, out[1] 1 if the x tile reads its twiddles from the
+// global table, out[2] 1 for the register z inverse (velocity tile loaded once),
+// out[3] lines per z-inverse tile, out[4] lines per inverse y tile (or whole
+// planes x NKI for short lines).  Returns -1 for lengths without a specialised
+// kernel.
+int spec_forms(int prec, int nx, int ny, int nz, int out[5]) {
+  int ok = 0;
+  auto fx = [&](auto tag, auto nn) {
+    using T = decltype(tag);
+    constexpr int n = decltype(nn)::value;
+    out[0] = xrows<T>(n);
+    out[1] = x_tw_global(n) ? 1 : 0;
+    ++ok;
+  };
+  auto fz = [&](auto tag, auto nn) {
+    using T = decltype(tag);
+    constexpr int n = decltype(nn)::value;
+    out[2] = (PDR_ZINV_REG && n <= 512) ? 1 : 0;
+    out[3] = (PDR_ZINV_REG && n <= 512) ? zinv_lines<T>(n) : tile_lines<T>(n);
+    ++ok;
+  };
+  auto fy = [&](auto tag, auto nn) {
+    using T = decltype(tag);
+    constexpr int n = decltype(nn)::value;
+    out[4] = y_lines<T>(n);
+    ++ok;
+  };
+  auto dispatch = [&](int n, auto f) {
+    switch (n) {
+#define PDR_SF(n)                                                    \
+  case n:                                                            \
+    if (prec)                                                        \
+      f(float(0), std::integral_constant<int, n>{});                 \
+    else                                                             \
+      f(double(0), std::integral_constant<int, n>{});                \
+    break;
+      PDR_SIZES(PDR_SF)
+#undef PDR_SF
+      default:
+        break;
+    }
+  };
+  dispatch(nx, fx);
+  dispatch(nz, fz);
+  dispatch(ny, fy);
+  return ok == 3 ? 0 : -1;
 }
 
 // The whole-plane tiles of the y passes of short lines take the i-lines per

@@ -22,6 +22,11 @@
 #include "pdr_rsstats.hpp"
 
 namespace pdr {
+
+void spectra_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6],
+                    const std::vector<double>& s12, double b11, double* spec_1i, double* spec_2i,
+                    pdr_homog_t* h);
+
 namespace {
 
 constexpr int kStatThreads = 256;
@@ -186,12 +191,7 @@ int kkmax_of(const Geo& g) {
   return int(std::llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk))));
 }
 
-// raw[7 * kkmax] (ken, ken1, ken2, ken3, dis, disskew, count) and uu[6]
-int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6], std::vector<double>* s12 = nullptr) {
-  const int kkmax = kkmax_of(c->g);
-  const int nki = c->g.nki;
-  const size_t lds = size_t(stats_hist(kkmax, nki) + 12 * kStatThreads) * sizeof(double);
-  if (lds > c->lds_max) return set_error(PDR_ERR_UNSUPPORTED, "statistics: spectrum too long for the LDS");
+StatsFrame make_stats_frame(const pdr_ctx* c) {
   StatsFrame sf;
   for (int m = 1; m <= 3; ++m)
     for (int n = 1; n <= 3; ++n) {
@@ -199,12 +199,53 @@ int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6], std::vector<do
       sf.bi[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->binv, m, n) : 0.0;
     }
   sf.nu = c->p.f_nu;
-  sf.kkmax = kkmax;
-  sf.nki = nki;
+  sf.kkmax = kkmax_of(c->g);
+  sf.nki = c->g.nki;
   sf.lab_state = rogallo_form(c) ? 1 : 0;
+  return sf;
+}
+
+// workgroups of the statistics pass and the width of one partial row
+int stats_nblk(const pdr_ctx* c) {
   const long long want = (c->g.mw + 4LL * kStatThreads - 1) / (4LL * kStatThreads);
-  const int nblk = int(want < kStatBlocks ? (want < 1 ? 1 : want) : kStatBlocks);
-  const int width = stats_hist(kkmax, nki) + 12;
+  return int(want < kStatBlocks ? (want < 1 ? 1 : want) : kStatBlocks);
+}
+int stats_width(const pdr_ctx* c) { return stats_hist(kkmax_of(c->g), c->g.nki) + 12; }
+
+bool stats_fits_lds(const pdr_ctx* c) {
+  const size_t lds = size_t(stats_hist(kkmax_of(c->g), c->g.nki) + 12 * kStatThreads) * sizeof(double);
+  return lds <= c->lds_max;
+}
+
+// the reduced row h[stats_width] of the pass -> raw[7 * kkmax] (ken, ken1, ken2,
+// ken3, dis, disskew, count), uu[6] and, on request, the streamwise bins
+void stats_unpack(const pdr_ctx* c, const double* h, std::vector<double>& raw, double uu[6],
+                  std::vector<double>* s12) {
+  const int kkmax = kkmax_of(c->g);
+  const int nki = c->g.nki;
+  raw.assign(size_t(7) * kkmax, 0.0);
+  for (int a = 0; a < 6; ++a)
+    for (int q = 0; q < kkmax; ++q)
+      raw[size_t(a * kkmax + q)] = h[size_t(a * kkmax + q)] + h[size_t((6 + a) * kkmax + q)];
+  for (int q = 0; q < kkmax; ++q) raw[size_t(6 * kkmax + q)] = h[size_t(12 * kkmax + q)];
+  const size_t nh = size_t(stats_hist(kkmax, nki));
+  for (int a = 0; a < 6; ++a) uu[a] = h[nh + a] + h[nh + 6 + a];
+  if (s12) {  // [s1 over i, s2 over i]
+    s12->assign(size_t(2) * nki, 0.0);
+    const size_t o = size_t(13) * kkmax;
+    for (int i = 0; i < nki; ++i) {
+      (*s12)[size_t(i)] = h[o + i] + h[o + nki + i];
+      (*s12)[size_t(nki + i)] = h[o + 2 * nki + i] + h[o + 3 * nki + i];
+    }
+  }
+}
+
+// raw[7 * kkmax] (ken, ken1, ken2, ken3, dis, disskew, count) and uu[6]
+int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6], std::vector<double>* s12 = nullptr) {
+  if (!stats_fits_lds(c)) return set_error(PDR_ERR_UNSUPPORTED, "statistics: spectrum too long for the LDS");
+  const StatsFrame sf = make_stats_frame(c);
+  const int nblk = stats_nblk(c);
+  const int width = stats_width(c);
   const size_t need = (size_t(nblk) + 1) * size_t(width) * sizeof(double);
   if (c->stat_bytes < need) {
     if (c->stat_buf) {
@@ -229,22 +270,13 @@ int stats_raw(pdr_ctx* c, std::vector<double>& raw, double uu[6], std::vector<do
   e = hipMemcpyAsync(h.data(), out, size_t(width) * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return set_error(PDR_ERR_HIP, std::string("statistics copy: ") + hipGetErrorString(e));
-  raw.assign(size_t(7) * kkmax, 0.0);
-  for (int a = 0; a < 6; ++a)
-    for (int q = 0; q < kkmax; ++q)
-      raw[size_t(a * kkmax + q)] = h[size_t(a * kkmax + q)] + h[size_t((6 + a) * kkmax + q)];
-  for (int q = 0; q < kkmax; ++q) raw[size_t(6 * kkmax + q)] = h[size_t(12 * kkmax + q)];
-  const size_t nh = size_t(stats_hist(kkmax, nki));
-  for (int a = 0; a < 6; ++a) uu[a] = h[nh + a] + h[nh + 6 + a];
-  if (s12) {  // [s1 over i, s2 over i]
-    s12->assign(size_t(2) * nki, 0.0);
-    const size_t o = size_t(13) * kkmax;
-    for (int i = 0; i < nki; ++i) {
-      (*s12)[size_t(i)] = h[o + i] + h[o + nki + i];
-      (*s12)[size_t(nki + i)] = h[o + 2 * nki + i] + h[o + 3 * nki + i];
-    }
-  }
+  stats_unpack(c, h.data(), raw, uu, s12);
   return PDR_OK;
+}
+
+// workgroup partials of the real-space moments: one per x-pass workgroup
+long long rs_nblk(const pdr_ctx* c) {
+  return c->spec ? (long long)xpass_tiles(c) * c->g.nz : (long long)c->g.ny * c->g.nz;
 }
 
 // Column q of the real-space partials: a sum (q < kRsSums, Neumaier per thread,
@@ -291,7 +323,7 @@ namespace pdr {
 namespace {
 
 // Mode-density normalisation and secondary quantities of f_FluidstatsFs
-void stats_finish(pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6], pdr_stats_t* q,
+void stats_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6], pdr_stats_t* q,
                   double* spec) {
   const int kkmax = pdr::kkmax_of(ctx->g);
   const double nu = ctx->p.f_nu;
@@ -381,9 +413,67 @@ int pdr_spectra_1d(pdr_ctx* ctx, double* spec_1i, double* spec_2i, pdr_homog_t* 
   double uu[6];
   int rc = pdr::stats_raw(ctx, raw, uu, &s12);
   if (rc != PDR_OK) return rc;
+  pdr::spectra_finish(ctx, raw, uu, s12, T9(ctx->binv, 1, 1), spec_1i, spec_2i, h);
+  return PDR_OK;
+}
+
+}  // extern "C"
+
+namespace pdr {
+namespace {
+
+// rs_raw[kRsWidth] (the reduced real-space partials) -> the pdr_rs_stats_t record
+// of f_FluidstatsRsCollect (src/pf_fluidstats.f90:992-1116); rs_rmsvort1 is
+// not divided by the number of points, as there
+void rs_finish(const pdr_ctx* c, const double* raw, pdr_rs_stats_t* out) {
+  const pdr::Geo& g = c->g;
+  const double n3 = (double(g.nx) * double(g.ny)) * double(g.nz);
+  double uu[6], sk[3], fl[3], rms[3], skv[3], flv[3];
+  for (int q = 0; q < 6; ++q) uu[q] = raw[q] / n3;
+  for (int q = 0; q < 3; ++q) {
+    sk[q] = (raw[6 + q] / n3) / std::pow(uu[q], 1.5);
+    fl[q] = (raw[9 + q] / n3) / (uu[q] * uu[q]);
+    rms[q] = (q == 0) ? raw[12] : raw[12 + q] / n3;
+    rms[q] = std::pow(rms[q], 0.5);
+    skv[q] = (raw[15 + q] / n3) / (rms[q] * rms[q] * rms[q]);
+    flv[q] = (raw[18 + q] / n3) / ((rms[q] * rms[q]) * (rms[q] * rms[q]));
+  }
+  out->rs_u1u1 = uu[0];
+  out->rs_u2u2 = uu[1];
+  out->rs_u3u3 = uu[2];
+  out->rs_u1u2 = uu[3];
+  out->rs_u1u3 = uu[4];
+  out->rs_u2u3 = uu[5];
+  out->rs_skew1 = sk[0];
+  out->rs_skew2 = sk[1];
+  out->rs_skew3 = sk[2];
+  out->rs_flat1 = fl[0];
+  out->rs_flat2 = fl[1];
+  out->rs_flat3 = fl[2];
+  out->rs_rmsvort1 = rms[0];
+  out->rs_rmsvort2 = rms[1];
+  out->rs_rmsvort3 = rms[2];
+  out->rs_skewvort1 = skv[0];
+  out->rs_skewvort2 = skv[1];
+  out->rs_skewvort3 = skv[2];
+  out->rs_flatvort1 = flv[0];
+  out->rs_flatvort2 = flv[1];
+  out->rs_flatvort3 = flv[2];
+  out->omegamax1 = raw[22];
+  out->omegamax2 = raw[23];
+  out->omegamax3 = raw[24];
+  out->ucfl = raw[21];
+}
+
+}  // namespace
+
+// streamwise 1-D spectra and the homogeneous-shear record from the unpacked
+// statistics pass; b11 = binv(1,1) of the frame the pass ran in
+void spectra_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6],
+                    const std::vector<double>& s12, double b11, double* spec_1i, double* spec_2i,
+                    pdr_homog_t* h) {
   const pdr::Geo& g = ctx->g;
   const bool shear = ctx->p.f_type != PDR_F_ISOTROPIC;
-  const double b11 = T9(ctx->binv, 1, 1);
   const double f = 2.0 * (double(g.ny) * double(g.nz)) / (double(g.nx) * double(g.nx)) *
                    (shear ? (b11 * b11) * b11 : 1.0);
   std::vector<double> s1(size_t(g.nki)), s2(size_t(g.nki));
@@ -417,14 +507,17 @@ int pdr_spectra_1d(pdr_ctx* ctx, double* spec_1i, double* spec_2i, pdr_homog_t* 
     h->fn_int2 = q.fn_int2;
     h->fn_int3 = q.fn_int3;
   }
-  return PDR_OK;
 }
+
+}  // namespace pdr
+
+extern "C" {
 
 int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out) {
   if (!ctx || !out) return pdr::set_error(PDR_ERR_ARG, "pdr_rs_stats: null argument");
   pdr_ctx* c = ctx;
   const pdr::Geo& g = c->g;
-  const long long nblk = c->spec ? (long long)pdr::xpass_tiles(c) * g.nz : (long long)g.ny * g.nz;
+  const long long nblk = pdr::rs_nblk(c);
   const size_t need = size_t(nblk + 1) * pdr::kRsWidth * sizeof(double);
   if (c->rs_bytes < need) {
     if (c->rs_buf) {
@@ -465,44 +558,7 @@ int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out) {
   if (e == hipSuccess) e = hipMemcpyAsync(raw, dres, sizeof(raw), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) return pdr::set_error(PDR_ERR_HIP, std::string("pdr_rs_stats: ") + hipGetErrorString(e));
-  // f_FluidstatsRsCollect (src/pf_fluidstats.f90:992-1116); rs_rmsvort1 is not
-  // divided by the number of points, as there
-  const double n3 = (double(g.nx) * double(g.ny)) * double(g.nz);
-  double uu[6], sk[3], fl[3], rms[3], skv[3], flv[3];
-  for (int q = 0; q < 6; ++q) uu[q] = raw[q] / n3;
-  for (int q = 0; q < 3; ++q) {
-    sk[q] = (raw[6 + q] / n3) / std::pow(uu[q], 1.5);
-    fl[q] = (raw[9 + q] / n3) / (uu[q] * uu[q]);
-    rms[q] = (q == 0) ? raw[12] : raw[12 + q] / n3;
-    rms[q] = std::pow(rms[q], 0.5);
-    skv[q] = (raw[15 + q] / n3) / (rms[q] * rms[q] * rms[q]);
-    flv[q] = (raw[18 + q] / n3) / ((rms[q] * rms[q]) * (rms[q] * rms[q]));
-  }
-  out->rs_u1u1 = uu[0];
-  out->rs_u2u2 = uu[1];
-  out->rs_u3u3 = uu[2];
-  out->rs_u1u2 = uu[3];
-  out->rs_u1u3 = uu[4];
-  out->rs_u2u3 = uu[5];
-  out->rs_skew1 = sk[0];
-  out->rs_skew2 = sk[1];
-  out->rs_skew3 = sk[2];
-  out->rs_flat1 = fl[0];
-  out->rs_flat2 = fl[1];
-  out->rs_flat3 = fl[2];
-  out->rs_rmsvort1 = rms[0];
-  out->rs_rmsvort2 = rms[1];
-  out->rs_rmsvort3 = rms[2];
-  out->rs_skewvort1 = skv[0];
-  out->rs_skewvort2 = skv[1];
-  out->rs_skewvort3 = skv[2];
-  out->rs_flatvort1 = flv[0];
-  out->rs_flatvort2 = flv[1];
-  out->rs_flatvort3 = flv[2];
-  out->omegamax1 = raw[22];
-  out->omegamax2 = raw[23];
-  out->omegamax3 = raw[24];
-  out->ucfl = raw[21];
+  pdr::rs_finish(c, raw, out);
   return PDR_OK;
 }
 
@@ -530,6 +586,114 @@ int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6
   tot[3] = uu[0];
   tot[4] = uu[1];
   tot[5] = uu[2];
+  return PDR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Statistics of stage 1 inside the step (f_RK3Stage1 and f_RK3RealSpace,
+// src/pf_rk3.f90:58-60, 1073-1078, 1124-1143): the wave-space pass of the state
+// after the remesh check, and the real-space moments accumulated by the stage's
+// own x pass.  Both reduce into a record buffer the context owns; nothing is
+// allocated or copied to the host during the step.
+// Record buffer: [wave partials nblk x width][wave row width][real-space
+// partials rs_nblk x kRsWidth][real-space row kRsWidth].
+// ---------------------------------------------------------------------------
+namespace pdr {
+
+static size_t sst_need(const pdr_ctx* c) {
+  return (size_t(stats_nblk(c)) + 1) * size_t(stats_width(c)) * sizeof(double) +
+         (size_t(rs_nblk(c)) + 1) * size_t(kRsWidth) * sizeof(double);
+}
+
+static double* sst_wave_part(pdr_ctx* c) { return (double*)c->sst_buf; }
+static double* sst_wave_row(pdr_ctx* c) { return sst_wave_part(c) + size_t(stats_nblk(c)) * stats_width(c); }
+double* stage_stats_rs_partials(pdr_ctx* c) { return sst_wave_row(c) + stats_width(c); }
+static double* sst_rs_row(pdr_ctx* c) { return stage_stats_rs_partials(c) + size_t(rs_nblk(c)) * kRsWidth; }
+
+bool stage_stats_due(const pdr_ctx* c, int stage, int tstep) {
+  return stage == 1 && c->sst_every > 0 && c->sst_buf && (tstep - 1) % c->sst_every == 0;
+}
+
+hipError_t stage_stats_wave(pdr_ctx* c, int tstep) {
+  const StatsFrame sf = make_stats_frame(c);
+  c->sst_tstep = tstep - 1;
+  c->sst_b11 = T9(c->binv, 1, 1);
+  return c->prec ? stats_t<float>(c, sf, stats_nblk(c), sst_wave_part(c), sst_wave_row(c))
+                 : stats_t<double>(c, sf, stats_nblk(c), sst_wave_part(c), sst_wave_row(c));
+}
+
+hipError_t stage_stats_rs_reduce(pdr_ctx* c) {
+  k_rs_reduce<<<dim3(unsigned(kRsWidth)), kStatThreads, 0, c->stream>>>(stage_stats_rs_partials(c), rs_nblk(c),
+                                                                          sst_rs_row(c));
+  return hipGetLastError();
+}
+
+void stage_stats_free(pdr_ctx* c) {
+  if (c->sst_buf) {
+    (void)hipFree(c->sst_buf);
+    c->dev_bytes -= c->sst_bytes;
+  }
+  c->sst_buf = nullptr;
+  c->sst_bytes = 0;
+}
+
+}  // namespace pdr
+
+extern "C" {
+
+int pdr_set_stage_stats(pdr_ctx* c, int32_t every) {
+  if (!c || every < 0) return pdr::set_error(PDR_ERR_ARG, "pdr_set_stage_stats: bad argument");
+  if (every > 0 && !pdr::stats_fits_lds(c))
+    return pdr::set_error(PDR_ERR_UNSUPPORTED, "statistics: spectrum too long for the LDS");
+  if (hipStreamSynchronize(c->stream) != hipSuccess)
+    return pdr::set_error(PDR_ERR_HIP, "pdr_set_stage_stats: stream synchronisation failed");
+  c->sst_every = every;
+  c->sst_tstep = -1;
+  if (every == 0) {
+    pdr::stage_stats_free(c);
+    return PDR_OK;
+  }
+  const size_t need = pdr::sst_need(c);
+  if (c->sst_bytes < need) {
+    pdr::stage_stats_free(c);
+    if (hipMalloc(&c->sst_buf, need) != hipSuccess) {
+      (void)hipGetLastError();
+      c->sst_buf = nullptr;
+      c->sst_every = 0;
+      return pdr::set_error(PDR_ERR_NOMEM, "pdr_set_stage_stats: device allocation failed");
+    }
+    c->sst_bytes = need;
+    c->dev_bytes += need;
+  }
+  return PDR_OK;
+}
+
+int pdr_stage_stats(pdr_ctx* c, int32_t* tstep, pdr_stats_t* fs, double* spec, pdr_rs_stats_t* rs,
+                    double* spec_1i, double* spec_2i, pdr_homog_t* h) {
+  if (!c) return pdr::set_error(PDR_ERR_ARG, "null context");
+  if (tstep) *tstep = c->sst_tstep;
+  if (c->sst_tstep < 0) return PDR_OK;
+  const int width = pdr::stats_width(c);
+  std::vector<double> row(size_t(width) + pdr::kRsWidth);
+  hipError_t e = hipMemcpyAsync(row.data(), pdr::sst_wave_row(c), size_t(width) * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(row.data() + width, pdr::sst_rs_row(c), pdr::kRsWidth * sizeof(double),
+                       hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return pdr::set_error(PDR_ERR_HIP, std::string("pdr_stage_stats: ") + hipGetErrorString(e));
+  std::vector<double> raw, s12;
+  double uu[6];
+  pdr::stats_unpack(c, row.data(), raw, uu, &s12);
+  if (fs || spec) {
+    pdr_stats_t q;
+    pdr::stats_finish(c, raw, uu, &q, spec);
+    if (fs) *fs = q;
+  }
+  if (spec_1i || spec_2i || h) pdr::spectra_finish(c, raw, uu, s12, c->sst_b11, spec_1i, spec_2i, h);
+  if (rs) pdr::rs_finish(c, row.data() + width, rs);
   return PDR_OK;
 }
 

@@ -3,7 +3,11 @@
 !> ISO_C_BINDING shim that keeps the reference's time-step interfaces
 !! (g_RK3Init / g_RK3Calc / g_RK3UpdateB in src/pg_rk3.f90:46-385 and
 !! f_RK3Stage1/2/3 in src/pf_rk3.f90:40-200) and runs them on the MI355X
-!! library declared in include/pandora_amd.h.  The Fortran caller keeps its
+!! library declared in include/pandora_amd.h, together with the tracer entry
+!! points p_ControlRK3 / p_RK3SolveParticle[BF|R] / p_ControlFinishTimeStep
+!! (src/pp_control.f90:518, 1440; src/pp_rk3.f90:57, 169, 307), which advance
+!! the tracers on the device when the caller drives them itself
+!! (pdr_SetTracersInStage(.false.)).  The Fortran caller keeps its
 !! TIMELOOP (src/pg_control.f90:177-194) and its pf_arrays-layout arrays
 !! arr(0:1, 0:NKK-1, 0:NKJ-1, 0:NKI-1); the whole box lives on one GPU.
 !!
@@ -51,6 +55,9 @@ module pdr_rk3_amd
   public :: pdr_InitField, pdr_RestartWrite, pdr_RestartRead, pdr_RestartReadScaled, pdr_Spectrum, pdr_ErrorMessage
   public :: pdr_GetFrame, pdr_PutForcing, pdr_GetForcing, pdr_Stats, pdr_RsStats
   public :: pdr_ParticleStatsConfig, pdr_ParticleStats, pdr_GetParticlesAux
+  public :: pdr_InitParticles, pdr_SetTracersInStage
+  public :: p_ControlRK3, p_ControlFinishTimeStep
+  public :: p_RK3SolveParticle, p_RK3SolveParticleBF, p_RK3SolveParticleR
 
   interface
     integer(c_int) function c_pdr_create(out, p) bind(C, name="pdr_create")
@@ -179,6 +186,36 @@ module pdr_rk3_amd
     integer(c_int) function c_pdr_particles_download_aux(c, up, xp0) bind(C, name="pdr_particles_download_aux")
       import :: c_int, c_ptr
       type(c_ptr), value :: c, up, xp0
+    end function
+    integer(c_int) function c_pdr_init_particles(c, seed, np) bind(C, name="pdr_init_particles")
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr), value :: c
+      integer(c_int64_t), value :: seed, np
+    end function
+    integer(c_int) function c_pdr_set_tracers_in_stage(c, on) bind(C, name="pdr_set_tracers_in_stage")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: on
+    end function
+    integer(c_int) function c_pdr_particles_control_rk3(c, stage, tstep, deltat) &
+        bind(C, name="pdr_particles_control_rk3")
+      import :: c_int, c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: stage, tstep
+      real(c_double), value :: deltat
+    end function
+    integer(c_int) function c_pdr_particles_solve(c, form, npin, stage, deltat) bind(C, name="pdr_particles_solve")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: form
+      integer(c_int64_t), value :: npin
+      integer(c_int32_t), value :: stage
+      real(c_double), value :: deltat
+    end function
+    integer(c_int) function c_pdr_particles_finish_step(c, tstep) bind(C, name="pdr_particles_finish_step")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: c
+      integer(c_int32_t), value :: tstep
     end function
     type(c_ptr) function c_pdr_last_error() bind(C, name="pdr_last_error")
       import :: c_ptr
@@ -404,6 +441,70 @@ contains
     integer, intent(inout)                   :: ierr
     ierr = int(c_pdr_particles_download_aux(ctx, c_loc(up), c_loc(xp0)))
   end subroutine pdr_GetParticlesAux
+
+  !> Uniform random tracer positions (P_INIT_RANDOM, src/pp_control.f90:657-686)
+  subroutine pdr_InitParticles ( seed, np, ierr )
+    integer(c_int64_t), intent(in) :: seed, np
+    integer, intent(inout)         :: ierr
+    ierr = int(c_pdr_init_particles(ctx, seed, np))
+  end subroutine pdr_InitParticles
+
+  !> .true. (default): f_RK3Stage1..3 and g_RK3Calc advance the tracers; .false.:
+  !! the caller does, with p_ControlRK3 or p_RK3SolveParticle[BF|R] and then
+  !! p_ControlFinishTimeStep after each f_RK3StageN, before g_RK3UpdateB
+  subroutine pdr_SetTracersInStage ( on, ierr )
+    logical, intent(in)    :: on
+    integer, intent(inout) :: ierr
+    integer(c_int32_t)     :: flag
+    flag = 0
+    if ( on ) flag = 1
+    ierr = int(c_pdr_set_tracers_in_stage(ctx, flag))
+  end subroutine pdr_SetTracersInStage
+
+  !> p_ControlRK3 (src/pp_control.f90:518-580): release handling at
+  !! P_TS_RELEASE, then the frame's p_RK3SolveParticle[BF|R] for every tracer
+  subroutine p_ControlRK3 ( stage, tstep, deltat, ierr )
+    integer, intent(in)             :: stage
+    integer, intent(in)             :: tstep
+    real(kind=c_double), intent(in) :: deltat
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_particles_control_rk3(ctx, int(stage, c_int32_t), int(tstep, c_int32_t), deltat))
+  end subroutine p_ControlRK3
+
+  !> p_ControlFinishTimeStep (src/pp_control.f90:1440-1450): periodic
+  !! correction of the positions (no ranks to redistribute to)
+  subroutine p_ControlFinishTimeStep ( tstep, ierr )
+    integer, intent(in)    :: tstep
+    integer, intent(inout) :: ierr
+    ierr = int(c_pdr_particles_finish_step(ctx, int(tstep, c_int32_t)))
+  end subroutine p_ControlFinishTimeStep
+
+  !> p_RK3SolveParticle (src/pp_rk3.f90:57-165), isotropic runs
+  subroutine p_RK3SolveParticle ( npin, stage, deltat, ierr )
+    integer, intent(in)             :: npin
+    integer, intent(in)             :: stage
+    real(kind=c_double), intent(in) :: deltat
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_particles_solve(ctx, 0_c_int32_t, int(npin, c_int64_t), int(stage, c_int32_t), deltat))
+  end subroutine p_RK3SolveParticle
+
+  !> p_RK3SolveParticleBF (src/pp_rk3.f90:169-303), NS_BF_ROT shear runs
+  subroutine p_RK3SolveParticleBF ( npin, stage, deltat, ierr )
+    integer, intent(in)             :: npin
+    integer, intent(in)             :: stage
+    real(kind=c_double), intent(in) :: deltat
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_particles_solve(ctx, 1_c_int32_t, int(npin, c_int64_t), int(stage, c_int32_t), deltat))
+  end subroutine p_RK3SolveParticleBF
+
+  !> p_RK3SolveParticleR (src/pp_rk3.f90:307-403), NS_R_ROT / NS_R_CONV shear runs
+  subroutine p_RK3SolveParticleR ( npin, stage, deltat, ierr )
+    integer, intent(in)             :: npin
+    integer, intent(in)             :: stage
+    real(kind=c_double), intent(in) :: deltat
+    integer, intent(inout)          :: ierr
+    ierr = int(c_pdr_particles_solve(ctx, 2_c_int32_t, int(npin, c_int64_t), int(stage, c_int32_t), deltat))
+  end subroutine p_RK3SolveParticleR
 
   !> Text of the last library error
   function pdr_ErrorMessage () result ( msg )

@@ -103,6 +103,8 @@ def load_library(path: str = LIB_PATH):
         "pdr_stats": (C.c_int, [vp, dp, dp]),
         "pdr_rs_stats": (C.c_int, [vp, dp]),
         "pdr_spectra_1d": (C.c_int, [vp, dp, dp, dp]),
+        "pdr_set_stage_stats": (C.c_int, [vp, i32]),
+        "pdr_stage_stats": (C.c_int, [vp, C.POINTER(i32), dp, dp, dp, dp, dp, dp]),
         "pdr_ensemble_create": (C.c_int, [C.POINTER(vp), C.POINTER(vp), i32]),
         "pdr_ensemble_run": (C.c_int, [vp, i32, i32, dp]),
         "pdr_ensemble_set_split": (C.c_int, [vp, i32, i32]),
@@ -128,6 +130,10 @@ def load_library(path: str = LIB_PATH):
         "pdr_particles_upload_aux": (C.c_int, [vp, vp, vp]),
         "pdr_particle_stats_config": (C.c_int, [vp, i32, i32, i32]),
         "pdr_set_particle_sort": (C.c_int, [vp, i32]),
+        "pdr_set_tracers_in_stage": (C.c_int, [vp, i32]),
+        "pdr_particles_control_rk3": (C.c_int, [vp, i32, i32, d]),
+        "pdr_particles_solve": (C.c_int, [vp, i32, i64, i32, d]),
+        "pdr_particles_finish_step": (C.c_int, [vp, i32]),
         "pdr_particle_stats": (C.c_int, [vp, C.POINTER(i32), dp]),
         "pdr_init_field": (C.c_int, [vp, C.c_uint64, d, d]),
         "pdr_init_particles": (C.c_int, [vp, C.c_uint64, i64]),
@@ -140,6 +146,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_restart_read_scaled": (C.c_int, [vp, C.c_char_p, i32]),
         "pdr_spectrum": (C.c_int, [vp, dp, dp, dp, dp]),
         "pdr_debug_prepare": (C.c_int, [vp, i32, dp]),
+        "pdr_kernel_forms": (C.c_int, [vp, np.ctypeslib.ndpointer(dtype=np.int32)]),
         "pdr_profile_enable": (C.c_int, [vp, i32]),
         "pdr_copy_bandwidth": (C.c_int, [i64, i32, C.POINTER(d)]),
         "pdr_profile_get": (C.c_int, [vp, i32, C.c_char_p, i32,
@@ -408,6 +415,23 @@ class Pandora:
         """Re-sort the tracer slots by grid row every `every` steps (0: never)."""
         _check(self.L.pdr_set_particle_sort(self.h, int(every)))
 
+    def set_tracers_in_stage(self, on: bool):
+        """True (default): pdr_stage advances the tracers; False: the caller does,
+        with control_rk3 / solve_particle and finish_time_step."""
+        _check(self.L.pdr_set_tracers_in_stage(self.h, 1 if on else 0))
+
+    def control_rk3(self, stage: int, tstep: int, deltat: float):
+        """p_ControlRK3(stage, tstep, deltat) (src/pp_control.f90:518-580)."""
+        _check(self.L.pdr_particles_control_rk3(self.h, stage, tstep, float(deltat)))
+
+    def solve_particle(self, form: int, npin: int, stage: int, deltat: float):
+        """p_RK3SolveParticle (form 0), ...BF (1), ...R (2) (src/pp_rk3.f90:57, 169, 307)."""
+        _check(self.L.pdr_particles_solve(self.h, form, npin, stage, float(deltat)))
+
+    def finish_time_step(self, tstep: int):
+        """p_ControlFinishTimeStep(tstep) (src/pp_control.f90:1440-1450)."""
+        _check(self.L.pdr_particles_finish_step(self.h, tstep))
+
     def particle_stats_config(self, enable=True, keep_initial=False, ts_release=0):
         """Tracer statistics in stage 1 (p_StatsPrimary/Dispersion/RelativeVelocity)."""
         _check(self.L.pdr_particle_stats_config(self.h, int(enable), int(keep_initial), int(ts_release)))
@@ -449,6 +473,24 @@ class Pandora:
         _check(self.L.pdr_spectra_1d(self.h, a, b, h))
         return a, b, dict(zip(HOMOG_NAMES, h.tolist()))
 
+    def set_stage_stats(self, every: int):
+        """Statistics of stage 1 inside the step every `every` steps (0: off)."""
+        _check(self.L.pdr_set_stage_stats(self.h, int(every)))
+
+    def stage_stats(self):
+        """The last stage-1 record: (tstep, stats dict, spectra dict, rs dict,
+        spec_1i, spec_2i, homogeneous dict); tstep -1 if none yet."""
+        t = C.c_int32(-1)
+        q = np.zeros(len(STATS_NAMES))
+        spec = np.zeros(7 * self.kkmax)
+        rs = np.zeros(len(RS_NAMES))
+        a = np.zeros(self.nki)
+        b = np.zeros(self.nki)
+        h = np.zeros(len(HOMOG_NAMES))
+        _check(self.L.pdr_stage_stats(self.h, C.byref(t), q, spec, rs, a, b, h))
+        return (int(t.value), dict(zip(STATS_NAMES, q)), dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))),
+                dict(zip(RS_NAMES, rs)), a, b, dict(zip(HOMOG_NAMES, h.tolist())))
+
     def rs_stats(self):
         """f_FluidstatsRs2D/RsCollect of the current state on the device (dict of RS_NAMES)."""
         out = np.zeros(len(RS_NAMES))
@@ -462,6 +504,14 @@ class Pandora:
         tot = np.zeros(6)
         _check(self.L.pdr_spectrum(self.h, ek, dk, ds, tot))
         return ek, dk, ds, tot
+
+    def kernel_forms(self):
+        """Kernel forms this box runs (pdr_kernel_forms) as a dict."""
+        out = np.zeros(8, dtype=np.int32)
+        _check(self.L.pdr_kernel_forms(self.h, out))
+        keys = ("specialised", "zchunk", "two_chunks", "fused_zformf", "xrows", "x_tw_global",
+                "zinv_register", "zinv_lines")
+        return dict(zip(keys, (int(v) for v in out)))
 
     def debug_prepare(self, field: int):
         out = np.zeros((self.nki, self.nz, self.nkj), np.complex128)

@@ -47,7 +47,8 @@ def _params(**kw):
     (dict(force_stop=10, force_tf=0.0, force_kf=2.5), -1),  # OU time scale must be > 0
     (dict(force_stop=10, force_tf=0.2, force_kf=0.0), -1),  # no forced band
     (dict(f_type=2, rk3_timestep=1, f_remesh_t=0, f_deform_s=1.0), -1),
-    (dict(p_interp_order=3), -4),
+    (dict(p_interp_order=0), -4),                         # P_INTERP_ORDER outside 1..8
+    (dict(p_interp_order=9), -4),
     (dict(precision=2), -1),
     (dict(b110=2), -4),
 ])

@@ -641,3 +641,154 @@ def test_spectrum_totals(oracle_mod):
     assert abs(ek.sum() - e[inside].sum()) < 1e-13 * e.sum()
     assert abs(dk.sum() - (2 * 0.01 * kabs**2 * e)[inside].sum()) < 1e-13 * dk.sum()
     assert abs(0.5 * (tot[3] + tot[4] + tot[5]) - e.sum()) < 1e-13 * e.sum()
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6, 7, 8])
+def test_neville_every_order_is_centred_lagrange(oracle_mod, order):
+    """P_INTERP_ORDER = P: the value at a tracer is the Lagrange interpolant of
+    the P x P x P nodes nearest to it, the molecule centred on the tracer's cell
+    for even P (nodes floor(x/dx) - P/2 + 1 .. floor(x/dx) + P/2) and on its
+    nearest node for odd P (round(x/dx) -+ (P-1)/2), periodic through the box
+    (p_InterpFindMolecule, src/pp_interp.f90:406-467, with g_DomainInitHalo,
+    src/pg_domain.f90:360-384).  Evaluated here by explicit Lagrange weights on
+    a random periodic field, not by a Neville tableau; P = 1 is the nearest
+    node.  Also: polynomials of degree < P are reproduced away from the wrap."""
+    P = order
+    n = 16
+    o = oracle_mod.Oracle(n, interp_order=P)
+    dx = TWOPI / n
+    rng = np.random.default_rng(100 + P)
+    f = [rng.standard_normal((n, n, n)) for _ in range(3)]  # [z][y][x]
+    xp = rng.uniform(-0.5 * dx, TWOPI - 0.5 * dx, size=(300, 3))
+    xp[:10, 0] = -0.5 * dx
+    xp[10:20, 1] = np.nextafter(TWOPI - 0.5 * dx, 0.0)
+    got = o.interpolate(*f, xp)
+
+    def stencil(x):
+        s = x / dx
+        first = (int(np.floor(s)) - P // 2 + 1) if P % 2 == 0 else (int(np.floor(s + 0.5)) - (P - 1) // 2)
+        nodes = np.arange(first, first + P)
+        w = np.ones(P)
+        for a in range(P):
+            for b in range(P):
+                if a != b:
+                    w[a] *= (x - nodes[b] * dx) / ((nodes[a] - nodes[b]) * dx)
+        return nodes % n, w
+
+    want = np.zeros_like(got)
+    for q in range(xp.shape[0]):
+        ix, wx = stencil(xp[q, 0])
+        iy, wy = stencil(xp[q, 1])
+        iz, wz = stencil(xp[q, 2])
+        for c in range(3):
+            blk = f[c][np.ix_(iz, iy, ix)]
+            want[q, c] = np.einsum("k,j,i,kji->", wz, wy, wx, blk)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+
+    # polynomials of degree P - 1 in each direction (inside the box, no wrap)
+    g = np.arange(n) * dx
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    d = P - 1
+    p1 = 0.5 + X**d - 0.3 * Y**d * (Z if d >= 1 else 1.0) + 0.1 * Z**d
+    xin = rng.uniform(0.35 * TWOPI, 0.6 * TWOPI, size=(100, 3))
+    uf = o.interpolate(p1, p1, p1, xin)
+    x, y, z = xin[:, 0], xin[:, 1], xin[:, 2]
+    e = 0.5 + x**d - 0.3 * y**d * (z if d >= 1 else 1.0) + 0.1 * z**d
+    assert np.abs(uf[:, 0] - e).max() < 1e-10 * max(1.0, np.abs(e).max())
+    o.close()
+
+
+def _rdt_ode(k, a0, S, nu, T):
+    """Rapid-distortion theory for one Fourier mode of homogeneous shear
+    dU1/dx3 = S, integrated to high accuracy: the wavevector kappa(t) =
+    (k1, k2, k3 - S t k1) and du/dt = -A u + 2 (kappa . A u) kappa / |kappa|^2
+    - nu |kappa|^2 u with (A u)_i = S delta_i1 u3 (Pope, Turbulent Flows,
+    sec. 11.4)."""
+    from scipy.integrate import solve_ivp
+
+    def rhs(t, y):
+        u = y[:3] + 1j * y[3:]
+        kap = np.array([k[0], k[1], k[2] - S * t * k[0]])
+        k2 = kap @ kap
+        au = np.array([S * u[2], 0.0, 0.0])
+        du = -au + 2.0 * (kap @ au) * kap / k2 - nu * k2 * u
+        return np.concatenate([du.real, du.imag])
+
+    y0 = np.concatenate([a0.real, a0.imag])
+    sol = solve_ivp(rhs, (0.0, T), y0, method="DOP853", rtol=1e-13, atol=1e-15)
+    return sol.y[:3, -1] + 1j * sol.y[3:, -1]
+
+
+def _rdt_oracle(oracle_mod, f_type, ns, S, nu, R, steps, k, a0, p3_two=0):
+    n = 16
+    o = oracle_mod.Oracle(n, f_type=f_type, ns_deformed=ns, rk3_timestep=1, deform_s=S, remesh_t=R,
+                          nu=nu, rrot_p3_two=p3_two)
+    u = [np.zeros(o.shape, complex) for _ in range(3)]
+    idx = (k[0], k[1] % o.nkj, k[2] % o.nkk)
+    for c in range(3):
+        u[c][idx] = a0[c]
+    o.set_state(*u)
+    t = 0.0
+    for s in range(1, steps + 1):
+        t = o.step(s, t)
+    b, bi = o.get_frame()
+    BI = bi.reshape(3, 3, order="F")
+    us = np.array([x[idx] for x in o.get_state()])
+    o.close()
+    ul = BI @ us if ns == 1 else us   # BF_ROT stores bmat u_lab; R forms store u_lab
+    return t, ul
+
+
+@pytest.mark.parametrize("f_type,ns,p3_two", [(4, 1, 0), (3, 1, 0), (4, 3, 1), (3, 3, 1)])
+def test_rdt_single_mode_follows_textbook_rdt(oracle_mod, f_type, ns, p3_two):
+    """F_TYPE 3 (viscous) and 4 (inviscid) rapid-distortion runs of one Fourier
+    mode against the textbook RDT solution, up to S t = 0.5 with no remesh.
+    BF_ROT converges at third order in dt (RK3 with the frame advanced at the
+    stage times).  R_ROT, with its pressure source's amat factor two restored
+    (the reference's R_ROT keeps factor one, tested below), converges to the
+    same solution at first order in dt."""
+    S, nu = 1.0, (0.02 if f_type == 3 else 0.0)
+    k = np.array([2, -3, 5])
+    a0 = np.array([1.0 + 0.5j, 0.3 - 0.2j, 0.0])
+    a0[2] = -(k[0] * a0[0] + k[1] * a0[1]) / k[2]
+    errs = []
+    for R in (50, 100):                                  # dt = 0.5 / (S R): 0.01, 0.005
+        T, ul = _rdt_oracle(oracle_mod, f_type, ns, S, nu, R, R, k, a0, p3_two)
+        assert abs(T - 0.5) < 1e-14
+        want = _rdt_ode(k.astype(float), a0, S, nu, T)
+        errs.append(np.abs(ul - want).max() / np.abs(a0).max())
+    if ns == 1:
+        assert errs[1] < 2e-7, errs
+        assert 5.0 < errs[0] / errs[1] < 11.0, errs      # third order: 8
+    else:
+        assert errs[1] < 1e-3, errs
+        assert 1.7 < errs[0] / errs[1] < 2.3, errs       # first order: 2
+
+
+def test_rdt_streamwise_independent_mode_is_exact(oracle_mod):
+    """A mode with k1 = 0 in inviscid RDT: the wavevector does not turn and no
+    pressure acts, u1(t) = u1(0) - S t u3(0) exactly; RK3 reproduces a linear
+    solution to rounding, for BF_ROT and both R_ROT pressure factors."""
+    S = 2.0
+    k = np.array([0, 2, -3])
+    a0 = np.array([0.4 - 0.1j, 0.75 + 0.3j, 0.0])
+    a0[2] = -(k[1] * a0[1]) / k[2]
+    for ns, p3 in ((1, 0), (3, 0), (3, 1)):
+        T, ul = _rdt_oracle(oracle_mod, 4, ns, S, 0.0, 40, 25, k, a0, p3)
+        want = a0.copy()
+        want[0] = a0[0] - S * T * a0[2]
+        assert np.abs(ul - want).max() < 1e-13, ns
+
+
+def test_rrot_reference_pressure_factor_departs_from_rdt(oracle_mod):
+    """With the reference's R_ROT pressure source (amat factor one,
+    src/pf_rk3.f90:3115-3121) a k1 != 0 mode leaves the textbook solution by far
+    more than the scheme's error: the oracle carries the reference's factor."""
+    S = 1.0
+    k = np.array([2, -3, 5])
+    a0 = np.array([1.0 + 0.5j, 0.3 - 0.2j, 0.0])
+    a0[2] = -(k[0] * a0[0] + k[1] * a0[1]) / k[2]
+    T, ul = _rdt_oracle(oracle_mod, 4, 3, S, 0.0, 100, 100, k, a0, 0)
+    want = _rdt_ode(k.astype(float), a0, S, 0.0, T)
+    assert np.abs(ul - want).max() / np.abs(a0).max() > 1e-3
