@@ -468,6 +468,13 @@ int pdr_kernel_forms(const pdr_ctx* ctx, int32_t out[8]);
  * No reference counterpart: the roofline reference for the kernels. */
 int pdr_copy_bandwidth(int64_t bytes, int32_t iters, double* gbps);
 
+/* HBM ceiling probe: kind 0 copy, 1 read, 2 write, 3 triad (d = a + s b) over
+ * `bytes`-byte buffers, each thread keeping eight 16-byte loads in flight and
+ * storing non-temporally; *gbps = bytes moved (reads + writes) per second over
+ * `iters` launches.  No reference counterpart: the measured HBM ceiling that
+ * kernel fractions are quoted against beside the nominal 8 TB/s. */
+int pdr_hbm_probe(int32_t kind, int64_t bytes, int32_t iters, double* gbps);
+
 /* Per-kernel timing with HIP events on the context's stream.  enable != 0
  * starts collection; pdr_profile_get returns, for up to n kernel classes,
  * names (comma-separated into names[namelen]), launch counts and total ms. */

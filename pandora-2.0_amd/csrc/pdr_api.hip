@@ -497,6 +497,8 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     for (void** q : {&c->w2b, &c->w2c})
       if (rc == PDR_OK) rc = dalloc(c, q, 6 * size_t(g.m2) * c->csz);
   }
+  // 256 bytes owned by the context: the CFL maximum at offset 0, the
+  // non-finite flag of pdr_check_finite at offset 64
   if (rc == PDR_OK) rc = dalloc(c, (void**)&c->umax, 256);
   const size_t fbytes = force_bytes(c);
   if (rc == PDR_OK && fbytes) {
@@ -938,14 +940,13 @@ int pdr_run(pdr_ctx* c, int32_t tstep0, int32_t nsteps, double* simtime) {
 
 int pdr_check_finite(pdr_ctx* c) {
   if (!c) return fail(PDR_ERR_ARG, "null context");
-  int* flag = nullptr;
-  HIPCHK(hipMalloc(&flag, sizeof(int)));
+  // the flag lives in the context's own small buffer: no allocation per call
+  int* flag = reinterpret_cast<int*>(reinterpret_cast<char*>(c->umax) + 64);
   int h = 0;
   hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), c->stream);
   if (e == hipSuccess) e = launch_finite(c, flag);
   if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(flag);
   if (e != hipSuccess) return fail(PDR_ERR_HIP, std::string("finite check: ") + hipGetErrorString(e));
   if (h) return fail(PDR_ERR_NONFINITE, "velocity state contains NaN or Inf");
   return PDR_OK;
@@ -1102,6 +1103,45 @@ int pdr_copy_bandwidth(int64_t bytes, int32_t iters, double* gbps) {
   if (b) (void)hipFree(b);
   if (e != hipSuccess) return fail(PDR_ERR_HIP, std::string("copy bandwidth: ") + hipGetErrorString(e));
   *gbps = 2.0 * double(bytes) * iters / (double(ms) * 1e-3) / 1e9;
+  return PDR_OK;
+}
+
+int pdr_hbm_probe(int32_t kind, int64_t bytes, int32_t iters, double* gbps) {
+  if (kind < 0 || kind > 3 || bytes < 16 || iters < 1 || !gbps) return fail(PDR_ERR_ARG, "pdr_hbm_probe: bad argument");
+  bytes &= ~int64_t(15);
+  // streams per launch: copy reads and writes, read and write one pass, triad reads two and writes one
+  const double passes[4] = {2.0, 1.0, 1.0, 3.0};
+  void *a = nullptr, *b = nullptr, *d = nullptr;
+  float* sink = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&a, size_t(bytes));
+  if (e == hipSuccess && kind == 3) e = hipMalloc(&b, size_t(bytes));
+  if (e == hipSuccess && kind != 1) e = hipMalloc(&d, size_t(bytes));
+  if (e == hipSuccess) e = hipMalloc((void**)&sink, 256 * sizeof(float));
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  // inputs hold 1.0f (the read probe's sum is then never -1)
+  if (e == hipSuccess) e = launch_probe(2, nullptr, nullptr, a, bytes, sink, s);
+  if (e == hipSuccess && b) e = launch_probe(2, nullptr, nullptr, b, bytes, sink, s);
+  if (e == hipSuccess) e = launch_probe(kind, a, b, d, bytes, sink, s);  // warm-up
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int q = 0; q < iters && e == hipSuccess; ++q) e = launch_probe(kind, a, b, d, bytes, sink, s);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  for (void* q : {a, b, d, (void*)sink})
+    if (q) (void)hipFree(q);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PDR_ERR_HIP, std::string("hbm probe: ") + hipGetErrorString(e));
+  }
+  *gbps = passes[kind] * double(bytes) * iters / (double(ms) * 1e-3) / 1e9;
   return PDR_OK;
 }
 

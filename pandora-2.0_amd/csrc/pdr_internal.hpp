@@ -245,6 +245,9 @@ double* stage_stats_rs_partials(pdr_ctx* c);
 hipError_t stage_stats_rs_reduce(pdr_ctx* c);
 void stage_stats_free(pdr_ctx* c);
 hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s);
+// HBM probes: kind 0 copy, 1 read, 2 write (fill with 1.0f), 3 triad
+hipError_t launch_probe(int kind, const void* a, const void* b, void* d, long long bytes, float* sink,
+                        hipStream_t s);
 
 // profiling hooks (pdr_api.hip)
 void prof_begin(pdr_ctx* c, int cls, hipStream_t s = nullptr);

@@ -601,8 +601,8 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // ---------------------------------------------------------------------------
-// Tracers, one lane per x column of the molecule: a group of G lanes (G = P
-// rounded up to a power of two) serves one tracer; lane a < P gathers the
+// Tracers, one lane per x column of the molecule: a group of G lanes
+// (G = tracer_group(P), at least 4) serves one tracer; lane a < P gathers the
 // P x P points (y, z) of column x_a, all three components per point (one
 // 3-element load each), and applies the y then z Neville passes to them; the
 // group exchanges the P column values with __shfl and every lane runs the x
@@ -610,12 +610,16 @@ __global__ void __launch_bounds__(kThreads)
 // operation sequence per column and component is that of the reference
 // (y for each (x, z), then z for each x, then x).
 // ---------------------------------------------------------------------------
+// Lanes per tracer: P rounded up to a power of two, and at least 4, since lane
+// c < 3 stores component c of the results.
+constexpr int tracer_group(int p) { return p <= 4 ? 4 : 8; }
+
 template <typename T, int P>
 __global__ void __launch_bounds__(kThreads)
     k_tracers_col(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
                   double* __restrict__ uprk, double* __restrict__ upa, double* __restrict__ xp0,
                   long long np, double beta, double gdt, int wrap) {
-  constexpr int G = P <= 1 ? 1 : (P <= 2 ? 2 : (P <= 4 ? 4 : 8));
+  constexpr int G = tracer_group(P);
   const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long q = gt / G;
   const int a = int(gt % G);
@@ -843,6 +847,67 @@ hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
+// HBM ceiling probes: every thread owns U 16-byte vectors spaced one workgroup
+// apart (coalesced), issues all U loads before any store, and writes with
+// non-temporal stores, so each wave keeps U * 1 KiB of reads in flight.
+// KIND 0: copy d = a; 1: read (sum of a, kept live through a never-taken
+// store); 2: write (fill d); 3: triad d = a + s b.
+template <int KIND, int U>
+__global__ void __launch_bounds__(kThreads)
+    k_probe(const float4* __restrict__ a, const float4* __restrict__ b, float4* __restrict__ d, long long n, float s,
+            float* __restrict__ sink) {
+  const long long base = (long long)blockIdx.x * (kThreads * U) + threadIdx.x;
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  if constexpr (KIND == 2) {
+    const v4 f = {s, s, s, s};
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long q = base + (long long)u * kThreads;
+      if (q < n) __builtin_nontemporal_store(f, reinterpret_cast<v4*>(d) + q);
+    }
+  } else {
+    v4 va[U], vb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long q = base + (long long)u * kThreads;
+      const v4 z = {0.f, 0.f, 0.f, 0.f};
+      va[u] = q < n ? reinterpret_cast<const v4*>(a)[q] : z;
+      if constexpr (KIND == 3) vb[u] = q < n ? reinterpret_cast<const v4*>(b)[q] : z;
+    }
+    if constexpr (KIND == 1) {
+      float acc = 0.f;
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc += (va[u].x + va[u].y) + (va[u].z + va[u].w);
+      if (acc == -1.0f) sink[threadIdx.x] = acc;  // never true for the probe's data (all ones)
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const long long q = base + (long long)u * kThreads;
+        v4 v = va[u];
+        if constexpr (KIND == 3) v = va[u] + s * vb[u];
+        if (q < n) __builtin_nontemporal_store(v, reinterpret_cast<v4*>(d) + q);
+      }
+    }
+  }
+}
+
+hipError_t launch_probe(int kind, const void* a, const void* b, void* d, long long bytes, float* sink,
+                        hipStream_t s) {
+  constexpr int U = 8;
+  const long long n = bytes / 16;
+  const dim3 grid(unsigned((n + kThreads * U - 1) / (kThreads * U)));
+  const float4* fa = (const float4*)a;
+  const float4* fb = (const float4*)b;
+  float4* fd = (float4*)d;
+  switch (kind) {
+    case 0: k_probe<0, U><<<grid, kThreads, 0, s>>>(fa, fb, fd, n, 1.0f, sink); break;
+    case 1: k_probe<1, U><<<grid, kThreads, 0, s>>>(fa, fb, fd, n, 1.0f, sink); break;
+    case 2: k_probe<2, U><<<grid, kThreads, 0, s>>>(fa, fb, fd, n, 1.0f, sink); break;
+    default: k_probe<3, U><<<grid, kThreads, 0, s>>>(fa, fb, fd, n, 1.0f, sink); break;
+  }
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -1054,18 +1119,18 @@ template <typename T>
 static hipError_t tracers_t(pdr_ctx* c, const TracerFrame& tf, double beta, double gdt, hipStream_t s,
                             int wrap) {
   const T* ur = (const T*)c->ur;
-  auto go = [&](auto kern, int lanes) {
-    const long long nb = (lanes * c->np + kThreads - 1) / kThreads;
+  auto go = [&](auto kern, int order) {
+    const long long nb = (tracer_group(order) * c->np + kThreads - 1) / kThreads;
     kern<<<dim3((unsigned)nb), kThreads, 0, s>>>(c->g, tf, ur, c->xp, c->uprk, c->up, c->xp0, c->np,
                                                          beta, gdt, wrap);
   };
   switch (tf.order) {
     case 1: go(k_tracers_col<T, 1>, 1); break;
     case 2: go(k_tracers_col<T, 2>, 2); break;
-    case 3: go(k_tracers_col<T, 3>, 4); break;
-    case 5: go(k_tracers_col<T, 5>, 8); break;
-    case 6: go(k_tracers_col<T, 6>, 8); break;
-    case 7: go(k_tracers_col<T, 7>, 8); break;
+    case 3: go(k_tracers_col<T, 3>, 3); break;
+    case 5: go(k_tracers_col<T, 5>, 5); break;
+    case 6: go(k_tracers_col<T, 6>, 6); break;
+    case 7: go(k_tracers_col<T, 7>, 7); break;
     case 8: go(k_tracers_col<T, 8>, 8); break;
     default: go(k_tracers_col<T, 4>, 4); break;
   }

@@ -149,6 +149,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_kernel_forms": (C.c_int, [vp, np.ctypeslib.ndpointer(dtype=np.int32)]),
         "pdr_profile_enable": (C.c_int, [vp, i32]),
         "pdr_copy_bandwidth": (C.c_int, [i64, i32, C.POINTER(d)]),
+        "pdr_hbm_probe": (C.c_int, [i32, i64, i32, C.POINTER(d)]),
         "pdr_profile_get": (C.c_int, [vp, i32, C.c_char_p, i32,
                                       np.ctypeslib.ndpointer(dtype=np.int64), dp]),
     }
@@ -220,6 +221,17 @@ def copy_bandwidth(nbytes=1 << 30, iters=20):
     """Measured HBM copy bandwidth in GB/s (bytes read + written per second)."""
     g = C.c_double()
     _check(load_library().pdr_copy_bandwidth(nbytes, iters, C.byref(g)))
+    return g.value
+
+
+HBM_PROBES = ("copy", "read", "write", "triad")
+
+
+def hbm_probe(kind, nbytes=1 << 32, iters=10):
+    """Measured HBM rate in GB/s (bytes read + written per second) of one of the
+    probe kernels: "copy", "read", "write" or "triad", on nbytes-byte buffers."""
+    g = C.c_double()
+    _check(load_library().pdr_hbm_probe(HBM_PROBES.index(kind), nbytes, iters, C.byref(g)))
     return g.value
 
 
