@@ -218,9 +218,74 @@ def ensemble_bench(pa, per_size=16, steps=50, warmup=5):
             "model_frac_of_peak": b * ens_sps / HBM_PEAK, "steps": steps}
 
 
-def cpu_baseline(n, ntr):
-    """One RK3 stage (a third of a step) of the same workload on the FP64 oracle,
-    OpenMP over the host cores; scaled to grid-point-steps/s."""
+def host_info():
+    """The host CPU as /proc describes it: model, logical CPUs, the CPUs this
+    process may run on, physical cores and memory."""
+    info = {"logical_cpus": os.cpu_count()}
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    try:
+        cores = set()
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and "cpu_model" not in info:
+                    info["cpu_model"] = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+        if cores:
+            info["physical_cores"] = len(cores)
+    except OSError:
+        pass
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    info["mem_gib"] = round(int(line.split()[1]) / 2**20, 1)
+    except OSError:
+        pass
+    return info
+
+
+def cpu_threads():
+    """OpenMP threads of the CPU leg: OMP_NUM_THREADS when the environment sets
+    it (the share of host cores given to this process), else every CPU this
+    process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def _median(v):
+    v = sorted(v)
+    m = len(v) // 2
+    return v[m] if len(v) % 2 else 0.5 * (v[m - 1] + v[m])
+
+
+def cpu_baseline(n, ntr, small=256, small_tracers=262144, small_steps=5):
+    """The reference's CPU path restated as the FP64 C oracle (OpenMP over the
+    host cores), on the host of the GPU run:
+    * the headline workload (n^3 shear, ntr tracers): one warm-up stage, then
+      the three stages of one step, each timed; value = n^3 / their sum;
+    * the FP64 drop-in workload (small^3 shear, small_tracers tracers): one
+      warm-up step, then the median of small_steps timed steps with the
+      statistics off, and of small_steps steps with the statistics of stage 1
+      on (f_FluidstatsFs and the real-space moments before each step)."""
     sys.path.insert(0, ROOT)
     import numpy as np
 
@@ -228,24 +293,107 @@ def cpu_baseline(n, ntr):
     import pandora_amd as pa
 
     oracle.build()
-    cores = min(os.cpu_count() or 1, 16)
-    u = pa.init_field_host(n, 2, 2.0 / 3.0, 6.0)
-    o = oracle.Oracle(n, f_type=2, rk3_timestep=1, remesh_t=250, deform_s=1.0, nu=0.005,
-                      nthreads=cores)
-    o.set_state(*u)
-    del u
-    rng = np.random.default_rng(102)
-    dx = pa.TWOPI_REF / n
-    o.set_particles(rng.uniform(-0.5 * dx, pa.TWOPI_REF - 0.5 * dx, size=(ntr, 3)))
-    t0 = time.perf_counter()
-    o.stage(1, 1, 0.0, 0.0)
-    t1 = time.perf_counter()
+    threads = cpu_threads()
+    kw = dict(f_type=2, rk3_timestep=1, remesh_t=250, deform_s=1.0, nu=0.005, nthreads=threads)
+
+    def make(nn, count, seed):
+        u = pa.init_field_host(nn, 2, 2.0 / 3.0, 6.0)
+        o = oracle.Oracle(nn, **kw)
+        o.set_state(*u)
+        del u
+        rng = np.random.default_rng(seed)
+        dx = pa.TWOPI_REF / nn
+        o.set_particles(rng.uniform(-0.5 * dx, pa.TWOPI_REF - 0.5 * dx, size=(count, 3)))
+        return o
+
+    print(f"[bench] CPU leg: {n}^3 stages on {threads} threads", file=sys.stderr, flush=True)
+    o = make(n, ntr, 102)
+    dt = o.stage(1, 1, 0.0, 0.0)  # warm-up
+    stage_s = []
+    for stage, tstep in ((2, 1), (3, 1), (1, 2)):
+        t0 = time.perf_counter()
+        o.stage(stage, tstep, 0.0, dt)
+        stage_s.append(time.perf_counter() - t0)
     o.close()
-    stage_s = t1 - t0
-    return {"value": n**3 / (3.0 * stage_s), "unit": "grid-point-steps/s", "cores": cores,
-            "kind": "port",
-            "sample": f"one RK3 stage (1/3 step) of the {n}^3 shear + {ntr} tracer workload on the "
-                      f"FP64 C oracle, {cores} OpenMP threads: {stage_s:.2f} s"}
+    step_s = sum(stage_s)
+    out = {"value": n**3 / step_s, "unit": "grid-point-steps/s", "cores": threads, "kind": "port",
+           "sample": f"{n}^3 shear + {ntr} tracers on the FP64 C oracle, {threads} OpenMP threads: one "
+                     f"warm-up stage, then the three stages of one step ({', '.join(f'{t:.2f}' for t in stage_s)} s)",
+           "host": host_info()}
+    if small:
+        print(f"[bench] CPU leg: {small}^3 steps, statistics off and on", file=sys.stderr, flush=True)
+        o = make(small, small_tracers, 101)
+        t = o.step(1, 0.0)  # warm-up
+        runs = {"stats_off": [], "stats_on": []}
+        s = 2
+        for key in ("stats_off", "stats_on"):
+            for _ in range(small_steps):
+                t0 = time.perf_counter()
+                if key == "stats_on":
+                    o.stats()
+                    o.rs_stats()
+                t = o.step(s, t)
+                runs[key].append(time.perf_counter() - t0)
+                s += 1
+        o.close()
+        out["drop_in_256"] = {
+            "workload": f"{small}^3 shear, {small_tracers} tracers, FP64, {threads} OpenMP threads",
+            **{k: {"median_s_per_step": _median(v), "steps_per_s": 1.0 / _median(v),
+                   "grid_point_steps_per_s": small**3 / _median(v), "samples_s": v} for k, v in runs.items()}}
+    return out
+
+
+def hbm_ceiling(pa, sizes_gib=(1, 4, 8), iters=10):
+    """Measured HBM rates of the probe kernels (copy, read, write, triad) over
+    several buffer sizes; the highest is the measured ceiling."""
+    probes = {}
+    best, where = 0.0, None
+    for gib in sizes_gib:
+        for kind in pa.HBM_PROBES:
+            r = pa.hbm_probe(kind, gib << 30, iters)
+            probes[f"{kind}_{gib}GiB"] = r
+            if r > best:
+                best, where = r, f"{kind}_{gib}GiB"
+    return {"GBps": best, "best": where, "probes_GBps": probes}
+
+
+def zstreams_ab(pa, n, ntr, steps=5, repeats=3):
+    """Steps/s of the FP32-mixed headline workload with the z chunks of a stage
+    one after another (PDR_ZSTREAMS=1) and two in flight (2), each the median
+    of `repeats` timed runs of `steps` steps in one context."""
+    env = os.environ.get("PDR_ZSTREAMS")
+    out = {}
+    try:
+        for zs in ("1", "2"):
+            os.environ["PDR_ZSTREAMS"] = zs
+            g = pa.Pandora(n, f_type=pa.F_SHEAR_DNS, rk3_timestep=pa.RK3_TIMESTEP_DECIDE, deform_s=1.0,
+                           remesh_t=250, nu=0.005, precision=pa.PDR_FP32_MIXED)
+            g.init_field(2, 2.0 / 3.0, 6.0)
+            if ntr:
+                g.init_particles(102, ntr)
+            t = g.run(1, 2, 0.0)
+            s = 3
+            rates = []
+            for _ in range(repeats):
+                g.synchronize()
+                t0 = time.perf_counter()
+                t = g.run(s, steps, t)
+                g.synchronize()
+                rates.append(steps / (time.perf_counter() - t0))
+                s += steps
+            g.close()
+            out[f"zstreams_{zs}"] = {"median_steps_per_s": _median(rates), "min": min(rates), "max": max(rates),
+                                     "samples": rates}
+    finally:
+        if env is None:
+            os.environ.pop("PDR_ZSTREAMS", None)
+        else:
+            os.environ["PDR_ZSTREAMS"] = env
+    a, b = out["zstreams_1"], out["zstreams_2"]
+    out["gain"] = b["median_steps_per_s"] / a["median_steps_per_s"] - 1.0
+    out["spread"] = max((a["max"] - a["min"]) / a["median_steps_per_s"], (b["max"] - b["min"]) / b["median_steps_per_s"])
+    out["steps"], out["repeats"] = steps, repeats
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -385,6 +533,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
+    ap.add_argument("--no-ab", action="store_true", help="skip the one/two chunk-stream comparison")
+    ap.add_argument("--no-cpu-small", action="store_true",
+                    help="CPU leg: skip the 256^3 statistics off/on medians")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched-ensemble measurement and print its JSON object")
     ap.add_argument("--case", choices=sorted(CASES), default=None,
@@ -410,14 +561,17 @@ def main():
     n, ntr = args.n, args.tracers
     print(f"[bench] FP32-mixed {n}^3 shear + {ntr} tracers: {args.warmup} warmup, {args.steps} timed",
           file=sys.stderr, flush=True)
-    dt32, prof_split, ok32 = run_gpu(pa, n, pa.PDR_FP32_MIXED, args.steps, args.warmup, ntr, True)
+    # the headline: no per-kernel events on the streams
+    dt32, _, ok32 = run_gpu(pa, n, pa.PDR_FP32_MIXED, args.steps, args.warmup, ntr, False)
     sps32 = args.steps / dt32
-    # Per-kernel rates and the roofline come from a short second timed run with
-    # the chunks of a stage one after another: in the run above two chunks share
-    # the GPU, so a launch's duration is not that kernel's own.
+    # Per-kernel times from two short profiled runs: the default schedule (two
+    # chunks in flight from 512 planes up, where launches overlap) and the
+    # chunks of a stage one after another, whose launch durations are each
+    # kernel's own (the rates and the roofline come from this one).
     rsteps = max(3, args.steps // 4)
-    print(f"[bench] FP32-mixed {n}^3, chunks one after another: 1 warmup, {rsteps} timed (kernel rates)",
+    print(f"[bench] FP32-mixed {n}^3, profiled: {rsteps} steps default schedule, {rsteps} chunks one after another",
           file=sys.stderr, flush=True)
+    _, prof_split, _ = run_gpu(pa, n, pa.PDR_FP32_MIXED, rsteps, 1, ntr, True)
     env_zs = os.environ.get("PDR_ZSTREAMS")
     os.environ["PDR_ZSTREAMS"] = "1"
     try:
@@ -427,12 +581,25 @@ def main():
             del os.environ["PDR_ZSTREAMS"]
         else:
             os.environ["PDR_ZSTREAMS"] = env_zs
+    ab = None
+    if not args.no_ab:
+        print("[bench] one against two z chunks in flight (medians)", file=sys.stderr, flush=True)
+        ab = zstreams_ab(pa, n, ntr)
     fp64 = None
     if not args.no_fp64:
         k64 = max(3, args.steps // 3)
         print(f"[bench] FP64 {n}^3: 1 warmup, {k64} timed", file=sys.stderr, flush=True)
-        dt64, prof64, ok64 = run_gpu(pa, n, pa.PDR_FP64, k64, 1, ntr, True)
+        dt64, _, ok64 = run_gpu(pa, n, pa.PDR_FP64, k64, 1, ntr, False)
         sps64 = k64 / dt64
+        print(f"[bench] FP64 {n}^3, profiled: {rsteps} steps, chunks one after another", file=sys.stderr, flush=True)
+        os.environ["PDR_ZSTREAMS"] = "1"
+        try:
+            _, prof64, _ = run_gpu(pa, n, pa.PDR_FP64, rsteps, 1, ntr, True)
+        finally:
+            if env_zs is None:
+                del os.environ["PDR_ZSTREAMS"]
+            else:
+                os.environ["PDR_ZSTREAMS"] = env_zs
         fp64 = {"steps_per_s": sps64, "grid_point_steps_per_s": sps64 * n**3, "ms_per_step": 1e3 / sps64,
                 "finite": bool(ok64), "prof": prof64}
 
@@ -463,9 +630,9 @@ def main():
         return out
 
     kernels = per_kernel(prof, 8, 4, rsteps)
-    kernels_split = per_kernel(prof_split, 8, 4, args.steps)
+    kernels_split = per_kernel(prof_split, 8, 4, rsteps)
     if fp64:
-        fp64["kernels"] = per_kernel(fp64.pop("prof"), 16, 8, k64)
+        fp64["kernels"] = per_kernel(fp64.pop("prof"), 16, 8, rsteps)
     if dom:
         cnt, ms = prof[dom]
         avg = ms / cnt * 1e-3
@@ -489,10 +656,18 @@ def main():
         ens = ensemble_bench(pa)
     bstep = model_bytes_per_step(g, 8, 4, ntr)
     copy_gbps = pa.copy_bandwidth(1 << 30, 20)
+    print("[bench] HBM ceiling probes", file=sys.stderr, flush=True)
+    ceiling = hbm_ceiling(pa)
+    if roof:
+        roof["frac_of_measured_ceiling"] = roof["achieved"] * 1e9 / (ceiling["GBps"] * 1e9)
+        roof["measured_ceiling_GBps"] = ceiling["GBps"]
+    for k in kernels.values():
+        if k.get("GBps"):
+            k["frac_of_peak"] = k["GBps"] * 1e9 / HBM_PEAK
+            k["frac_of_measured_ceiling"] = k["GBps"] / ceiling["GBps"]
     cpu = None
     if not args.no_cpu:
-        print("[bench] CPU baseline (FP64 oracle, one stage)", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(n, ntr)
+        cpu = cpu_baseline(n, ntr, small=0 if args.no_cpu_small else 256)
     out = {
         "metric": METRIC,
         "value": sps32 * n**3,
@@ -513,10 +688,13 @@ def main():
         "fp64": fp64,
         "fp32_over_fp64": (sps32 / fp64["steps_per_s"]) if fp64 else None,
         "step_model": {"bytes_per_step": bstep, "achieved_GBps": bstep * sps32 / 1e9,
-                       "frac_of_peak": bstep * sps32 / HBM_PEAK},
+                       "frac_of_peak": bstep * sps32 / HBM_PEAK,
+                       "frac_of_measured_ceiling": bstep * sps32 / (ceiling["GBps"] * 1e9)},
         "kernels": kernels,
         "kernels_two_chunks_in_flight": kernels_split,
         "measured_copy_GBps": copy_gbps,
+        "hbm_ceiling": ceiling,
+        "zstreams_ab": ab,
         "roofline": roof,
         "cpu_baseline": cpu,
         "ensemble": ens,

@@ -754,32 +754,45 @@ void orc_rs_raw(orc* o, double* raw) {
   orc_prepare_realspace(o);
   const int nx = o->nx, ny = o->ny, nz = o->nz;
   const long plane = (long)nx * ny;
-  for (int q = 0; q < 25; q++) raw[q] = 0.0;
-  double* in = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
-  double* out = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
-  double* pc = (double*)malloc(sizeof(double) * 2 * (size_t)ny * o->nki);
-  double* r = (double*)malloc(sizeof(double) * (size_t)o->nf * (size_t)plane);
-  for (int z = 0; z < nz; z++) {
-    plane_fields(o, z, in, out, pc, r);
-    for (long q = 0; q < plane; q++) {
-      double v[3];
-      lab_velocity(o, r[q], r[plane + q], r[2 * plane + q], v);
-      const double w[3] = {r[3 * plane + q], r[4 * plane + q], r[5 * plane + q]};
-      raw[0] += v[0] * v[0]; raw[1] += v[1] * v[1]; raw[2] += v[2] * v[2];
-      raw[3] += v[0] * v[1]; raw[4] += v[0] * v[2]; raw[5] += v[1] * v[2];
-      for (int c = 0; c < 3; c++) {
-        raw[6 + c] += v[c] * v[c] * v[c];
-        raw[9 + c] += v[c] * v[c] * v[c] * v[c];
-        raw[12 + c] += w[c] * w[c];
-        raw[15 + c] += w[c] * w[c] * w[c];
-        raw[18 + c] += w[c] * w[c] * w[c] * w[c];
-        if (fabs(w[c]) > raw[22 + c]) raw[22 + c] = fabs(w[c]);
+  /* one row of 25 per z plane, planes in parallel, rows combined in z order */
+  double* zr = (double*)calloc(25 * (size_t)nz, sizeof(double));
+#pragma omp parallel num_threads(nthr(o))
+  {
+    double* in = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+    double* out = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+    double* pc = (double*)malloc(sizeof(double) * 2 * (size_t)ny * o->nki);
+    double* r = (double*)malloc(sizeof(double) * (size_t)o->nf * (size_t)plane);
+#pragma omp for schedule(dynamic)
+    for (int z = 0; z < nz; z++) {
+      double* rz = &zr[25 * (size_t)z];
+      plane_fields(o, z, in, out, pc, r);
+      for (long q = 0; q < plane; q++) {
+        double v[3];
+        lab_velocity(o, r[q], r[plane + q], r[2 * plane + q], v);
+        const double w[3] = {r[3 * plane + q], r[4 * plane + q], r[5 * plane + q]};
+        rz[0] += v[0] * v[0]; rz[1] += v[1] * v[1]; rz[2] += v[2] * v[2];
+        rz[3] += v[0] * v[1]; rz[4] += v[0] * v[2]; rz[5] += v[1] * v[2];
+        for (int c = 0; c < 3; c++) {
+          rz[6 + c] += v[c] * v[c] * v[c];
+          rz[9 + c] += v[c] * v[c] * v[c] * v[c];
+          rz[12 + c] += w[c] * w[c];
+          rz[15 + c] += w[c] * w[c] * w[c];
+          rz[18 + c] += w[c] * w[c] * w[c] * w[c];
+          if (fabs(w[c]) > rz[22 + c]) rz[22 + c] = fabs(w[c]);
+        }
+        const double u = fabs(v[0]) + fabs(v[1]) + fabs(v[2]);
+        if (u > rz[21]) rz[21] = u;
       }
-      const double u = fabs(v[0]) + fabs(v[1]) + fabs(v[2]);
-      if (u > raw[21]) raw[21] = u;
     }
+    free(in); free(out); free(pc); free(r);
   }
-  free(in); free(out); free(pc); free(r);
+  for (int q = 0; q < 25; q++) raw[q] = 0.0;
+  for (int z = 0; z < nz; z++) {
+    const double* rz = &zr[25 * (size_t)z];
+    for (int q = 0; q < 21; q++) raw[q] += rz[q];
+    for (int q = 21; q < 25; q++) if (rz[q] > raw[q]) raw[q] = rz[q];
+  }
+  free(zr);
 }
 
 /* f_FluidstatsRsCollect normalisation (src/pf_fluidstats.f90:992-1116),
