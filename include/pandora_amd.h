@@ -355,6 +355,18 @@ typedef struct pdr_stats_t {
  * spec_ken1..3, spec_dis, spec_disskew and dens_mod, bin kk at index kk-1. */
 int pdr_stats(pdr_ctx* ctx, pdr_stats_t* out, double* spec);
 
+/* pdr_stats without any device allocation.  No counterpart in the reference:
+ * an entry point for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The workgroup partials live in a fixed 16 MiB device array of the library
+ * (2^21 doubles per GPU, held until the process ends) instead of the
+ * context's lazily allocated buffer; boxes whose statistics rows do not fit
+ * are refused with PDR_ERR_UNSUPPORTED.  spec (7 * kkmax doubles, as above) is
+ * required: it is also the working buffer.  Results equal pdr_stats' to
+ * rounding (the partials may be summed over fewer workgroups).  All calls on
+ * one GPU share the array: calls that overlap on one GPU race. */
+int pdr_stats_noalloc(pdr_ctx* ctx, pdr_stats_t* out, double* spec);
+
 /* Streamwise 1-D spectra of f_FluidstatsFs2D + f_FluidstatsFsCollect
  * (src/pf_fluidstats.f90:1154-1283) for the current state: spec_1i[i],
  * spec_2i[i], i = 0..NKI-1, the sums over the (y, z) points of 2 |u_lab,c|^2 of

@@ -16,6 +16,7 @@
 // Neumaier summation, and the host finishes the kkmax-sized arrays in FP64.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "pdr_internal.hpp"
@@ -322,26 +323,28 @@ extern "C" {
 namespace pdr {
 namespace {
 
-// Mode-density normalisation and secondary quantities of f_FluidstatsFs
-void stats_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6], pdr_stats_t* q,
-                  double* spec) {
+// Mode-density normalisation and secondary quantities of f_FluidstatsFs, in
+// place: sp[7 * kkmax] holds the raw shell sums (ken, ken1, ken2, ken3, dis,
+// disskew, count) on entry and the normalised spectra and mode density on exit.
+void stats_finish_inplace(const pdr_ctx* ctx, double* sp, const double uu[6], pdr_stats_t* q) {
   const int kkmax = pdr::kkmax_of(ctx->g);
   const double nu = ctx->p.f_nu;
-  std::vector<double> sp(size_t(7) * kkmax, 0.0);
   // normalisation by the mode density (src/pf_fluidstats.f90:1547-1575, 767-769)
   double f_ken = 0, f_dis = 0, f_ds = 0, fn[6] = {0, 0, 0, 0, 0, 0};
   for (int kk = 1; kk <= kkmax; ++kk) {
     const double up = double(kk) + 0.5, lo = double(kk) - 0.5;
-    const double dens = raw[size_t(6 * kkmax + kk - 1)] / (4.0 / 3.0 * pdr::kOnePiRef * (up * up * up - lo * lo * lo));
+    const double dens = sp[size_t(6 * kkmax + kk - 1)] / (4.0 / 3.0 * pdr::kOnePiRef * (up * up * up - lo * lo * lo));
     sp[size_t(6 * kkmax + kk - 1)] = dens;
     if (dens != 0.0) {
-      f_ken = f_ken + raw[size_t(kk - 1)];
-      f_dis = f_dis + raw[size_t(4 * kkmax + kk - 1)];
-      f_ds = f_ds + raw[size_t(5 * kkmax + kk - 1)];
+      f_ken = f_ken + sp[size_t(kk - 1)];
+      f_dis = f_dis + sp[size_t(4 * kkmax + kk - 1)];
+      f_ds = f_ds + sp[size_t(5 * kkmax + kk - 1)];
       for (int a = 0; a < 6; ++a) {
-        sp[size_t(a * kkmax + kk - 1)] = raw[size_t(a * kkmax + kk - 1)] / dens;
+        sp[size_t(a * kkmax + kk - 1)] = sp[size_t(a * kkmax + kk - 1)] / dens;
         fn[a] = fn[a] + sp[size_t(a * kkmax + kk - 1)];
       }
+    } else {
+      for (int a = 0; a < 6; ++a) sp[size_t(a * kkmax + kk - 1)] = 0.0;
     }
   }
   q->tke = 0.5 * (uu[0] + uu[1] + uu[2]);
@@ -384,6 +387,12 @@ void stats_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const doub
   q->tn_kol = std::sqrt(nu / fn[4]);
   q->disskew = fn[5] * std::pow(15.0 / fn[4], 3.0 / 2.0) * std::sqrt(nu) / 35.0;
   q->c_epsilon = fn[4] * li[0] / (fint * fint * fint);
+}
+
+void stats_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6], pdr_stats_t* q,
+                  double* spec) {
+  std::vector<double> sp(raw);
+  stats_finish_inplace(ctx, sp.data(), uu, q);
   if (spec) std::copy(sp.begin(), sp.end(), spec);
 }
 
@@ -694,6 +703,111 @@ int pdr_stage_stats(pdr_ctx* c, int32_t* tstep, pdr_stats_t* fs, double* spec, p
   }
   if (spec_1i || spec_2i || h) pdr::spectra_finish(c, raw, uu, s12, c->sst_b11, spec_1i, spec_2i, h);
   if (rs) pdr::rs_finish(c, row.data() + width, rs);
+  return PDR_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free wave-space statistics (pdr_stats_noalloc).  The workgroup
+// partials, the reduced row and the folded shell sums live in `scratch`, a
+// fixed device array of kScratchDoubles doubles (16 MiB) that every GPU the
+// library runs on holds until the process ends.  Its device address is looked
+// up once per device with hipGetSymbolAddress (with that device current) and
+// kept in g_scratch; the kernels take that address, never the array's name.
+// Every call on a device shares that device's array and leaves the last
+// call's contents there: each call writes every element it later reads
+// (the partial rows, then the row, then the folded sums), so nothing needs
+// clearing, and calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+namespace pdr {
+constexpr size_t kScratchDoubles = size_t(2) << 20;
+}  // namespace pdr
+
+static __device__ __align__(256) double scratch[pdr::kScratchDoubles];
+
+namespace pdr {
+namespace {
+
+constexpr int kScratchMaxDevices = 64;
+double* g_scratch[kScratchMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// this device's copy of scratch (the current device is the context's)
+hipError_t scratch_ptr(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kScratchMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
+
+// the reduced row h[stats_hist + 12] -> raw[7 * kkmax] (hi + lo of the six shell
+// sums, then the mode counts) and uu[6] at raw + 7 * kkmax: stats_unpack's
+// additions on the device
+__global__ void __launch_bounds__(kStatThreads)
+    k_stats_fold(const double* __restrict__ h, int kkmax, int nki, double* __restrict__ raw) {
+  const int q = int(blockIdx.x * kStatThreads + threadIdx.x);
+  if (q < 6 * kkmax) {
+    raw[q] = h[q] + h[6 * kkmax + q];
+  } else if (q < 7 * kkmax) {
+    raw[q] = h[6 * kkmax + q];
+  } else if (q < 7 * kkmax + 6) {
+    const int nh = stats_hist(kkmax, nki);
+    const int a = q - 7 * kkmax;
+    raw[q] = h[nh + a] + h[nh + 6 + a];
+  }
+}
+
+}  // namespace
+}  // namespace pdr
+
+extern "C" {
+
+int pdr_stats_noalloc(pdr_ctx* ctx, pdr_stats_t* q, double* spec) {
+  if (!ctx || !q || !spec)
+    return pdr::set_error(PDR_ERR_ARG, "pdr_stats_noalloc: null argument (spec is required: it is the working buffer)");
+  if (!pdr::stats_fits_lds(ctx)) return pdr::set_error(PDR_ERR_UNSUPPORTED, "statistics: spectrum too long for the LDS");
+  const int kkmax = pdr::kkmax_of(ctx->g);
+  const long long width = pdr::stats_width(ctx);
+  const long long tail = 7LL * kkmax + 6;
+  // rows of workgroup partials that fit beside the reduced row and the folded sums
+  const long long rows = (static_cast<long long>(pdr::kScratchDoubles) - width - tail) / width;
+  if (rows < 1)
+    return pdr::set_error(PDR_ERR_UNSUPPORTED,
+                          "pdr_stats_noalloc: the statistics of this box need " + std::to_string(2 * width + tail) +
+                              " doubles of scratch, more than its maximum of " +
+                              std::to_string(pdr::kScratchDoubles));
+  const int nblk = int(std::min<long long>(pdr::stats_nblk(ctx), rows));
+  double* base = nullptr;
+  hipError_t e = pdr::scratch_ptr(&base);
+  if (e != hipSuccess) return pdr::set_error(PDR_ERR_HIP, std::string("pdr_stats_noalloc: scratch address: ") + hipGetErrorString(e));
+  double* part = base;
+  double* row = part + size_t(nblk) * size_t(width);
+  double* fold = row + width;
+  const pdr::StatsFrame sf = pdr::make_stats_frame(ctx);
+  e = ctx->prec ? pdr::stats_t<float>(ctx, sf, nblk, part, row) : pdr::stats_t<double>(ctx, sf, nblk, part, row);
+  if (e == hipSuccess) {
+    pdr::k_stats_fold<<<dim3(unsigned((tail + pdr::kStatThreads - 1) / pdr::kStatThreads)), pdr::kStatThreads, 0,
+                        ctx->stream>>>(row, kkmax, ctx->g.nki, fold);
+    e = hipGetLastError();
+  }
+  double uu[6];
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(spec, fold, size_t(7) * kkmax * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(uu, fold + size_t(7) * kkmax, sizeof(uu), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return pdr::set_error(PDR_ERR_HIP, std::string("pdr_stats_noalloc: ") + hipGetErrorString(e));
+  pdr::stats_finish_inplace(ctx, spec, uu, q);
   return PDR_OK;
 }
 

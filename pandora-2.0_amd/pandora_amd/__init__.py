@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH):
         "pdr_upload_rk3": (C.c_int, [vp, dp, dp, dp]),
         "pdr_download_rk3": (C.c_int, [vp, dp, dp, dp]),
         "pdr_stats": (C.c_int, [vp, dp, dp]),
+        "pdr_stats_noalloc": (C.c_int, [vp, dp, dp]),
         "pdr_rs_stats": (C.c_int, [vp, dp]),
         "pdr_spectra_1d": (C.c_int, [vp, dp, dp, dp]),
         "pdr_set_stage_stats": (C.c_int, [vp, i32]),
@@ -475,6 +476,14 @@ class Pandora:
         q = np.zeros(len(STATS_NAMES))
         spec = np.zeros(7 * self.kkmax)
         _check(self.L.pdr_stats(self.h, q, spec))
+        return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
+
+    def stats_noalloc(self):
+        """As stats(), through pdr_stats_noalloc (no device allocation: the
+        partials live in the library's fixed device scratch array)."""
+        q = np.zeros(len(STATS_NAMES))
+        spec = np.zeros(7 * self.kkmax)
+        _check(self.L.pdr_stats_noalloc(self.h, q, spec))
         return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
 
     def spectra_1d(self):
