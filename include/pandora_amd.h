@@ -315,6 +315,27 @@ int pdr_restart_write_host(const char* path, int64_t n, const double* u1, const 
                            const double* u3, int32_t int64_header);
 int pdr_restart_read_host(const char* path, int64_t n, double* u1, double* u2, double* u3);
 
+/* Particle restart: g_RestartPetscParticle(WRITE_FILE) (src/pg_restart.f90:264-318)
+ * of the arrays p_ControlPrepareWriteRestart gathers (src/pp_control.f90:1219-1430):
+ * PETSc Vec records of np doubles, in order particle_xp(1,:), (2,:), (3,:),
+ * particle_up(1..3,:), the diameter (0 for tracers), particle_status(1,:) (the
+ * particle number 1..np, src/pp_control.f90:646, 690) and, with
+ * P_KEEP_INITIAL_POSITION, particle_xp0(1..3,:): 8 or 11 records, the
+ * particles in the caller's numbering.  PDR_ERR_STATE without particles. */
+int pdr_particles_restart_write(pdr_ctx* ctx, const char* path, int32_t int64_header);
+/* Reads such a file of np particles (either header width, 8 or 11 records),
+ * placing each entry by its particle number: positions, particle_up and, when
+ * the file has them and the context keeps initial positions, particle_xp0;
+ * particle_up_rk3 starts at zero.  No reference counterpart: the reference's
+ * read branch is commented out (src/pg_restart.f90:220-262). */
+int pdr_particles_restart_read(pdr_ctx* ctx, const char* path, int64_t np);
+/* Host-only forms (no context or GPU): xp, up, xp0 as (3, np) arrays; xp0 may
+ * be NULL (8 records); *has_xp0 (may be NULL) tells whether the file had them. */
+int pdr_particles_restart_write_host(const char* path, int64_t np, const double* xp, const double* up,
+                                     const double* xp0, int32_t int64_header);
+int pdr_particles_restart_read_host(const char* path, int64_t np, double* xp, double* up, double* xp0,
+                                    int32_t* has_xp0);
+
 /* Scaled restart (F_RESTART_SCALE > 1): g_RestartPetscFluid of a file written on
  * the grid (nx, ny, nz) / scale, then f_InitialiseCopyRestart
  * (src/pf_initialise.f90:1184-1299, src/pf_arrays.f90:1047-1312): each coarse
@@ -407,6 +428,15 @@ typedef struct pdr_rs_stats_t {
 } pdr_rs_stats_t;
 int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out);
 
+/* The TS_CORR two-point correlation of f_FluidstatsRs2D / f_FluidstatsRsCollect
+ * (src/pf_fluidstats.f90:933-945, 1092-1106) of the current state:
+ * r1i[NX/2 + 1] = sum over grid points of u1_lab(x) u1(x + s) / (rs_u1u1 NX NY NZ)
+ * (laboratory u1 at x, stored u1 at x + s, x periodic) and lint11 = dx_lab
+ * times that sum over s = 0 .. (NX-1)/2 + 1.  Evaluated from per-i
+ * correlation sums of the wave-space statistics pass (an exact trigonometric
+ * sum on the grid), with no inverse transform. */
+int pdr_rs_correlation(pdr_ctx* ctx, double* r1i, double* lint11);
+
 /* Statistics of stage 1 inside the step, as f_RK3Stage1 takes them
  * (src/pf_rk3.f90:58-60 and f_RK3RealSpace, 1073-1078, 1124-1143): with
  * every > 0, stage 1 of each tstep with (tstep - 1) mod every = 0 runs the
@@ -425,6 +455,9 @@ int pdr_set_stage_stats(pdr_ctx* ctx, int32_t every);
  * and pdr_spectra_1d for that state.  Any output pointer may be NULL. */
 int pdr_stage_stats(pdr_ctx* ctx, int32_t* tstep, pdr_stats_t* fs, double* spec, pdr_rs_stats_t* rs,
                     double* spec_1i, double* spec_2i, pdr_homog_t* h);
+/* The same correlation from the stage-1 record (as pdr_rs_correlation of the
+ * state the record's stage 1 started from); *tstep as in pdr_stage_stats. */
+int pdr_stage_correlation(pdr_ctx* ctx, int32_t* tstep, double* r1i, double* lint11);
 
 /* Shell sums before mode-density normalisation from the same device pass:
  * ek, dk, dsk have kkmax entries (energy, 2 nu k^2 E, 4 nu^2 k^4 E);

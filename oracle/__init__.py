@@ -90,6 +90,7 @@ def lib():
         L.orc_rs_raw.argtypes = [vp, dp]
         L.orc_rs_finish.argtypes = [C.c_int, C.c_int, C.c_int, dp, dp]
         L.orc_rs_stats.argtypes = [vp, dp]
+        L.orc_rs_correlation.argtypes = [vp, dp, C.POINTER(C.c_double)]
         L.orc_spectra_1d.argtypes = [vp, dp, dp]
         L.orc_homogeneous.argtypes = [vp, dp]
         L.orc_force_nk.restype = C.c_int
@@ -237,6 +238,14 @@ class Oracle:
         out = np.zeros(25)
         lib().orc_rs_stats(self.h, out)
         return dict(zip(RS_NAMES, out))
+
+    def rs_correlation(self):
+        """(r1i over separations 0..NX/2, lint11): the TS_CORR two-point
+        correlation of f_FluidstatsRs2D/RsCollect of the current state."""
+        r = np.zeros(self.nx // 2 + 1)
+        li = C.c_double()
+        lib().orc_rs_correlation(self.h, r, C.byref(li))
+        return r, li.value
 
     def spectra_1d(self):
         """(spec_1i, spec_2i) over i = 0..K (f_FluidstatsFs2D/FsCollect)."""

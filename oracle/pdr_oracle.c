@@ -795,6 +795,59 @@ void orc_rs_raw(orc* o, double* raw) {
   free(zr);
 }
 
+/* Longitudinal two-point correlation of f_FluidstatsRs2D on TS_CORR steps
+ * (src/pf_fluidstats.f90:933-945) and its normalisation in f_FluidstatsRsCollect
+ * (1092-1106), on one box (x_min_2r = 0, x_width_2r = NX): for every grid point,
+ * the laboratory-frame u1 there times the stored u1 at x + s (x periodic), for
+ * s = 0 .. (NX-1)/2 + 1, summed into r1i(s) and, each product times dx, into
+ * lint11; both divided by rs_u1u1 * NX NY NZ.  r1i holds NX/2 + 1 entries (the
+ * reference's array extent: for odd NX its loop's last separation enters lint11
+ * only).  Planes in parallel, each plane's sums combined in z order. */
+void orc_rs_correlation(orc* o, double* r1i, double* lint11) {
+  orc_prepare_realspace(o);
+  const int nx = o->nx, ny = o->ny, nz = o->nz;
+  const long plane = (long)nx * ny;
+  const int smax = (nx - 1) / 2 + 1;
+  const int w = smax + 3; /* separations 0..smax, lint11, u1u1 */
+  double* zr = (double*)calloc((size_t)w * (size_t)nz, sizeof(double));
+#pragma omp parallel num_threads(nthr(o))
+  {
+    double* in = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+    double* out = (double*)malloc(sizeof(double) * 2 * (size_t)(ny > nx ? ny : nx));
+    double* pc = (double*)malloc(sizeof(double) * 2 * (size_t)ny * o->nki);
+    double* r = (double*)malloc(sizeof(double) * (size_t)o->nf * (size_t)plane);
+#pragma omp for schedule(dynamic)
+    for (int z = 0; z < nz; z++) {
+      double* rz = &zr[(size_t)w * z];
+      plane_fields(o, z, in, out, pc, r);
+      for (int y = 0; y < ny; y++)
+        for (int x = 0; x < nx; x++) {
+          const long q = (long)y * nx + x;
+          double v[3];
+          lab_velocity(o, r[q], r[plane + q], r[2 * plane + q], v);
+          rz[smax + 2] += v[0] * v[0];
+          for (int s = 0; s <= smax; s++) {
+            int xx = x + s;
+            if (xx >= nx) xx -= nx;
+            const double c = v[0] * r[(long)y * nx + xx];
+            rz[s] += c;
+            rz[smax + 1] += c * o->dx;
+          }
+        }
+    }
+    free(in); free(out); free(pc); free(r);
+  }
+  double* t = (double*)calloc((size_t)w, sizeof(double));
+  for (int z = 0; z < nz; z++)
+    for (int q = 0; q < w; q++) t[q] += zr[(size_t)w * z + q];
+  const double n3 = ((double)nx * (double)ny) * (double)nz;
+  const double u1u1 = t[smax + 2] / n3;
+  for (int s = 0; s <= nx / 2; s++) r1i[s] = t[s] / (u1u1 * n3);
+  *lint11 = t[smax + 1] / (u1u1 * n3);
+  free(t);
+  free(zr);
+}
+
 /* f_FluidstatsRsCollect normalisation (src/pf_fluidstats.f90:992-1116),
  * including its rs_rmsvort1 that is not divided by the number of points.
  * out[25]: rs_u1u1 rs_u2u2 rs_u3u3 rs_u1u2 rs_u1u3 rs_u2u3, rs_skew1..3,

@@ -130,6 +130,8 @@ void   orc_stats(const orc* o, double* spec, double* q);
 void   orc_rs_raw(orc* o, double* raw);
 void   orc_rs_finish(int nx, int ny, int nz, const double* raw, double* out);
 void   orc_rs_stats(orc* o, double* out);
+/* TS_CORR two-point correlation: r1i[NX/2 + 1] and lint11 (overwrites the 1r2w arrays) */
+void   orc_rs_correlation(orc* o, double* r1i, double* lint11);
 /* f_FluidstatsFs2D/FsCollect: streamwise spectra spec_1i, spec_2i (nki each;
  * overwrites the 1r2w arrays); f_FluidstatsHomogeneous record out[15] */
 void   orc_spectra_1d(orc* o, double* s1, double* s2);

@@ -1,5 +1,6 @@
 // pdr_host.hip -- host-side pieces of the C-ABI that sit either side of the
 // time step: seeded initial conditions and PETSc-binary restart I/O.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -214,6 +215,122 @@ int pdr_restart_write(pdr_ctx* ctx, const char* path, int32_t int64_header) {
   int rc = pdr_download_state(ctx, u[0].data(), u[1].data(), u[2].data());
   if (rc != PDR_OK) return rc;
   return pdr_restart_write_host(path, n, u[0].data(), u[1].data(), u[2].data(), int64_header);
+}
+
+// Particle restart records (g_RestartPetscParticle, src/pg_restart.f90:264-318):
+// xp1, xp2, xp3, up1, up2, up3, dp, status[, xp01, xp02, xp03], each a PETSc Vec
+// record {int32 VEC_FILE_CLASSID, PetscInt np, np big-endian doubles}.
+int pdr_particles_restart_write_host(const char* path, int64_t np, const double* xp, const double* up,
+                                     const double* xp0, int32_t int64_header) {
+  if (!path || np <= 0 || !xp || !up)
+    return pdr::set_error(PDR_ERR_ARG, "pdr_particles_restart_write_host: bad argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return pdr::set_error(PDR_ERR_IO, std::string("cannot open for writing: ") + path);
+  const int hl = int64_header ? 8 : 4;
+  const int nrec = xp0 ? 11 : 8;
+  std::vector<unsigned char> buf(size_t(np) * 8);
+  bool ok = true;
+  for (int r = 0; r < nrec && ok; ++r) {
+    unsigned char h[12];
+    be_store(h, uint64_t(uint32_t(kVecFileClassId)), 4);
+    be_store(h + 4, uint64_t(np), hl);
+    ok = std::fwrite(h, 1, size_t(4 + hl), f) == size_t(4 + hl);
+    for (int64_t q = 0; q < np && ok; ++q) {
+      double v;
+      if (r < 3)
+        v = xp[3 * q + r];
+      else if (r < 6)
+        v = up[3 * q + (r - 3)];
+      else if (r == 6)
+        v = 0.0;  // particle diameter: tracers
+      else if (r == 7)
+        v = double(q + 1);  // particle_status(1,:): the particle number
+      else
+        v = xp0[3 * q + (r - 8)];
+      uint64_t bits;
+      std::memcpy(&bits, &v, 8);
+      be_store(&buf[size_t(8 * q)], bits, 8);
+    }
+    if (ok) ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  }
+  if (std::fclose(f) != 0) ok = false;
+  return ok ? PDR_OK : pdr::set_error(PDR_ERR_IO, std::string("write failed: ") + path);
+}
+
+int pdr_particles_restart_read_host(const char* path, int64_t np, double* xp, double* up, double* xp0,
+                                    int32_t* has_xp0) {
+  if (!path || np <= 0 || !xp || !up) return pdr::set_error(PDR_ERR_ARG, "pdr_particles_restart_read_host: bad argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return pdr::set_error(PDR_ERR_IO, std::string("cannot open: ") + path);
+  std::fseek(f, 0, SEEK_END);
+  const long long size = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  int hl = 0, nrec = 0;
+  for (int r : {8, 11})
+    for (int w : {4, 8})
+      if (size == (long long)r * (4 + w + 8 * np)) {
+        hl = w;
+        nrec = r;
+      }
+  if (!hl) {
+    std::fclose(f);
+    return pdr::set_error(PDR_ERR_IO, "particle restart size does not match 8 or 11 PETSc Vec records of np particles");
+  }
+  std::vector<double> rec(size_t(nrec) * size_t(np));
+  std::vector<unsigned char> buf(size_t(np) * 8);
+  bool ok = true;
+  for (int r = 0; r < nrec && ok; ++r) {
+    unsigned char h[12];
+    ok = std::fread(h, 1, size_t(4 + hl), f) == size_t(4 + hl);
+    if (ok) ok = int32_t(be_load(h, 4)) == kVecFileClassId && (int64_t)be_load(h + 4, hl) == np;
+    if (ok) ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+    for (int64_t q = 0; q < np && ok; ++q) {
+      const uint64_t bits = be_load(&buf[size_t(8 * q)], 8);
+      std::memcpy(&rec[size_t(r) * size_t(np) + size_t(q)], &bits, 8);
+    }
+  }
+  std::fclose(f);
+  if (!ok) return pdr::set_error(PDR_ERR_IO, "bad PETSc Vec header or short read in particle restart file");
+  // place every entry by its particle number (status record); it must be a permutation of 1..np
+  std::vector<char> seen(size_t(np), 0);
+  const double* st = &rec[size_t(7) * size_t(np)];
+  for (int64_t q = 0; q < np; ++q) {
+    const double v = st[q];
+    const int64_t id = int64_t(v) - 1;
+    if (!(v == std::floor(v)) || id < 0 || id >= np || seen[size_t(id)])
+      return pdr::set_error(PDR_ERR_IO, "particle restart: status record is not a numbering 1..np");
+    seen[size_t(id)] = 1;
+    for (int c = 0; c < 3; ++c) {
+      xp[3 * id + c] = rec[size_t(c) * size_t(np) + size_t(q)];
+      up[3 * id + c] = rec[size_t(3 + c) * size_t(np) + size_t(q)];
+      if (xp0 && nrec == 11) xp0[3 * id + c] = rec[size_t(8 + c) * size_t(np) + size_t(q)];
+    }
+  }
+  if (has_xp0) *has_xp0 = nrec == 11 ? 1 : 0;
+  return PDR_OK;
+}
+
+int pdr_particles_restart_write(pdr_ctx* ctx, const char* path, int32_t int64_header) {
+  if (!ctx || !path) return pdr::set_error(PDR_ERR_ARG, "pdr_particles_restart_write: null argument");
+  if (ctx->np <= 0) return pdr::set_error(PDR_ERR_STATE, "pdr_particles_restart_write: no particles set");
+  const size_t n3 = size_t(3) * size_t(ctx->np);
+  std::vector<double> xp(n3), up(n3), xp0(ctx->xp0 ? n3 : 0);
+  int rc = pdr_particles_download(ctx, xp.data(), nullptr);
+  if (rc == PDR_OK) rc = pdr_particles_download_aux(ctx, up.data(), ctx->xp0 ? xp0.data() : nullptr);
+  if (rc != PDR_OK) return rc;
+  return pdr_particles_restart_write_host(path, ctx->np, xp.data(), up.data(), ctx->xp0 ? xp0.data() : nullptr,
+                                          int64_header);
+}
+
+int pdr_particles_restart_read(pdr_ctx* ctx, const char* path, int64_t np) {
+  if (!ctx || !path || np <= 0) return pdr::set_error(PDR_ERR_ARG, "pdr_particles_restart_read: bad argument");
+  const size_t n3 = size_t(3) * size_t(np);
+  std::vector<double> xp(n3), up(n3), xp0(n3);
+  int32_t has0 = 0;
+  int rc = pdr_particles_restart_read_host(path, np, xp.data(), up.data(), xp0.data(), &has0);
+  if (rc == PDR_OK) rc = pdr_particles_upload(ctx, xp.data(), np);
+  if (rc == PDR_OK) rc = pdr_particles_upload_aux(ctx, up.data(), (has0 && ctx->xp0) ? xp0.data() : nullptr);
+  return rc;
 }
 
 int pdr_restart_read_scaled_host(const char* path, int32_t nx, int32_t ny, int32_t nz, int32_t scale,

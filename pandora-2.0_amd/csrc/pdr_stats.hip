@@ -27,6 +27,7 @@ namespace pdr {
 void spectra_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const double uu[6],
                     const std::vector<double>& s12, double b11, double* spec_1i, double* spec_2i,
                     pdr_homog_t* h);
+void corr_finish(const pdr_ctx* ctx, const std::vector<double>& s12, double* r1i, double* lint11);
 
 namespace {
 
@@ -44,8 +45,9 @@ struct StatsFrame {
 };
 
 // Width of the per-workgroup histogram: 13 shell arrays of kkmax, then the
-// streamwise bins (hi and lo of sum |u1_lab|^2, then of |u2_lab|^2) over i.
-__host__ __device__ inline int stats_hist(int kkmax, int nki) { return 13 * kkmax + 4 * nki; }
+// streamwise bins over i: hi and lo of sum |u1_lab|^2, of |u2_lab|^2, and of the
+// real and imaginary parts of conj(u1_lab) u1 (the two-point correlation).
+__host__ __device__ inline int stats_hist(int kkmax, int nki) { return 13 * kkmax + 8 * nki; }
 
 __device__ __forceinline__ double wvn_s(int k, int kmax) {
   return (k <= kmax) ? double(k) : double(k - (2 * kmax + 1));
@@ -102,6 +104,11 @@ __global__ void __launch_bounds__(kStatThreads)
     // f_FluidstatsFs2D maps with binv in every sheared form)
     two_sum_add(&s1[i], &s1[sf.nki + i], ul[0][0] * ul[0][0] + ul[0][1] * ul[0][1]);
     two_sum_add(&s1[2 * sf.nki + i], &s1[3 * sf.nki + i], ul[1][0] * ul[1][0] + ul[1][1] * ul[1][1]);
+    // two-point correlation of the laboratory u1 with the stored u1 along x:
+    // sum over (j, k) of conj(u1_lab) u1 per i (Wiener-Khinchin form of the
+    // real-space sum of f_FluidstatsRs2D, src/pf_fluidstats.f90:933-945)
+    two_sum_add(&s1[4 * sf.nki + i], &s1[5 * sf.nki + i], ul[0][0] * um[0][0] + ul[0][1] * um[0][1]);
+    two_sum_add(&s1[6 * sf.nki + i], &s1[7 * sf.nki + i], ul[0][0] * um[0][1] - ul[0][1] * um[0][0]);
     const double(*v)[2] = sf.lab_state ? um : ul;
     double ph[6];
     ph[0] = v[0][0] * v[0][0] + v[0][1] * v[0][1];
@@ -231,13 +238,12 @@ void stats_unpack(const pdr_ctx* c, const double* h, std::vector<double>& raw, d
   for (int q = 0; q < kkmax; ++q) raw[size_t(6 * kkmax + q)] = h[size_t(12 * kkmax + q)];
   const size_t nh = size_t(stats_hist(kkmax, nki));
   for (int a = 0; a < 6; ++a) uu[a] = h[nh + a] + h[nh + 6 + a];
-  if (s12) {  // [s1 over i, s2 over i]
-    s12->assign(size_t(2) * nki, 0.0);
+  if (s12) {  // [s1 over i, s2 over i, Re and Im of the correlation sums over i]
+    s12->assign(size_t(4) * nki, 0.0);
     const size_t o = size_t(13) * kkmax;
-    for (int i = 0; i < nki; ++i) {
-      (*s12)[size_t(i)] = h[o + i] + h[o + nki + i];
-      (*s12)[size_t(nki + i)] = h[o + 2 * nki + i] + h[o + 3 * nki + i];
-    }
+    for (int i = 0; i < nki; ++i)
+      for (int a = 0; a < 4; ++a)
+        (*s12)[size_t(a * nki + i)] = h[o + size_t(2 * a * nki + i)] + h[o + size_t((2 * a + 1) * nki + i)];
   }
 }
 
@@ -426,6 +432,19 @@ int pdr_spectra_1d(pdr_ctx* ctx, double* spec_1i, double* spec_2i, pdr_homog_t* 
   return PDR_OK;
 }
 
+// The TS_CORR two-point correlation of f_FluidstatsRs2D / RsCollect
+// (src/pf_fluidstats.f90:933-945, 1092-1106) of the current state, from the
+// per-i correlation bins of the statistics pass.
+int pdr_rs_correlation(pdr_ctx* ctx, double* r1i, double* lint11) {
+  if (!ctx || !r1i || !lint11) return pdr::set_error(PDR_ERR_ARG, "pdr_rs_correlation: null argument");
+  std::vector<double> raw, s12;
+  double uu[6];
+  int rc = pdr::stats_raw(ctx, raw, uu, &s12);
+  if (rc != PDR_OK) return rc;
+  pdr::corr_finish(ctx, s12, r1i, lint11);
+  return PDR_OK;
+}
+
 }  // extern "C"
 
 namespace pdr {
@@ -516,6 +535,38 @@ void spectra_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const do
     h->fn_int2 = q.fn_int2;
     h->fn_int3 = q.fn_int3;
   }
+}
+
+// r1i(s) for s = 0..NX/2 and lint11 from the per-i sums C(i) = sum over (j, k)
+// of conj(u1_lab) u1 and S(i) = sum |u1_lab|^2 of the statistics pass.  On the
+// grid the real-space sums of f_FluidstatsRs2D are exact trigonometric sums:
+// sum_x u1_lab(x) u1(x + s) = N^3 (Re C(0) + 2 sum_{i>=1} Re(C(i) e^{2 pi i i s / NX}))
+// and sum_x u1_lab^2 = N^3 (S(0) + 2 sum_{i>=1} S(i)), so r1i(s) is their ratio
+// and lint11 = dx_lab times the sum of the ratio over the reference loop's
+// separations s = 0 .. (NX-1)/2 + 1 (for odd NX the last one is beyond the
+// r1i array and enters lint11 only).
+void corr_finish(const pdr_ctx* ctx, const std::vector<double>& s12, double* r1i, double* lint11) {
+  const int nx = ctx->g.nx, nki = ctx->g.nki;
+  const double* S = s12.data();
+  const double* cre = S + 2 * nki;
+  const double* cim = S + 3 * nki;
+  double den = S[0];
+  for (int i = 1; i < nki; ++i) den += 2.0 * S[i];
+  const int smax = (nx - 1) / 2 + 1;
+  const double twopi = 2.0 * 3.14159265358979323846;
+  double lsum = 0.0;
+  for (int s = 0; s <= smax; ++s) {
+    double num = cre[0];
+    for (int i = 1; i < nki; ++i) {
+      const double th = twopi * double((static_cast<long long>(i) * s) % nx) / double(nx);
+      num += 2.0 * (cre[i] * std::cos(th) - cim[i] * std::sin(th));
+    }
+    const double r = num / den;
+    if (s <= nx / 2) r1i[s] = r;
+    lsum += r;
+  }
+  // dx_laboratoryframe = TWOPI B110 / NODES_X (src/pg_domain.f90:178-183), B110 = 1
+  *lint11 = lsum * (2.0 * kOnePiRef / double(nx));
 }
 
 }  // namespace pdr
@@ -703,6 +754,25 @@ int pdr_stage_stats(pdr_ctx* c, int32_t* tstep, pdr_stats_t* fs, double* spec, p
   }
   if (spec_1i || spec_2i || h) pdr::spectra_finish(c, raw, uu, s12, c->sst_b11, spec_1i, spec_2i, h);
   if (rs) pdr::rs_finish(c, row.data() + width, rs);
+  return PDR_OK;
+}
+
+// The TS_CORR two-point correlation of the stage-1 record (the state that
+// stage 1 of the record's step started from), as pdr_rs_correlation.
+int pdr_stage_correlation(pdr_ctx* c, int32_t* tstep, double* r1i, double* lint11) {
+  if (!c || !r1i || !lint11) return pdr::set_error(PDR_ERR_ARG, "pdr_stage_correlation: null argument");
+  if (tstep) *tstep = c->sst_tstep;
+  if (c->sst_tstep < 0) return PDR_OK;
+  const int width = pdr::stats_width(c);
+  std::vector<double> row(static_cast<size_t>(width));
+  hipError_t e = hipMemcpyAsync(row.data(), pdr::sst_wave_row(c), size_t(width) * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return pdr::set_error(PDR_ERR_HIP, std::string("pdr_stage_correlation: ") + hipGetErrorString(e));
+  std::vector<double> raw, s12;
+  double uu[6];
+  pdr::stats_unpack(c, row.data(), raw, uu, &s12);
+  pdr::corr_finish(c, s12, r1i, lint11);
   return PDR_OK;
 }
 

@@ -58,6 +58,7 @@ module pdr_rk3_amd
   public :: pdr_InitParticles, pdr_SetTracersInStage
   public :: p_ControlRK3, p_ControlFinishTimeStep
   public :: p_RK3SolveParticle, p_RK3SolveParticleBF, p_RK3SolveParticleR
+  public :: pdr_RsCorrelation, pdr_ParticleRestartWrite, pdr_ParticleRestartRead
 
   interface
     integer(c_int) function c_pdr_create(out, p) bind(C, name="pdr_create")
@@ -170,6 +171,23 @@ module pdr_rk3_amd
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: c
       real(c_double), intent(out) :: q(25)
+    end function
+    integer(c_int) function c_pdr_rs_correlation(c, r1i, lint11) bind(C, name="pdr_rs_correlation")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: c
+      real(c_double), intent(out) :: r1i(*), lint11
+    end function
+    integer(c_int) function c_pdr_particles_restart_write(c, path, i64) bind(C, name="pdr_particles_restart_write")
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: c
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int32_t), value :: i64
+    end function
+    integer(c_int) function c_pdr_particles_restart_read(c, path, np) bind(C, name="pdr_particles_restart_read")
+      import :: c_int, c_ptr, c_char, c_int64_t
+      type(c_ptr), value :: c
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: np
     end function
     integer(c_int) function c_pdr_particle_stats_config(c, enable, keep, release) &
         bind(C, name="pdr_particle_stats_config")
@@ -386,6 +404,27 @@ contains
     ierr = int(c_pdr_restart_read_scaled(ctx, trim(path)//c_null_char, int(scale, c_int32_t)))
   end subroutine pdr_RestartReadScaled
 
+  !> g_RestartPetscParticle(WRITE_FILE) (src/pg_restart.f90:264-318) after
+  !! p_ControlPrepareWriteRestart: xp, up, dp, status[, xp0] records
+  subroutine pdr_ParticleRestartWrite ( path, int64header, ierr )
+    character(len=*), intent(in) :: path
+    logical, intent(in)          :: int64header
+    integer, intent(inout)       :: ierr
+    integer(c_int32_t)           :: flag
+    flag = 0
+    if ( int64header ) flag = 1
+    ierr = int(c_pdr_particles_restart_write(ctx, trim(path)//c_null_char, flag))
+  end subroutine pdr_ParticleRestartWrite
+
+  !> Reads a particle restart file of np particles (positions, particle_up,
+  !! particle_xp0) back onto the device
+  subroutine pdr_ParticleRestartRead ( path, np, ierr )
+    character(len=*), intent(in) :: path
+    integer, intent(in)          :: np
+    integer, intent(inout)       :: ierr
+    ierr = int(c_pdr_particles_restart_read(ctx, trim(path)//c_null_char, int(np, c_int64_t)))
+  end subroutine pdr_ParticleRestartRead
+
   !> E(k), D(k) shell sums and totals (f_FluidstatsFsPrimaryQuants)
   subroutine pdr_Spectrum ( ek, dk, dsk, tot, ierr )
     real(kind=c_double), intent(out) :: ek(:), dk(:), dsk(:), tot(6)
@@ -412,6 +451,14 @@ contains
     integer, intent(inout)           :: ierr
     ierr = int(c_pdr_rs_stats(ctx, q))
   end subroutine pdr_RsStats
+
+  !> The TS_CORR part of f_FluidstatsRs2D / f_FluidstatsRsCollect: r1i(0:NODES_X/2)
+  !! and rs_lint11 of the current state
+  subroutine pdr_RsCorrelation ( r1i, lint11, ierr )
+    real(kind=c_double), intent(out) :: r1i(0:), lint11
+    integer, intent(inout)           :: ierr
+    ierr = int(c_pdr_rs_correlation(ctx, r1i, lint11))
+  end subroutine pdr_RsCorrelation
 
   !> Tracer statistics of f_RK3Stage1 (p_StatsPrimary, p_StatsDispersion,
   !! p_StatsRelativeVelocity): track = P_TRACK_PART statistics on/off,

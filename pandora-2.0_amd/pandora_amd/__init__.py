@@ -103,6 +103,8 @@ def load_library(path: str = LIB_PATH):
         "pdr_stats": (C.c_int, [vp, dp, dp]),
         "pdr_stats_noalloc": (C.c_int, [vp, dp, dp]),
         "pdr_rs_stats": (C.c_int, [vp, dp]),
+        "pdr_rs_correlation": (C.c_int, [vp, dp, C.POINTER(d)]),
+        "pdr_stage_correlation": (C.c_int, [vp, C.POINTER(i32), dp, C.POINTER(d)]),
         "pdr_spectra_1d": (C.c_int, [vp, dp, dp, dp]),
         "pdr_set_stage_stats": (C.c_int, [vp, i32]),
         "pdr_stage_stats": (C.c_int, [vp, C.POINTER(i32), dp, dp, dp, dp, dp, dp]),
@@ -141,6 +143,10 @@ def load_library(path: str = LIB_PATH):
         "pdr_init_field_host": (C.c_int, [i32, i32, i32, C.c_uint64, d, d, dp, dp, dp]),
         "pdr_restart_write_host": (C.c_int, [C.c_char_p, i64, dp, dp, dp, i32]),
         "pdr_restart_read_host": (C.c_int, [C.c_char_p, i64, dp, dp, dp]),
+        "pdr_particles_restart_write": (C.c_int, [vp, C.c_char_p, i32]),
+        "pdr_particles_restart_read": (C.c_int, [vp, C.c_char_p, i64]),
+        "pdr_particles_restart_write_host": (C.c_int, [C.c_char_p, i64, vp, vp, vp, i32]),
+        "pdr_particles_restart_read_host": (C.c_int, [C.c_char_p, i64, vp, vp, vp, C.POINTER(i32)]),
         "pdr_restart_write": (C.c_int, [vp, C.c_char_p, i32]),
         "pdr_restart_read": (C.c_int, [vp, C.c_char_p]),
         "pdr_restart_read_scaled_host": (C.c_int, [C.c_char_p, i32, i32, i32, i32, dp, dp, dp]),
@@ -205,6 +211,28 @@ def restart_read_host(path, shape):
     flat = [o.view(np.float64).reshape(-1) for o in out]
     _check(L.pdr_restart_read_host(path.encode(), flat[0].size, *flat))
     return out
+
+
+def particles_restart_write_host(path, xp, up, xp0=None, int64_header=False):
+    """Particle restart file (g_RestartPetscParticle) from host arrays of shape
+    (np, 3): positions, particle_up and, optionally, particle_xp0."""
+    L = load_library()
+    xp = np.ascontiguousarray(xp, np.float64)
+    up = np.ascontiguousarray(up, np.float64)
+    x0 = None if xp0 is None else np.ascontiguousarray(xp0, np.float64)
+    _check(L.pdr_particles_restart_write_host(path.encode(), len(xp), xp.ctypes.data, up.ctypes.data,
+                                              None if x0 is None else x0.ctypes.data, 1 if int64_header else 0))
+
+
+def particles_restart_read_host(path, count):
+    """(xp, up, xp0 or None) of a particle restart file of `count` particles,
+    each (count, 3), in particle-number order."""
+    L = load_library()
+    xp, up, x0 = (np.zeros((count, 3)) for _ in range(3))
+    has = C.c_int32(0)
+    _check(L.pdr_particles_restart_read_host(path.encode(), count, xp.ctypes.data, up.ctypes.data,
+                                             x0.ctypes.data, C.byref(has)))
+    return xp, up, (x0 if has.value else None)
 
 
 def restart_read_scaled_host(path, nx, ny, nz, scale):
@@ -467,6 +495,12 @@ class Pandora:
     def restart_read(self, path: str):
         _check(self.L.pdr_restart_read(self.h, path.encode()))
 
+    def particles_restart_write(self, path: str, int64_header: bool = False):
+        _check(self.L.pdr_particles_restart_write(self.h, path.encode(), 1 if int64_header else 0))
+
+    def particles_restart_read(self, path: str, count: int):
+        _check(self.L.pdr_particles_restart_read(self.h, path.encode(), count))
+
     def restart_read_scaled(self, path: str, scale: int):
         _check(self.L.pdr_restart_read_scaled(self.h, path.encode(), scale))
 
@@ -485,6 +519,22 @@ class Pandora:
         spec = np.zeros(7 * self.kkmax)
         _check(self.L.pdr_stats_noalloc(self.h, q, spec))
         return dict(zip(SPEC_NAMES, spec.reshape(7, self.kkmax))), dict(zip(STATS_NAMES, q))
+
+    def rs_correlation(self):
+        """(r1i over separations 0..NX/2, lint11): the TS_CORR two-point
+        correlation of the current state on the device."""
+        r = np.zeros(self.params.nodes_x // 2 + 1)
+        li = C.c_double()
+        _check(self.L.pdr_rs_correlation(self.h, r, C.byref(li)))
+        return r, li.value
+
+    def stage_correlation(self):
+        """(tstep of the record or -1, r1i, lint11) of the stage-1 record."""
+        r = np.zeros(self.params.nodes_x // 2 + 1)
+        li = C.c_double()
+        t = C.c_int32()
+        _check(self.L.pdr_stage_correlation(self.h, C.byref(t), r, C.byref(li)))
+        return t.value, r, li.value
 
     def spectra_1d(self):
         """(spec_1i, spec_2i, homogeneous-record dict) of the current state on the device."""

@@ -242,6 +242,60 @@ def test_rs_stats_parseval_iso_and_sheared(oracle_mod):
     assert q["rs_u1u1"] != pytest.approx(o.rs_stats()["rs_u1u1"], rel=1e-3)
 
 
+# ---- TS_CORR two-point correlation (f_FluidstatsRs2D / RsCollect) --------------
+
+def _corr_numpy(a, b, n):
+    """r1i(s) = sum a(x) b(x + s) / sum a^2 over the grid (x first axis, periodic),
+    s = 0 .. (n-1)//2 + 1, with numpy from real-space fields."""
+    den = float(np.sum(a * a))
+    return np.array([float(np.sum(a * np.roll(b, -s, axis=0))) / den for s in range((n - 1) // 2 + 2)])
+
+
+def test_rs_correlation_single_mode(oracle_mod):
+    n, A = 16, 0.8
+    o = oracle_mod.Oracle(n)
+    dx = 2.0 * ONEPI / n
+    o.set_state(*_single_mode(o, 0, 0, A))        # u1 = A cos x
+    r, li = o.rs_correlation()
+    s = np.arange(n // 2 + 1)
+    assert np.abs(r - np.cos(2 * np.pi * s / n)).max() < 1e-14
+    assert li == pytest.approx(dx * np.cos(2 * np.pi * s / n).sum(), rel=1e-13, abs=1e-15)
+    o.set_state(*_single_mode(o, 0, 1, A))        # u1 = A cos y: no x dependence
+    r, li = o.rs_correlation()
+    assert np.abs(r - 1.0).max() < 1e-14
+    assert li == pytest.approx(dx * (n // 2 + 1), rel=1e-14)
+
+
+@pytest.mark.parametrize("n", [16, 15])
+def test_rs_correlation_against_numpy(oracle_mod, n):
+    """Random field, isotropic and in a sheared frame (binv(1,3) = 0.7, the
+    laboratory u1 correlated with the stored u1); odd n: the reference loop's
+    last separation (n-1)//2 + 1 enters lint11 but not the NX/2 + 1 array."""
+    o = oracle_mod.Oracle(n, nu=0.01)
+    u = _random_state(o, 40 + n)
+    o.set_state(*u)
+    dx = 2.0 * ONEPI / n
+    u1, _ = _real_field(o, u[0])
+    u3, _ = _real_field(o, u[2])
+    for frame in ("iso", "sheared"):
+        if frame == "sheared":
+            o = oracle_mod.Oracle(n, f_type=2, rk3_timestep=2, deform_s=1.0, nu=0.01)
+            o.set_state(*u)
+            b, bi = o.get_frame()
+            b[6], bi[6] = -0.7, 0.7
+            o.set_frame(b, bi)
+            a = u1 + 0.7 * u3
+        else:
+            a = u1
+        want = _corr_numpy(a, u1, n)
+        r, li = o.rs_correlation()
+        assert len(r) == n // 2 + 1
+        assert np.abs(r - want[:n // 2 + 1]).max() < 1e-12, frame
+        assert li == pytest.approx(dx * want.sum(), rel=1e-12), frame
+    # the stored u1 enters at x + s: the laboratory u1 there would give another r1i
+    assert np.abs(r - _corr_numpy(a, a, n)[:n // 2 + 1]).max() > 1e-6
+
+
 # ---- streamwise 1-D spectra (f_FluidstatsFs2D / FsCollect) --------------------
 
 @pytest.mark.parametrize("shear", [False, True])
