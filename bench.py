@@ -357,6 +357,35 @@ def hbm_ceiling(pa, sizes_gib=(1, 4, 8), iters=10):
     return {"GBps": best, "best": where, "probes_GBps": probes}
 
 
+def stats_on_off(pa, n, ntr, steps=5, repeats=3):
+    """Steps/s of the FP32-mixed headline workload with the statistics of stage 1
+    taken inside every step (f_FluidstatsFs of the starting state and the
+    real-space moments of stage 1's planes, pdr_set_stage_stats(1)) and without,
+    each the median of `repeats` timed runs of `steps` steps in one context."""
+    out = {}
+    for every in (0, 1):
+        g = pa.Pandora(n, f_type=pa.F_SHEAR_DNS, rk3_timestep=pa.RK3_TIMESTEP_DECIDE, deform_s=1.0,
+                       remesh_t=250, nu=0.005, precision=pa.PDR_FP32_MIXED)
+        g.init_field(2, 2.0 / 3.0, 6.0)
+        if ntr:
+            g.init_particles(102, ntr)
+        g.set_stage_stats(every)
+        t = g.run(1, 2, 0.0)
+        s = 3
+        rates = []
+        for _ in range(repeats):
+            g.synchronize()
+            t0 = time.perf_counter()
+            t = g.run(s, steps, t)
+            g.synchronize()
+            rates.append(steps / (time.perf_counter() - t0))
+            s += steps
+        g.close()
+        out["stats_on" if every else "stats_off"] = {"median_steps_per_s": _median(rates), "samples": rates}
+    out["cost"] = 1.0 - out["stats_on"]["median_steps_per_s"] / out["stats_off"]["median_steps_per_s"]
+    return out
+
+
 def zstreams_ab(pa, n, ntr, steps=5, repeats=3):
     """Steps/s of the FP32-mixed headline workload with the z chunks of a stage
     one after another (PDR_ZSTREAMS=1) and two in flight (2), each the median
@@ -533,7 +562,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
-    ap.add_argument("--no-ab", action="store_true", help="skip the one/two chunk-stream comparison")
+    ap.add_argument("--no-ab", action="store_true",
+                    help="skip the one/two chunk-stream and the statistics on/off comparisons")
     ap.add_argument("--no-cpu-small", action="store_true",
                     help="CPU leg: skip the 256^3 statistics off/on medians")
     ap.add_argument("--ensemble-only", action="store_true",
@@ -585,6 +615,10 @@ def main():
     if not args.no_ab:
         print("[bench] one against two z chunks in flight (medians)", file=sys.stderr, flush=True)
         ab = zstreams_ab(pa, n, ntr)
+    stats = None
+    if not args.no_ab:
+        print("[bench] statistics of stage 1 in every step, on and off (medians)", file=sys.stderr, flush=True)
+        stats = stats_on_off(pa, n, ntr)
     fp64 = None
     if not args.no_fp64:
         k64 = max(3, args.steps // 3)
@@ -695,6 +729,7 @@ def main():
         "measured_copy_GBps": copy_gbps,
         "hbm_ceiling": ceiling,
         "zstreams_ab": ab,
+        "stage_statistics": stats,
         "roofline": roof,
         "cpu_baseline": cpu,
         "ensemble": ens,
