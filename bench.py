@@ -594,14 +594,12 @@ def main():
     # the headline: no per-kernel events on the streams
     dt32, _, ok32 = run_gpu(pa, n, pa.PDR_FP32_MIXED, args.steps, args.warmup, ntr, False)
     sps32 = args.steps / dt32
-    # Per-kernel times from two short profiled runs: the default schedule (two
-    # chunks in flight from 512 planes up, where launches overlap) and the
-    # chunks of a stage one after another, whose launch durations are each
-    # kernel's own (the rates and the roofline come from this one).
+    # Per-kernel times from a short profiled run with the chunks of a stage one
+    # after another (the default at this size), whose launch durations are each
+    # kernel's own: the rates and the roofline come from it.
     rsteps = max(3, args.steps // 4)
-    print(f"[bench] FP32-mixed {n}^3, profiled: {rsteps} steps default schedule, {rsteps} chunks one after another",
+    print(f"[bench] FP32-mixed {n}^3, profiled: {rsteps} steps, chunks one after another",
           file=sys.stderr, flush=True)
-    _, prof_split, _ = run_gpu(pa, n, pa.PDR_FP32_MIXED, rsteps, 1, ntr, True)
     env_zs = os.environ.get("PDR_ZSTREAMS")
     os.environ["PDR_ZSTREAMS"] = "1"
     try:
@@ -664,7 +662,6 @@ def main():
         return out
 
     kernels = per_kernel(prof, 8, 4, rsteps)
-    kernels_split = per_kernel(prof_split, 8, 4, rsteps)
     if fp64:
         fp64["kernels"] = per_kernel(fp64.pop("prof"), 16, 8, rsteps)
     if dom:
@@ -725,7 +722,6 @@ def main():
                        "frac_of_peak": bstep * sps32 / HBM_PEAK,
                        "frac_of_measured_ceiling": bstep * sps32 / (ceiling["GBps"] * 1e9)},
         "kernels": kernels,
-        "kernels_two_chunks_in_flight": kernels_split,
         "measured_copy_GBps": copy_gbps,
         "hbm_ceiling": ceiling,
         "zstreams_ab": ab,

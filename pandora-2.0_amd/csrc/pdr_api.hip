@@ -444,12 +444,14 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     // Two chunks in flight: the even chunks' y/x/y passes on the main stream,
     // the odd ones' on a second stream with their own x-pencil buffer, so that
     // the HBM-bound y passes of one run beside the issue-bound x pass of the
-    // other (512^3 FP32: 84.8 / 85.9 against 83.8 / 84.9 steps/s, FP64 +1 %;
-    // 1024^3 FP64 +3 %).  On from 512 planes up when the whole-box buffer fits
-    // (PDR_ZSTREAMS=2 forces it, 1 turns it off), in the same buffer bytes; not
-    // over chunks cut for memory (1536^3 FP64: 0.750 against 0.771 steps/s).
+    // other.  Medians of same-process runs at 512^3 FP32 with 1M tracers show
+    // no gain (83.34 against 83.47 steps/s, spread 0.4 %), so it is on by
+    // default only where it was measured faster: FP64 from 1024 planes up
+    // (+3 %) when the whole-box buffer fits; not over chunks cut for memory
+    // (1536^3 FP64: 0.750 against 0.771 steps/s).  PDR_ZSTREAMS=2 forces it,
+    // 1 turns it off; the same buffer bytes either way.
     const char* zs = std::getenv("PDR_ZSTREAMS");
-    c->zs2 = (zs && zs[0] ? zs[0] == '2' : (nz >= 512 && zc == nz)) && nz > 1;
+    c->zs2 = (zs && zs[0] ? zs[0] == '2' : (c->prec == 0 && nz >= 1024 && zc == nz)) && nz > 1;
     if (c->zs2) zc = std::min(zc, (nz + 1) / 2);
     g.zc = zc;
     g.m2 = (long long)zc * g.ny * g.ldi;

@@ -78,20 +78,21 @@ __device__ __forceinline__ int lds_off(int o) {
 // stride LD, padded layout above).  The per-element offsets o(r) = B + r*S are
 // turned into a base plus compile-time constants wherever the pad positions
 // allow it (all passes of power-of-two lengths), which keeps the address
-// registers of the butterflies few.
-template <typename T, int N, int LINES, int LD, int R, int NS, int SIGN>
+// registers of the butterflies few.  Twiddles: PPT = true reads the per-pass
+// table of load_tw (LDS), false the plain table exp(-2 pi i m / N) (global).
+template <typename T, int N, int LINES, int LD, int R, int NS, int SIGN, bool PPT, int NT = kBlock2>
 __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
   constexpr int NBL = N / R;
   constexpr int TOTAL = NBL * LINES;
-  constexpr int NB = (TOTAL + kBlock2 - 1) / kBlock2;
+  constexpr int NB = (TOTAL + NT - 1) / NT;
   constexpr int TWS = N / (NS * R);
   constexpr int PS = lds_pad_shift<T>();
   constexpr int P2 = 1 << PS;
   cx<T> v[NB][R];
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int t = int(threadIdx.x) + kBlock2 * b;
-    if (TOTAL % kBlock2 == 0 || t < TOTAL) {
+    const int t = int(threadIdx.x) + NT * b;
+    if (TOTAL % NT == 0 || t < TOTAL) {
       const int line = t / NBL;
       const int j = t - line * NBL;
       if constexpr (NBL % P2 == 0) {
@@ -107,15 +108,15 @@ __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __
   __syncthreads();
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const int t = int(threadIdx.x) + kBlock2 * b;
-    if (TOTAL % kBlock2 == 0 || t < TOTAL) {
+    const int t = int(threadIdx.x) + NT * b;
+    if (TOTAL % NT == 0 || t < TOTAL) {
       const int line = t / NBL;
       const int j = t - line * NBL;
       const int k = j % NS;
       if (NS > 1) {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cx<T> w = tw[k * r * TWS];
+          cx<T> w = PPT ? tw[(NS - 1) + k * (R - 1) + (r - 1)] : tw[k * r * TWS];
           if (SIGN > 0) w.y = -w.y;
           v[b][r] = mul(v[b][r], w);
         }
@@ -143,12 +144,12 @@ __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __
   __syncthreads();
 }
 
-template <typename T, int N, int LINES, int LD, int SIGN, int P = 0, int NS = 1>
+template <typename T, int N, int LINES, int LD, int SIGN, bool PPT = true, int NT = kBlock2, int P = 0, int NS = 1>
 __device__ __forceinline__ void fft_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
   if constexpr (P < plan_npass(N)) {
     constexpr int R = plan_radix(N, P);
-    pass_ip<T, N, LINES, LD, R, NS, SIGN>(buf, tw);
-    fft_ip<T, N, LINES, LD, SIGN, P + 1, NS * R>(buf, tw);
+    pass_ip<T, N, LINES, LD, R, NS, SIGN, PPT, NT>(buf, tw);
+    fft_ip<T, N, LINES, LD, SIGN, PPT, NT, P + 1, NS * R>(buf, tw);
   }
 }
 
@@ -182,10 +183,27 @@ __device__ __forceinline__ void put1(T* p, T v) {
     *p = v;
 }
 
-// Copy the length-N twiddle table into LDS (the caller synchronises).
-template <typename T, int N>
+// Per-pass twiddle table in LDS, built from the plain table src[m] =
+// exp(-2 pi i m / N) (the caller synchronises).  Pass p (NS > 1) keeps its
+// NS (R - 1) twiddles w^(k r N / (NS R)) at NS - 1 + k (R - 1) + (r - 1): the
+// lanes of a butterfly wave (consecutive k) then read at stride R - 1, which is
+// odd for radix 8, 4 and 2 (no bank conflicts), where the plain table's
+// stride k r N / (NS R) puts many lanes on one bank.  The passes' tables tile
+// [0, N - 1) exactly (sum of NS (R - 1) over the passes is N - 1).
+template <typename T, int N, int NT = kBlock2, int P = 0, int NS = 1>
 __device__ __forceinline__ void load_tw(cx<T>* __restrict__ dst, const cx<T>* __restrict__ src) {
-  for (int m = threadIdx.x; m < N; m += kBlock2) dst[m] = src[m];
+  if constexpr (P < plan_npass(N)) {
+    constexpr int R = plan_radix(N, P);
+    if constexpr (NS > 1) {
+      constexpr int TWS = N / (NS * R);
+      for (int e = threadIdx.x; e < NS * (R - 1); e += NT) {
+        const int k = e / (R - 1);
+        const int r = e - k * (R - 1) + 1;
+        dst[(NS - 1) + e] = src[k * r * TWS];
+      }
+    }
+    load_tw<T, N, NT, P + 1, NS * R>(dst, src);
+  }
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs (each with its own L2);

@@ -303,15 +303,34 @@ constexpr int lds_zinv_bytes(int n) {
   return (n + zinv_lines<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
 }
 #define PDR_ZINV_WAVES __attribute__((amdgpu_waves_per_eu(waves_for(lds_zinv_bytes<T>(N)))))
+// The single-box register z inverse runs 512-thread workgroups over twice the
+// lines (16 FP32 / 8 FP64): each thread keeps the same velocity elements as
+// with 256 threads and 8 / 4 lines, and occupancy stays two waves per SIMD, but
+// a tile's store of one z row is a whole 128-byte line of the 1r2w array
+// instead of half of one (which the neighbouring tile wrote at another time).
+// Batched ensembles keep the 256-thread form.
+#ifndef PDR_ZINV_THREADS
+#define PDR_ZINV_THREADS 512
+#endif
+template <typename T, int ZT>
+constexpr int zinv_lines_t(int n) {
+  return zinv_lines<T>(n) * (ZT / kBlock2);
+}
+template <typename T, int ZT>
+constexpr int lds_zinv_bytes_t(int n) {
+  return (n + zinv_lines_t<T, ZT>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>));
+}
+#define PDR_ZINV_WAVES_T \
+  __attribute__((amdgpu_waves_per_eu(PDR_ZINV_THREADS > kBlock2 ? PDR_ZINV_THREADS / kBlock2 : waves_for(lds_zinv_bytes<T>(N)))))
 
-template <typename T, int N>
+template <typename T, int N, int ZT = kBlock2>
 __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const Coef& cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
               const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
-  constexpr int L = zinv_lines<T>(N);
+  constexpr int L = zinv_lines_t<T, ZT>(N);
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKK = 2 * K + 1;
-  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
+  constexpr int QE = (L * N + ZT - 1) / ZT;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -319,17 +338,17 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
   const int i = int(bid / unsigned(ntile));
   const int j0 = tile * L;
   const int nkj = g.nkj;
-  load_tw<T, N>(tw, twg);
+  load_tw<T, N, ZT>(tw, twg);
   const long long base = ((long long)i * nkj + j0) * NKK;
   const cx<T>* uu[3] = {u0 + base, u1 + base, u2 + base};
   cx<T> ua[QE], ub[QE], uc[QE];  // u1, u2, u3 of this thread's elements
 #pragma unroll
   for (int q = 0; q < QE; ++q) {
-    const int idx = int(threadIdx.x) + kBlock2 * q;
+    const int idx = int(threadIdx.x) + ZT * q;
     const int line = idx / N;
     const int z = idx - line * N;
     const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : -1);
-    const bool ok = (L * N % kBlock2 == 0 || idx < L * N) && j0 + line < nkj && k >= 0;
+    const bool ok = (L * N % ZT == 0 || idx < L * N) && j0 + line < nkj && k >= 0;
     const int e = line * NKK + k;
     ua[q] = ok ? get<kNrZinv>(&uu[0][e]) : czero<T>();
     ub[q] = ok ? get<kNrZinv>(&uu[1][e]) : czero<T>();
@@ -340,8 +359,8 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
   for (int f = 0; f < 6; ++f) {  // unrolled: every field selection is static
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
-      const int idx = int(threadIdx.x) + kBlock2 * q;
-      if (L * N % kBlock2 == 0 || idx < L * N) {
+      const int idx = int(threadIdx.x) + ZT * q;
+      if (L * N % ZT == 0 || idx < L * N) {
         const int line = idx / N;
         const int z = idx - line * N;
         const cx<T> un[3] = {ua[q], ub[q], uc[q]};
@@ -366,9 +385,9 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
       }
     }
     __syncthreads();
-    fft_ip<T, N, L, LD, +1>(buf, tw);
+    fft_ip<T, N, L, LD, +1, true, ZT>(buf, tw);
     cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
-    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+    for (int idx = threadIdx.x; idx < L * N; idx += ZT) {
       const int z = idx / L;
       const int line = idx - z * L;
       if (j0 + line < nkj) put<kNtZinv>(&dst[z * g.ldj + line], buf[line * LD + lds_off<T>(z)]);
@@ -378,10 +397,10 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
 }
 
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_ZINV_WAVES
+__global__ void __launch_bounds__(PDR_ZINV_THREADS) PDR_ZINV_WAVES_T
     k2_zinv_r(Geo g, Coef cf, const cx<T>* __restrict__ u0, const cx<T>* __restrict__ u1,
               const cx<T>* __restrict__ u2, cx<T>* __restrict__ w1, const cx<T>* __restrict__ twg, int ntile) {
-  k2_zinv_r_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, cf, u0, u1, u2, w1, twg, ntile);
+  k2_zinv_r_body<T, N, PDR_ZINV_THREADS>(xcd_remap(blockIdx.x, gridDim.x), g, cf, u0, u1, u2, w1, twg, ntile);
 }
 
 // ---------------------------------------------------------------------------
@@ -470,7 +489,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_Y_WAVES
 // ---------------------------------------------------------------------------
 // pass 3: RB rows (z, y0..y0+RB-1) per block: c2r, product, r2c
 // ---------------------------------------------------------------------------
-template <typename T, int N, bool RS>
+template <typename T, int N, bool RS, int XT = kBlock2>
 __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const Coef& cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
              unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile, int z0,
              double* __restrict__ rsp) {
@@ -478,9 +497,9 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   constexpr int LD = lds_ld<T>(N);
   constexpr int KI = N / 3;
   constexpr int NKI = KI + 1;
-  constexpr int IT = (RB * N + kBlock2 - 1) / kBlock2;  // product items per thread
+  constexpr int IT = (RB * N + XT - 1) / XT;  // product items per thread
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ T red[kBlock2];
+  __shared__ T red[XT];
   constexpr bool TWG = x_tw_global(N);
   cx<T>* const lds0 = reinterpret_cast<cx<T>*>(smem);
   const cx<T>* const tw = TWG ? twg : lds0;
@@ -490,7 +509,7 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   const int z = z0 + zl;
   const int y0 = tile * RB;
   const int ny = g.ny;
-  if constexpr (!TWG) load_tw<T, N>(lds0, twg);
+  if constexpr (!TWG) load_tw<T, N, XT>(lds0, twg);
   cx<T>* rowp = w2 + ((long long)zl * ny + y0) * g.ldi;
   // fill: the 3 * RB lines pack pairs of real inputs (p = 0: u1 + i u2,
   // 1: u3 + i w1, 2: w2 + i w3).  Each stored mode m' = 0..KI of a pair (a, b) is
@@ -499,18 +518,18 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   // KI < m < N - KI is zero.  All loads of a thread precede its LDS stores.
   constexpr int NZB = N - 2 * KI - 1;
   constexpr int EL = 3 * RB * NKI;
-  constexpr int QL = (EL + kBlock2 - 1) / kBlock2;
+  constexpr int QL = (EL + XT - 1) / XT;
   cx<T> A[QL], B[QL];
 #pragma unroll
   for (int q = 0; q < QL; ++q) {
-    const int idx = int(threadIdx.x) + kBlock2 * q;
+    const int idx = int(threadIdx.x) + XT * q;
     const int line = idx / NKI;
     const int m = idx - line * NKI;
     const int r = line / 3;
     const int p = line - 3 * r;
     A[q] = czero<T>();
     B[q] = czero<T>();
-    if ((EL % kBlock2 == 0 || idx < EL) && y0 + r < ny) {
+    if ((EL % XT == 0 || idx < EL) && y0 + r < ny) {
       const cx<T>* fa = rowp + (long long)(2 * p) * g.m2 + r * g.ldi;
       A[q] = get<kNrXpass>(&fa[m]);
       B[q] = get<kNrXpass>(&fa[g.m2 + m]);
@@ -518,8 +537,8 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   }
 #pragma unroll
   for (int q = 0; q < QL; ++q) {
-    const int idx = int(threadIdx.x) + kBlock2 * q;
-    if (EL % kBlock2 == 0 || idx < EL) {
+    const int idx = int(threadIdx.x) + XT * q;
+    if (EL % XT == 0 || idx < EL) {
       const int line = idx / NKI;
       const int m = idx - line * NKI;
       cx<T>* row = buf + line * LD;
@@ -531,12 +550,12 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
       }
     }
   }
-  for (int idx = threadIdx.x; idx < 3 * RB * NZB; idx += kBlock2) {
+  for (int idx = threadIdx.x; idx < 3 * RB * NZB; idx += XT) {
     const int line = idx / NZB;
     buf[line * LD + lds_off<T>(KI + 1 + idx - line * NZB)] = czero<T>();
   }
   __syncthreads();
-  fft_ip<T, N, 3 * RB, LD, +1>(buf, tw);
+  fft_ip<T, N, 3 * RB, LD, +1, !TWG, XT>(buf, tw);
   const T V[9] = {T(cf.V[0]), T(cf.V[1]), T(cf.V[2]), T(cf.V[3]), T(cf.V[4]),
                   T(cf.V[5]), T(cf.V[6]), T(cf.V[7]), T(cf.V[8])};
   T nl[IT][3];
@@ -546,8 +565,8 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   if (RS) rs_init(acc);
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
-    const int idx = int(threadIdx.x) + kBlock2 * q;
-    if ((RB * N) % kBlock2 == 0 || idx < RB * N) {
+    const int idx = int(threadIdx.x) + XT * q;
+    if ((RB * N) % XT == 0 || idx < RB * N) {
       const int r = idx / N;
       const int x = idx - r * N;
       const cx<T> a = buf[(3 * r) * LD + lds_off<T>(x)], b = buf[(3 * r + 1) * LD + lds_off<T>(x)],
@@ -576,15 +595,15 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   }
   if (RS) {
     // partial of this workgroup at its place in the whole box: (z, tile)
-    __shared__ double rs_lds[kRsWidth * (kBlock2 / 64)];
+    __shared__ double rs_lds[kRsWidth * (XT / 64)];
     rs_block_out(acc, rs_lds, rsp + ((long long)z * ntile + tile) * kRsWidth);
   }
   __syncthreads();
   // forward lines: r -> n1 + i n2 of row r; RB + r/2 -> n3 of rows (r, r+1)
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
-    const int idx = int(threadIdx.x) + kBlock2 * q;
-    if ((RB * N) % kBlock2 == 0 || idx < RB * N) {
+    const int idx = int(threadIdx.x) + XT * q;
+    if ((RB * N) % XT == 0 || idx < RB * N) {
       const int r = idx / N;
       const int x = idx - r * N;
       buf[r * LD + lds_off<T>(x)] = mk<T>(nl[q][0], nl[q][1]);
@@ -596,8 +615,8 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
     }
   }
   __syncthreads();
-  fft_ip<T, N, 3 * RB / 2, LD, -1>(buf, tw);
-  for (int idx = threadIdx.x; idx < RB * NKI; idx += kBlock2) {
+  fft_ip<T, N, 3 * RB / 2, LD, -1, !TWG, XT>(buf, tw);
+  for (int idx = threadIdx.x; idx < RB * NKI; idx += XT) {
     const int r = idx / NKI;
     const int m = idx - r * NKI;
     if (y0 + r >= ny) continue;
@@ -613,13 +632,20 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
       put<kNtXpass>(d0 + 2 * g.m2, mk<T>(T(0.5) * (W.x + Wr.x), T(0.5) * (W.y - Wr.y)));
   }
   if (want_cfl) {
-    red[threadIdx.x] = lmax;
-    __syncthreads();
-    for (int s = kBlock2 / 2; s > 0; s >>= 1) {
-      if (int(threadIdx.x) < s) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + s] ? red[threadIdx.x] : red[threadIdx.x + s];
-      __syncthreads();
+    // maximum over the wave, then over the XT / 64 waves of the workgroup
+    T m = lmax;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T o = __shfl_xor(m, off);
+      m = o > m ? o : m;
     }
-    if (threadIdx.x == 0) atomicMax(umax, (unsigned long long)__double_as_longlong(double(red[0])));
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T r = red[0];
+      for (int w = 1; w < XT / 64; ++w) r = red[w] > r ? red[w] : r;
+      atomicMax(umax, (unsigned long long)__double_as_longlong(double(r)));
+    }
   }
 }
 
@@ -628,7 +654,8 @@ __global__ void __launch_bounds__(kBlock2) PDR_X_WAVES
     k2_xpass(Geo g, Coef cf, cx<T>* __restrict__ w2, T* __restrict__ ur, int want_tr, int want_cfl,
              unsigned long long* __restrict__ umax, const cx<T>* __restrict__ twg, int ntile, int z0,
              double* __restrict__ rsp) {
-  k2_xpass_body<T, N, RS>(xcd_remap(blockIdx.x, gridDim.x), g, cf, w2, ur, want_tr, want_cfl, umax, twg, ntile, z0, rsp);
+  k2_xpass_body<T, N, RS>(xcd_remap(blockIdx.x, gridDim.x), g, cf, w2, ur, want_tr, want_cfl, umax, twg, ntile, z0,
+                          rsp);
 }
 
 // ---------------------------------------------------------------------------
@@ -877,12 +904,12 @@ template <typename T, int N>
 hipError_t zinv_n(pdr_ctx* c, const Coef& cf) {
   const Geo& g = c->g;
   if constexpr (PDR_ZINV_REG && N <= 512) {  // longer lines hold too many values per thread
-    constexpr int L = zinv_lines<T>(N);
+    constexpr int L = zinv_lines_t<T, PDR_ZINV_THREADS>(N);
     const int nt = (g.nkj + L - 1) / L;
-    const size_t lds = size_t(lds_zinv_bytes<T>(N));
+    const size_t lds = size_t(lds_zinv_bytes_t<T, PDR_ZINV_THREADS>(N));
     hipError_t e = big_lds((const void*)k2_zinv_r<T, N>, lds);
     if (e != hipSuccess) return e;
-    k2_zinv_r<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
+    k2_zinv_r<T, N><<<dim3(unsigned(nt * g.nki)), PDR_ZINV_THREADS, lds, c->stream>>>(
         g, cf, (const cx<T>*)c->u[0], (const cx<T>*)c->u[1], (const cx<T>*)c->u[2], (cx<T>*)c->w1,
         (const cx<T>*)c->tab.twz, nt);
     return hipGetLastError();
@@ -1236,7 +1263,7 @@ int spec_forms(int prec, int nx, int ny, int nz, int out[5]) {
     using T = decltype(tag);
     constexpr int n = decltype(nn)::value;
     out[2] = (PDR_ZINV_REG && n <= 512) ? 1 : 0;
-    out[3] = (PDR_ZINV_REG && n <= 512) ? zinv_lines<T>(n) : tile_lines<T>(n);
+    out[3] = (PDR_ZINV_REG && n <= 512) ? zinv_lines_t<T, PDR_ZINV_THREADS>(n) : tile_lines<T>(n);
     ++ok;
   };
   auto fy = [&](auto tag, auto nn) {
