@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(PDR_ZINV_THREADS) PDR_ZINV_WAVES_T
 // ---------------------------------------------------------------------------
 // pass 2: inverse y FFT, (z, i-tile) per block
 // ---------------------------------------------------------------------------
-template <typename T, int N>
+template <typename T, int N, int YT = kBlock2>
 __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
   constexpr int PP = y_planes<T>(N);
@@ -414,7 +414,7 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKJ = 2 * K + 1;
-  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
+  constexpr int QE = (L * N + YT - 1) / YT;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -426,13 +426,13 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   const int i0 = PP ? 0 : tile * L;
   const int nki = g.nki;
   const int lstride = g.nz * g.ldj;
-  load_tw<T, N>(tw, twg);
+  load_tw<T, N, YT>(tw, twg);
   cx<T> pre[QE];
   auto load = [&](int f) {
     const cx<T>* src = w1 + (long long)f * g.m1 + ((long long)i0 * g.nz + z) * g.ldj;
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
-      const int idx = int(threadIdx.x) + kBlock2 * q;
+      const int idx = int(threadIdx.x) + YT * q;
       const int line = idx / N;
       const int y = idx - line * N;
       const int j = (y <= K) ? y : ((y >= N - K) ? y - (N - NKJ) : -1);
@@ -441,10 +441,10 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
         const int p = line / (K + 1);
         const int i = line - p * (K + 1);
         const int zz = tile * PP + p;
-        if ((L * N % kBlock2 == 0 || idx < L * N) && zz < nzc && j >= 0)
+        if ((L * N % YT == 0 || idx < L * N) && zz < nzc && j >= 0)
           v = get<kNrYinv>(&src[((long long)i * g.nz + zz) * g.ldj + j]);
       } else {
-        if ((L * N % kBlock2 == 0 || idx < L * N) && i0 + line < nki && j >= 0)
+        if ((L * N % YT == 0 || idx < L * N) && i0 + line < nki && j >= 0)
           v = get<kNrYinv>(&src[line * lstride + j]);
       }
       pre[q] = v;
@@ -454,16 +454,16 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
     load(f);
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
-      const int idx = int(threadIdx.x) + kBlock2 * q;
-      if (L * N % kBlock2 == 0 || idx < L * N) {
+      const int idx = int(threadIdx.x) + YT * q;
+      if (L * N % YT == 0 || idx < L * N) {
         const int line = idx / N;
         buf[line * LD + lds_off<T>(idx - line * N)] = pre[q];
       }
     }
     __syncthreads();
-    fft_ip<T, N, L, LD, +1>(buf, tw);
+    fft_ip<T, N, L, LD, +1, true, YT>(buf, tw);
     cx<T>* dst = w2 + (long long)f * g.m2 + (long long)zl * N * g.ldi + i0;
-    for (int idx = threadIdx.x; idx < L * N; idx += kBlock2) {
+    for (int idx = threadIdx.x; idx < L * N; idx += YT) {
       const int y = idx / L;
       const int line = idx - y * L;
       if constexpr (PP > 0) {
@@ -479,11 +479,33 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   }
 }
 
+// The single-box inverse y pass of tiles of i-lines runs 512-thread workgroups
+// over the same tile: half the elements per thread, so the registers of a wave
+// fall below half of what 256 threads need and the CU holds twice the waves
+// (the tile's LDS then bounds it) to keep more loads in flight.  Whole-plane
+// tiles of short lines and the batched ensembles keep 256 threads.
+#ifndef PDR_YINV_THREADS
+#define PDR_YINV_THREADS 512
+#endif
 template <typename T, int N>
-__global__ void __launch_bounds__(kBlock2) PDR_Y_WAVES
+constexpr int yinv_threads() {
+  return y_planes<T>(N) ? kBlock2 : PDR_YINV_THREADS;
+}
+// waves per SIMD declared to the register allocator: as for the other passes
+// (at most two 256-thread workgroups), or the two waves per SIMD of each
+// 512-thread workgroup times the workgroups the LDS admits, at most four
+template <typename T, int N>
+constexpr int yinv_waves() {
+  constexpr int wg = kLdsPerCu / lds_y_bytes<T>(N) < 1 ? 1 : kLdsPerCu / lds_y_bytes<T>(N);
+  return yinv_threads<T, N>() == kBlock2 ? waves_for(lds_y_bytes<T>(N))
+                                          : (wg * (yinv_threads<T, N>() / kBlock2) > 4 ? 4 : wg * (yinv_threads<T, N>() / kBlock2));
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__((yinv_threads<T, N>())) __attribute__((amdgpu_waves_per_eu(yinv_waves<T, N>())))
     k2_yinv(Geo g, const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ twg,
             int ntile, int z0, int nzc, int fpb) {
-  k2_yinv_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, w1, w2, twg, ntile, z0, nzc, fpb);
+  k2_yinv_body<T, N, yinv_threads<T, N>()>(xcd_remap(blockIdx.x, gridDim.x), g, w1, w2, twg, ntile, z0, nzc, fpb);
 }
 
 // ---------------------------------------------------------------------------
@@ -517,42 +539,83 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   // (conj(a) + i conj(b)); c2r uses only the real part of the DC term; the band
   // KI < m < N - KI is zero.  All loads of a thread precede its LDS stores.
   constexpr int NZB = N - 2 * KI - 1;
-  constexpr int EL = 3 * RB * NKI;
-  constexpr int QL = (EL + XT - 1) / XT;
-  cx<T> A[QL], B[QL];
+  if constexpr (NKI >= 128) {
+    // long lines: thread m takes mode m of every line, so the global row bases
+    // are workgroup-uniform, the LDS line offsets compile-time constants and
+    // the per-element index arithmetic is one address add
+    constexpr int NL = 3 * RB;
+    const cx<T>* fa[NL];
 #pragma unroll
-  for (int q = 0; q < QL; ++q) {
-    const int idx = int(threadIdx.x) + XT * q;
-    const int line = idx / NKI;
-    const int m = idx - line * NKI;
-    const int r = line / 3;
-    const int p = line - 3 * r;
-    A[q] = czero<T>();
-    B[q] = czero<T>();
-    if ((EL % XT == 0 || idx < EL) && y0 + r < ny) {
-      const cx<T>* fa = rowp + (long long)(2 * p) * g.m2 + r * g.ldi;
-      A[q] = get<kNrXpass>(&fa[m]);
-      B[q] = get<kNrXpass>(&fa[g.m2 + m]);
-    }
-  }
+    for (int line = 0; line < NL; ++line) fa[line] = rowp + (long long)(2 * (line % 3)) * g.m2 + (line / 3) * g.ldi;
+    for (int m0 = 0; m0 < NKI; m0 += XT) {
+      const int m = m0 + int(threadIdx.x);
+      if (m < NKI) {
+        cx<T> A[NL], B[NL];
 #pragma unroll
-  for (int q = 0; q < QL; ++q) {
-    const int idx = int(threadIdx.x) + XT * q;
-    if (EL % XT == 0 || idx < EL) {
-      const int line = idx / NKI;
-      const int m = idx - line * NKI;
-      cx<T>* row = buf + line * LD;
-      if (m == 0) {
-        row[0] = mk<T>(A[q].x, B[q].x);
-      } else {
-        row[lds_off<T>(m)] = mk<T>(A[q].x - B[q].y, A[q].y + B[q].x);
-        row[lds_off<T>(N - m)] = mk<T>(A[q].x + B[q].y, B[q].x - A[q].y);
+        for (int line = 0; line < NL; ++line) {
+          A[line] = czero<T>();
+          B[line] = czero<T>();
+          if (y0 + line / 3 < ny) {
+            A[line] = get<kNrXpass>(&fa[line][m]);
+            B[line] = get<kNrXpass>(&fa[line][g.m2 + m]);
+          }
+        }
+        const int om = lds_off<T>(m), on = lds_off<T>(N - m);
+#pragma unroll
+        for (int line = 0; line < NL; ++line) {
+          cx<T>* row = buf + line * LD;
+          if (m == 0) {
+            row[0] = mk<T>(A[line].x, B[line].x);
+          } else {
+            row[om] = mk<T>(A[line].x - B[line].y, A[line].y + B[line].x);
+            row[on] = mk<T>(A[line].x + B[line].y, B[line].x - A[line].y);
+          }
+        }
       }
     }
-  }
-  for (int idx = threadIdx.x; idx < 3 * RB * NZB; idx += XT) {
-    const int line = idx / NZB;
-    buf[line * LD + lds_off<T>(KI + 1 + idx - line * NZB)] = czero<T>();
+    for (int mm = threadIdx.x; mm < NZB; mm += XT) {
+      const int o = lds_off<T>(KI + 1 + mm);
+#pragma unroll
+      for (int line = 0; line < NL; ++line) buf[line * LD + o] = czero<T>();
+    }
+  } else {
+    constexpr int EL = 3 * RB * NKI;
+    constexpr int QL = (EL + XT - 1) / XT;
+    cx<T> A[QL], B[QL];
+#pragma unroll
+    for (int q = 0; q < QL; ++q) {
+      const int idx = int(threadIdx.x) + XT * q;
+      const int line = idx / NKI;
+      const int m = idx - line * NKI;
+      const int r = line / 3;
+      const int p = line - 3 * r;
+      A[q] = czero<T>();
+      B[q] = czero<T>();
+      if ((EL % XT == 0 || idx < EL) && y0 + r < ny) {
+        const cx<T>* fa = rowp + (long long)(2 * p) * g.m2 + r * g.ldi;
+        A[q] = get<kNrXpass>(&fa[m]);
+        B[q] = get<kNrXpass>(&fa[g.m2 + m]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < QL; ++q) {
+      const int idx = int(threadIdx.x) + XT * q;
+      if (EL % XT == 0 || idx < EL) {
+        const int line = idx / NKI;
+        const int m = idx - line * NKI;
+        cx<T>* row = buf + line * LD;
+        if (m == 0) {
+          row[0] = mk<T>(A[q].x, B[q].x);
+        } else {
+          row[lds_off<T>(m)] = mk<T>(A[q].x - B[q].y, A[q].y + B[q].x);
+          row[lds_off<T>(N - m)] = mk<T>(A[q].x + B[q].y, B[q].x - A[q].y);
+        }
+      }
+    }
+    for (int idx = threadIdx.x; idx < 3 * RB * NZB; idx += XT) {
+      const int line = idx / NZB;
+      buf[line * LD + lds_off<T>(KI + 1 + idx - line * NZB)] = czero<T>();
+    }
   }
   __syncthreads();
   fft_ip<T, N, 3 * RB, LD, +1, !TWG, XT>(buf, tw);
@@ -616,13 +679,11 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   }
   __syncthreads();
   fft_ip<T, N, 3 * RB / 2, LD, -1, !TWG, XT>(buf, tw);
-  for (int idx = threadIdx.x; idx < RB * NKI; idx += XT) {
-    const int r = idx / NKI;
-    const int m = idx - r * NKI;
-    if (y0 + r >= ny) continue;
-    const int mr = (m == 0) ? 0 : N - m;
-    const cx<T> Z = buf[r * LD + lds_off<T>(m)], Zr = buf[r * LD + lds_off<T>(mr)];
-    const cx<T> W = buf[(RB + r / 2) * LD + lds_off<T>(m)], Wr = buf[(RB + r / 2) * LD + lds_off<T>(mr)];
+  // unpack the pairs: row r of n1 + i n2 and half of line RB + r/2 give the
+  // stored modes m = 0..KI of the three forward fields
+  auto unpack = [&](int r, int m, int om, int omr) {
+    const cx<T> Z = buf[r * LD + om], Zr = buf[r * LD + omr];
+    const cx<T> W = buf[(RB + r / 2) * LD + om], Wr = buf[(RB + r / 2) * LD + omr];
     cx<T>* d0 = rowp + r * g.ldi + m;
     put<kNtXpass>(d0, mk<T>(T(0.5) * (Z.x + Zr.x), T(0.5) * (Z.y - Zr.y)));
     put<kNtXpass>(d0 + g.m2, mk<T>(T(0.5) * (Z.y + Zr.y), T(-0.5) * (Z.x - Zr.x)));
@@ -630,6 +691,25 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
       put<kNtXpass>(d0 + 2 * g.m2, mk<T>(T(0.5) * (W.y + Wr.y), T(-0.5) * (W.x - Wr.x)));
     else
       put<kNtXpass>(d0 + 2 * g.m2, mk<T>(T(0.5) * (W.x + Wr.x), T(0.5) * (W.y - Wr.y)));
+  };
+  if constexpr (NKI >= 128) {
+    for (int m0 = 0; m0 < NKI; m0 += XT) {
+      const int m = m0 + int(threadIdx.x);
+      if (m < NKI) {
+        const int om = lds_off<T>(m), omr = lds_off<T>((m == 0) ? 0 : N - m);
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+          if (y0 + r < ny) unpack(r, m, om, omr);
+      }
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < RB * NKI; idx += XT) {
+      const int r = idx / NKI;
+      const int m = idx - r * NKI;
+      if (y0 + r >= ny) continue;
+      const int mr = (m == 0) ? 0 : N - m;
+      unpack(r, m, lds_off<T>(m), lds_off<T>(mr));
+    }
   }
   if (want_cfl) {
     // maximum over the wave, then over the XT / 64 waves of the workgroup
@@ -802,7 +882,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
 // does the same per i slice).  The modes of a tile, (i, j0..j0+L-1, all k), are
 // contiguous in the wave arrays.
 // ---------------------------------------------------------------------------
-template <typename T, int N>
+template <typename T, int N, int FT = kBlock2>
 __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, const Coef& cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
                   cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
                   const double2* __restrict__ frc, int fnk, double fkf, double beta, double gdt,
@@ -811,8 +891,8 @@ __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, c
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKK = 2 * K + 1;
-  constexpr int QE = (L * N + kBlock2 - 1) / kBlock2;
-  constexpr int QF = (L * NKK + kBlock2 - 1) / kBlock2;  // modes per thread
+  constexpr int QE = (L * N + FT - 1) / FT;
+  constexpr int QF = (L * NKK + FT - 1) / FT;  // modes per thread
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -821,17 +901,17 @@ __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, c
   const int j0 = tile * L;
   const int nkj = g.nkj;
   const int nl_lines = (nkj - j0) < L ? (nkj - j0) : L;
-  load_tw<T, N>(tw, twg);
+  load_tw<T, N, FT>(tw, twg);
   cx<T> pre[QE];
   cx<T> nlv[3][QF];
   auto load = [&](int f) {
     const cx<T>* src = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
-      const int idx = int(threadIdx.x) + kBlock2 * q;
+      const int idx = int(threadIdx.x) + FT * q;
       const int z = idx / L;
       const int line = idx - z * L;
-      pre[q] = ((L * N % kBlock2 == 0 || idx < L * N) && j0 + line < nkj) ? src[z * g.ldj + line] : czero<T>();
+      pre[q] = ((L * N % FT == 0 || idx < L * N) && j0 + line < nkj) ? src[z * g.ldj + line] : czero<T>();
     }
   };
 #pragma unroll
@@ -839,17 +919,17 @@ __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, c
     load(f);
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
-      const int idx = int(threadIdx.x) + kBlock2 * q;
-      if (L * N % kBlock2 == 0 || idx < L * N) {
+      const int idx = int(threadIdx.x) + FT * q;
+      if (L * N % FT == 0 || idx < L * N) {
         const int z = idx / L;
         buf[(idx - z * L) * LD + lds_off<T>(z)] = pre[q];
       }
     }
     __syncthreads();
-    fft_ip<T, N, L, LD, -1>(buf, tw);
+    fft_ip<T, N, L, LD, -1, true, FT>(buf, tw);
 #pragma unroll
     for (int q = 0; q < QF; ++q) {
-      const int idx = int(threadIdx.x) + kBlock2 * q;
+      const int idx = int(threadIdx.x) + FT * q;
       const int line = idx / NKK;
       const int k = idx - line * NKK;
       const int zz = (k <= K) ? k : N - NKK + k;
@@ -860,7 +940,7 @@ __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, c
   const long long e0 = ((long long)i * nkj + j0) * NKK;
 #pragma unroll
   for (int q = 0; q < QF; ++q) {
-    const int idx = int(threadIdx.x) + kBlock2 * q;
+    const int idx = int(threadIdx.x) + FT * q;
     if (idx < nl_lines * NKK) {
       const int line = idx / NKK;
       formf_mode<T>(g, cf, i, j0 + line, idx - line * NKK, e0 + idx, nlv[0][q], nlv[1][q], nlv[2][q], u0, u1, u2,
@@ -935,7 +1015,7 @@ hipError_t yinv_n(pdr_ctx* c, int z0, int nzc) {
   const size_t lds = size_t(lds_y_bytes<T>(N));
   hipError_t e = big_lds((const void*)k2_yinv<T, N>, lds);
   if (e != hipSuccess) return e;
-  k2_yinv<T, N><<<dim3(unsigned(nt * (PP ? 1 : nzc) * (6 / c->fpb_y))), kBlock2, lds, c->stream>>>(
+  k2_yinv<T, N><<<dim3(unsigned(nt * (PP ? 1 : nzc) * (6 / c->fpb_y))), yinv_threads<T, N>(), lds, c->stream>>>(
       g, (const cx<T>*)c->w1, (cx<T>*)c->w2, (const cx<T>*)c->tab.twy, nt, z0, nzc, c->fpb_y);
   return hipGetLastError();
 }
