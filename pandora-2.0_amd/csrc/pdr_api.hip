@@ -116,7 +116,7 @@ static void frame_init(pdr_ctx* c) {
 }
 
 // Operator tensors from the current frame (see Coef).
-Coef pdr::make_coef(const pdr_ctx* c) {
+static Coef make_coef_fp64(const pdr_ctx* c) {
   Coef f;
   std::memset(&f, 0, sizeof(f));
   const double* B = c->bmat;
@@ -198,6 +198,12 @@ Coef pdr::make_coef(const pdr_ctx* c) {
   f.use_nl = (c->p.f_type == PDR_F_ISOTROPIC || c->p.f_type == PDR_F_SHEAR_DNS) ? 1 : 0;
   f.use_visc = (c->p.f_type != PDR_F_SHEAR_RDT_INVISCID) ? 1 : 0;
   f.has_hom = (c->p.f_type != PDR_F_ISOTROPIC) ? 1 : 0;
+  return f;
+}
+
+Coef pdr::make_coef(const pdr_ctx* c) {
+  Coef f = make_coef_fp64(c);
+  for (int q = 0; q < 9; ++q) f.Vf[q] = float(f.V[q]);
   return f;
 }
 
@@ -688,7 +694,11 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   // real-space moments then ride on this stage's x pass (1073-1078)
   const bool sst = stage_stats_due(c, stage, tstep);
   double* rsp = sst ? stage_stats_rs_partials(c) : nullptr;
-  if (sst) HIPCHK(stage_stats_wave(c, tstep));
+  if (sst) {
+    prof_begin(c, K_STATS);
+    HIPCHK(stage_stats_wave(c, tstep));
+    prof_end(c, K_STATS);
+  }
   // f_RK3Stage1: p_StatsPrimary after the remesh, p_StatsRelativeVelocity after
   // the particle advance (src/pf_rk3.f90:62-99), for tstep > P_TS_RELEASE
   if (stage == 1 && c->np > 1 && c->sort_every > 0 && ++c->since_sort >= c->sort_every) {
@@ -1172,8 +1182,8 @@ int pdr_profile_get(pdr_ctx* c, int32_t n, char* names, int32_t namelen, int64_t
     c->ev_pool.push_back(e.second.second);
   }
   c->ev_pending.clear();
-  static const char* kNames[K_NCLASS] = {"zinv",    "yinv",    "xpass",  "yfwd", "zfwd", "formf",
-                                         "zfwdformf", "tracers", "remesh", "misc"};
+  static const char* kNames[K_NCLASS] = {"zinv",      "yinv",    "xpass",  "yfwd", "zfwd", "formf",
+                                         "zfwdformf", "tracers", "remesh", "misc", "stats"};
   std::string all;
   for (int k = 0; k < K_NCLASS && k < n; ++k) {
     if (counts) counts[k] = c->prof_cnt[k];

@@ -40,6 +40,7 @@ struct Geo {
 struct Coef {
   double G[27];  // [c][n][m]
   double V[9];   // [r][m] masked binv
+  float Vf[9];   // V rounded to FP32 (the FP32-mixed x pass)
   double Bm[9];  // [i][c] masked bmat
   double M[9];   // [l][n] sum_j bmat(l,j) bmat(n,j)
   double P[9];   // [m][c]
@@ -95,6 +96,7 @@ enum KernelClass {
   K_TRACER,
   K_REMESH,
   K_MISC,
+  K_STATS,  // statistics of stage 1 inside the step (wave-space pass)
   K_NCLASS
 };
 

@@ -619,13 +619,22 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   }
   __syncthreads();
   fft_ip<T, N, 3 * RB, LD, +1, !TWG, XT>(buf, tw);
-  const T V[9] = {T(cf.V[0]), T(cf.V[1]), T(cf.V[2]), T(cf.V[3]), T(cf.V[4]),
-                  T(cf.V[5]), T(cf.V[6]), T(cf.V[7]), T(cf.V[8])};
+  T V[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    if constexpr (sizeof(T) == 4)
+      V[q] = cf.Vf[q];
+    else
+      V[q] = T(cf.V[q]);
+  }
   T nl[IT][3];
   T lmax = T(0);
   T* urow = ur + ((long long)z * ny + y0) * N * 3;
   RsAcc acc;
   if (RS) rs_init(acc);
+  // the product, with the frame's form a compile-time choice (no per-point select)
+  auto product = [&](auto form_c) {
+    constexpr bool FORM = decltype(form_c)::value;
 #pragma unroll
   for (int q = 0; q < IT; ++q) {
     const int idx = int(threadIdx.x) + XT * q;
@@ -639,7 +648,7 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
       const T v2 = V[3] * u1 + V[4] * u2 + V[5] * u3;
       const T v3 = V[6] * u1 + V[7] * u2 + V[8] * u3;
       // omega x (binv u) (BF_ROT), or omega x u (isotropic with binv = I, R_ROT)
-      const T w1 = cf.form ? u1 : v1, w2 = cf.form ? u2 : v2, w3 = cf.form ? u3 : v3;
+      const T w1 = FORM ? u1 : v1, w2 = FORM ? u2 : v2, w3 = FORM ? u3 : v3;
       nl[q][0] = o2 * w3 - o3 * w2;
       nl[q][1] = o3 * w1 - o1 * w3;
       nl[q][2] = o1 * w2 - o2 * w1;
@@ -656,6 +665,11 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
       if (RS && y0 + r < ny) rs_add(acc, double(v1), double(v2), double(v3), double(o1), double(o2), double(o3));
     }
   }
+  };
+  if (cf.form)
+    product(std::true_type{});
+  else
+    product(std::false_type{});
   if (RS) {
     // partial of this workgroup at its place in the whole box: (z, tile)
     __shared__ double rs_lds[kRsWidth * (XT / 64)];

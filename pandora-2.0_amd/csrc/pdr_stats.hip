@@ -81,12 +81,36 @@ __global__ void __launch_bounds__(kStatThreads)
   for (int q = threadIdx.x; q < nh; q += kStatThreads) sh[q] = 0.0;
   __syncthreads();
   double uu[6] = {0, 0, 0, 0, 0, 0}, uc[6] = {0, 0, 0, 0, 0, 0};
-  const long long stride = (long long)gridDim.x * kStatThreads;
-  for (long long e = (long long)blockIdx.x * kStatThreads + threadIdx.x; e < g.mw; e += stride) {
+  // Each workgroup takes a contiguous range of modes (the lanes of a wave still
+  // read consecutive modes), so a thread's modes stay in one i plane for long
+  // stretches: the per-i bins (streamwise spectra and the correlation sums) are
+  // summed in registers (Neumaier) and added to the LDS bins only when the plane
+  // changes, instead of one contended LDS atomic per mode.
+  const long long per = (g.mw + gridDim.x - 1) / gridDim.x;
+  const long long e_beg = (long long)blockIdx.x * per;
+  const long long e_end = e_beg + per < g.mw ? e_beg + per : g.mw;
+  int icur = -1;
+  double pi_s[4] = {0, 0, 0, 0}, pi_c[4] = {0, 0, 0, 0};
+  auto flush_i = [&]() {
+    if (icur >= 0) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a) two_sum_add(&s1[2 * a * sf.nki + icur], &s1[(2 * a + 1) * sf.nki + icur], pi_s[a] + pi_c[a]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      pi_s[a] = 0.0;
+      pi_c[a] = 0.0;
+    }
+  };
+  for (long long e = e_beg + threadIdx.x; e < e_end; e += kStatThreads) {
     const int k = int(e % g.nkk);
     const long long ij = e / g.nkk;
     const int j = int(ij % g.nkj);
     const int i = int(ij / g.nkj);
+    if (i != icur) {
+      flush_i();
+      icur = i;
+    }
     const double kv[3] = {double(i), wvn_s(j, g.kj), wvn_s(k, g.kk)};
     const cx<T> a = u0[e], b = u1[e], c = u2[e];
     const double um[3][2] = {{double(a.x), double(a.y)}, {double(b.x), double(b.y)}, {double(c.x), double(c.y)}};
@@ -100,15 +124,21 @@ __global__ void __launch_bounds__(kStatThreads)
         ul[m][1] = ul[m][1] + sf.bi[3 * m + n] * um[n][1];
       }
     const double km = sqrt((kl[0] * kl[0]) + (kl[1] * kl[1]) + (kl[2] * kl[2]));
-    // streamwise spectra: sum over (j, k) of |binv u|^2 per i (all stored modes;
-    // f_FluidstatsFs2D maps with binv in every sheared form)
-    two_sum_add(&s1[i], &s1[sf.nki + i], ul[0][0] * ul[0][0] + ul[0][1] * ul[0][1]);
-    two_sum_add(&s1[2 * sf.nki + i], &s1[3 * sf.nki + i], ul[1][0] * ul[1][0] + ul[1][1] * ul[1][1]);
-    // two-point correlation of the laboratory u1 with the stored u1 along x:
-    // sum over (j, k) of conj(u1_lab) u1 per i (Wiener-Khinchin form of the
-    // real-space sum of f_FluidstatsRs2D, src/pf_fluidstats.f90:933-945)
-    two_sum_add(&s1[4 * sf.nki + i], &s1[5 * sf.nki + i], ul[0][0] * um[0][0] + ul[0][1] * um[0][1]);
-    two_sum_add(&s1[6 * sf.nki + i], &s1[7 * sf.nki + i], ul[0][0] * um[0][1] - ul[0][1] * um[0][0]);
+    // per i, over all stored modes: |u1_lab|^2 and |u2_lab|^2 (streamwise spectra;
+    // f_FluidstatsFs2D maps with binv in every sheared form) and the two-point
+    // correlation of the laboratory u1 with the stored u1 along x, conj(u1_lab) u1
+    // (Wiener-Khinchin form of the real-space sum of f_FluidstatsRs2D,
+    // src/pf_fluidstats.f90:933-945)
+    {
+      const double pv[4] = {ul[0][0] * ul[0][0] + ul[0][1] * ul[0][1], ul[1][0] * ul[1][0] + ul[1][1] * ul[1][1],
+                            ul[0][0] * um[0][0] + ul[0][1] * um[0][1], ul[0][0] * um[0][1] - ul[0][1] * um[0][0]};
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const double t = pi_s[a] + pv[a];
+        pi_c[a] += (fabs(pi_s[a]) >= fabs(pv[a])) ? (pi_s[a] - t) + pv[a] : (pv[a] - t) + pi_s[a];
+        pi_s[a] = t;
+      }
+    }
     const double(*v)[2] = sf.lab_state ? um : ul;
     double ph[6];
     ph[0] = v[0][0] * v[0][0] + v[0][1] * v[0][1];
@@ -145,6 +175,7 @@ __global__ void __launch_bounds__(kStatThreads)
       atomicAdd(&sh[12 * kkmax + o], (i != 0) ? 2.0 : 1.0);
     }
   }
+  flush_i();
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     red[q * kStatThreads + threadIdx.x] = uu[q];
