@@ -896,6 +896,9 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
 // does the same per i slice).  The modes of a tile, (i, j0..j0+L-1, all k), are
 // contiguous in the wave arrays.
 // ---------------------------------------------------------------------------
+#ifndef PDR_ZFF_PF
+#define PDR_ZFF_PF 1
+#endif
 template <typename T, int N, int FT = kBlock2>
 __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, const Coef& cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
                   cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
@@ -928,9 +931,14 @@ __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, c
       pre[q] = ((L * N % FT == 0 || idx < L * N) && j0 + line < nkj) ? src[z * g.ldj + line] : czero<T>();
     }
   };
+  // FP32 lines of 256..512 points: the next field's loads are issued before
+  // this field's transform, whose work then hides their latency (the registers
+  // of one field's elements fit beside the transform's)
+  constexpr bool PF = PDR_ZFF_PF && sizeof(T) == 4 && N >= 256 && N <= 512;
+  if constexpr (PF) load(0);
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    load(f);
+    if constexpr (!PF) load(f);
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
       const int idx = int(threadIdx.x) + FT * q;
@@ -940,6 +948,9 @@ __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, c
       }
     }
     __syncthreads();
+    if constexpr (PF) {
+      if (f < 2) load(f + 1);
+    }
     fft_ip<T, N, L, LD, -1, true, FT>(buf, tw);
 #pragma unroll
     for (int q = 0; q < QF; ++q) {
