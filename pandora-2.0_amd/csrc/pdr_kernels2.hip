@@ -630,7 +630,7 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   T nl[IT][3];
   T lmax = T(0);
   T* urow = ur + ((long long)z * ny + y0) * N * 3;
-  RsAcc acc;
+  RsAccT<T> acc;
   if (RS) rs_init(acc);
   // the product, with the frame's form a compile-time choice (no per-point select)
   auto product = [&](auto form_c) {
@@ -662,7 +662,7 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
         const T s = fabs(v1) + fabs(v2) + fabs(v3);  // lab-frame velocity (src/pf_fluidstats.f90:853-866, 914)
         lmax = s > lmax ? s : lmax;
       }
-      if (RS && y0 + r < ny) rs_add(acc, double(v1), double(v2), double(v3), double(o1), double(o2), double(o3));
+      if (RS && y0 + r < ny) rs_add(acc, v1, v2, v3, o1, o2, o3);
     }
   }
   };

@@ -16,39 +16,48 @@ namespace pdr {
 constexpr int kRsSums = 21;
 constexpr int kRsWidth = 25;
 
-struct RsAcc {
-  double s[kRsSums];
-  double m[4];
+// A thread's partials are of type A: double, or float in the FP32-mixed x pass,
+// where a thread sums the few points of its own (4 at 512 points per line, from
+// FP32 velocities) and every sum across threads and workgroups is FP64.  The
+// float partials halve the accumulator's registers (25 instead of 50): the
+// FP32 x pass with the moments then needs 92 VGPRs at 512 points instead of
+// 130, five waves per SIMD (as many as its LDS admits) instead of three.
+template <typename A = double>
+struct RsAccT {
+  A s[kRsSums];
+  A m[4];
 };
+using RsAcc = RsAccT<double>;
 
-__device__ __forceinline__ void rs_init(RsAcc& a) {
+template <typename A>
+__device__ __forceinline__ void rs_init(RsAccT<A>& a) {
 #pragma unroll
-  for (int q = 0; q < kRsSums; ++q) a.s[q] = 0.0;
+  for (int q = 0; q < kRsSums; ++q) a.s[q] = A(0);
 #pragma unroll
-  for (int q = 0; q < 4; ++q) a.m[q] = 0.0;
+  for (int q = 0; q < 4; ++q) a.m[q] = A(0);
 }
 
-__device__ __forceinline__ void rs_add(RsAcc& a, double v1, double v2, double v3, double w1, double w2,
-                                       double w3) {
+template <typename A>
+__device__ __forceinline__ void rs_add(RsAccT<A>& a, A v1, A v2, A v3, A w1, A w2, A w3) {
   a.s[0] += v1 * v1;
   a.s[1] += v2 * v2;
   a.s[2] += v3 * v3;
   a.s[3] += v1 * v2;
   a.s[4] += v1 * v3;
   a.s[5] += v2 * v3;
-  const double v[3] = {v1, v2, v3}, w[3] = {w1, w2, w3};
+  const A v[3] = {v1, v2, v3}, w[3] = {w1, w2, w3};
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const double v2c = v[c] * v[c], w2c = w[c] * w[c];
+    const A v2c = v[c] * v[c], w2c = w[c] * w[c];
     a.s[6 + c] += v2c * v[c];
     a.s[9 + c] += v2c * v2c;
     a.s[12 + c] += w2c;
     a.s[15 + c] += w2c * w[c];
     a.s[18 + c] += w2c * w2c;
-    const double aw = fabs(w[c]);
+    const A aw = fabs(w[c]);
     a.m[1 + c] = aw > a.m[1 + c] ? aw : a.m[1 + c];
   }
-  const double u = fabs(v1) + fabs(v2) + fabs(v3);
+  const A u = fabs(v1) + fabs(v2) + fabs(v3);
   a.m[0] = u > a.m[0] ? u : a.m[0];
 }
 
@@ -60,20 +69,21 @@ __device__ __forceinline__ void rs_add(RsAcc& a, double v1, double v2, double v3
 // keeps the upper half), so lane l ends with slot l >> 1 summed over 32 lanes and
 // one more exchange (offset 1) completes it: 32 shuffles instead of 21 x 6.
 // The 4 maxima take a plain butterfly.
-__device__ __forceinline__ void rs_block_out(RsAcc& a, double* lds, double* __restrict__ out) {
+template <typename A>
+__device__ __forceinline__ void rs_block_out(RsAccT<A>& a, double* lds, double* __restrict__ out) {
   const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
   const int nw = int(blockDim.x >> 6);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const double o = __shfl_xor(a.m[q], off);
+      const A o = __shfl_xor(a.m[q], off);
       a.m[q] = o > a.m[q] ? o : a.m[q];
     }
   }
   double v[32];
 #pragma unroll
-  for (int q = 0; q < 32; ++q) v[q] = q < kRsSums ? a.s[q] : 0.0;
+  for (int q = 0; q < 32; ++q) v[q] = q < kRsSums ? double(a.s[q]) : 0.0;
 #pragma unroll
   for (int r = 0; r < 5; ++r) {
     const int off = 32 >> r, len = 16 >> r;
@@ -90,7 +100,7 @@ __device__ __forceinline__ void rs_block_out(RsAcc& a, double* lds, double* __re
   if (!(lane & 1) && (lane >> 1) < kRsSums) lds[wave * kRsWidth + (lane >> 1)] = sum;
   if (lane == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) lds[wave * kRsWidth + kRsSums + q] = a.m[q];
+    for (int q = 0; q < 4; ++q) lds[wave * kRsWidth + kRsSums + q] = double(a.m[q]);
   }
   __syncthreads();
   if (int(threadIdx.x) < kRsWidth) {

@@ -9,11 +9,13 @@
 // existing entries; the identity for isotropic runs), binned by |k_lab| into
 // [kk - 1/2, kk + 1/2), and its energy, component energies, dissipation and
 // dissipation-skewness terms and its mode count (1 on i = 0, 2 elsewhere) are
-// added to a per-workgroup LDS histogram.  The LDS adds are FP64 atomics that
-// return the old value, so each thread recovers the exact rounding error of its
-// add (TwoSum) and adds it to a second histogram: the per-workgroup sums are
-// compensated.  Workgroup partials are then reduced in workgroup order with
-// Neumaier summation, and the host finishes the kkmax-sized arrays in FP64.
+// added to a per-workgroup LDS histogram.  A thread sums the terms of
+// consecutive modes of one shell in registers (Neumaier) and adds each such run
+// to the histogram.  The LDS adds are FP64 atomics that return the old value,
+// so each thread recovers the exact rounding error of its add (TwoSum) and adds
+// it to a second histogram: the per-workgroup sums are compensated.  Workgroup
+// partials are then reduced in workgroup order with Neumaier summation, and the
+// host finishes the kkmax-sized arrays in FP64.
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -33,6 +35,10 @@ namespace {
 
 constexpr int kStatThreads = 256;
 constexpr int kStatBlocks = 1024;
+#ifndef PDR_STATS_RUN
+#define PDR_STATS_RUN 4
+#endif
+constexpr int kStatRun = PDR_STATS_RUN;
 
 struct StatsFrame {
   double bm[9];  // masked bmat, [m][n]
@@ -62,8 +68,8 @@ __device__ __forceinline__ void two_sum_add(double* hi, double* lo, double v) {
 }
 
 // LDS: hist[13][kkmax] = hi sums of 6 quantities, their lo parts, mode counts;
-// the streamwise bins s1 (hi, lo), s2 (hi, lo) of nki each; then 12 doubles x
-// kStatThreads for the Reynolds-stress tree reduction.
+// the streamwise bins s1 (hi, lo), s2 (hi, lo), cre (hi, lo), cim (hi, lo) of
+// nki each; then 12 doubles per wave for the Reynolds-stress sums.
 // Partial of workgroup b: part[b][stats_hist + 12].
 template <typename T>
 __global__ void __launch_bounds__(kStatThreads)
@@ -102,17 +108,31 @@ __global__ void __launch_bounds__(kStatThreads)
       pi_c[a] = 0.0;
     }
   };
-  for (long long e = e_beg + threadIdx.x; e < e_end; e += kStatThreads) {
-    const int k = int(e % g.nkk);
-    const long long ij = e / g.nkk;
-    const int j = int(ij % g.nkj);
-    const int i = int(ij / g.nkj);
+  // Shell bins: a thread takes runs of kStatRun consecutive modes (consecutive
+  // k3, so |k| moves slowly and neighbouring modes mostly share a shell); it sums
+  // a shell's terms in registers (Neumaier) while the shell stays the same and
+  // adds them to the LDS bins when it changes, instead of seven contended LDS
+  // atomics per mode.
+  int kcur = 0;
+  double bs[7], bc[7];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) bs[q] = bc[q] = 0.0;
+  auto flush_k = [&]() {
+    if (kcur) {
+      const int o = kcur - 1;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) two_sum_add(&sh[q * kkmax + o], &sh[(6 + q) * kkmax + o], bs[q] + bc[q]);
+      atomicAdd(&sh[12 * kkmax + o], bs[6]);  // mode count: integers, exact
+    }
+#pragma unroll
+    for (int q = 0; q < 7; ++q) bs[q] = bc[q] = 0.0;
+  };
+  auto mode = [&](int i, int j, int k, const cx<T>& a, const cx<T>& b, const cx<T>& c) {
     if (i != icur) {
       flush_i();
       icur = i;
     }
     const double kv[3] = {double(i), wvn_s(j, g.kj), wvn_s(k, g.kk)};
-    const cx<T> a = u0[e], b = u1[e], c = u2[e];
     const double um[3][2] = {{double(a.x), double(a.y)}, {double(b.x), double(b.y)}, {double(c.x), double(c.y)}};
     double kl[3] = {0, 0, 0}, ul[3][2] = {{0, 0}, {0, 0}, {0, 0}};
 #pragma unroll
@@ -164,34 +184,76 @@ __global__ void __launch_bounds__(kStatThreads)
     const int kc = int(floor(km + 0.5));
     for (int kb = kc - 1; kb <= kc + 1; ++kb)
       if (kb >= 1 && kb <= kkmax && km >= double(kb) - 0.5 && km < double(kb) + 0.5) kk = kb;
+    if (kk != kcur) {
+      flush_k();
+      kcur = kk;
+    }
     if (kk) {
-      const int o = kk - 1;
-      two_sum_add(&sh[o], &sh[6 * kkmax + o], ek);
-      two_sum_add(&sh[kkmax + o], &sh[7 * kkmax + o], 0.5 * ph[0]);
-      two_sum_add(&sh[2 * kkmax + o], &sh[8 * kkmax + o], 0.5 * ph[1]);
-      two_sum_add(&sh[3 * kkmax + o], &sh[9 * kkmax + o], 0.5 * ph[2]);
-      two_sum_add(&sh[4 * kkmax + o], &sh[10 * kkmax + o], dk);
-      two_sum_add(&sh[5 * kkmax + o], &sh[11 * kkmax + o], dsk);
-      atomicAdd(&sh[12 * kkmax + o], (i != 0) ? 2.0 : 1.0);
+      const double tv[6] = {ek, 0.5 * ph[0], 0.5 * ph[1], 0.5 * ph[2], dk, dsk};
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        const double t = bs[q] + tv[q];
+        bc[q] += (fabs(bs[q]) >= fabs(tv[q])) ? (bs[q] - t) + tv[q] : (tv[q] - t) + bs[q];
+        bs[q] = t;
+      }
+      bs[6] += (i != 0) ? 2.0 : 1.0;
+    }
+  };
+  for (long long eb = e_beg + (long long)threadIdx.x * kStatRun; eb < e_end; eb += (long long)kStatThreads * kStatRun) {
+    // the run's loads first, all in flight together; (i, j, k) from one
+    // division, then stepped
+    cx<T> ra[kStatRun], rb[kStatRun], rc[kStatRun];
+#pragma unroll
+    for (int r = 0; r < kStatRun; ++r) {
+      const bool ok = eb + r < e_end;
+      ra[r] = ok ? u0[eb + r] : cx<T>{T(0), T(0)};
+      rb[r] = ok ? u1[eb + r] : cx<T>{T(0), T(0)};
+      rc[r] = ok ? u2[eb + r] : cx<T>{T(0), T(0)};
+    }
+    const long long ij = eb / g.nkk;
+    int k = int(eb - ij * g.nkk);
+    int j = int(ij % g.nkj);
+    int i = int(ij / g.nkj);
+#pragma unroll
+    for (int r = 0; r < kStatRun; ++r) {
+      if (eb + r < e_end) mode(i, j, k, ra[r], rb[r], rc[r]);
+      if (++k == g.nkk) {
+        k = 0;
+        if (++j == g.nkj) {
+          j = 0;
+          ++i;
+        }
+      }
     }
   }
   flush_i();
+  flush_k();
+  // the 12 Reynolds-stress sums: over each wave (butterfly, every lane ends with
+  // the same sums), then over the waves in wave order
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    red[q * kStatThreads + threadIdx.x] = uu[q];
-    red[(6 + q) * kStatThreads + threadIdx.x] = uc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      uu[q] += __shfl_xor(uu[q], off);
+      uc[q] += __shfl_xor(uc[q], off);
+    }
+  }
+  const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      red[q * (kStatThreads / 64) + wave] = uu[q];
+      red[(6 + q) * (kStatThreads / 64) + wave] = uc[q];
+    }
   }
   __syncthreads();
-  for (int s = kStatThreads / 2; s > 0; s >>= 1) {
-    if (int(threadIdx.x) < s) {
-#pragma unroll
-      for (int q = 0; q < 12; ++q) red[q * kStatThreads + threadIdx.x] += red[q * kStatThreads + threadIdx.x + s];
-    }
-    __syncthreads();
-  }
   double* out = part + (size_t)blockIdx.x * size_t(nh + 12);
   for (int q = threadIdx.x; q < nh; q += kStatThreads) out[q] = sh[q];
-  if (threadIdx.x < 12) out[nh + threadIdx.x] = red[threadIdx.x * kStatThreads];
+  if (threadIdx.x < 12) {
+    double r = red[threadIdx.x * (kStatThreads / 64)];
+    for (int w = 1; w < kStatThreads / 64; ++w) r += red[threadIdx.x * (kStatThreads / 64) + w];
+    out[nh + threadIdx.x] = r;
+  }
 }
 
 // out[q] = Neumaier sum over workgroups b = 0, 1, ... of part[b][q]
@@ -200,6 +262,8 @@ __global__ void __launch_bounds__(kStatThreads)
   const int q = int(blockIdx.x * kStatThreads + threadIdx.x);
   if (q >= width) return;
   double s = 0.0, c = 0.0;
+  // unrolled so that eight loads are in flight ahead of the dependent sums
+#pragma unroll 8
   for (int b = 0; b < nblk; ++b) {
     const double v = part[(size_t)b * width + q];
     const double t = s + v;
@@ -212,7 +276,7 @@ __global__ void __launch_bounds__(kStatThreads)
 template <typename T>
 hipError_t stats_t(pdr_ctx* c, const StatsFrame& sf, int nblk, double* part, double* out) {
   const int width = stats_hist(sf.kkmax, sf.nki) + 12;
-  const size_t lds = size_t(width - 12 + 12 * kStatThreads) * sizeof(double);
+  const size_t lds = size_t(width - 12 + 12 * (kStatThreads / 64)) * sizeof(double);
   if (lds > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)k_stats<T>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     if (e != hipSuccess) return e;
@@ -252,7 +316,7 @@ int stats_nblk(const pdr_ctx* c) {
 int stats_width(const pdr_ctx* c) { return stats_hist(kkmax_of(c->g), c->g.nki) + 12; }
 
 bool stats_fits_lds(const pdr_ctx* c) {
-  const size_t lds = size_t(stats_hist(kkmax_of(c->g), c->g.nki) + 12 * kStatThreads) * sizeof(double);
+  const size_t lds = size_t(stats_hist(kkmax_of(c->g), c->g.nki) + 12 * (kStatThreads / 64)) * sizeof(double);
   return lds <= c->lds_max;
 }
 
@@ -345,6 +409,64 @@ __global__ void __launch_bounds__(kStatThreads)
     __syncthreads();
   }
   if (threadIdx.x == 0) out[q] = sh[0];
+}
+
+// First level for many partials: workgroup w folds rows [w per, (w + 1) per)
+// into one row.  A thread takes whole rows (25 consecutive doubles, so a wave's
+// loads are contiguous), Neumaier per column, then a fixed-order tree over the
+// workgroup; the maxima by maximum.  k_rs_reduce then reduces the folded rows.
+constexpr int kRsFold = 512;
+__global__ void __launch_bounds__(kStatThreads)
+    k_rs_fold(const double* __restrict__ part, long long nblk, long long per, double* __restrict__ out) {
+  __shared__ double sh[kRsWidth * kStatThreads];
+  const long long b0 = (long long)blockIdx.x * per;
+  const long long b1 = b0 + per < nblk ? b0 + per : nblk;
+  double s[kRsWidth], c[kRsSums];
+#pragma unroll
+  for (int q = 0; q < kRsWidth; ++q) s[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < kRsSums; ++q) c[q] = 0.0;
+  for (long long b = b0 + threadIdx.x; b < b1; b += kStatThreads) {
+    const double* row = part + b * kRsWidth;
+#pragma unroll
+    for (int q = 0; q < kRsWidth; ++q) {
+      const double v = row[q];
+      if (q >= kRsSums) {
+        s[q] = v > s[q] ? v : s[q];
+      } else {
+        const double t = s[q] + v;
+        c[q] += (fabs(s[q]) >= fabs(v)) ? (s[q] - t) + v : (v - t) + s[q];
+        s[q] = t;
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kRsWidth; ++q) sh[q * kStatThreads + threadIdx.x] = q < kRsSums ? s[q] + c[q] : s[q];
+  __syncthreads();
+  for (int w = kStatThreads / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+#pragma unroll
+      for (int q = 0; q < kRsWidth; ++q) {
+        const double a = sh[q * kStatThreads + threadIdx.x], b = sh[q * kStatThreads + threadIdx.x + w];
+        sh[q * kStatThreads + threadIdx.x] = q >= kRsSums ? (b > a ? b : a) : a + b;
+      }
+    }
+    __syncthreads();
+  }
+  if (int(threadIdx.x) < kRsWidth) out[(long long)blockIdx.x * kRsWidth + threadIdx.x] = sh[threadIdx.x * kStatThreads];
+}
+
+// Real-space moments: the nblk partials (fold: room for kRsFold rows) -> out.
+hipError_t rs_reduce(const double* part, long long nblk, double* fold, double* out, hipStream_t st) {
+  if (nblk > 4 * kRsFold) {
+    const long long per = (nblk + kRsFold - 1) / kRsFold;
+    const long long nf = (nblk + per - 1) / per;
+    k_rs_fold<<<dim3(unsigned(nf)), kStatThreads, 0, st>>>(part, nblk, per, fold);
+    part = fold;
+    nblk = nf;
+  }
+  k_rs_reduce<<<dim3(unsigned(kRsWidth)), kStatThreads, 0, st>>>(part, nblk, out);
+  return hipGetLastError();
 }
 
 // float32 pi widened to double (ONEPI, src/pg_constants.f90:126)
@@ -609,7 +731,7 @@ int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out) {
   pdr_ctx* c = ctx;
   const pdr::Geo& g = c->g;
   const long long nblk = pdr::rs_nblk(c);
-  const size_t need = size_t(nblk + 1) * pdr::kRsWidth * sizeof(double);
+  const size_t need = size_t(nblk + 1 + pdr::kRsFold) * pdr::kRsWidth * sizeof(double);
   if (c->rs_bytes < need) {
     if (c->rs_buf) {
       (void)hipStreamSynchronize(c->stream);
@@ -641,10 +763,7 @@ int pdr_rs_stats(pdr_ctx* ctx, pdr_rs_stats_t* out) {
       if (e == hipSuccess) e = pdr::launch_xpass(c, cf, false, false, z0, nzc, part);
     }
   }
-  if (e == hipSuccess) {
-    pdr::k_rs_reduce<<<dim3(unsigned(pdr::kRsWidth)), pdr::kStatThreads, 0, c->stream>>>(part, nblk, dres);
-    e = hipGetLastError();
-  }
+  if (e == hipSuccess) e = pdr::rs_reduce(part, nblk, dres + pdr::kRsWidth, dres, c->stream);
   double raw[pdr::kRsWidth];
   if (e == hipSuccess) e = hipMemcpyAsync(raw, dres, sizeof(raw), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -689,13 +808,14 @@ int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6
 // own x pass.  Both reduce into a record buffer the context owns; nothing is
 // allocated or copied to the host during the step.
 // Record buffer: [wave partials nblk x width][wave row width][real-space
-// partials rs_nblk x kRsWidth][real-space row kRsWidth].
+// partials rs_nblk x kRsWidth][real-space row kRsWidth][kRsFold x kRsWidth of
+// folded real-space partials].
 // ---------------------------------------------------------------------------
 namespace pdr {
 
 static size_t sst_need(const pdr_ctx* c) {
   return (size_t(stats_nblk(c)) + 1) * size_t(stats_width(c)) * sizeof(double) +
-         (size_t(rs_nblk(c)) + 1) * size_t(kRsWidth) * sizeof(double);
+         (size_t(rs_nblk(c)) + 1 + kRsFold) * size_t(kRsWidth) * sizeof(double);
 }
 
 static double* sst_wave_part(pdr_ctx* c) { return (double*)c->sst_buf; }
@@ -716,9 +836,7 @@ hipError_t stage_stats_wave(pdr_ctx* c, int tstep) {
 }
 
 hipError_t stage_stats_rs_reduce(pdr_ctx* c) {
-  k_rs_reduce<<<dim3(unsigned(kRsWidth)), kStatThreads, 0, c->stream>>>(stage_stats_rs_partials(c), rs_nblk(c),
-                                                                          sst_rs_row(c));
-  return hipGetLastError();
+  return rs_reduce(stage_stats_rs_partials(c), rs_nblk(c), sst_rs_row(c) + kRsWidth, sst_rs_row(c), c->stream);
 }
 
 void stage_stats_free(pdr_ctx* c) {
