@@ -670,10 +670,19 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
     product(std::true_type{});
   else
     product(std::false_type{});
-  if (RS) {
-    // partial of this workgroup at its place in the whole box: (z, tile)
-    __shared__ double rs_lds[kRsWidth * (XT / 64)];
-    rs_block_out(acc, rs_lds, rsp + ((long long)z * ntile + tile) * kRsWidth);
+  if constexpr (RS) {
+    // partial of this workgroup at its place in the whole box: (z, tile); where
+    // the x tile can hold the threads' partials (the product has read it), they
+    // are reduced through it
+    double* const rso = rsp + ((long long)z * ntile + tile) * kRsWidth;
+    __shared__ double rs_lds[kRsWidth * 4];
+    if constexpr (kRsWidth * (XT + 1) * sizeof(T) <= 3 * RB * LD * sizeof(cx<T>)) {
+      __syncthreads();
+      rs_block_out_lds<XT>(acc, reinterpret_cast<T*>(buf), rs_lds, rso);
+    } else {
+      static_assert(kRsWidth * (XT / 64) <= kRsWidth * 4, "rs_block_out's LDS");
+      rs_block_out(acc, rs_lds, rso);
+    }
   }
   __syncthreads();
   // forward lines: r -> n1 + i n2 of row r; RB + r/2 -> n3 of rows (r, r+1)

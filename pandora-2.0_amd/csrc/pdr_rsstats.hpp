@@ -114,4 +114,46 @@ __device__ __forceinline__ void rs_block_out(RsAccT<A>& a, double* lds, double* 
   }
 }
 
+// Workgroup reduction through LDS, for a caller whose LDS tile is free at this
+// point (every thread has finished its reads of it): each thread stores its 25
+// partials as A (slot-major rows of NT + 1 values, so that the lanes reading
+// different slot rows fall on different banks), then 4 threads per slot each sum
+// every fourth value of the row in FP64 (maxima by maximum) and one thread per
+// slot combines the 4 in order.  About a third of the instructions of the
+// shuffle reduction above.  scratch holds kRsWidth * (NT + 1) values of A,
+// done kRsWidth * 4 doubles.  Ends past a barrier, with the tile free again.
+template <int NT, typename A>
+__device__ __forceinline__ void rs_block_out_lds(const RsAccT<A>& a, A* scratch, double* done,
+                                                 double* __restrict__ out) {
+  static_assert(NT % 4 == 0 && kRsWidth * 4 <= NT, "one group of 4 threads per slot");
+  constexpr int RL = NT + 1;
+  const int t = int(threadIdx.x);
+#pragma unroll
+  for (int q = 0; q < kRsSums; ++q) scratch[q * RL + t] = a.s[q];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) scratch[(kRsSums + q) * RL + t] = a.m[q];
+  __syncthreads();
+  if (t < kRsWidth * 4) {
+    const int q = t >> 2, p = t & 3;
+    const A* row = scratch + q * RL + p;
+    double r = 0.0;
+    if (q < kRsSums) {
+#pragma unroll 8
+      for (int k = 0; k < NT / 4; ++k) r += double(row[4 * k]);
+    } else {
+#pragma unroll 8
+      for (int k = 0; k < NT / 4; ++k) r = double(row[4 * k]) > r ? double(row[4 * k]) : r;
+    }
+    done[t] = r;
+  }
+  __syncthreads();
+  if (t < kRsWidth) {
+    double r = done[4 * t];
+#pragma unroll
+    for (int p = 1; p < 4; ++p) r = (t < kRsSums) ? r + done[4 * t + p] : (done[4 * t + p] > r ? done[4 * t + p] : r);
+    out[t] = r;
+  }
+  __syncthreads();
+}
+
 }  // namespace pdr
