@@ -49,6 +49,7 @@ extern "C" {
 #define PDR_F_SHEAR_DNS           2
 #define PDR_F_SHEAR_RDT_VISCOUS   3
 #define PDR_F_SHEAR_RDT_INVISCID  4
+#define PDR_F_ISOTROPIC_DEFORMED  5
 #define PDR_NS_BF_ROT             1
 #define PDR_NS_R_CONV             2
 #define PDR_NS_R_ROT              3
@@ -66,14 +67,16 @@ typedef struct pdr_ctx pdr_ctx;
  * that the time step uses. */
 typedef struct pdr_params {
   int32_t nodes_x, nodes_y, nodes_z; /* NODES_X/Y/Z: 2^a * {1,3,5} (radix 2/3/5 FFTs)  */
-  int32_t f_type;                    /* F_TYPE: 1 isotropic, 2 shear DNS, 3/4 shear RDT */
+  int32_t f_type;                    /* F_TYPE: 1 isotropic, 2 shear DNS, 3/4 shear RDT,
+                                        5 isotropic on a deformed grid (BF_ROT)          */
   int32_t ns_deformed;               /* NS_DEFORMED: 1 BF_ROT, 2 R_CONV, 3 R_ROT        */
   int32_t dealiasing;                /* DEALIASING: 1 cubic 2/3 rule by storage; 2 also
                                         spherical |k_lab| <= KMAX; 3 LES |k_lab| <=
                                         LES_CUTOFF KMAX (KMAX = min(KI, KJ, KK))         */
   int32_t rk3_timestep;              /* RK3_TIMESTEP: 1 DECIDE, 2 FIXED, 3 CFL          */
   int32_t f_remesh_t;                /* F_REMESH_T (shear, DECIDE)                      */
-  int32_t b110;                      /* B110 (box stretch in x1; 1 for all configs)     */
+  int32_t b110;                      /* B110: box stretch in x1 (<= 0 taken as 1; other
+                                        than 1 only with F_TYPE 5)                       */
   int32_t p_interp_order;            /* P_INTERP_ORDER: 1..8 (reference default 4)      */
   int32_t precision;                 /* PDR_FP64 or PDR_FP32_MIXED                      */
   int32_t force_stop;                /* FORCE_STOP: OU forcing while tstep < FORCE_STOP */
@@ -86,6 +89,7 @@ typedef struct pdr_params {
   double  force_sigma;               /* FORCE_SIGMA: OU amplitude                       */
   int64_t force_seed;                /* seed of the forcing draws (counter-based RNG)   */
   double  les_cutoff;                /* LES_CUTOFF (DEALIASING = 3; reference default 1) */
+  int32_t b220, b330;                /* B220, B330: box stretch in x2, x3 (as B110)     */
 } pdr_params;
 
 /* ---- library / context -------------------------------------------------- */
@@ -287,7 +291,10 @@ int pdr_particle_stats(pdr_ctx* ctx, int32_t* tstep, pdr_pstats_t* out);
  * E(k) = 16 sqrt(2/pi) (3/2) U0 k^4/kp^5 exp(-2k^2/kp^2).  The gfortran
  * intrinsic RNG is replaced by a counter-based generator, so fields are
  * reproducible from (seed) on any machine but are not the reference's stream.
- * Hermitian symmetry on the i = 0 plane is enforced as in the reference. */
+ * Hermitian symmetry on the i = 0 plane is enforced as in the reference.  A
+ * stretched box (B110 B220 B330 > 1, F_TYPE 5) returns PDR_ERR_UNSUPPORTED: the
+ * reference's stretched variant of the spectrum (src/pf_initialise.f90:67-71,
+ * 550-632) is not built, such a run starts from an uploaded state. */
 int pdr_init_field(pdr_ctx* ctx, uint64_t seed, double init_u0, double init_kp);
 
 /* Host-only form of pdr_init_field: fills u1..u3 (pf_arrays layout for an

@@ -29,7 +29,7 @@ class OrcParams(C.Structure):
         ("force_tf", C.c_double), ("p_interp_order", C.c_int32), ("nthreads", C.c_int32),
         ("force_stop", C.c_int32), ("rrot_p3_two", C.c_int32), ("force_kf", C.c_double),
         ("force_sigma", C.c_double), ("force_seed", C.c_int64), ("dealiasing", C.c_int32),
-        ("pad1", C.c_int32), ("les_cutoff", C.c_double),
+        ("pad1", C.c_int32), ("les_cutoff", C.c_double), ("b220", C.c_int32), ("b330", C.c_int32),
     ]
 
 
@@ -179,17 +179,18 @@ class Oracle:
     def __init__(self, n=None, *, nx=None, ny=None, nz=None, f_type=1, ns_deformed=1,
                  rk3_timestep=2, rk3_dt=0.005, rk3_cfl=0.75, nu=0.01, deform_s=0.0,
                  remesh_t=0, b110=1, force_tf=0.0, interp_order=4, nthreads=0, force_stop=0,
-                 force_kf=0.0, force_sigma=0.0, force_seed=0, rrot_p3_two=0, dealiasing=1, les_cutoff=1.0):
+                 force_kf=0.0, force_sigma=0.0, force_seed=0, rrot_p3_two=0, dealiasing=1, les_cutoff=1.0,
+                 b220=1, b330=1):
         nx = nx or n
         ny = ny or n
         nz = nz or n
         self.p = OrcParams(nx, ny, nz, f_type, ns_deformed, rk3_timestep, rk3_dt, rk3_cfl,
                            nu, deform_s, remesh_t, b110, force_tf, interp_order, nthreads,
                            force_stop, rrot_p3_two, force_kf, force_sigma, force_seed, dealiasing, 0,
-                           les_cutoff)
+                           les_cutoff, b220, b330)
         self.h = lib().orc_create(C.byref(self.p))
         if not self.h:
-            raise ValueError(f"oracle: NS_DEFORMED={ns_deformed} is not restated")
+            raise ValueError(f"oracle: F_TYPE={f_type} with NS_DEFORMED={ns_deformed} is not restated")
         s = np.zeros(6, dtype=np.int64)
         lib().orc_sizes(self.h, s)
         self.nki, self.nkj, self.nkk, self.mw, self.m1, self.kkmax = (int(v) for v in s)

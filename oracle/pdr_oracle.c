@@ -129,6 +129,7 @@ struct orc {
   double dt;            /* sim_dt, SAVEd in g_RK3Calc (src/pg_rk3.f90:92) */
   double ucfl;
   double dx, dy, dz, lx, ly, lz;
+  double dx_lab;        /* dx_laboratoryframe = TWOPI B110 / NODES_X */
   double* u[3];
   double* rk[3];
   double* w[12];        /* u1..u6_1r2w (u1..u12 for R_CONV) */
@@ -167,6 +168,7 @@ static int nthr(const orc* o) {
 orc* orc_create(const orc_params* p) {
   /* NS_DEFORMED: 1 BF_ROT, 2 R_CONV, 3 R_ROT */
   if (p->f_type != 1 && (p->ns_deformed < 1 || p->ns_deformed > 3)) return NULL;
+  if (p->f_type == 5 && p->ns_deformed != 1) return NULL; /* the deformed isotropic form: BF_ROT */
   orc* o = (orc*)calloc(1, sizeof(orc));
   o->p = *p;
   o->pst_tstep = -1;
@@ -176,16 +178,25 @@ orc* orc_create(const orc_params* p) {
   o->nki = o->ki + 1; o->nkj = 2 * o->kj + 1; o->nkk = 2 * o->kk + 1;
   o->mw = (long)o->nki * o->nkj * o->nkk;
   o->m1 = (long)o->nki * o->nkj * o->nz;
-  /* src/pf_fluidstats.f90:343-374 (k_min = 1) */
-  o->kkmax = (int)lround(sqrt((double)(o->ki * o->ki + o->kj * o->kj + o->kk * o->kk)));
+  const int b110 = p->b110 > 0 ? p->b110 : 1;
+  const int b220 = p->b220 > 0 ? p->b220 : 1;
+  const int b330 = p->b330 > 0 ? p->b330 : 1;
+  /* src/pf_fluidstats.f90:343-374 (k_min = 1): k_max over the stretched index
+   * ranges outside the isotropic form */
+  if (p->f_type == 1)
+    o->kkmax = (int)lround(sqrt((double)(o->ki * o->ki + o->kj * o->kj + o->kk * o->kk)));
+  else
+    o->kkmax = (int)lround(sqrt(((double)(o->ki * o->ki) / (double)(b110 * b110) +
+                                 (double)(o->kj * o->kj) / (double)(b220 * b220)) +
+                                (double)(o->kk * o->kk) / (double)(b330 * b330)));
   /* src/pg_domain.f90:168-216 */
   o->lx = TWOPI_REF; o->ly = TWOPI_REF; o->lz = TWOPI_REF;
   o->dx = o->lx / (double)o->nx; o->dy = o->ly / (double)o->ny; o->dz = o->lz / (double)o->nz;
+  o->dx_lab = (TWOPI_REF * (double)b110) / (double)o->nx;
   for (int a = 0; a < 9; a++) { o->bmat[a] = o->binv[a] = o->amat[a] = 0.0; o->bval[a] = o->aval[a] = 0; }
-  const int b110 = p->b110 > 0 ? p->b110 : 1;
   T9(o->binv, 1, 1) = (double)b110; T9(o->bval, 1, 1) = 1;
-  T9(o->binv, 2, 2) = 1.0; T9(o->bval, 2, 2) = 1;
-  T9(o->binv, 3, 3) = 1.0; T9(o->bval, 3, 3) = 1;
+  T9(o->binv, 2, 2) = (double)b220; T9(o->bval, 2, 2) = 1;
+  T9(o->binv, 3, 3) = (double)b330; T9(o->bval, 3, 3) = 1;
   if (p->f_type != 1 && p->f_type != 5) {
     T9(o->bval, 1, 3) = 1;
     T9(o->amat, 1, 3) = p->f_deform_s;
@@ -211,7 +222,7 @@ orc* orc_create(const orc_params* p) {
 #else
   o->nthreads = 1;
 #endif
-  o->forcing = (p->f_type == 1 && p->force_stop > 0);
+  o->forcing = ((p->f_type == 1 || p->f_type == 5) && p->force_stop > 0);
   o->force_nk = o->forcing ? (int)trunc(p->force_kf) : 0;
   if (o->forcing) {
     const long w = 2 * o->force_nk + 1;
@@ -730,10 +741,12 @@ void orc_realspace(orc* o, int stage, int tstep, double* dt) {
       case 2: *dt = o->p.rk3_dt; break;
       case 3: *dt = (o->ucfl == 0.0) ? (double)0.02f : o->p.rk3_cfl * o->dx / o->ucfl; break;
       default:
-        if (o->p.f_type == 1) {
+        if (o->p.f_type == 1 || o->p.f_type == 5) {
+          /* the deformed form: min(DX, DY, DZ) of the moving frame (3905-3919) */
+          const double h = (o->p.f_type == 1) ? o->dx : fmin(o->dx, fmin(o->dy, o->dz));
           if (o->ucfl == 0.0) *dt = (double)0.02f;
           else {
-            *dt = o->p.rk3_cfl * o->dx / o->ucfl;
+            *dt = o->p.rk3_cfl * h / o->ucfl;
             if (*dt > o->p.force_tf) *dt = o->p.force_tf;
           }
         } else {
@@ -831,7 +844,7 @@ void orc_rs_correlation(orc* o, double* r1i, double* lint11) {
             if (xx >= nx) xx -= nx;
             const double c = v[0] * r[(long)y * nx + xx];
             rz[s] += c;
-            rz[smax + 1] += c * o->dx;
+            rz[smax + 1] += c * o->dx_lab;
           }
         }
     }
@@ -1203,10 +1216,33 @@ static inline void rk_update(double* rk[3], double* u[3], long e, const double t
   }
 }
 
+/* f_RK3BFRotForcing (src/pf_rk3.f90:3256-3349), the forcing of the deformed
+ * isotropic form: a mode (i, j, k) whose indices B110, B220, B330 divide takes the
+ * forcing arrays at (i/B110, j/B220, k/B330) if its laboratory wavenumber
+ * wvnlab(m) = bmat(m,m) k_m has 0 < |wvnlab| <= FORCE_KF, made solenoidal along
+ * wvnlab. */
+static void bf_forcing(const orc* o, int i, int j, int k, const double* kv, double f[3][2]) {
+  for (int c = 0; c < 3; c++) f[c][0] = f[c][1] = 0.0;
+  if (!o->forcing) return;
+  const int b1 = (int)lround(T9(o->binv, 1, 1)), b2 = (int)lround(T9(o->binv, 2, 2)), b3 = (int)lround(T9(o->binv, 3, 3));
+  if (i % b1 != 0 || j % b2 != 0 || k % b3 != 0) return;
+  double wl[3] = {0.0, 0.0, 0.0};
+  for (int m = 0; m < 3; m++) wl[m] = wl[m] + T9(o->bmat, m + 1, m + 1) * kv[m];
+  const double kmag2 = ((wl[0] * wl[0]) + (wl[1] * wl[1])) + (wl[2] * wl[2]);
+  if (!(sqrt(kmag2) <= o->p.force_kf) || !(sqrt(kmag2) > 0.0)) return;
+  const int il = i / b1, jl = j / b2, kl = k / b3, nk = o->force_nk;
+  if (il > nk || jl < -nk || jl > nk || kl < -nk || kl > nk) return;
+  const double* ff[3] = {fat(o, 0, il, jl, kl), fat(o, 1, il, jl, kl), fat(o, 2, il, jl, kl)};
+  for (int r = 0; r < 2; r++) {
+    const double kb = (wl[0] * ff[0][r] + wl[1] * ff[1][r]) + wl[2] * ff[2][r];
+    for (int c = 0; c < 3; c++) f[c][r] = ff[c][r] - ((wl[c] / kmag2) * kb);
+  }
+}
+
 /* f_RK3SolveNavierStokes, src/pf_rk3.f90:1996-2142 with f_RK3FormFIsotropic
  * (2323-2487) and f_RK3FormFBFRot (2494-2673).  The isotropic form adds the
- * OU forcing; the shear forms carry none (F_TYPE 2..4 do not call
- * f_RK3BFRotForcing); the two-way coupling term is zero. */
+ * OU forcing, the deformed isotropic one (F_TYPE 5) f_RK3BFRotForcing's; the
+ * shear forms carry none; the two-way coupling term is zero. */
 static void solve_ns(orc* o, double dt, int stage) {
   double beta = 0.0, gamma = THIRD;
   if (stage == 2) { beta = -FIVE_OVER_NINE; gamma = FIFTEEN_OVER_SIXTEEN; }
@@ -1284,15 +1320,18 @@ static void solve_ns(orc* o, double dt, int stage) {
             continue;
           } else {
             double nlf[3][2], pf[3][2], hom[3][2], visc[3][2];
-            if (o->p.f_type == 2) bf_nonlinear(o, nn, nlf);
+            if (o->p.f_type == 2 || o->p.f_type == 5) bf_nonlinear(o, nn, nlf);
             else for (int c = 0; c < 3; c++) nlf[c][0] = nlf[c][1] = 0.0;
             bf_pressure(o, uu, kv, nlf, pf);
             bf_source(o, uu, hom);
-            if (o->p.f_type == 2 || o->p.f_type == 3) bf_viscous(o, uu, kv, visc);
+            if (o->p.f_type == 2 || o->p.f_type == 3 || o->p.f_type == 5) bf_viscous(o, uu, kv, visc);
+            double fo[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            if (o->p.f_type == 5) bf_forcing(o, i, (int)wj, (int)wk, kv, fo);
             for (int c = 0; c < 3; c++)
               for (int r = 0; r < 2; r++) {
                 if (o->p.f_type == 2) tot[c][r] = (((visc[c][r] - nlf[c][r]) + pf[c][r]) + 0.0) - hom[c][r];
                 else if (o->p.f_type == 3) tot[c][r] = ((visc[c][r] + pf[c][r]) + 0.0) - hom[c][r];
+                else if (o->p.f_type == 5) tot[c][r] = (((visc[c][r] - nlf[c][r]) + pf[c][r]) + 0.0) + fo[c][r];
                 else tot[c][r] = pf[c][r] - hom[c][r];
               }
           }

@@ -28,7 +28,8 @@ extern "C" {
 
 typedef struct orc_params {
   int32_t nodes_x, nodes_y, nodes_z;
-  int32_t f_type;          /* 1 iso, 2 shear DNS, 3 shear RDT viscous, 4 shear RDT inviscid */
+  int32_t f_type;          /* 1 iso, 2 shear DNS, 3 shear RDT viscous, 4 shear RDT inviscid,
+                              5 isotropic on a deformed grid */
   int32_t ns_deformed;     /* 1 BF_ROT, 2 R_CONV, 3 R_ROT */
   int32_t rk3_timestep;    /* 1 DECIDE, 2 FIXED, 3 CFL */
   double  rk3_dt;
@@ -36,7 +37,7 @@ typedef struct orc_params {
   double  f_nu;
   double  f_deform_s;
   int32_t f_remesh_t;
-  int32_t b110;
+  int32_t b110;            /* B110 (<= 0 taken as 1) */
   double  force_tf;
   int32_t p_interp_order;  /* even orders (4 is the reference default) */
   int32_t nthreads;        /* 0 = OpenMP default */
@@ -49,6 +50,7 @@ typedef struct orc_params {
   int32_t dealiasing;      /* DEALIASING: 1 two-thirds (storage), 2 spherical, 3 LES */
   int32_t pad1;
   double  les_cutoff;      /* LES_CUTOFF */
+  int32_t b220, b330;      /* B220, B330 (<= 0 taken as 1) */
 } orc_params;
 
 typedef struct orc orc;

@@ -98,14 +98,14 @@ static void frame_init(pdr_ctx* c) {
     c->bmat[a] = c->binv[a] = c->amat[a] = 0.0;
     c->bval[a] = c->aval[a] = 0;
   }
-  const int b110 = c->p.b110 > 0 ? c->p.b110 : 1;
-  T9(c->binv, 1, 1) = double(b110);
+  T9(c->binv, 1, 1) = double(box_stretch(c->p, 1));
   T9(c->bval, 1, 1) = 1;
-  T9(c->binv, 2, 2) = 1.0;
+  T9(c->binv, 2, 2) = double(box_stretch(c->p, 2));
   T9(c->bval, 2, 2) = 1;
-  T9(c->binv, 3, 3) = 1.0;
+  T9(c->binv, 3, 3) = double(box_stretch(c->p, 3));
   T9(c->bval, 3, 3) = 1;
-  if (c->p.f_type != PDR_F_ISOTROPIC) {
+  // the shear entries exist in every form but the deformed isotropic one
+  if (c->p.f_type != PDR_F_ISOTROPIC && c->p.f_type != PDR_F_ISOTROPIC_DEFORMED) {
     T9(c->bval, 1, 3) = 1;
     T9(c->amat, 1, 3) = c->p.f_deform_s;
     T9(c->aval, 1, 3) = 1;
@@ -195,9 +195,21 @@ static Coef make_coef_fp64(const pdr_ctx* c) {
       f.H[3 * (m - 1) + (cc - 1)] = h;
     }
   f.nu = c->p.f_nu;
-  f.use_nl = (c->p.f_type == PDR_F_ISOTROPIC || c->p.f_type == PDR_F_SHEAR_DNS) ? 1 : 0;
+  f.use_nl = (c->p.f_type == PDR_F_ISOTROPIC || c->p.f_type == PDR_F_SHEAR_DNS ||
+              c->p.f_type == PDR_F_ISOTROPIC_DEFORMED) ? 1 : 0;
   f.use_visc = (c->p.f_type != PDR_F_SHEAR_RDT_INVISCID) ? 1 : 0;
-  f.has_hom = (c->p.f_type != PDR_F_ISOTROPIC) ? 1 : 0;
+  // the deformed isotropic form adds no homogeneous source term (its amat is zero)
+  f.has_hom = (c->p.f_type != PDR_F_ISOTROPIC && c->p.f_type != PDR_F_ISOTROPIC_DEFORMED) ? 1 : 0;
+  if (c->p.f_type == PDR_F_ISOTROPIC_DEFORMED) {
+    // f_RK3BFRotForcing (src/pf_rk3.f90:3256-3349): forced modes are those whose
+    // indices the stretches divide, read at the quotient, projected along the
+    // laboratory wavenumber bmat(m,m) k_m
+    f.fdef = 1;
+    for (int m = 1; m <= 3; ++m) {
+      f.fdiv[m - 1] = box_stretch(c->p, m);
+      f.fbm[m - 1] = T9(B, m, m);
+    }
+  }
   return f;
 }
 
@@ -339,8 +351,10 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
   const int nx = p->nodes_x, ny = p->nodes_y, nz = p->nodes_z;
   if (!radix235(nx) || !radix235(ny) || !radix235(nz) || nx % 2 || nx < 8 || ny < 8 || nz < 8)
     return fail(PDR_ERR_UNSUPPORTED, "grid sizes must be even, >= 8 and of the form 2^a 3^b 5^c");
-  if (p->f_type < 1 || p->f_type > 4)
-    return fail(PDR_ERR_UNSUPPORTED, "F_TYPE must be 1 (isotropic) or 2..4 (shear DNS/RDT)");
+  if (p->f_type < 1 || p->f_type > 5)
+    return fail(PDR_ERR_UNSUPPORTED, "F_TYPE must be 1 (isotropic), 2..4 (shear DNS/RDT) or 5 (deformed isotropic)");
+  if (p->f_type == PDR_F_ISOTROPIC_DEFORMED && p->ns_deformed != PDR_NS_BF_ROT)
+    return fail(PDR_ERR_UNSUPPORTED, "F_TYPE = 5 is built for NS_DEFORMED = 1 (BF_ROT) only");
   if (p->f_type != PDR_F_ISOTROPIC && (p->ns_deformed < PDR_NS_BF_ROT || p->ns_deformed > PDR_NS_R_ROT))
     return fail(PDR_ERR_UNSUPPORTED, "NS_DEFORMED must be 1 (BF_ROT), 2 (R_CONV) or 3 (R_ROT)");
   if (p->dealiasing < PDR_DEALIASING_TWOTHIRD || p->dealiasing > PDR_DEALIASING_LES)
@@ -356,8 +370,11 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     return fail(PDR_ERR_UNSUPPORTED, "P_INTERP_ORDER must be 1..8");
   if (p->precision != PDR_FP64 && p->precision != PDR_FP32_MIXED)
     return fail(PDR_ERR_ARG, "precision must be PDR_FP64 or PDR_FP32_MIXED");
-  if (p->b110 != 1) return fail(PDR_ERR_UNSUPPORTED, "B110 must be 1");
-  const bool forcing = p->f_type == PDR_F_ISOTROPIC && p->force_stop > 0;
+  if ((p->b110 > 1 || p->b220 > 1 || p->b330 > 1) && p->f_type != PDR_F_ISOTROPIC_DEFORMED)
+    return fail(PDR_ERR_UNSUPPORTED, "B110, B220, B330 other than 1 are built for F_TYPE = 5 only");
+  // f_ForceCalc runs in every stage; its arrays enter FormF in the isotropic and
+  // the deformed isotropic forms (src/pf_rk3.f90:2063, 2421-2467, 3256-3349)
+  const bool forcing = (p->f_type == PDR_F_ISOTROPIC || p->f_type == PDR_F_ISOTROPIC_DEFORMED) && p->force_stop > 0;
   if (forcing && !(p->force_tf > 0.0 && p->force_kf > 0.0 && p->force_kf <= 64.0 && p->force_sigma >= 0.0))
     return fail(PDR_ERR_ARG, "forcing needs FORCE_TF > 0, 0 < FORCE_KF <= 64 and FORCE_SIGMA >= 0");
 
@@ -543,7 +560,7 @@ int pdr_sizes(const pdr_ctx* c, int64_t out[8]) {
   out[2] = g.nkk;
   out[3] = g.mw;
   out[4] = g.m1;
-  out[5] = (int64_t)llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk)));
+  out[5] = kkmax_of(c);
   out[6] = (int64_t)c->dev_bytes;
   out[7] = c->np;
   return PDR_OK;
@@ -660,7 +677,7 @@ int pdr_synchronize(pdr_ctx* c) {
 static int maybe_remesh(pdr_ctx* c, int tstep, int32_t* done) {
   if (done) *done = 0;
   if (c->p.f_type == PDR_F_ISOTROPIC || c->p.rk3_timestep != PDR_RK3_TIMESTEP_DECIDE) return PDR_OK;
-  const int b110 = c->p.b110;
+  const int b110 = box_stretch(c->p, 1);
   const int a = tstep - (b110 * c->p.f_remesh_t) - 1;
   const int b = 2 * b110 * c->p.f_remesh_t;
   if (a % b != 0) return PDR_OK;  // Fortran mod truncates toward zero like C %
@@ -716,7 +733,8 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
   const bool tracers = field && c->tracers_in_stage;
   const bool need_cfl =
       stage == 1 && (c->p.rk3_timestep == PDR_RK3_TIMESTEP_CFL ||
-                     (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE && c->p.f_type == PDR_F_ISOTROPIC));
+                     (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE &&
+                      (c->p.f_type == PDR_F_ISOTROPIC || c->p.f_type == PDR_F_ISOTROPIC_DEFORMED)));
   if (conv_form(c)) {
     if (need_cfl) HIPCHK(hipMemsetAsync(c->umax, 0, sizeof(unsigned long long), c->stream));
     HIPCHK(launch_conv_front(c, cf, field, need_cfl, rsp));
@@ -778,11 +796,15 @@ int pdr_stage(pdr_ctx* c, int32_t stage, int32_t tstep, double time, double* dt)
       case PDR_RK3_TIMESTEP_FIXED: *dt = c->p.rk3_dt; break;
       case PDR_RK3_TIMESTEP_CFL: *dt = (c->ucfl == 0.0) ? double(0.02f) : c->p.rk3_cfl * dx / c->ucfl; break;
       default:
-        if (c->p.f_type == PDR_F_ISOTROPIC) {
+        if (c->p.f_type == PDR_F_ISOTROPIC || c->p.f_type == PDR_F_ISOTROPIC_DEFORMED) {
+          // the deformed form takes the smallest moving-frame spacing (3905-3919)
+          const double h = c->p.f_type == PDR_F_ISOTROPIC
+                               ? dx
+                               : std::min(dx, std::min(kTwoPiRef / double(c->g.ny), kTwoPiRef / double(c->g.nz)));
           if (c->ucfl == 0.0) {
             *dt = double(0.02f);
           } else {
-            *dt = c->p.rk3_cfl * dx / c->ucfl;
+            *dt = c->p.rk3_cfl * h / c->ucfl;
             if (*dt > c->p.force_tf) *dt = c->p.force_tf;
           }
         } else {
@@ -903,7 +925,7 @@ int pdr_remesh(pdr_ctx* c, int32_t tstep, int32_t* done) {
 int pdr_update_b(pdr_ctx* c, int32_t tstep, double simtime) {
   if (!c) return fail(PDR_ERR_ARG, "null context");
   if (c->p.f_type == PDR_F_ISOTROPIC) return PDR_OK;
-  const int b110 = c->p.b110;
+  const int b110 = box_stretch(c->p, 1);
   double localtime;
   if (c->p.rk3_timestep == PDR_RK3_TIMESTEP_DECIDE) {
     if (tstep <= (b110 * c->p.f_remesh_t)) {

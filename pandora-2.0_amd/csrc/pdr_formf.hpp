@@ -4,7 +4,8 @@
 // the host-built operator tensors of Coef:
 //   f_RK3FormFIsotropic / f_RK3FormFBFRot (src/pf_rk3.f90:2323-2673) with
 //   f_RK3BFRot{Nonlinear,Viscous,Pressure,Source} (2864-3209), the isotropic
-//   forcing (2421-2467), f_RK3RungeKutta (3457-3498) and
+//   forcing (2421-2467) and that of the deformed isotropic form
+//   (f_RK3BFRotForcing, 3256-3349), f_RK3RungeKutta (3457-3498) and
 //   f_RK3ContinuityFixIsotropic (3504-3531).
 #pragma once
 #include "pdr_internal.hpp"
@@ -74,8 +75,29 @@ __device__ __forceinline__ void formf_mode(const Geo& g, const Coef& cf, int i, 
     tot[c] = t;
   }
   // forcing on 0 < |k| <= FORCE_KF, made solenoidal (src/pf_rk3.f90:2421-2467);
-  // frc is [i][j+NK][k+NK][c] (isotropic frame only, fnk = 0 otherwise)
-  if (fnk > 0) {
+  // frc is [i][j+NK][k+NK][c] (isotropic forms only, fnk = 0 otherwise)
+  if (fnk > 0 && cf.fdef) {
+    // deformed isotropic (f_RK3BFRotForcing, 3256-3349): modes whose indices the
+    // stretches divide take the forcing at the quotient, made solenoidal along the
+    // laboratory wavenumber bmat(m,m) k_m
+    const int jj = int(kv[1]), kq = int(kv[2]);
+    if (i % cf.fdiv[0] == 0 && jj % cf.fdiv[1] == 0 && kq % cf.fdiv[2] == 0) {
+      const double wl[3] = {cf.fbm[0] * kv[0], cf.fbm[1] * kv[1], cf.fbm[2] * kv[2]};
+      const double kl2 = (wl[0] * wl[0] + wl[1] * wl[1]) + wl[2] * wl[2];
+      const int il = i / cf.fdiv[0], jl = jj / cf.fdiv[1], kl = kq / cf.fdiv[2];
+      if (sqrt(kl2) <= fkf && sqrt(kl2) > 0.0 && il <= fnk && jl >= -fnk && jl <= fnk && kl >= -fnk && kl <= fnk) {
+        const int w = 2 * fnk + 1;
+        const double2* f = frc + ((long long)(il * w + jl + fnk) * w + kl + fnk) * 3;
+        const double kbr = (wl[0] * f[0].x + wl[1] * f[1].x) + wl[2] * f[2].x;
+        const double kbi = (wl[0] * f[0].y + wl[1] * f[1].y) + wl[2] * f[2].y;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double q = wl[c] / kl2;
+          tot[c] = add(tot[c], mk<T>(T(f[c].x - q * kbr), T(f[c].y - q * kbi)));
+        }
+      }
+    }
+  } else if (fnk > 0) {
     const int jj = int(kv[1]), kq = int(kv[2]);
     if (i <= fnk && jj >= -fnk && jj <= fnk && kq >= -fnk && kq <= fnk && sqrt(kmag2) <= fkf) {
       const int w = 2 * fnk + 1;

@@ -119,6 +119,10 @@ int pdr_init_field_host(int32_t nx, int32_t ny, int32_t nz, uint64_t seed, doubl
 
 int pdr_init_field(pdr_ctx* ctx, uint64_t seed, double init_u0, double init_kp) {
   if (!ctx) return pdr::set_error(PDR_ERR_ARG, "pdr_init_field: null context");
+  // a stretched box takes f_InitialiseSpec's deformed variant (src/pf_initialise.f90:
+  // 67-71, 550-632), which is not built: such a run starts from an uploaded state
+  if (pdr::box_stretch(ctx->p, 1) * pdr::box_stretch(ctx->p, 2) * pdr::box_stretch(ctx->p, 3) != 1)
+    return pdr::set_error(PDR_ERR_UNSUPPORTED, "pdr_init_field: the stretched-box initial field is not built; upload a state");
   const pdr::Geo& g = ctx->g;
   std::vector<double> u[3];
   for (int c = 0; c < 3; ++c) u[c].assign(size_t(2 * g.mw), 0.0);

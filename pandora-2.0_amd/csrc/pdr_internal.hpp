@@ -52,7 +52,16 @@ struct Coef {
   int form;      // 0: isotropic / BF_ROT tensors; 1: NS_R_ROT (nonlinear term of the
                  // unmapped velocity, pressure along M k = bmat^T k, continuity
                  // fix through Bm and V)
+  int fdef;      // F_TYPE 5: forcing read at (i, j, k) / fdiv where fdiv divides them,
+  int fdiv[3];   // projected along the laboratory wavenumber fbm[m] k_m
+  double fbm[3];
 };
+
+// B110, B220, B330 (m = 1, 2, 3); values <= 0 stand for the reference default 1
+inline int box_stretch(const pdr_params& p, int m) {
+  const int b = m == 1 ? p.b110 : (m == 2 ? p.b220 : p.b330);
+  return b > 0 ? b : 1;
+}
 
 // One member of a batched ensemble (pdr_ens.hip): everything a batched pass
 // kernel needs to run that member's tiles, read from device memory by the
@@ -244,6 +253,7 @@ hipError_t launch_finite(pdr_ctx* c, int* flag);
 bool stage_stats_due(const pdr_ctx* c, int stage, int tstep);
 hipError_t stage_stats_wave(pdr_ctx* c, int tstep);
 double* stage_stats_rs_partials(pdr_ctx* c);
+int kkmax_of(const pdr_ctx* c);  // spectrum bins of the box (f_FluidstatsAlloc)
 hipError_t stage_stats_rs_reduce(pdr_ctx* c);
 void stage_stats_free(pdr_ctx* c);
 hipError_t launch_copy(const void* a, void* b, long long bytes, hipStream_t s);

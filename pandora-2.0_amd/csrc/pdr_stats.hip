@@ -31,6 +31,17 @@ void spectra_finish(const pdr_ctx* ctx, const std::vector<double>& raw, const do
                     pdr_homog_t* h);
 void corr_finish(const pdr_ctx* ctx, const std::vector<double>& s12, double* r1i, double* lint11);
 
+// spectrum bins (f_FluidstatsAlloc, src/pf_fluidstats.f90:348-374): kkmax =
+// nint(k_max) with k_max the largest laboratory |k| of the box, each index range
+// divided by its stretch outside the isotropic form
+int kkmax_of(const pdr_ctx* c) {
+  const Geo& g = c->g;
+  if (c->p.f_type == PDR_F_ISOTROPIC) return int(std::llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk))));
+  const double b1 = box_stretch(c->p, 1), b2 = box_stretch(c->p, 2), b3 = box_stretch(c->p, 3);
+  const double k2 = (double(g.ki * g.ki) / (b1 * b1) + double(g.kj * g.kj) / (b2 * b2)) + double(g.kk * g.kk) / (b3 * b3);
+  return int(std::llround(std::sqrt(k2)));
+}
+
 namespace {
 
 constexpr int kStatThreads = 256;
@@ -290,9 +301,6 @@ hipError_t stats_t(pdr_ctx* c, const StatsFrame& sf, int nblk, double* part, dou
   return hipGetLastError();
 }
 
-int kkmax_of(const Geo& g) {
-  return int(std::llround(std::sqrt(double(g.ki * g.ki + g.kj * g.kj + g.kk * g.kk))));
-}
 
 StatsFrame make_stats_frame(const pdr_ctx* c) {
   StatsFrame sf;
@@ -302,7 +310,7 @@ StatsFrame make_stats_frame(const pdr_ctx* c) {
       sf.bi[3 * (m - 1) + (n - 1)] = T9(c->bval, m, n) ? T9(c->binv, m, n) : 0.0;
     }
   sf.nu = c->p.f_nu;
-  sf.kkmax = kkmax_of(c->g);
+  sf.kkmax = kkmax_of(c);
   sf.nki = c->g.nki;
   sf.lab_state = rogallo_form(c) ? 1 : 0;
   return sf;
@@ -313,10 +321,10 @@ int stats_nblk(const pdr_ctx* c) {
   const long long want = (c->g.mw + 4LL * kStatThreads - 1) / (4LL * kStatThreads);
   return int(want < kStatBlocks ? (want < 1 ? 1 : want) : kStatBlocks);
 }
-int stats_width(const pdr_ctx* c) { return stats_hist(kkmax_of(c->g), c->g.nki) + 12; }
+int stats_width(const pdr_ctx* c) { return stats_hist(kkmax_of(c), c->g.nki) + 12; }
 
 bool stats_fits_lds(const pdr_ctx* c) {
-  const size_t lds = size_t(stats_hist(kkmax_of(c->g), c->g.nki) + 12 * (kStatThreads / 64)) * sizeof(double);
+  const size_t lds = size_t(stats_hist(kkmax_of(c), c->g.nki) + 12 * (kStatThreads / 64)) * sizeof(double);
   return lds <= c->lds_max;
 }
 
@@ -324,7 +332,7 @@ bool stats_fits_lds(const pdr_ctx* c) {
 // ken3, dis, disskew, count), uu[6] and, on request, the streamwise bins
 void stats_unpack(const pdr_ctx* c, const double* h, std::vector<double>& raw, double uu[6],
                   std::vector<double>* s12) {
-  const int kkmax = kkmax_of(c->g);
+  const int kkmax = kkmax_of(c);
   const int nki = c->g.nki;
   raw.assign(size_t(7) * kkmax, 0.0);
   for (int a = 0; a < 6; ++a)
@@ -486,7 +494,7 @@ namespace {
 // place: sp[7 * kkmax] holds the raw shell sums (ken, ken1, ken2, ken3, dis,
 // disskew, count) on entry and the normalised spectra and mode density on exit.
 void stats_finish_inplace(const pdr_ctx* ctx, double* sp, const double uu[6], pdr_stats_t* q) {
-  const int kkmax = pdr::kkmax_of(ctx->g);
+  const int kkmax = pdr::kkmax_of(ctx);
   const double nu = ctx->p.f_nu;
   // normalisation by the mode density (src/pf_fluidstats.f90:1547-1575, 767-769)
   double f_ken = 0, f_dis = 0, f_ds = 0, fn[6] = {0, 0, 0, 0, 0, 0};
@@ -718,8 +726,8 @@ void corr_finish(const pdr_ctx* ctx, const std::vector<double>& s12, double* r1i
     if (s <= nx / 2) r1i[s] = r;
     lsum += r;
   }
-  // dx_laboratoryframe = TWOPI B110 / NODES_X (src/pg_domain.f90:178-183), B110 = 1
-  *lint11 = lsum * (2.0 * kOnePiRef / double(nx));
+  // dx_laboratoryframe = TWOPI B110 / NODES_X (src/pg_domain.f90:178-183)
+  *lint11 = lsum * ((2.0 * kOnePiRef * double(box_stretch(ctx->p, 1))) / double(nx));
 }
 
 }  // namespace pdr
@@ -778,7 +786,7 @@ int pdr_spectrum(pdr_ctx* ctx, double* ek, double* dk, double* dsk, double tot[6
   double uu[6];
   int rc = pdr::stats_raw(ctx, raw, uu);
   if (rc != PDR_OK) return rc;
-  const int kkmax = pdr::kkmax_of(ctx->g);
+  const int kkmax = pdr::kkmax_of(ctx);
   double fk = 0, fd = 0, fs = 0;
   for (int q = 0; q < kkmax; ++q) {
     ek[q] = raw[size_t(q)];
@@ -995,7 +1003,7 @@ int pdr_stats_noalloc(pdr_ctx* ctx, pdr_stats_t* q, double* spec) {
   if (!ctx || !q || !spec)
     return pdr::set_error(PDR_ERR_ARG, "pdr_stats_noalloc: null argument (spec is required: it is the working buffer)");
   if (!pdr::stats_fits_lds(ctx)) return pdr::set_error(PDR_ERR_UNSUPPORTED, "statistics: spectrum too long for the LDS");
-  const int kkmax = pdr::kkmax_of(ctx->g);
+  const int kkmax = pdr::kkmax_of(ctx);
   const long long width = pdr::stats_width(ctx);
   const long long tail = 7LL * kkmax + 6;
   // rows of workgroup partials that fit beside the reduced row and the folded sums

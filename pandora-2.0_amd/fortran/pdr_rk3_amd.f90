@@ -42,6 +42,7 @@ module pdr_rk3_amd
     real(c_double)     :: force_sigma = 0.0d0
     integer(c_int64_t) :: force_seed = 0
     real(c_double)     :: les_cutoff = 1.0d0
+    integer(c_int32_t) :: b220 = 1, b330 = 1
   end type pdr_params_t
 
   integer(c_int32_t), parameter, public :: PDR_FP64 = 0, PDR_FP32_MIXED = 1

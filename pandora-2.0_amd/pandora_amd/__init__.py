@@ -26,7 +26,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_PKG_DIR), "include", "pandora_amd.h"
 
 PDR_FP64 = 0
 PDR_FP32_MIXED = 1
-F_ISOTROPIC, F_SHEAR_DNS, F_SHEAR_RDT_VISCOUS, F_SHEAR_RDT_INVISCID = 1, 2, 3, 4
+F_ISOTROPIC, F_SHEAR_DNS, F_SHEAR_RDT_VISCOUS, F_SHEAR_RDT_INVISCID, F_ISOTROPIC_DEFORMED = 1, 2, 3, 4, 5
 DEALIASING_TWOTHIRD, DEALIASING_SPHERICAL, DEALIASING_LES = 1, 2, 3
 NS_BF_ROT, NS_R_CONV, NS_R_ROT = 1, 2, 3
 RK3_TIMESTEP_DECIDE, RK3_TIMESTEP_FIXED, RK3_TIMESTEP_CFL = 1, 2, 3
@@ -73,6 +73,7 @@ class Params(C.Structure):
         ("rk3_dt", C.c_double), ("rk3_cfl", C.c_double), ("f_nu", C.c_double),
         ("f_deform_s", C.c_double), ("force_tf", C.c_double), ("force_kf", C.c_double),
         ("force_sigma", C.c_double), ("force_seed", C.c_int64), ("les_cutoff", C.c_double),
+        ("b220", C.c_int32), ("b330", C.c_int32),
     ]
 
 
@@ -273,15 +274,15 @@ class Pandora:
     def __init__(self, n=None, *, nx=None, ny=None, nz=None, f_type=F_ISOTROPIC, ns_deformed=1,
                  rk3_timestep=RK3_TIMESTEP_FIXED, rk3_dt=0.005, rk3_cfl=0.75, nu=0.01, deform_s=0.0,
                  remesh_t=0, b110=1, interp_order=4, precision=PDR_FP64, force_tf=0.0, force_stop=0,
-                 force_kf=0.0, force_sigma=0.0, force_seed=0, dealiasing=1, les_cutoff=1.0):
+                 force_kf=0.0, force_sigma=0.0, force_seed=0, dealiasing=1, les_cutoff=1.0, b220=1, b330=1):
         L = load_library()
         nx = nx or n
         ny = ny or n
         nz = nz or n
         self.params = Params(nx, ny, nz, f_type, ns_deformed, dealiasing, rk3_timestep, remesh_t, b110,
                              interp_order, precision, force_stop, rk3_dt, rk3_cfl, nu, deform_s,
-                             force_tf, force_kf, force_sigma, force_seed, les_cutoff)
-        self.force_nk = int(force_kf) if (f_type == F_ISOTROPIC and force_stop > 0) else -1
+                             force_tf, force_kf, force_sigma, force_seed, les_cutoff, b220, b330)
+        self.force_nk = int(force_kf) if (f_type in (F_ISOTROPIC, F_ISOTROPIC_DEFORMED) and force_stop > 0) else -1
         h = C.c_void_p()
         _check(L.pdr_create(C.byref(h), C.byref(self.params)))
         self.h = h

@@ -39,7 +39,8 @@ def _params(**kw):
 @pytest.mark.parametrize("kw,code", [
     (dict(nodes_x=28, nodes_y=28, nodes_z=28), -4),        # 7 is not a 2/3/5 radix
     (dict(nodes_x=15), -4),                               # odd x length (real transforms)
-    (dict(f_type=5), -4),                                 # isotropic deformed grid
+    (dict(f_type=6), -4),                                 # no such flow type
+    (dict(f_type=5, ns_deformed=3), -4),                  # deformed isotropic: BF_ROT only
     (dict(f_type=2, ns_deformed=4), -4),                  # no such NS_DEFORMED form
     (dict(dealiasing=4), -4),                             # no such dealiasing form
     (dict(dealiasing=3, les_cutoff=0.0), -1),             # LES cutoff must be > 0
@@ -50,7 +51,8 @@ def _params(**kw):
     (dict(p_interp_order=0), -4),                         # P_INTERP_ORDER outside 1..8
     (dict(p_interp_order=9), -4),
     (dict(precision=2), -1),
-    (dict(b110=2), -4),
+    (dict(b110=2), -4),                                   # stretches: deformed isotropic only
+    (dict(f_type=2, b220=2), -4),
 ])
 def test_create_rejects_invalid_parameters(kw, code):
     L = pa.load_library()
