@@ -68,7 +68,7 @@ typedef struct pdr_ctx pdr_ctx;
 typedef struct pdr_params {
   int32_t nodes_x, nodes_y, nodes_z; /* NODES_X/Y/Z: 2^a * {1,3,5} (radix 2/3/5 FFTs)  */
   int32_t f_type;                    /* F_TYPE: 1 isotropic, 2 shear DNS, 3/4 shear RDT,
-                                        5 isotropic on a deformed grid (BF_ROT)          */
+                                        5 isotropic on a deformed grid                   */
   int32_t ns_deformed;               /* NS_DEFORMED: 1 BF_ROT, 2 R_CONV, 3 R_ROT        */
   int32_t dealiasing;                /* DEALIASING: 1 cubic 2/3 rule by storage; 2 also
                                         spherical |k_lab| <= KMAX; 3 LES |k_lab| <=

@@ -168,7 +168,6 @@ static int nthr(const orc* o) {
 orc* orc_create(const orc_params* p) {
   /* NS_DEFORMED: 1 BF_ROT, 2 R_CONV, 3 R_ROT */
   if (p->f_type != 1 && (p->ns_deformed < 1 || p->ns_deformed > 3)) return NULL;
-  if (p->f_type == 5 && p->ns_deformed != 1) return NULL; /* the deformed isotropic form: BF_ROT */
   orc* o = (orc*)calloc(1, sizeof(orc));
   o->p = *p;
   o->pst_tstep = -1;
@@ -1305,14 +1304,18 @@ static void solve_ns(orc* o, double dt, int stage) {
              * only for F_SHEAR_DNS, unmapped */
             double nlf[3][2], pf[3][2], hom[3][2], visc[3][2];
             for (int c = 0; c < 3; c++)
-              for (int r = 0; r < 2; r++) nlf[c][r] = (o->p.f_type == 2) ? nn[c][r] : 0.0;
+              for (int r = 0; r < 2; r++) nlf[c][r] = (o->p.f_type == 2 || o->p.f_type == 5) ? nn[c][r] : 0.0;
             rr_pressure(o, uu, kv, nlf, pf);
             rr_source(o, uu, hom);
-            if (o->p.f_type == 2 || o->p.f_type == 3) rr_viscous(o, uu, kv, visc);
+            if (o->p.f_type == 2 || o->p.f_type == 3 || o->p.f_type == 5) rr_viscous(o, uu, kv, visc);
+            /* f_RK3RRotForcing is f_RK3BFRotForcing's twin (the deformed branch, 2820-2828) */
+            double fo[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            if (o->p.f_type == 5) bf_forcing(o, i, (int)wj, (int)wk, kv, fo);
             for (int c = 0; c < 3; c++)
               for (int r = 0; r < 2; r++) {
                 if (o->p.f_type == 2) tot[c][r] = (((visc[c][r] - nlf[c][r]) + pf[c][r]) + 0.0) - hom[c][r];
                 else if (o->p.f_type == 3) tot[c][r] = ((visc[c][r] + pf[c][r]) + 0.0) - hom[c][r];
+                else if (o->p.f_type == 5) tot[c][r] = (((visc[c][r] - nlf[c][r]) + pf[c][r]) + 0.0) + fo[c][r];
                 else tot[c][r] = pf[c][r] - hom[c][r];
               }
             rk_only(o->rk, o->u, e, tot, beta, gdt);

@@ -115,6 +115,20 @@ static void frame_init(pdr_ctx* c) {
   T9(c->bmat, 3, 3) = 1.0 / T9(c->binv, 3, 3);
 }
 
+// The forcing of the deformed isotropic form, f_RK3BFRotForcing and the identical
+// f_RK3RRotForcing (src/pf_rk3.f90:3256-3349, and its R_ROT twin): forced modes are
+// those whose indices the stretches divide, read at the quotient, made solenoidal
+// along the laboratory wavenumber bmat(m,m) k_m.
+static void deformed_forcing(const pdr_ctx* c, Coef& f) {
+  const bool def = c->p.f_type == PDR_F_ISOTROPIC_DEFORMED;
+  f.fdef = def ? 1 : 0;
+  for (int m = 1; m <= 3; ++m) {
+    f.fdiv[m - 1] = def ? box_stretch(c->p, m) : 1;
+    f.finv[m - 1] = 1.0 / double(f.fdiv[m - 1]);
+    f.fbm[m - 1] = def ? T9(c->bmat, m, m) : 1.0;  // the isotropic form: unit stretch
+  }
+}
+
 // Operator tensors from the current frame (see Coef).
 static Coef make_coef_fp64(const pdr_ctx* c) {
   Coef f;
@@ -148,10 +162,12 @@ static Coef make_coef_fp64(const pdr_ctx* c) {
         f.H[3 * (r - 1) + (m - 1)] = T9(av, r, m) ? 0.5 * T9(A, r, m) : 0.0;
       }
     f.nu = c->p.f_nu;
-    f.use_nl = (c->p.f_type == PDR_F_SHEAR_DNS) ? 1 : 0;
+    f.use_nl = (c->p.f_type == PDR_F_SHEAR_DNS || c->p.f_type == PDR_F_ISOTROPIC_DEFORMED) ? 1 : 0;
     f.use_visc = (c->p.f_type != PDR_F_SHEAR_RDT_INVISCID) ? 1 : 0;
-    f.has_hom = 1;
+    // the deformed isotropic branch of f_RK3FormFRRot adds no source term (2820-2828)
+    f.has_hom = (c->p.f_type != PDR_F_ISOTROPIC_DEFORMED) ? 1 : 0;
     f.form = 1;
+    deformed_forcing(c, f);
     return f;
   }
   for (int n = 1; n <= 3; ++n)
@@ -200,16 +216,7 @@ static Coef make_coef_fp64(const pdr_ctx* c) {
   f.use_visc = (c->p.f_type != PDR_F_SHEAR_RDT_INVISCID) ? 1 : 0;
   // the deformed isotropic form adds no homogeneous source term (its amat is zero)
   f.has_hom = (c->p.f_type != PDR_F_ISOTROPIC && c->p.f_type != PDR_F_ISOTROPIC_DEFORMED) ? 1 : 0;
-  if (c->p.f_type == PDR_F_ISOTROPIC_DEFORMED) {
-    // f_RK3BFRotForcing (src/pf_rk3.f90:3256-3349): forced modes are those whose
-    // indices the stretches divide, read at the quotient, projected along the
-    // laboratory wavenumber bmat(m,m) k_m
-    f.fdef = 1;
-    for (int m = 1; m <= 3; ++m) {
-      f.fdiv[m - 1] = box_stretch(c->p, m);
-      f.fbm[m - 1] = T9(B, m, m);
-    }
-  }
+  deformed_forcing(c, f);
   return f;
 }
 
@@ -353,8 +360,6 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     return fail(PDR_ERR_UNSUPPORTED, "grid sizes must be even, >= 8 and of the form 2^a 3^b 5^c");
   if (p->f_type < 1 || p->f_type > 5)
     return fail(PDR_ERR_UNSUPPORTED, "F_TYPE must be 1 (isotropic), 2..4 (shear DNS/RDT) or 5 (deformed isotropic)");
-  if (p->f_type == PDR_F_ISOTROPIC_DEFORMED && p->ns_deformed != PDR_NS_BF_ROT)
-    return fail(PDR_ERR_UNSUPPORTED, "F_TYPE = 5 is built for NS_DEFORMED = 1 (BF_ROT) only");
   if (p->f_type != PDR_F_ISOTROPIC && (p->ns_deformed < PDR_NS_BF_ROT || p->ns_deformed > PDR_NS_R_ROT))
     return fail(PDR_ERR_UNSUPPORTED, "NS_DEFORMED must be 1 (BF_ROT), 2 (R_CONV) or 3 (R_ROT)");
   if (p->dealiasing < PDR_DEALIASING_TWOTHIRD || p->dealiasing > PDR_DEALIASING_LES)

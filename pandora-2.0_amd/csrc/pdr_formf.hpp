@@ -18,7 +18,10 @@ __device__ __forceinline__ double wvn_m(int k, int kmax) {
 
 // Mode (i, j, k) at element e of the wave arrays; a, b, c are its three
 // components of the nonlinear term NL (already normalised and truncated).
-template <typename T>
+// FRC: 0 leaves the forcing code out (callers without forcing arrays), so that
+// it costs the fused kernels' unforced runs no registers; 1 the isotropic
+// forcing; 2 that of the deformed isotropic form (cf.fdef).
+template <typename T, int FRC = 2>
 __device__ __forceinline__ void formf_mode(const Geo& g, const Coef& cf, int i, int j, int k, long long e, cx<T> a,
                                            cx<T> b, cx<T> c, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
                                            cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1,
@@ -75,38 +78,35 @@ __device__ __forceinline__ void formf_mode(const Geo& g, const Coef& cf, int i, 
     tot[c] = t;
   }
   // forcing on 0 < |k| <= FORCE_KF, made solenoidal (src/pf_rk3.f90:2421-2467);
-  // frc is [i][j+NK][k+NK][c] (isotropic forms only, fnk = 0 otherwise)
-  if (fnk > 0 && cf.fdef) {
-    // deformed isotropic (f_RK3BFRotForcing, 3256-3349): modes whose indices the
-    // stretches divide take the forcing at the quotient, made solenoidal along the
-    // laboratory wavenumber bmat(m,m) k_m
-    const int jj = int(kv[1]), kq = int(kv[2]);
-    if (i % cf.fdiv[0] == 0 && jj % cf.fdiv[1] == 0 && kq % cf.fdiv[2] == 0) {
-      const double wl[3] = {cf.fbm[0] * kv[0], cf.fbm[1] * kv[1], cf.fbm[2] * kv[2]};
-      const double kl2 = (wl[0] * wl[0] + wl[1] * wl[1]) + wl[2] * wl[2];
-      const int il = i / cf.fdiv[0], jl = jj / cf.fdiv[1], kl = kq / cf.fdiv[2];
-      if (sqrt(kl2) <= fkf && sqrt(kl2) > 0.0 && il <= fnk && jl >= -fnk && jl <= fnk && kl >= -fnk && kl <= fnk) {
-        const int w = 2 * fnk + 1;
-        const double2* f = frc + ((long long)(il * w + jl + fnk) * w + kl + fnk) * 3;
-        const double kbr = (wl[0] * f[0].x + wl[1] * f[1].x) + wl[2] * f[2].x;
-        const double kbi = (wl[0] * f[0].y + wl[1] * f[1].y) + wl[2] * f[2].y;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double q = wl[c] / kl2;
-          tot[c] = add(tot[c], mk<T>(T(f[c].x - q * kbr), T(f[c].y - q * kbi)));
-        }
-      }
+  // frc is [i][j+NK][k+NK][c] (isotropic forms only, fnk = 0 otherwise).  The
+  // deformed isotropic form (f_RK3BFRotForcing, 3256-3349, and its R_ROT twin)
+  // forces only modes whose indices the stretches fdiv divide, reads the arrays
+  // at the quotient and projects along the laboratory wavenumber fbm[m] k_m; the
+  // isotropic form is its unit-stretch case (fbm = 1: the same wavenumber)
+  if (FRC != 0 && fnk > 0) {
+    int il = i, jl = int(kv[1]), kl = int(kv[2]);
+    bool hit = true;
+    if (FRC == 2 && cf.fdef) {
+      // quotients by rounding the product with the reciprocal (exact for the
+      // divisible indices, which are the only ones kept): no integer division
+      const int qi = int(rint(double(il) * cf.finv[0])), qj = int(rint(double(jl) * cf.finv[1])),
+                qk = int(rint(double(kl) * cf.finv[2]));
+      hit = qi * cf.fdiv[0] == il && qj * cf.fdiv[1] == jl && qk * cf.fdiv[2] == kl;
+      il = qi;
+      jl = qj;
+      kl = qk;
     }
-  } else if (fnk > 0) {
-    const int jj = int(kv[1]), kq = int(kv[2]);
-    if (i <= fnk && jj >= -fnk && jj <= fnk && kq >= -fnk && kq <= fnk && sqrt(kmag2) <= fkf) {
+    const double wl[3] = {FRC == 2 ? cf.fbm[0] * kv[0] : kv[0], FRC == 2 ? cf.fbm[1] * kv[1] : kv[1],
+                          FRC == 2 ? cf.fbm[2] * kv[2] : kv[2]};
+    const double kl2 = (wl[0] * wl[0] + wl[1] * wl[1]) + wl[2] * wl[2];
+    if (hit && il <= fnk && jl >= -fnk && jl <= fnk && kl >= -fnk && kl <= fnk && sqrt(kl2) <= fkf && kl2 > 0.0) {
       const int w = 2 * fnk + 1;
-      const double2* f = frc + ((long long)(i * w + jj + fnk) * w + kq + fnk) * 3;
-      const double kbr = (kv[0] * f[0].x + kv[1] * f[1].x) + kv[2] * f[2].x;
-      const double kbi = (kv[0] * f[0].y + kv[1] * f[1].y) + kv[2] * f[2].y;
+      const double2* f = frc + ((long long)(il * w + jl + fnk) * w + kl + fnk) * 3;
+      const double kbr = (wl[0] * f[0].x + wl[1] * f[1].x) + wl[2] * f[2].x;
+      const double kbi = (wl[0] * f[0].y + wl[1] * f[1].y) + wl[2] * f[2].y;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double q = kv[c] / kmag2;
+        const double q = wl[c] / kl2;
         tot[c] = add(tot[c], mk<T>(T(f[c].x - q * kbr), T(f[c].y - q * kbi)));
       }
     }

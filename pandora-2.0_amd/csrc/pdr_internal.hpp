@@ -53,8 +53,9 @@ struct Coef {
                  // unmapped velocity, pressure along M k = bmat^T k, continuity
                  // fix through Bm and V)
   int fdef;      // F_TYPE 5: forcing read at (i, j, k) / fdiv where fdiv divides them,
-  int fdiv[3];   // projected along the laboratory wavenumber fbm[m] k_m
-  double fbm[3];
+  int fdiv[3];   // projected along the laboratory wavenumber fbm[m] k_m (1 and 1.0
+  double fbm[3]; // for the isotropic form); finv = 1 / fdiv
+  double finv[3];
 };
 
 // B110, B220, B330 (m = 1, 2, 3); values <= 0 stand for the reference default 1

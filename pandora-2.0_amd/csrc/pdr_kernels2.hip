@@ -908,7 +908,7 @@ __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
 #ifndef PDR_ZFF_PF
 #define PDR_ZFF_PF 1
 #endif
-template <typename T, int N, int FT = kBlock2>
+template <typename T, int N, int FT = kBlock2, int FRC = 2>
 __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, const Coef& cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
                   cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
                   const double2* __restrict__ frc, int fnk, double fkf, double beta, double gdt,
@@ -977,19 +977,21 @@ __device__ __forceinline__ void k2_zfwd_formf_body(unsigned bid, const Geo& g, c
     const int idx = int(threadIdx.x) + FT * q;
     if (idx < nl_lines * NKK) {
       const int line = idx / NKK;
-      formf_mode<T>(g, cf, i, j0 + line, idx - line * NKK, e0 + idx, nlv[0][q], nlv[1][q], nlv[2][q], u0, u1, u2,
+      formf_mode<T, FRC>(g, cf, i, j0 + line, idx - line * NKK, e0 + idx, nlv[0][q], nlv[1][q], nlv[2][q], u0, u1, u2,
                     r0, r1, r2, frc, fnk, fkf, beta, gdt);
     }
   }
 }
 
-template <typename T, int N>
+// FRC: 0 unforced, 1 isotropic forcing, 2 deformed isotropic forcing
+template <typename T, int N, int FRC>
 __global__ void __launch_bounds__(kBlock2) PDR_FWD_WAVES
     k2_zfwd_formf(Geo g, Coef cf, const cx<T>* __restrict__ w1, cx<T>* __restrict__ u0, cx<T>* __restrict__ u1,
                   cx<T>* __restrict__ u2, cx<T>* __restrict__ r0, cx<T>* __restrict__ r1, cx<T>* __restrict__ r2,
                   const double2* __restrict__ frc, int fnk, double fkf, double beta, double gdt,
                   const cx<T>* __restrict__ twg, int ntile, T norm) {
-  k2_zfwd_formf_body<T, N>(xcd_remap(blockIdx.x, gridDim.x), g, cf, w1, u0, u1, u2, r0, r1, r2, frc, fnk, fkf, beta, gdt, twg, ntile, norm);
+  k2_zfwd_formf_body<T, N, kBlock2, FRC>(xcd_remap(blockIdx.x, gridDim.x), g, cf, w1, u0, u1, u2, r0, r1, r2, frc,
+                                         fnk, fkf, beta, gdt, twg, ntile, norm);
 }
 
 template <typename T, int N>
@@ -1111,14 +1113,20 @@ hipError_t zfwd_formf_n(pdr_ctx* c, const Coef& cf, double beta, double gdt) {
   constexpr int L = tile_lines_fwd<T>(N);
   const int nt = (g.nkj + L - 1) / L;
   const size_t lds = lds_fwd<T, N>();
-  hipError_t e = big_lds((const void*)k2_zfwd_formf<T, N>, lds);
-  if (e != hipSuccess) return e;
   const double norm = 1.0 / (double(g.nx) * double(g.ny) * double(g.nz));
-  k2_zfwd_formf<T, N><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
-      g, cf, (const cx<T>*)c->w1, (cx<T>*)c->u[0], (cx<T>*)c->u[1], (cx<T>*)c->u[2], (cx<T>*)c->rk[0],
-      (cx<T>*)c->rk[1], (cx<T>*)c->rk[2], (const double2*)c->force, c->forcing ? c->force_nk : 0, c->p.force_kf,
-      beta, gdt, (const cx<T>*)c->tab.twz, nt, T(norm));
-  return hipGetLastError();
+  auto launch = [&](auto frc_c) -> hipError_t {
+    constexpr int F = decltype(frc_c)::value;
+    hipError_t e = big_lds((const void*)k2_zfwd_formf<T, N, F>, lds);
+    if (e != hipSuccess) return e;
+    k2_zfwd_formf<T, N, F><<<dim3(unsigned(nt * g.nki)), kBlock2, lds, c->stream>>>(
+        g, cf, (const cx<T>*)c->w1, (cx<T>*)c->u[0], (cx<T>*)c->u[1], (cx<T>*)c->u[2], (cx<T>*)c->rk[0],
+        (cx<T>*)c->rk[1], (cx<T>*)c->rk[2], F ? (const double2*)c->force : nullptr, F ? c->force_nk : 0,
+        F ? c->p.force_kf : 0.0, beta, gdt, (const cx<T>*)c->tab.twz, nt, T(norm));
+    return hipGetLastError();
+  };
+  if (!c->forcing) return launch(std::integral_constant<int, 0>{});
+  if (cf.fdef) return launch(std::integral_constant<int, 2>{});
+  return launch(std::integral_constant<int, 1>{});
   }
 }
 
@@ -1228,8 +1236,9 @@ __global__ void __launch_bounds__(kBlock2) k_ens_zfwd_formf(const EnsMember* __r
   switch (e.n) {
 #define PDR_EC(n)                                                                                                \
   case n:                                                                                                        \
-    k2_zfwd_formf_body<T, n>(lb, e.g, e.cf, (const cx<T>*)e.w1, (cx<T>*)e.u[0], (cx<T>*)e.u[1], (cx<T>*)e.u[2], \
-                             (cx<T>*)e.rk[0], (cx<T>*)e.rk[1], (cx<T>*)e.rk[2], nullptr, 0, 0.0, beta, gdt,     \
+    k2_zfwd_formf_body<T, n, kBlock2, 0>(lb, e.g, e.cf, (const cx<T>*)e.w1, (cx<T>*)e.u[0], (cx<T>*)e.u[1],  \
+                             (cx<T>*)e.u[2], (cx<T>*)e.rk[0], (cx<T>*)e.rk[1], (cx<T>*)e.rk[2], nullptr, 0, 0.0, \
+                             beta, gdt,                                                                          \
                              (const cx<T>*)e.twz, e.nt[4], T(e.norm));                                           \
     break;
     PDR_ENS_SIZES(PDR_EC)

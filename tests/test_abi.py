@@ -40,7 +40,7 @@ def _params(**kw):
     (dict(nodes_x=28, nodes_y=28, nodes_z=28), -4),        # 7 is not a 2/3/5 radix
     (dict(nodes_x=15), -4),                               # odd x length (real transforms)
     (dict(f_type=6), -4),                                 # no such flow type
-    (dict(f_type=5, ns_deformed=3), -4),                  # deformed isotropic: BF_ROT only
+    (dict(f_type=5, ns_deformed=4), -4),                  # no such NS_DEFORMED form
     (dict(f_type=2, ns_deformed=4), -4),                  # no such NS_DEFORMED form
     (dict(dealiasing=4), -4),                             # no such dealiasing form
     (dict(dealiasing=3, les_cutoff=0.0), -1),             # LES cutoff must be > 0

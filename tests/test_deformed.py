@@ -1,5 +1,6 @@
-"""Pins for the oracle's deformed isotropic form (F_TYPE = 5, NS_BF_ROT, box
-stretches B110/B220/B330; src/pf_rk3.f90:2567-2648, 3256-3349, 3905-3919,
+"""Pins for the oracle's deformed isotropic form (F_TYPE = 5 over BF_ROT, R_ROT
+and R_CONV, box stretches B110/B220/B330; src/pf_rk3.f90:2567-2648, 2751-2828,
+3256-3349 and its R_ROT twin, 3905-3919,
 src/pg_domain.f90:177-216, src/pf_fluidstats.f90:348-374).
 
 The deformed form is checked against things it shares no branch with:
@@ -16,7 +17,10 @@ The deformed form is checked against things it shares no branch with:
 * the forcing of a stretched box after one stage from rest is the isotropic
   box's at (i/B110, j, k), made solenoidal by the continuity fix along k;
 * the DECIDE time step of the stretched box uses min(DX, DY, DZ): half the
-  isotropic one for the same field.
+  isotropic one for the same field;
+* over the R forms (laboratory velocities) the stretched box holds the
+  unit-stretch run's state, forcing included, and unit-stretch R_ROT is the
+  isotropic form.
 
 CPU only (the oracle is FP64 C)."""
 import numpy as np
@@ -149,6 +153,35 @@ def test_decide_step_takes_the_smallest_spacing(oracle_mod):
     assert 0.0 < td < 0.1 and td * 2.0 == ti
 
 
-def test_deformed_form_needs_bf_rot(oracle_mod):
-    with pytest.raises(ValueError):
-        oracle_mod.Oracle(16, f_type=5, ns_deformed=3, b110=2)
+@pytest.mark.parametrize("ns", [3, 2], ids=["R_ROT", "R_CONV"])
+def test_rogallo_forms_stretched_twin_with_forcing(oracle_mod, ns):
+    """The deformed branch of f_RK3FormFRRot with f_RK3RRotForcing (the R forms
+    hold laboratory velocities): the (2M, M, M) box with B110 = 2 holds the
+    unit-stretch M^3 run's state at i = 2i' through 4 forced steps; with unit
+    stretch the R_ROT form is the isotropic one."""
+    kw = dict(rk3_timestep=2, rk3_dt=0.01, nu=0.01, force_stop=100, force_kf=2.5, force_tf=0.2,
+              force_sigma=0.7, force_seed=5)
+    u = pa.init_field_host(M, 11, 2.0 / 3.0, 4.0)
+    ou = oracle_mod.Oracle(M, f_type=5, ns_deformed=ns, **kw)
+    od = oracle_mod.Oracle(nx=2 * M, ny=M, nz=M, f_type=5, ns_deformed=ns, b110=2, **kw)
+    ou.set_state(*u)
+    ud = [np.zeros(od.shape, np.complex128) for _ in range(3)]
+    for c in range(3):
+        ud[c][0::2] = u[c]
+    od.set_state(*ud)
+    tu = td = 0.0
+    for s in range(1, 5):
+        tu, td = ou.step(s, tu), od.step(s, td)
+    su, sd = ou.get_state(), od.get_state()
+    assert max(np.abs(a).max() for a in su) > 0.01
+    for c in range(3):
+        assert rel(sd[c][0::2], su[c]) < 1e-14
+        assert np.abs(sd[c][1::2]).max() < 1e-13 * np.abs(su[c]).max()
+    if ns == 3:
+        oi = oracle_mod.Oracle(M, f_type=1, **kw)
+        oi.set_state(*u)
+        t = 0.0
+        for s in range(1, 5):
+            t = oi.step(s, t)
+        for a, b in zip(su, oi.get_state()):
+            assert rel(a, b) < 1e-13

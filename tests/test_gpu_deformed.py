@@ -1,6 +1,6 @@
-"""GPU parity of the deformed isotropic form (F_TYPE = 5, NS_BF_ROT, boxes
-stretched by B110 / B220: src/pf_rk3.f90:2567-2648 with f_RK3BFRotForcing
-3256-3349, the DECIDE step 3905-3919, the frame of src/pg_domain.f90:177-216,
+"""GPU parity of the deformed isotropic form (F_TYPE = 5 over BF_ROT, R_ROT and
+R_CONV, boxes stretched by B110 / B220: src/pf_rk3.f90:2567-2648, 2751-2828 with
+f_RK3BFRotForcing 3256-3349 and its R_ROT twin, the DECIDE step 3905-3919, the frame of src/pg_domain.f90:177-216,
 the spectrum bins of src/pf_fluidstats.f90:348-374) against the oracle, whose
 deformed form is pinned by tests/test_deformed.py.  FP64 <= 1e-12 norm-wise
 (tracers <= 1e-11), FP32-mixed <= 1e-5 (tracers <= 1e-6); statistics <= 1e-11."""
@@ -50,24 +50,30 @@ def _tracers(count, seed, nx):
 
 
 CASES = [
-    # (nx, ny, nz), (B110, B220, B330): the runtime-radix kernels (short non-cubic
-    # lines) and the length-specialised ones, an x and a y stretch
-    ((48, 24, 24), (2, 1, 1)),
-    ((24, 48, 24), (1, 2, 1)),
-    ((256, 128, 128), (2, 1, 1)),
+    # (nx, ny, nz), (B110, B220, B330), NS_DEFORMED: the runtime-radix kernels
+    # (short non-cubic lines) and the length-specialised ones, an x and a y
+    # stretch; BF_ROT, and the R forms (laboratory velocities)
+    ((48, 24, 24), (2, 1, 1), 1),
+    ((24, 48, 24), (1, 2, 1), 1),
+    ((256, 128, 128), (2, 1, 1), 1),
+    ((48, 24, 24), (2, 1, 1), 3),
+    ((48, 24, 24), (2, 1, 1), 2),
 ]
 
 
-@pytest.mark.parametrize("shape,b", CASES, ids=lambda v: "x".join(map(str, v)))
-def test_deformed_forced_with_tracers_against_oracle(oracle_mod, shape, b):
+@pytest.mark.parametrize("shape,b,ns", CASES, ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else f"ns{v}")
+def test_deformed_forced_with_tracers_against_oracle(oracle_mod, shape, b, ns):
     """Forcing, 2000 tracers, FIXED dt, 5 steps (3 at 256x128x128); then the
     statistics of the final state."""
     nx, ny, nz = shape
     steps = 3 if nx >= 256 else 5
-    note(f"deformed isotropic {shape} B={b}: forcing + tracers, {steps} steps, oracle vs FP64 and FP32-mixed")
-    kw = dict(f_type=5, ns_deformed=1, rk3_timestep=2, rk3_dt=0.004, nu=0.01, b110=b[0], b220=b[1], b330=b[2])
+    note(f"deformed isotropic {shape} B={b} NS_DEFORMED={ns}: forcing + tracers, {steps} steps, "
+         "oracle vs FP64 and FP32-mixed")
+    kw = dict(f_type=5, ns_deformed=ns, rk3_timestep=2, rk3_dt=0.004, nu=0.01, b110=b[0], b220=b[1], b330=b[2])
     kw.update(FORCE)
     u = _field(nx, ny, nz, b, 40 + nx)
+    if ns != 1:  # the R forms store the laboratory velocity
+        u = [a * bc for a, bc in zip(u, b)]
     xp = _tracers(2000, nx + ny, nx)
     o = oracle_mod.Oracle(nx=nx, ny=ny, nz=nz, **kw)
     o.set_state(*u)
