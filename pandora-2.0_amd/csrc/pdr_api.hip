@@ -375,8 +375,8 @@ int pdr_create(pdr_ctx** out, const pdr_params* p) {
     return fail(PDR_ERR_UNSUPPORTED, "P_INTERP_ORDER must be 1..8");
   if (p->precision != PDR_FP64 && p->precision != PDR_FP32_MIXED)
     return fail(PDR_ERR_ARG, "precision must be PDR_FP64 or PDR_FP32_MIXED");
-  if ((p->b110 > 1 || p->b220 > 1 || p->b330 > 1) && p->f_type != PDR_F_ISOTROPIC_DEFORMED)
-    return fail(PDR_ERR_UNSUPPORTED, "B110, B220, B330 other than 1 are built for F_TYPE = 5 only");
+  if ((p->b110 > 1 || p->b220 > 1 || p->b330 > 1) && p->f_type == PDR_F_ISOTROPIC)
+    return fail(PDR_ERR_UNSUPPORTED, "box stretches B110, B220, B330 need a deformed frame (F_TYPE 2..5)");
   // f_ForceCalc runs in every stage; its arrays enter FormF in the isotropic and
   // the deformed isotropic forms (src/pf_rk3.f90:2063, 2421-2467, 3256-3349)
   const bool forcing = (p->f_type == PDR_F_ISOTROPIC || p->f_type == PDR_F_ISOTROPIC_DEFORMED) && p->force_stop > 0;

@@ -52,7 +52,7 @@ def _params(**kw):
     (dict(p_interp_order=9), -4),
     (dict(precision=2), -1),
     (dict(b110=2), -4),                                   # stretches: deformed isotropic only
-    (dict(f_type=2, b220=2), -4),
+    (dict(b220=2), -4),
 ])
 def test_create_rejects_invalid_parameters(kw, code):
     L = pa.load_library()

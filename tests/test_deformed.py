@@ -185,3 +185,94 @@ def test_rogallo_forms_stretched_twin_with_forcing(oracle_mod, ns):
             t = oi.step(s, t)
         for a, b in zip(su, oi.get_state()):
             assert rel(a, b) < 1e-13
+
+
+# ---- stretched shear boxes (B110 > 1 with the shear forms) -------------------
+
+def _shear_pair(oracle_mod, ns, T):
+    kw = dict(f_type=2, ns_deformed=ns, rk3_timestep=1, deform_s=1.0, remesh_t=T, nu=0.01)
+    u = pa.init_field_host(M, 11, 2.0 / 3.0, 4.0)
+    ou = oracle_mod.Oracle(M, **kw)
+    os_ = oracle_mod.Oracle(nx=2 * M, ny=M, nz=M, b110=2, **kw)
+    ou.set_state(*u)
+    scale = (0.5, 1.0, 1.0) if ns == 1 else (1.0, 1.0, 1.0)  # R forms: laboratory velocities
+    us = [np.zeros(os_.shape, np.complex128) for _ in range(3)]
+    for c in range(3):
+        us[c][0::2] = scale[c] * u[c]
+    os_.set_state(*us)
+    return ou, os_, scale
+
+
+def test_stretched_shear_box_is_its_twin_until_the_twin_remeshes(oracle_mod):
+    """BF_ROT shear DNS with DECIDE (S = 1, F_REMESH_T = 5, so the unstretched box
+    remeshes at step 6): the (48, 24, 24) box with B110 = 2 holds the 24^3 run's
+    state at i = 2i' for the first 5 steps; the same lab shear, bmat(1,3) =
+    -S t bmat(1,1).  (The R_ROT form is no such twin under shear: its state
+    leaves the solenoidal set within a stage and its continuity fix projects
+    along the stored k, which the stretch changes.)"""
+    T = 5
+    ou, os_, scale = _shear_pair(oracle_mod, 1, T)
+    tu = ts = 0.0
+    for s in range(1, T + 1):
+        tu, ts = ou.step(s, tu), os_.step(s, ts)
+    assert tu == ts
+    su, ss = ou.get_state(), os_.get_state()
+    for c in range(3):
+        assert rel(ss[c][0::2], scale[c] * su[c]) < 1e-14
+    bu, bs = ou.get_frame()[0], os_.get_frame()[0]
+    assert bs[6] == pytest.approx(0.5 * bu[6], rel=1e-15)  # bmat(1,3), column-major
+
+
+def test_stretched_shear_remesh_keeps_the_frame_within_half_a_unit(oracle_mod):
+    """B110 = 2, F_REMESH_T = 5, 22 steps: the frame's shear entry bmat(1,3) =
+    -S t bmat(1,1) stays within [-1/2, 1/2]; each remesh raises it by exactly 1
+    (so the stored k3 move by whole multiples of i and keep their laboratory
+    wavenumbers), and a remesh happens exactly when the entry would leave the
+    interval (strain B110 / 2 since the last one)."""
+    T = 5
+    _, o, _ = _shear_pair(oracle_mod, 1, T)
+    t = 0.0
+    prev = o.get_frame()[0][6]
+    jumps = []
+    for s in range(1, 4 * T + 3):
+        t = o.step(s, t)
+        b = o.get_frame()[0][6]
+        assert -0.5 - 1e-12 <= b <= 0.5 + 1e-12
+        d = b - prev
+        # 0.05 = S dt bmat(1,1) with dt = (1/2S)/F_REMESH_T
+        if abs(d + 0.05) > 1e-12:
+            assert d == pytest.approx(1.0 - 0.05, abs=1e-12)
+            assert prev == pytest.approx(-0.5, abs=1e-12)
+            jumps.append(s)
+        prev = b
+    assert jumps == [2 * T + 1]
+
+
+def test_stretched_shear_remesh_keeps_the_laboratory_energy(oracle_mod):
+    """Across the stretched box's remesh the laboratory-frame energy changes only
+    by that of the modes the re-indexing drops (found by enumerating the map)."""
+    T = 3
+    _, o, _ = _shear_pair(oracle_mod, 1, T)
+    t = 0.0
+    for s in range(1, 2 * T + 1):
+        t = o.step(s, t)
+    c = oracle_mod.Oracle(nx=2 * M, ny=M, nz=M, b110=2, f_type=2, ns_deformed=1, rk3_timestep=1,
+                          deform_s=1.0, remesh_t=T, nu=0.01)
+    c.set_state(*o.get_state())
+    c.set_frame(*o.get_frame())
+    u = c.get_state()
+    kmax = c.nkk // 2
+    lost = np.zeros(c.shape, bool)
+    for i in range(c.nki):
+        for k in range(c.nkk):
+            if oracle_mod.remesh_target(kmax, i, k) < 0:
+                lost[i, :, k] = True
+    b = np.asarray(c.get_frame()[1]).reshape(3, 3, order="F")  # binv
+    ul = [sum(b[m, n] * u[n] for n in range(3)) for m in range(3)]
+    w = np.where(np.arange(c.nki) > 0, 2.0, 1.0)[:, None, None]
+    e_lost = 0.5 * sum((w * np.abs(a) ** 2)[lost].sum() for a in ul)
+    e0 = c.stats()[1]["tke"]
+    assert c.remesh(2 * T + 1)
+    e1 = c.stats()[1]["tke"]
+    assert e_lost > 1e-8 * e0  # the map drops something
+    assert e1 == pytest.approx(e0 - e_lost, rel=1e-12)

@@ -1,8 +1,10 @@
 """GPU parity of the deformed isotropic form (F_TYPE = 5 over BF_ROT, R_ROT and
 R_CONV, boxes stretched by B110 / B220: src/pf_rk3.f90:2567-2648, 2751-2828 with
 f_RK3BFRotForcing 3256-3349 and its R_ROT twin, the DECIDE step 3905-3919, the frame of src/pg_domain.f90:177-216,
-the spectrum bins of src/pf_fluidstats.f90:348-374) against the oracle, whose
-deformed form is pinned by tests/test_deformed.py.  FP64 <= 1e-12 norm-wise
+the spectrum bins of src/pf_fluidstats.f90:348-374), and the shear forms on a
+stretched box (the remesh clock of B110 F_REMESH_T, src/pf_rk3.f90:3958-4178,
+src/pg_rk3.f90:280-385), against the oracle, whose deformed and stretched forms
+are pinned by tests/test_deformed.py.  FP64 <= 1e-12 norm-wise
 (tracers <= 1e-11), FP32-mixed <= 1e-5 (tracers <= 1e-6); statistics <= 1e-11."""
 import sys
 
@@ -147,3 +149,46 @@ def test_deformed_decide_step_against_oracle(oracle_mod):
             assert rel(a, c) < 1e-12
         g.close()
         o.close()
+
+
+@pytest.mark.parametrize("shape,ns", [((48, 24, 24), 1), ((48, 24, 24), 3), ((256, 128, 128), 1)],
+                         ids=["48x24x24-BF_ROT", "48x24x24-R_ROT", "256x128x128-BF_ROT"])
+def test_stretched_shear_with_remesh_against_oracle(oracle_mod, shape, ns):
+    """Shear DNS on a box stretched by B110 = 2 (S = 10, DECIDE with F_REMESH_T = 3,
+    so dt = 1/60 and the remesh comes at step B110 F_REMESH_T + 1 = 7), 2000
+    tracers, 8 steps: FP64 <= 1e-12 and the frame bit for bit, FP32-mixed <= 1e-5."""
+    nx, ny, nz = shape
+    note(f"stretched shear {shape} B110=2 NS_DEFORMED={ns}: remesh at step 7, 8 steps")
+    kw = dict(f_type=2, ns_deformed=ns, rk3_timestep=1, deform_s=10.0, remesh_t=3, nu=0.005, b110=2)
+    u = _field(nx, ny, nz, (2, 1, 1), 90 + nx)
+    if ns != 1:
+        u = [u[0] * 2.0, u[1], u[2]]
+    xp = _tracers(2000, 5 + nx, nx)
+    o = oracle_mod.Oracle(nx=nx, ny=ny, nz=nz, **kw)
+    o.set_state(*u)
+    o.set_particles(xp)
+    to = 0.0
+    for s in range(1, 9):
+        to = o.step(s, to)
+    assert all(np.isfinite(a).all() for a in o.get_state())
+    so, fo = o.get_state(), o.get_frame()
+    xo, _ = o.get_particles()
+    o.close()
+    for prec in (pa.PDR_FP64, pa.PDR_FP32_MIXED):
+        g = pa.Pandora(nx=nx, ny=ny, nz=nz, precision=prec, **kw)
+        g.set_state(*u)
+        g.set_particles(xp)
+        tg = 0.0
+        for s in range(1, 9):
+            tg = g.rk3_calc(s, tg)
+        assert tg == to
+        fg = g.get_frame()
+        assert np.array_equal(fg[0], fo[0]) and np.array_equal(fg[1], fo[1])
+        xg, _ = g.get_particles()
+        d = np.abs(xg - xo)
+        d = np.minimum(d, TWOPI - d)
+        tol, ttol = (1e-12, 1e-11) if prec == pa.PDR_FP64 else (1e-5, 1e-6)
+        for a, c in zip(g.get_state(), so):
+            assert rel(a, c) < tol
+        assert d.max() < ttol
+        g.close()
