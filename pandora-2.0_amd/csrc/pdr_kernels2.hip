@@ -125,10 +125,12 @@ constexpr int lds_yf_bytes(int n) {
 #ifndef PDR_X_TWG_MIN
 #define PDR_X_TWG_MIN 1024
 #endif
+template <typename T>
 constexpr bool x_tw_global(int n) { return n > PDR_X_TWG_MIN; }
 template <typename T>
 constexpr int lds_x_bytes(int n) {
-  return ((x_tw_global(n) ? 0 : n) + 3 * xrows<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>)) + kBlock2 * int(sizeof(T));
+  return ((x_tw_global<T>(n) ? 0 : n) + 3 * xrows<T>(n) * lds_ld<T>(n)) * int(sizeof(cx<T>)) +
+         (kBlock2 / 64) * int(sizeof(T));
 }
 // Streaming (non-temporal) stores per kernel, bit mask: 1 zinv, 2 yinv,
 // 4 xpass, 8 yfwd, 16 zfwd.  Every pass writes data that only the next pass
@@ -521,8 +523,8 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   constexpr int NKI = KI + 1;
   constexpr int IT = (RB * N + XT - 1) / XT;  // product items per thread
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ T red[XT];
-  constexpr bool TWG = x_tw_global(N);
+  __shared__ T red[XT / 64];  // the CFL maximum of each wave
+  constexpr bool TWG = x_tw_global<T>(N);
   cx<T>* const lds0 = reinterpret_cast<cx<T>*>(smem);
   const cx<T>* const tw = TWG ? twg : lds0;
   cx<T>* const buf = TWG ? lds0 : lds0 + N;
@@ -1008,7 +1010,7 @@ size_t lds_y() {
 }
 template <typename T, int N>
 size_t lds_x() {
-  return size_t(lds_x_bytes<T>(N) - kBlock2 * int(sizeof(T)));  // dynamic part
+  return size_t(lds_x_bytes<T>(N) - (kBlock2 / 64) * int(sizeof(T)));  // dynamic part
 }
 
 hipError_t big_lds(const void* fn, size_t bytes) {
@@ -1379,7 +1381,7 @@ int spec_forms(int prec, int nx, int ny, int nz, int out[5]) {
     using T = decltype(tag);
     constexpr int n = decltype(nn)::value;
     out[0] = xrows<T>(n);
-    out[1] = x_tw_global(n) ? 1 : 0;
+    out[1] = x_tw_global<T>(n) ? 1 : 0;
     ++ok;
   };
   auto fz = [&](auto tag, auto nn) {
