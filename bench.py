@@ -462,6 +462,8 @@ def rocpd_summary(trace_db, fetch_db, write_db, out_json, sq_dbs=(), copy_bytes=
         m = re.search(r"k2?_[a-z0-9_]+<(float|double)", name)
         if m:
             base += "<f32>" if m.group(1) == "float" else "<f64>"
+        if re.search(r"k2_xpass<[a-z]+, \d+, true>", name):
+            base += " with moments"  # the stage-1 x pass of a step with statistics
         a = agg.setdefault(base, [0, 0.0, 1e30, 0.0])
         a[0] += 1
         a[1] += dur
