@@ -80,7 +80,11 @@ __device__ __forceinline__ int lds_off(int o) {
 // allow it (all passes of power-of-two lengths), which keeps the address
 // registers of the butterflies few.  Twiddles: PPT = true reads the per-pass
 // table of load_tw (LDS), false the plain table exp(-2 pi i m / N) (global).
-template <typename T, int N, int LINES, int LD, int R, int NS, int SIGN, bool PPT, int NT = kBlock2>
+// ZB > 0 declares the band ZB < o < N - ZB of every line zero on entry (the
+// modes a 2/3 truncation drops): the pass then reads none of it, so the caller
+// need not write it; positions of the band whose butterfly input is known zero
+// at compile time cost nothing, the others a select.
+template <typename T, int N, int LINES, int LD, int R, int NS, int SIGN, bool PPT, int NT = kBlock2, int ZB = 0>
 __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
   constexpr int NBL = N / R;
   constexpr int TOTAL = NBL * LINES;
@@ -95,7 +99,19 @@ __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __
     if (TOTAL % NT == 0 || t < TOTAL) {
       const int line = t / NBL;
       const int j = t - line * NBL;
-      if constexpr (NBL % P2 == 0) {
+      if constexpr (ZB > 0) {
+        static_assert(NS == 1, "the zero band is known only on entry to the first pass");
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int lo = r * NBL, hi = r * NBL + NBL - 1, o = j + r * NBL;
+          if (lo > ZB && hi < N - ZB)
+            v[b][r] = mk<T>(T(0), T(0));
+          else if (hi <= ZB || lo >= N - ZB)
+            v[b][r] = buf[line * LD + lds_off<T>(o)];
+          else
+            v[b][r] = (o > ZB && o < N - ZB) ? mk<T>(T(0), T(0)) : buf[line * LD + lds_off<T>(o)];
+        }
+      } else if constexpr (NBL % P2 == 0) {
         const cx<T>* s = buf + line * LD + lds_off<T>(j);
 #pragma unroll
         for (int r = 0; r < R; ++r) v[b][r] = s[r * (NBL + (NBL >> PS))];
@@ -144,12 +160,13 @@ __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __
   __syncthreads();
 }
 
-template <typename T, int N, int LINES, int LD, int SIGN, bool PPT = true, int NT = kBlock2, int P = 0, int NS = 1>
+template <typename T, int N, int LINES, int LD, int SIGN, bool PPT = true, int NT = kBlock2, int ZB = 0, int P = 0,
+          int NS = 1>
 __device__ __forceinline__ void fft_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
   if constexpr (P < plan_npass(N)) {
     constexpr int R = plan_radix(N, P);
-    pass_ip<T, N, LINES, LD, R, NS, SIGN, PPT, NT>(buf, tw);
-    fft_ip<T, N, LINES, LD, SIGN, PPT, NT, P + 1, NS * R>(buf, tw);
+    pass_ip<T, N, LINES, LD, R, NS, SIGN, PPT, NT, P == 0 ? ZB : 0>(buf, tw);
+    fft_ip<T, N, LINES, LD, SIGN, PPT, NT, ZB, P + 1, NS * R>(buf, tw);
   }
 }
 

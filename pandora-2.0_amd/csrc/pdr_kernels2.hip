@@ -332,7 +332,6 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
   constexpr int LD = lds_ld<T>(N);
   constexpr int K = N / 3;
   constexpr int NKK = 2 * K + 1;
-  constexpr int QE = (L * N + ZT - 1) / ZT;
   extern __shared__ __align__(16) unsigned char smem[];
   cx<T>* tw = reinterpret_cast<cx<T>*>(smem);
   cx<T>* buf = tw + N;
@@ -343,15 +342,18 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
   load_tw<T, N, ZT>(tw, twg);
   const long long base = ((long long)i * nkj + j0) * NKK;
   const cx<T>* uu[3] = {u0 + base, u1 + base, u2 + base};
-  cx<T> ua[QE], ub[QE], uc[QE];  // u1, u2, u3 of this thread's elements
+  // A thread keeps the stored modes e = tid + ZT q of the tile (line e / NKK,
+  // mode k = e % NKK): the tile's velocity is one contiguous run of L NKK modes
+  // per component, and the truncated band of each line is neither held nor
+  // written to the tile (the first FFT pass takes it as zero).
+  constexpr int EM = L * NKK;
+  constexpr int QM = (EM + ZT - 1) / ZT;
+  cx<T> ua[QM], ub[QM], uc[QM];  // u1, u2, u3 of this thread's modes
 #pragma unroll
-  for (int q = 0; q < QE; ++q) {
-    const int idx = int(threadIdx.x) + ZT * q;
-    const int line = idx / N;
-    const int z = idx - line * N;
-    const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : -1);
-    const bool ok = (L * N % ZT == 0 || idx < L * N) && j0 + line < nkj && k >= 0;
-    const int e = line * NKK + k;
+  for (int q = 0; q < QM; ++q) {
+    const int e = int(threadIdx.x) + ZT * q;
+    const int line = e / NKK;
+    const bool ok = (EM % ZT == 0 || e < EM) && j0 + line < nkj;
     ua[q] = ok ? get<kNrZinv>(&uu[0][e]) : czero<T>();
     ub[q] = ok ? get<kNrZinv>(&uu[1][e]) : czero<T>();
     uc[q] = ok ? get<kNrZinv>(&uu[2][e]) : czero<T>();
@@ -360,11 +362,12 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
 #pragma unroll
   for (int f = 0; f < 6; ++f) {  // unrolled: every field selection is static
 #pragma unroll
-    for (int q = 0; q < QE; ++q) {
-      const int idx = int(threadIdx.x) + ZT * q;
-      if (L * N % ZT == 0 || idx < L * N) {
-        const int line = idx / N;
-        const int z = idx - line * N;
+    for (int q = 0; q < QM; ++q) {
+      const int e = int(threadIdx.x) + ZT * q;
+      if (EM % ZT == 0 || e < EM) {
+        const int line = e / NKK;
+        const int k = e - line * NKK;
+        const int z = (k <= K) ? k : k + (N - NKK);
         const cx<T> un[3] = {ua[q], ub[q], uc[q]};
         cx<T> v;
         if (f < 3) {
@@ -372,7 +375,6 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
         } else {
           // w_c = i sum_n (G[c][n] . k) u_n, the same operations as k2_zinv
           const int c = f - 3;
-          const int k = (z <= K) ? z : ((z >= N - K) ? z - (N - NKK) : 0);
           const double kv[3] = {wi, wvn2(j0 + line, g.kj), wvn2(k, K)};
           cx<T> acc = czero<T>();
 #pragma unroll
@@ -387,7 +389,7 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
       }
     }
     __syncthreads();
-    fft_ip<T, N, L, LD, +1, true, ZT>(buf, tw);
+    fft_ip<T, N, L, LD, +1, true, ZT, K>(buf, tw);
     cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
     for (int idx = threadIdx.x; idx < L * N; idx += ZT) {
       const int z = idx / L;
@@ -429,9 +431,26 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   const int nki = g.nki;
   const int lstride = g.nz * g.ldj;
   load_tw<T, N, YT>(tw, twg);
-  cx<T> pre[QE];
+  // Tiles of i-lines: a thread takes the stored modes e = tid + YT q of the tile
+  // (line e / NKJ, mode j = e % NKJ), and the truncated band of each line is not
+  // written to the tile (the first FFT pass takes it as zero).  Whole-plane
+  // tiles fill every position.
+  constexpr int EM = L * NKJ;
+  constexpr int QM = PP > 0 ? QE : (EM + YT - 1) / YT;
+  cx<T> pre[QM];
   auto load = [&](int f) {
     const cx<T>* src = w1 + (long long)f * g.m1 + ((long long)i0 * g.nz + z) * g.ldj;
+    if constexpr (PP == 0) {
+#pragma unroll
+      for (int q = 0; q < QM; ++q) {
+        const int e = int(threadIdx.x) + YT * q;
+        const int line = e / NKJ;
+        cx<T> v = czero<T>();
+        if ((EM % YT == 0 || e < EM) && i0 + line < nki) v = get<kNrYinv>(&src[line * lstride + (e - line * NKJ)]);
+        pre[q] = v;
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < QE; ++q) {
       const int idx = int(threadIdx.x) + YT * q;
@@ -454,16 +473,28 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   };
   for (int f = fg * fpb; f < (fg + 1) * fpb; ++f) {
     load(f);
+    if constexpr (PP == 0) {
 #pragma unroll
-    for (int q = 0; q < QE; ++q) {
-      const int idx = int(threadIdx.x) + YT * q;
-      if (L * N % YT == 0 || idx < L * N) {
-        const int line = idx / N;
-        buf[line * LD + lds_off<T>(idx - line * N)] = pre[q];
+      for (int q = 0; q < QM; ++q) {
+        const int e = int(threadIdx.x) + YT * q;
+        if (EM % YT == 0 || e < EM) {
+          const int line = e / NKJ;
+          const int j = e - line * NKJ;
+          buf[line * LD + lds_off<T>((j <= K) ? j : j + (N - NKJ))] = pre[q];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < QE; ++q) {
+        const int idx = int(threadIdx.x) + YT * q;
+        if (L * N % YT == 0 || idx < L * N) {
+          const int line = idx / N;
+          buf[line * LD + lds_off<T>(idx - line * N)] = pre[q];
+        }
       }
     }
     __syncthreads();
-    fft_ip<T, N, L, LD, +1, true, YT>(buf, tw);
+    fft_ip<T, N, L, LD, +1, true, YT, (PP > 0 ? 0 : K)>(buf, tw);
     cx<T>* dst = w2 + (long long)f * g.m2 + (long long)zl * N * g.ldi + i0;
     for (int idx = threadIdx.x; idx < L * N; idx += YT) {
       const int y = idx / L;
@@ -539,8 +570,8 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   // 1: u3 + i w1, 2: w2 + i w3).  Each stored mode m' = 0..KI of a pair (a, b) is
   // loaded once and written to both m' and its Hermitian image N - m'
   // (conj(a) + i conj(b)); c2r uses only the real part of the DC term; the band
-  // KI < m < N - KI is zero.  All loads of a thread precede its LDS stores.
-  constexpr int NZB = N - 2 * KI - 1;
+  // KI < m < N - KI is zero and not written (the first FFT pass takes it as
+  // zero).  All loads of a thread precede its LDS stores.
   if constexpr (NKI >= 128) {
     // long lines: thread m takes mode m of every line, so the global row bases
     // are workgroup-uniform, the LDS line offsets compile-time constants and
@@ -574,11 +605,6 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
           }
         }
       }
-    }
-    for (int mm = threadIdx.x; mm < NZB; mm += XT) {
-      const int o = lds_off<T>(KI + 1 + mm);
-#pragma unroll
-      for (int line = 0; line < NL; ++line) buf[line * LD + o] = czero<T>();
     }
   } else {
     constexpr int EL = 3 * RB * NKI;
@@ -614,13 +640,9 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
         }
       }
     }
-    for (int idx = threadIdx.x; idx < 3 * RB * NZB; idx += XT) {
-      const int line = idx / NZB;
-      buf[line * LD + lds_off<T>(KI + 1 + idx - line * NZB)] = czero<T>();
-    }
   }
   __syncthreads();
-  fft_ip<T, N, 3 * RB, LD, +1, !TWG, XT>(buf, tw);
+  fft_ip<T, N, 3 * RB, LD, +1, !TWG, XT, KI>(buf, tw);
   T V[9];
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
