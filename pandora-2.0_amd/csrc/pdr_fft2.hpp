@@ -81,9 +81,10 @@ __device__ __forceinline__ int lds_off(int o) {
 // registers of the butterflies few.  Twiddles: PPT = true reads the per-pass
 // table of load_tw (LDS), false the plain table exp(-2 pi i m / N) (global).
 // ZB > 0 declares the band ZB < o < N - ZB of every line zero on entry (the
-// modes a 2/3 truncation drops): the pass then reads none of it, so the caller
-// need not write it; positions of the band whose butterfly input is known zero
-// at compile time cost nothing, the others a select.
+// modes a 2/3 truncation drops): the inputs r of a butterfly whose whole block
+// r NBL .. r NBL + NBL - 1 lies in the band are zero at compile time and not
+// read, so the caller writes zeros only to the band positions of the blocks
+// that hold stored modes as well (zero_band_edges).
 template <typename T, int N, int LINES, int LD, int R, int NS, int SIGN, bool PPT, int NT = kBlock2, int ZB = 0>
 __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __restrict__ tw) {
   constexpr int NBL = N / R;
@@ -99,25 +100,18 @@ __device__ __forceinline__ void pass_ip(cx<T>* __restrict__ buf, const cx<T>* __
     if (TOTAL % NT == 0 || t < TOTAL) {
       const int line = t / NBL;
       const int j = t - line * NBL;
-      if constexpr (ZB > 0) {
-        static_assert(NS == 1, "the zero band is known only on entry to the first pass");
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const int lo = r * NBL, hi = r * NBL + NBL - 1, o = j + r * NBL;
-          if (lo > ZB && hi < N - ZB)
-            v[b][r] = mk<T>(T(0), T(0));
-          else if (hi <= ZB || lo >= N - ZB)
-            v[b][r] = buf[line * LD + lds_off<T>(o)];
-          else
-            v[b][r] = (o > ZB && o < N - ZB) ? mk<T>(T(0), T(0)) : buf[line * LD + lds_off<T>(o)];
-        }
-      } else if constexpr (NBL % P2 == 0) {
+      static_assert(ZB == 0 || NS == 1, "the zero band is known only on entry to the first pass");
+      if constexpr (NBL % P2 == 0) {
         const cx<T>* s = buf + line * LD + lds_off<T>(j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = s[r * (NBL + (NBL >> PS))];
+        for (int r = 0; r < R; ++r)
+          v[b][r] = (ZB > 0 && r * NBL > ZB && r * NBL + NBL - 1 < N - ZB) ? mk<T>(T(0), T(0))
+                                                                           : s[r * (NBL + (NBL >> PS))];
       } else {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = buf[line * LD + lds_off<T>(j + r * NBL)];
+        for (int r = 0; r < R; ++r)
+          v[b][r] = (ZB > 0 && r * NBL > ZB && r * NBL + NBL - 1 < N - ZB) ? mk<T>(T(0), T(0))
+                                                                           : buf[line * LD + lds_off<T>(j + r * NBL)];
       }
     }
   }
@@ -167,6 +161,24 @@ __device__ __forceinline__ void fft_ip(cx<T>* __restrict__ buf, const cx<T>* __r
     constexpr int R = plan_radix(N, P);
     pass_ip<T, N, LINES, LD, R, NS, SIGN, PPT, NT, P == 0 ? ZB : 0>(buf, tw);
     fft_ip<T, N, LINES, LD, SIGN, PPT, NT, ZB, P + 1, NS * R>(buf, tw);
+  }
+}
+
+// Zeros into the band positions ZB < o < N - ZB of LINES lines that the first
+// pass of fft_ip<..., ZB> reads: those in the first-pass blocks (N / R0
+// positions each) that also hold stored modes, at most two runs per line
+// ([A, E1] and [S2, B]).  The caller synchronises before the transform.
+template <typename T, int N, int LINES, int LD, int ZB, int NT = kBlock2>
+__device__ __forceinline__ void zero_band_edges(cx<T>* __restrict__ buf) {
+  constexpr int NBL = N / plan_radix(N, 0);
+  constexpr int A = ZB + 1, B = N - ZB - 1;  // the band, inclusive
+  constexpr int E1 = (A % NBL == 0) ? A - 1 : ((A / NBL + 1) * NBL - 1 < B ? (A / NBL + 1) * NBL - 1 : B);
+  constexpr int S2 = ((B + 1) % NBL == 0) ? B + 1 : ((B / NBL) * NBL > E1 ? (B / NBL) * NBL : E1 + 1);
+  constexpr int C1 = E1 - A + 1, C = C1 + (B - S2 + 1);
+  static_assert(ZB > 0 && A <= B && C1 >= 0 && C >= C1, "a band inside the line");
+  for (int t = threadIdx.x; t < LINES * C; t += NT) {
+    const int line = t / C, p = t - line * C;
+    buf[line * LD + lds_off<T>(p < C1 ? A + p : S2 + (p - C1))] = mk<T>(T(0), T(0));
   }
 }
 

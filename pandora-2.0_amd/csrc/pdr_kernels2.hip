@@ -344,8 +344,8 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
   const cx<T>* uu[3] = {u0 + base, u1 + base, u2 + base};
   // A thread keeps the stored modes e = tid + ZT q of the tile (line e / NKK,
   // mode k = e % NKK): the tile's velocity is one contiguous run of L NKK modes
-  // per component, and the truncated band of each line is neither held nor
-  // written to the tile (the first FFT pass takes it as zero).
+  // per component, and of the truncated band of each line only the positions
+  // the first FFT pass reads are written (zeros).
   constexpr int EM = L * NKK;
   constexpr int QM = (EM + ZT - 1) / ZT;
   cx<T> ua[QM], ub[QM], uc[QM];  // u1, u2, u3 of this thread's modes
@@ -388,6 +388,7 @@ __device__ __forceinline__ void k2_zinv_r_body(unsigned bid, const Geo& g, const
         buf[line * LD + lds_off<T>(z)] = v;
       }
     }
+    zero_band_edges<T, N, L, LD, K, ZT>(buf);
     __syncthreads();
     fft_ip<T, N, L, LD, +1, true, ZT, K>(buf, tw);
     cx<T>* dst = w1 + (long long)f * g.m1 + (long long)i * N * g.ldj + j0;
@@ -432,8 +433,8 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   const int lstride = g.nz * g.ldj;
   load_tw<T, N, YT>(tw, twg);
   // Tiles of i-lines: a thread takes the stored modes e = tid + YT q of the tile
-  // (line e / NKJ, mode j = e % NKJ), and the truncated band of each line is not
-  // written to the tile (the first FFT pass takes it as zero).  Whole-plane
+  // (line e / NKJ, mode j = e % NKJ); of the truncated band of each line only
+  // the positions the first FFT pass reads are written (zeros).  Whole-plane
   // tiles fill every position.
   constexpr int EM = L * NKJ;
   constexpr int QM = PP > 0 ? QE : (EM + YT - 1) / YT;
@@ -483,6 +484,7 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
           buf[line * LD + lds_off<T>((j <= K) ? j : j + (N - NKJ))] = pre[q];
         }
       }
+      zero_band_edges<T, N, L, LD, K, YT>(buf);
     } else {
 #pragma unroll
       for (int q = 0; q < QE; ++q) {
@@ -570,8 +572,8 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
   // 1: u3 + i w1, 2: w2 + i w3).  Each stored mode m' = 0..KI of a pair (a, b) is
   // loaded once and written to both m' and its Hermitian image N - m'
   // (conj(a) + i conj(b)); c2r uses only the real part of the DC term; the band
-  // KI < m < N - KI is zero and not written (the first FFT pass takes it as
-  // zero).  All loads of a thread precede its LDS stores.
+  // KI < m < N - KI is zero, and only its positions that the first FFT pass
+  // reads are written.  All loads of a thread precede its LDS stores.
   if constexpr (NKI >= 128) {
     // long lines: thread m takes mode m of every line, so the global row bases
     // are workgroup-uniform, the LDS line offsets compile-time constants and
@@ -641,6 +643,7 @@ __device__ __forceinline__ void k2_xpass_body(unsigned bid, const Geo& g, const 
       }
     }
   }
+  zero_band_edges<T, N, 3 * RB, LD, KI, XT>(buf);
   __syncthreads();
   fft_ip<T, N, 3 * RB, LD, +1, !TWG, XT, KI>(buf, tw);
   T V[9];
