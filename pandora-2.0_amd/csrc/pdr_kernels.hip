@@ -25,6 +25,7 @@
 #include "pdr_rng.hpp"
 #include "pdr_rsstats.hpp"
 #include "pdr_formf.hpp"
+#include "pdr_fft2.hpp"
 
 namespace pdr {
 
@@ -614,13 +615,21 @@ __global__ void __launch_bounds__(kThreads)
 // c < 3 stores component c of the results.
 constexpr int tracer_group(int p) { return p <= 4 ? 4 : 8; }
 
+// Workgroups take consecutive tracer slots; with the slots sorted by grid row,
+// the workgroups of one XCD (renumbered by xcd_remap) then walk one contiguous
+// slab of rows, and the stencil rows that neighbouring workgroups share are
+// fetched into that XCD's L2 once instead of into several.
+#ifndef PDR_TR_XCD
+#define PDR_TR_XCD 1
+#endif
 template <typename T, int P>
 __global__ void __launch_bounds__(kThreads)
     k_tracers_col(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
                   double* __restrict__ uprk, double* __restrict__ upa, double* __restrict__ xp0,
                   long long np, double beta, double gdt, int wrap) {
   constexpr int G = tracer_group(P);
-  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned blk = PDR_TR_XCD ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+  const long long gt = (long long)blk * blockDim.x + threadIdx.x;
   const long long q = gt / G;
   const int a = int(gt % G);
   const bool live = q < np;
