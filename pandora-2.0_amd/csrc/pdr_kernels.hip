@@ -622,8 +622,17 @@ constexpr int tracer_group(int p) { return p <= 4 ? 4 : 8; }
 #ifndef PDR_TR_XCD
 #define PDR_TR_XCD 1
 #endif
+#ifndef PDR_TR_RCP
+#define PDR_TR_RCP 1
+#endif
+// Four waves per SIMD (at most 128 registers) for the orders up to 4: the gather
+// is latency-bound, and without the hint the FP64 form takes 134 registers and
+// three waves.
+#ifndef PDR_TR_WAVES
+#define PDR_TR_WAVES 4
+#endif
 template <typename T, int P>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(P <= 4 ? PDR_TR_WAVES : 1)))
     k_tracers_col(Geo g, TracerFrame tf, const T* __restrict__ ur, double* __restrict__ xpa,
                   double* __restrict__ uprk, double* __restrict__ upa, double* __restrict__ xp0,
                   long long np, double beta, double gdt, int wrap) {
@@ -662,8 +671,21 @@ __global__ void __launch_bounds__(kThreads)
     iy[m] = double(j0 + m) * tf.dy;
     iz[m] = double(k0 + m) * tf.dz;
   }
+  // Each tableau entry is ((d1 v) + (d2 v')) / d3 with d3 a node difference that
+  // depends only on (col, nv) and the axis: the P (P - 1) / 2 reciprocals of an
+  // axis are formed once and the entries multiply by them (one FP64 division per
+  // (col, nv) instead of one per row, component and entry; the entries then
+  // differ from the quotients by at most an ulp of rounding).
+  auto recips = [&](const double* nd, double (&rd)[P][P]) {
+#pragma unroll
+    for (int col = 1; col <= P - 1; ++col)
+#pragma unroll
+      for (int nv = 1; nv <= P - col; ++nv) rd[col][nv - 1] = 1.0 / (nd[nv - 1] - nd[nv + col - 1]);
+  };
   double vc[3] = {0.0, 0.0, 0.0};  // this lane's column after the y and z passes
   if (a < P) {
+    double ry[P][P];
+    if (PDR_TR_RCP) recips(iy, ry);
     const long long gx3 = 3 * pmod(i0 + a, g.nx);
     double vz[3][P];  // [component][z]
 #pragma unroll
@@ -689,11 +711,14 @@ __global__ void __launch_bounds__(kThreads)
             const double d1 = x[1] - iy[nv + col - 1];
             const double d2 = iy[nv - 1] - x[1];
             const double d3 = iy[nv - 1] - iy[nv + col - 1];
-            v[nv - 1] = ((d1 * v[nv - 1]) + (d2 * v[nv])) / d3;
+            v[nv - 1] = PDR_TR_RCP ? ((d1 * v[nv - 1]) + (d2 * v[nv])) * ry[col][nv - 1]
+                                   : ((d1 * v[nv - 1]) + (d2 * v[nv])) / d3;
           }
         vz[c][qz] = v[0];
       }
     }
+    double rz[P][P];
+    if (PDR_TR_RCP) recips(iz, rz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
@@ -703,7 +728,8 @@ __global__ void __launch_bounds__(kThreads)
           const double d1 = x[2] - iz[nv + col - 1];
           const double d2 = iz[nv - 1] - x[2];
           const double d3 = iz[nv - 1] - iz[nv + col - 1];
-          vz[c][nv - 1] = ((d1 * vz[c][nv - 1]) + (d2 * vz[c][nv])) / d3;
+          vz[c][nv - 1] = PDR_TR_RCP ? ((d1 * vz[c][nv - 1]) + (d2 * vz[c][nv])) * rz[col][nv - 1]
+                                     : ((d1 * vz[c][nv - 1]) + (d2 * vz[c][nv])) / d3;
         }
       vc[c] = vz[c][0];
     }
@@ -711,6 +737,10 @@ __global__ void __launch_bounds__(kThreads)
   // x pass over the P columns of the group
   const int lane0 = int(threadIdx.x & 63u) - a;  // first lane of the group within the wave
   double uf[3];
+  double ix[P], rx[P][P];
+#pragma unroll
+  for (int m = 0; m < P; ++m) ix[m] = double(i0 + m) * tf.dx;
+  if (PDR_TR_RCP) recips(ix, rx);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     double vx[P];
@@ -720,11 +750,12 @@ __global__ void __launch_bounds__(kThreads)
     for (int col = 1; col <= P - 1; ++col)
 #pragma unroll
       for (int nv = 1; nv <= P - col; ++nv) {
-        const double xa = double(i0 + nv + col - 1) * tf.dx, xb = double(i0 + nv - 1) * tf.dx;
+        const double xa = ix[nv + col - 1], xb = ix[nv - 1];
         const double d1 = x[0] - xa;
         const double d2 = xb - x[0];
         const double d3 = xb - xa;
-        vx[nv - 1] = ((d1 * vx[nv - 1]) + (d2 * vx[nv])) / d3;
+        vx[nv - 1] = PDR_TR_RCP ? ((d1 * vx[nv - 1]) + (d2 * vx[nv])) * rx[col][nv - 1]
+                                : ((d1 * vx[nv - 1]) + (d2 * vx[nv])) / d3;
       }
     uf[c] = vx[0];
   }
