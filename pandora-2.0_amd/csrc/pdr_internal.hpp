@@ -190,6 +190,9 @@ struct pdr_ctx {
   bool fuse_zf = true;  // fused z-forward + FormF (specialised path; PDR_FUSE_ZFORMF=0 disables)
   int tj_z = 1, ti_y = 1;
   size_t lds_max = 65536;
+  // workgroups of the statistics pass that the device holds at once (found on
+  // first use: 0 not yet, -1 unknown)
+  mutable int stats_resident = 0;
   hipStream_t stream = nullptr;
   // the tracer advance runs on its own stream beside the forward passes of the
   // stage: ev_ur orders it after the x pass that writes the tracer field,

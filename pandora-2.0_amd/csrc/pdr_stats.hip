@@ -316,17 +316,40 @@ StatsFrame make_stats_frame(const pdr_ctx* c) {
   return sf;
 }
 
-// workgroups of the statistics pass and the width of one partial row
-int stats_nblk(const pdr_ctx* c) {
-  const long long want = (c->g.mw + 4LL * kStatThreads - 1) / (4LL * kStatThreads);
-  return int(want < kStatBlocks ? (want < 1 ? 1 : want) : kStatBlocks);
-}
-int stats_width(const pdr_ctx* c) { return stats_hist(kkmax_of(c), c->g.nki) + 12; }
-
 bool stats_fits_lds(const pdr_ctx* c) {
   const size_t lds = size_t(stats_hist(kkmax_of(c), c->g.nki) + 12 * (kStatThreads / 64)) * sizeof(double);
   return lds <= c->lds_max;
 }
+
+// Workgroups of the statistics pass that the device runs at once (its LDS
+// histogram admits three per CU at 512^3), or -1 if the runtime cannot say.
+static int stats_resident_of(const pdr_ctx* c) {
+  if (!stats_fits_lds(c)) return -1;
+  const size_t lds = size_t(stats_hist(kkmax_of(c), c->g.nki) + 12 * (kStatThreads / 64)) * sizeof(double);
+  const void* f = c->prec ? (const void*)k_stats<float> : (const void*)k_stats<double>;
+  int per_cu = 0, dev = 0, ncu = 0;
+  bool ok = lds <= 65536 || hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)) == hipSuccess;
+  ok = ok && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kStatThreads, lds) == hipSuccess && per_cu > 0;
+  ok = ok && hipGetDevice(&dev) == hipSuccess &&
+       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0;
+  if (!ok) {
+    (void)hipGetLastError();  // leave no error for the next launch check to report
+    return -1;
+  }
+  return per_cu * ncu;
+}
+
+// Workgroups of the statistics pass and the width of one partial row.  Every
+// workgroup takes an equal range of modes, so the pass runs in one round: at
+// most as many workgroups as the device holds at once (1024 at three per CU on
+// 256 CUs would leave a second round a third full).
+int stats_nblk(const pdr_ctx* c) {
+  const long long want = (c->g.mw + 4LL * kStatThreads - 1) / (4LL * kStatThreads);
+  if (c->stats_resident == 0) c->stats_resident = stats_resident_of(c);
+  const long long cap = c->stats_resident > 0 ? std::min<long long>(kStatBlocks, c->stats_resident) : kStatBlocks;
+  return int(want < cap ? (want < 1 ? 1 : want) : cap);
+}
+int stats_width(const pdr_ctx* c) { return stats_hist(kkmax_of(c), c->g.nki) + 12; }
 
 // the reduced row h[stats_width] of the pass -> raw[7 * kkmax] (ken, ken1, ken2,
 // ken3, dis, disskew, count), uu[6] and, on request, the streamwise bins
