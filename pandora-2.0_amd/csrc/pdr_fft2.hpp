@@ -183,6 +183,15 @@ __device__ __forceinline__ void zero_band_edges(cx<T>* __restrict__ buf) {
   }
 }
 
+// Whether position o of a line lies in a first-pass block wholly inside the
+// band ZB < o < N - ZB (a position fft_ip<..., ZB> never reads).
+template <int N, int ZB>
+__device__ __forceinline__ bool in_skipped_block(int o) {
+  constexpr int NBL = N / plan_radix(N, 0);
+  const int lo = (o / NBL) * NBL;
+  return lo > ZB && lo + NBL - 1 < N - ZB;
+}
+
 // Global store / load of one complex element; NT selects the non-temporal
 // (streaming) form, which does not keep the line in the caches.
 template <bool NT, typename T>

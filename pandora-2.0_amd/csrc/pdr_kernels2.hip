@@ -435,7 +435,7 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
   // Tiles of i-lines: a thread takes the stored modes e = tid + YT q of the tile
   // (line e / NKJ, mode j = e % NKJ); of the truncated band of each line only
   // the positions the first FFT pass reads are written (zeros).  Whole-plane
-  // tiles fill every position.
+  // tiles fill every position the first pass reads.
   constexpr int EM = L * NKJ;
   constexpr int QM = PP > 0 ? QE : (EM + YT - 1) / YT;
   cx<T> pre[QM];
@@ -486,17 +486,20 @@ __device__ __forceinline__ void k2_yinv_body(unsigned bid, const Geo& g, const c
       }
       zero_band_edges<T, N, L, LD, K, YT>(buf);
     } else {
+      // whole-plane tiles: every position but those of the skipped band blocks
+      // (the band's other positions hold the zeros of the load)
 #pragma unroll
       for (int q = 0; q < QE; ++q) {
         const int idx = int(threadIdx.x) + YT * q;
         if (L * N % YT == 0 || idx < L * N) {
           const int line = idx / N;
-          buf[line * LD + lds_off<T>(idx - line * N)] = pre[q];
+          const int y = idx - line * N;
+          if (!in_skipped_block<N, K>(y)) buf[line * LD + lds_off<T>(y)] = pre[q];
         }
       }
     }
     __syncthreads();
-    fft_ip<T, N, L, LD, +1, true, YT, (PP > 0 ? 0 : K)>(buf, tw);
+    fft_ip<T, N, L, LD, +1, true, YT, K>(buf, tw);
     cx<T>* dst = w2 + (long long)f * g.m2 + (long long)zl * N * g.ldi + i0;
     for (int idx = threadIdx.x; idx < L * N; idx += YT) {
       const int y = idx / L;
