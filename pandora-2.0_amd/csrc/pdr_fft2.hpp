@@ -176,10 +176,12 @@ __device__ __forceinline__ void zero_band_edges(cx<T>* __restrict__ buf) {
   constexpr int S2 = ((B + 1) % NBL == 0) ? B + 1 : ((B / NBL) * NBL > E1 ? (B / NBL) * NBL : E1 + 1);
   constexpr int C1 = E1 - A + 1, C = C1 + (B - S2 + 1);
   static_assert(ZB > 0 && A <= B && C1 >= 0 && C >= C1, "a band inside the line");
-  for (int p = threadIdx.x; p < C; p += NT) {  // a position of every line, constant line offsets
-    cx<T>* const d = buf + lds_off<T>(p < C1 ? A + p : S2 + (p - C1));
-#pragma unroll
-    for (int line = 0; line < LINES; ++line) d[line * LD] = mk<T>(T(0), T(0));
+  // one (line, position) per thread: short lines have few band positions (3 at
+  // 32 and 48 points) and many lines per tile, which a thread per position
+  // would write one after another
+  for (int t = threadIdx.x; t < LINES * C; t += NT) {
+    const int line = t / C, p = t - line * C;
+    buf[line * LD + lds_off<T>(p < C1 ? A + p : S2 + (p - C1))] = mk<T>(T(0), T(0));
   }
 }
 
