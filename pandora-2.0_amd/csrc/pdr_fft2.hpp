@@ -179,9 +179,11 @@ __device__ __forceinline__ void zero_band_edges(cx<T>* __restrict__ buf) {
   // one (line, position) per thread: short lines have few band positions (3 at
   // 32 and 48 points) and many lines per tile, which a thread per position
   // would write one after another
-  for (int t = threadIdx.x; t < LINES * C; t += NT) {
-    const int line = t / C, p = t - line * C;
-    buf[line * LD + lds_off<T>(p < C1 ? A + p : S2 + (p - C1))] = mk<T>(T(0), T(0));
+  if constexpr (C > 0) {
+    for (int t = threadIdx.x; t < LINES * C; t += NT) {
+      const int line = t / C, p = t - line * C;
+      buf[line * LD + lds_off<T>(p < C1 ? A + p : S2 + (p - C1))] = mk<T>(T(0), T(0));
+    }
   }
 }
 
